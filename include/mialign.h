@@ -1,0 +1,199 @@
+/*
+ * mialign — MI355X (gfx950) seed -> extend -> sort path for linear WGBS reads.
+ *
+ * Plain C ABI: pointers and sizes only. Every call returns 0 on success or a
+ * negative status; ma_last_error() then holds the message (the reference
+ * raises a Mojo Error with the same text role). Host-pointer calls copy in and
+ * out; *_dev calls take device pointers already resident in HBM and enqueue on
+ * the given stream without synchronising.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   ma_index_open ........... LinearIndex._load_dense_pack + index upload,
+ *                             fq2bam-meth/src/linear_index.mojo:217-303 and
+ *                             linear_gpu_locate.mojo:1609-1703
+ *   ma_index_build .......... build_dense_pack, fq2bam-meth/python/mojo_linear_pack.py:208-287
+ *   ma_index_export_dense ... the dense-v1 files written by build_dense_pack (:271-273)
+ *   ma_map_batch(_dev) ...... per-batch body of map_fastq_dense_gpu_locate,
+ *                             linear_gpu_locate.mojo:1836-2720 (pack, FW/RC
+ *                             encode+locate+vote_extend passes, FW/RC merge :2101-2113)
+ *   ma_sort_keys_se_dev ..... SE key builders, linear_gpu_fm.mojo:371-405, 800-823
+ *   ma_sort_markdup(_dev) ... gpu_sort_markdup, fq2bam-meth/src/linear_gpu_sort.mojo:24-41
+ *   ma_seed_kmers ........... seed_kmers_on_device (pack + k-mer + Wang hash),
+ *                             gpu-common/src/gpu_kernels.mojo:81-281
+ *   ma_device_info .......... select_device / probe_device_context + hbm_info,
+ *                             gpu-common/src/gpu_device.mojo:19-63, gpu_kernels.mojo:42-65,
+ *                             gpu-common/python/gpu_mem.py:88-121
+ *   ma_serve_* .............. no single reference counterpart: the request loop
+ *                             generalises the batch loop of map_fastq_dense_gpu_locate
+ *                             with admission by free HBM (gpu_mem.py:203-377)
+ */
+#ifndef MIALIGN_H
+#define MIALIGN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MA_OK 0
+#define MA_ERR_ARG (-1)
+#define MA_ERR_HIP (-2)
+#define MA_ERR_IO (-3)
+#define MA_ERR_NOMEM (-4)
+#define MA_ERR_UNSUPPORTED (-5)
+
+typedef struct ma_index ma_index;
+typedef struct ma_workspace ma_workspace;
+typedef struct ma_server ma_server;
+
+/* Mapping knobs; ma_params_default gives the reference defaults
+ * (linear_gpu_locate.mojo:1729-1751, linear_index.mojo:425-434). max_indel must
+ * be 0 here: the gapless contract (indel branches are not built). */
+typedef struct {
+    int32_t k;         /* k-mer length (index k) */
+    int32_t stride;    /* seed stride (METHYLGRAPHER_LINEAR_SEED_STRIDE, 5) */
+    int32_t max_occ;   /* locate cap (METHYLGRAPHER_LINEAR_MAX_OCC, 16384) */
+    int32_t vote_occ;  /* vote / high-occ split (VOTE_OCC, 256) */
+    int32_t vote_fast; /* pass-0 vote cap (VOTE_OCC_FAST, = vote_occ) */
+    int32_t max_diff;  /* mismatch budget (MAX_DIFF, 6) */
+    int32_t max_soft;  /* soft-clip cap (MAX_SOFT, 8) */
+    int32_t max_indel; /* 0 = gapless */
+    int32_t do_rc;     /* reverse-complement pass (LINEAR_RC, 1) */
+} ma_params;
+
+/* One record per read, structure of arrays (vote_extend_kernel outputs,
+ * linear_gpu_locate.mojo:778-787). cid -1 = unmapped; cid indexes the index's
+ * contigs, so it also names the conversion copy (f* = C->T, r* = G->A). */
+typedef struct {
+    int32_t *cid;
+    uint32_t *pos;  /* 0-based leftmost aligned base on the contig */
+    uint32_t *mapq;
+    int32_t *flag;  /* 0, 16 (reverse) or 4 (unmapped) */
+    uint32_t *sl;   /* soft clip, left (reference orientation) */
+    uint32_t *sr;   /* soft clip, right */
+    uint32_t *nm;   /* mismatches */
+    uint32_t *iop;  /* indel op (always 0 in the gapless contract) */
+    uint32_t *iat;
+    uint32_t *ilen;
+} ma_recs;
+
+enum { MA_ENC_ASCII = 0, MA_ENC_2BIT = 1 };
+enum { MA_BS_NONE = 0, MA_BS_C2T = 1, MA_BS_G2A = 2 };
+
+/* ---- library / device ---- */
+void ma_params_default(ma_params *p);
+const char *ma_last_error(void);
+const char *ma_version(void);
+int ma_device_count(int *n);
+int ma_device_info(int device, char *name, int name_len, int64_t *hbm_total, int64_t *hbm_free,
+                   int32_t *n_cu, int32_t *lds_per_block);
+
+/* ---- device memory, streams and events (so hosts need no other GPU runtime) ---- */
+int ma_dev_alloc(int device, int64_t bytes, void **ptr);
+int ma_dev_free(void *ptr);
+int ma_memcpy_h2d(void *dst, const void *src, int64_t bytes, void *stream);
+int ma_memcpy_d2h(void *dst, const void *src, int64_t bytes, void *stream);
+int ma_memcpy_d2d(void *dst, const void *src, int64_t bytes, void *stream);
+int ma_stream_create(int device, void **stream);
+int ma_stream_destroy(void *stream);
+int ma_stream_sync(void *stream);
+int ma_event_create(void **ev);
+int ma_event_destroy(void *ev);
+int ma_event_record(void *ev, void *stream);
+int ma_event_sync(void *ev);
+int ma_event_elapsed_ms(void *start, void *stop, float *ms);
+int ma_event_query(void *ev); /* 1 = complete, 0 = pending, <0 error */
+/* Pinned host memory (page-locked; lets copies overlap kernels). */
+int ma_host_alloc(int64_t bytes, void **ptr);
+int ma_host_free(void *ptr);
+
+/* ---- index ---- */
+/* Load a dense-v1 directory (meta.json, kmers.bin, offsets.bin, postings.bin,
+ * sequences.bin, contig_offsets.bin) and make it resident on `device`. */
+int ma_index_open(const char *dense_v1_dir, int device, ma_index **out);
+/* Same, from dense-v1 arrays already in host memory. */
+int ma_index_from_dense(int device, int32_t k, const uint64_t *kmers, int64_t n_keys,
+                        const uint64_t *offsets, const uint32_t *postings, int64_t n_postings,
+                        const uint8_t *seq, int64_t n_bases, const uint64_t *contig_off,
+                        int64_t n_contigs, ma_index **out);
+/* Build the dense-v1 index on the GPU from (already converted) ASCII contig
+ * bases; byte-identical to build_dense_pack's kmers/offsets/postings. */
+int ma_index_build(int device, int32_t k, const uint8_t *seq, int64_t n_bases,
+                   const uint64_t *contig_off, int64_t n_contigs, ma_index **out);
+int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int64_t *n_bases,
+                   int64_t *n_contigs, int64_t *device_bytes, int32_t *k);
+/* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
+ * postings[2*n_postings]); any pointer may be NULL. */
+int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings);
+int ma_index_close(ma_index *ix);
+
+/* ---- map ---- */
+/* Workspace for up to max_reads reads of at most max_len bases. */
+int ma_workspace_create(ma_index *ix, int64_t max_reads, int32_t max_len, ma_workspace **out);
+int ma_workspace_destroy(ma_workspace *ws);
+/* Bytes a workspace of this shape holds on the device (for admission). */
+int64_t ma_workspace_bytes(int64_t max_reads, int32_t max_len);
+
+/* Reads in the reference's padded layout: read i at bases + i*stride, lens[i]
+ * bases. encoding MA_ENC_ASCII (stride >= max_len, any byte; non-ACGT = N) or
+ * MA_ENC_2BIT (4 bases/byte, base 4j in bits 7-6 of byte j, A0 C1 G2 T3).
+ * In-silico conversion bs_r1 applies to reads [0, n_r1), bs_r2 to [n_r1, n)
+ * (R1 then R2, linear_gpu_locate.mojo:1838-1876). Outputs: merged records
+ * (required); fw / rc per-strand records (may be NULL). Host pointers. */
+int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint32_t *lens,
+                 int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
+                 int64_t n_r1, const ma_params *p, ma_recs *merged, ma_recs *fw, ma_recs *rc);
+/* Device-pointer form; record arrays are device pointers; enqueued on stream. */
+int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t *d_bases,
+                     int64_t stride, const uint32_t *d_lens, int64_t n, int32_t max_len,
+                     int32_t encoding, int32_t bs_r1, int32_t bs_r2, int64_t n_r1,
+                     const ma_params *p, ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc);
+
+/* ---- sort keys, sort, duplicate marking ---- */
+/* SE keys from merged records (device pointers): coord, dhi, dlo, score, pair
+ * (pair = pair_base + i). d_tid_of_cid maps index contig -> canonical target. */
+int ma_sort_keys_se_dev(void *stream, int64_t n, const ma_recs *d_merged, const uint32_t *d_lens,
+                        const int32_t *d_tid_of_cid, int64_t pair_base, uint64_t *d_coord,
+                        uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair);
+/* gpu_sort_markdup contract: is_dup (original order), perm (stable coord
+ * order), dup_perm (stable (dhi, dlo) order, identity when !do_markdup; may be
+ * NULL). Host pointers. */
+int ma_sort_markdup(int device, int64_t n, const uint64_t *coord, const uint64_t *dhi,
+                    const uint64_t *dlo, const uint32_t *score, const uint32_t *pair,
+                    uint32_t *is_dup, uint32_t *perm, int32_t do_markdup, uint32_t *dup_perm);
+/* Device form; scratch sized by ma_sort_scratch_bytes(n) (caller-owned). */
+int64_t ma_sort_scratch_bytes(int64_t n);
+int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const uint64_t *d_dhi,
+                        const uint64_t *d_dlo, const uint32_t *d_score, const uint32_t *d_pair,
+                        uint32_t *d_is_dup, uint32_t *d_perm, int32_t do_markdup,
+                        uint32_t *d_dup_perm, void *d_scratch, int64_t scratch_bytes);
+/* Stable LSD radix sort of u64 keys carrying u32 values (device pointers). */
+int ma_radix_sort_u64_dev(void *stream, int64_t n, uint64_t *d_keys, uint32_t *d_vals,
+                          void *d_scratch, int64_t scratch_bytes);
+
+/* ---- gpu-common seed kernels ---- */
+/* ASCII reads [n x max_len] -> per-position Wang hash of the MSB-first k-mer
+ * (0 = invalid window), out_hash[n*max_len]. Host pointers. */
+int ma_seed_kmers(int device, const uint8_t *bases, int64_t n, int32_t max_len, int32_t k,
+                  uint64_t *out_hash);
+
+/* ---- served request loop ---- */
+/* A server owns `slots` concurrent in-flight batches on their own streams and
+ * admits requests while the free-HBM budget (bytes) allows. */
+int ma_serve_open(ma_index *ix, int32_t slots, int64_t hbm_budget, int32_t max_len,
+                  ma_server **out);
+/* Submit host reads (same layout as ma_map_batch); returns a ticket >= 0, or
+ * MA_ERR_NOMEM when the request cannot be admitted now (retry after poll). */
+int64_t ma_serve_submit(ma_server *s, const uint8_t *bases, int64_t stride, const uint32_t *lens,
+                        int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
+                        int64_t n_r1, const ma_params *p, ma_recs *merged_out);
+/* Poll: returns 1 and fills ms when the ticket is complete (outputs written),
+ * 0 while pending. wait != 0 blocks until complete. */
+int ma_serve_poll(ma_server *s, int64_t ticket, int32_t wait, double *latency_ms);
+int ma_serve_close(ma_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

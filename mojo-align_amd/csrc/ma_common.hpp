@@ -1,0 +1,149 @@
+// Shared host/device helpers for the mialign library (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include <string>
+
+#include "../../include/mialign.h"
+
+namespace ma {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string &msg);
+
+#define MA_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (call);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ::ma::set_error(std::string(#call) + ": " + hipGetErrorString(_e) + " at " + \
+                            __FILE__ + ":" + std::to_string(__LINE__));                \
+            return MA_ERR_HIP;                                                         \
+        }                                                                              \
+    } while (0)
+
+#define MA_REQUIRE(cond, msg)            \
+    do {                                 \
+        if (!(cond)) {                   \
+            ::ma::set_error(msg);        \
+            return MA_ERR_ARG;           \
+        }                                \
+    } while (0)
+
+#define MA_TRY(call)           \
+    do {                       \
+        int _rc = (call);      \
+        if (_rc != MA_OK) return _rc; \
+    } while (0)
+
+// Launch check: reports the launch error of the kernel just enqueued.
+#define MA_LAUNCHED() MA_HIP(hipGetLastError())
+
+// ---------------------------------------------------------- device memory
+// Owning device allocation (freed on destruction).
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t n) {
+        release();
+        if (n == 0) n = 256;
+        hipError_t e = hipMalloc(&p, n);
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error(std::string("hipMalloc(") + std::to_string(n) + "): " + hipGetErrorString(e));
+            return MA_ERR_NOMEM;
+        }
+        bytes = n;
+        return MA_OK;
+    }
+    // Grow-only reuse.
+    int reserve(size_t n) { return (n <= bytes && p) ? MA_OK : alloc(n); }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// ------------------------------------------------------------- constants
+constexpr int WAVE = 64;
+constexpr uint64_t LOW2 = 0x5555555555555555ull;  // low bit of each 2-bit group
+constexpr uint64_t EMPTY_KEY = ~0ull;
+
+// Hash index slot: key (MSB-first 2-bit k-mer) and value = occ(24) << 40 | start(40).
+struct HashSlot {
+    uint64_t key;
+    uint64_t val;
+};
+constexpr int BUCKET_SLOTS = 4;  // 64-byte buckets
+constexpr uint64_t START_MASK = (1ull << 40) - 1;
+constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
+
+// Resident index (device pointers) — passed by value to kernels.
+struct IndexView {
+    const HashSlot *table;
+    uint64_t n_buckets;
+    int32_t max_probe;
+    int32_t k;
+    const uint32_t *postings;  // (cid, pos) pairs
+    int64_t n_postings;
+    const uint64_t *ref2;  // 2-bit MSB-first, 32 bases / word (+2 pad words)
+    const uint64_t *refN;  // N flags in the same geometry (low bit of the group)
+    int32_t ref_has_n;
+    const uint64_t *contig_off;  // n_contigs + 1
+    int64_t n_contigs;
+    int64_t n_bases;
+};
+
+// ------------------------------------------------------- integer hashing
+// Thomas Wang 64-bit mix (the seed hash of gpu-common/src/gpu_kernels.mojo:128-137).
+__host__ __device__ inline uint64_t wang_hash(uint64_t key) {
+    key = (~key) + (key << 21);
+    key = key ^ (key >> 24);
+    key = (key + (key << 3)) + (key << 8);
+    key = key ^ (key >> 14);
+    key = (key + (key << 2)) + (key << 4);
+    key = key ^ (key >> 28);
+    key = key + (key << 31);
+    return key;
+}
+
+__host__ __device__ inline uint64_t bucket_of(uint64_t key, uint64_t n_buckets) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __umul64hi(wang_hash(key), n_buckets);
+#else
+    return (uint64_t)(((unsigned __int128)wang_hash(key) * n_buckets) >> 64);
+#endif
+}
+
+// ------------------------------------------------------ 2-bit word helpers
+// 32 bases starting at base offset `g` of a word array (MSB-first groups).
+__device__ inline uint64_t window32(const uint64_t *w, int64_t g) {
+    int64_t wi = g >> 5;
+    int sh = (int)(g & 31) * 2;
+    uint64_t a = w[wi];
+    if (sh == 0) return a;
+    return (a << sh) | (w[wi + 1] >> (64 - sh));
+}
+
+// Reverse the order of the 32 two-bit groups of a word.
+__device__ inline uint64_t reverse_groups(uint64_t x) {
+    uint64_t y = __builtin_bitreverse64(x);
+    return ((y >> 1) & LOW2) | ((y & LOW2) << 1);
+}
+
+// Mask with the low flag bit set for groups j < n (group j at bit 62-2j).
+__device__ inline uint64_t first_groups_mask(int n) {
+    if (n <= 0) return 0;
+    if (n >= 32) return LOW2;
+    return LOW2 & ~(~0ull >> (2 * n));
+}
+
+}  // namespace ma

@@ -1,0 +1,36 @@
+// Resident index object (one per device).
+#pragma once
+
+#include "ma_common.hpp"
+
+struct ma_index {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int32_t k = 0;
+    int64_t n_keys = 0;
+    int64_t n_postings = 0;
+    int64_t n_bases = 0;
+    int64_t n_contigs = 0;
+    int32_t ref_has_n = 0;
+    int32_t max_probe = 1;
+    uint64_t n_buckets = 0;
+    ma::DevBuf kmers;     // u64[n_keys]       (dense-v1)
+    ma::DevBuf offsets;   // u64[n_keys + 1]   (dense-v1)
+    ma::DevBuf postings;  // u32[2 * n_postings] (dense-v1)
+    ma::DevBuf ref2;      // 2-bit reference words
+    ma::DevBuf refN;      // N flags, same geometry
+    ma::DevBuf coff;      // u64[n_contigs + 1]
+    ma::DevBuf table;     // HashSlot[n_buckets * BUCKET_SLOTS]
+};
+
+// Per-launch state of the map kernel (its work counter).
+struct ma_workspace {
+    ma_index *ix = nullptr;
+    int64_t max_reads = 0;  // 0 = no limit
+    int32_t max_len = 0;
+    ma::DevBuf counter;
+};
+
+namespace ma {
+IndexView index_view(const ma_index *ix);
+}

@@ -1,0 +1,1079 @@
+/*
+ * ORACLE — CPU restatement of the reference's linear WGBS seed -> extend ->
+ * sort path. TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg load this library as the checker. The product
+ * (mojo-align_amd/) never links, loads or calls it.
+ *
+ * The reference is Mojo (fq2bam-meth/src/ .mojo files) and cannot be compiled here,
+ * so every function below restates the reference's arithmetic from its source
+ * text; each cites the file:line it follows. Knob defaults are the reference's
+ * (linear_gpu_locate.mojo:1729-1751): k=21 (caller), stride 5, max_occ 16384,
+ * vote_occ = vote_fast = 256, max_diff 6, max_soft 8, RC on; indel_cap is a
+ * parameter (0 = gapless contract).
+ *
+ * Parity status per function (how each is pinned by tests/test_oracle_*.py):
+ *   or_code_of / or_pack_bases / or_bs_convert ..... reference known answers +
+ *                                                    closed form (per-base table)
+ *   or_wang_hash ..................................... published integer hash;
+ *                                                    pinned by bijectivity /
+ *                                                    avalanche checks only
+ *   or_encode_strided ................................ brute-force string k-mers
+ *   or_build_dense / or_locate_bsearch ............... brute-force window scan
+ *   or_vote_extend / or_map_batch .................... PARITY UNPINNED by any
+ *        reference vector (none exists); pinned by invariants, brute-force
+ *        best-locus search on unique references and a hand-computed branch suite
+ *   or_sort_keys_se .................................. closed-form key formulas
+ *   or_sort_markdup .................................. reference known answers
+ *                                                    (test_bam_emit.py) + brute force
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define OR_EMPTY_KEY 0xFFFFFFFFFFFFFFFFull
+
+/* ------------------------------------------------------------------ A1/A2 */
+
+/* ASCII -> 2-bit code; anything but A/C/G/T (either case) -> 255.
+ * Follows pack_bases_kernel, gpu_kernels.mojo:108-126 (= linear_gpu_locate.mojo:399-417). */
+uint8_t or_code_of(uint8_t b) {
+    switch (b) {
+    case 'A': case 'a': return 0;
+    case 'C': case 'c': return 1;
+    case 'G': case 'g': return 2;
+    case 'T': case 't': return 3;
+    default: return 255;
+    }
+}
+
+void or_pack_bases(const uint8_t *bases, uint8_t *codes, int64_t n) {
+    for (int64_t i = 0; i < n; i++) codes[i] = or_code_of(bases[i]);
+}
+
+/* In-silico bisulfite conversion of ASCII text. mode 1 = C2T (Cc -> Tt),
+ * mode 2 = G2A (Gg -> Aa); other bytes untouched.
+ * Follows _bs_convert, linear_gpu_locate.mojo:58-70 (str.translate tables). */
+void or_bs_convert(uint8_t *seq, int64_t n, int mode) {
+    for (int64_t i = 0; i < n; i++) {
+        uint8_t b = seq[i];
+        if (mode == 1) {
+            if (b == 'C') seq[i] = 'T';
+            else if (b == 'c') seq[i] = 't';
+        } else if (mode == 2) {
+            if (b == 'G') seq[i] = 'A';
+            else if (b == 'g') seq[i] = 'a';
+        }
+    }
+}
+
+/* Reference-FASTA conversion: only C/c -> T (uppercase T even for 'c').
+ * Follows convert_fasta_c2t, fq2bam_meth.py:79-88. */
+void or_convert_fasta_c2t(uint8_t *seq, int64_t n) {
+    for (int64_t i = 0; i < n; i++)
+        if (seq[i] == 'C' || seq[i] == 'c') seq[i] = 'T';
+}
+
+/* ------------------------------------------------------------------ A3 */
+
+/* Thomas Wang 64-bit integer mix; follows wang_hash_u64, gpu_kernels.mojo:128-137. */
+uint64_t or_wang_hash(uint64_t key) {
+    key = (~key) + (key << 21);
+    key = key ^ (key >> 24);
+    key = (key + (key << 3)) + (key << 8);
+    key = key ^ (key >> 14);
+    key = (key + (key << 2)) + (key << 4);
+    key = key ^ (key >> 28);
+    key = key + (key << 31);
+    return key;
+}
+
+/* Strided k-mer keys, MSB-first 2-bit; ~0 when the window overruns the read or
+ * holds a code > 3. slot s -> q_off = s*stride.
+ * Follows encode_strided_kernel, linear_gpu_locate.mojo:419-451. */
+static void encode_one_read(const uint8_t *codes, int64_t rid, uint32_t L, int64_t max_len,
+                            int64_t k, int64_t stride, int64_t slots_per,
+                            uint64_t *keys, uint32_t *qoffs) {
+    for (int64_t s = 0; s < slots_per; s++) {
+        int64_t idx = rid * slots_per + s;
+        int64_t q_off = s * stride;
+        keys[idx] = OR_EMPTY_KEY;
+        qoffs[idx] = (uint32_t)q_off;
+        if (q_off + k > (int64_t)L) continue;
+        const uint8_t *p = codes + rid * max_len + q_off;
+        uint64_t key = 0;
+        int ok = 1;
+        for (int64_t j = 0; j < k; j++) {
+            if (p[j] > 3) { ok = 0; break; }
+            key = (key << 2) | p[j];
+        }
+        if (ok) keys[idx] = key;
+    }
+}
+
+void or_encode_strided(const uint8_t *codes, const uint32_t *lens, int64_t n_reads,
+                       int64_t max_len, int64_t k, int64_t stride, int64_t slots_per,
+                       uint64_t *keys, uint32_t *qoffs) {
+    for (int64_t r = 0; r < n_reads; r++)
+        encode_one_read(codes, r, lens[r], max_len, k, stride, slots_per, keys, qoffs);
+}
+
+/* ------------------------------------------------------------------ A4 */
+
+/* lower_bound over sorted unique kmers -> CSR [start, end); zeroed when the key
+ * is absent, invalid, or occ > max_occ.
+ * Follows locate_bsearch_kernel, linear_gpu_locate.mojo:453-490 and
+ * _dense_find_key, linear_index.mojo:436-453. */
+static void locate_one(uint64_t key, const uint64_t *kmers, const uint64_t *offsets,
+                       int64_t n_table, int64_t max_occ, uint64_t *st, uint64_t *en) {
+    *st = 0;
+    *en = 0;
+    if (key == OR_EMPTY_KEY || n_table <= 0) return;
+    int64_t lo = 0, hi = n_table;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        if (kmers[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= n_table || kmers[lo] != key) return;
+    uint64_t a = offsets[lo], b = offsets[lo + 1];
+    int64_t occ = (int64_t)(b - a);
+    if (occ <= 0 || occ > max_occ) return;
+    *st = a;
+    *en = b;
+}
+
+void or_locate_bsearch(const uint64_t *keys, int64_t n_slots, const uint64_t *kmers,
+                       const uint64_t *offsets, int64_t n_table, int64_t max_occ,
+                       uint64_t *out_start, uint64_t *out_end) {
+    for (int64_t i = 0; i < n_slots; i++)
+        locate_one(keys[i], kmers, offsets, n_table, max_occ, &out_start[i], &out_end[i]);
+}
+
+/* ------------------------------------------------- dense-v1 index build */
+
+/* Number of k-mer windows (before N filtering) over all contigs. */
+int64_t or_count_windows(const uint64_t *contig_off, int64_t n_contigs, int k) {
+    int64_t n = 0;
+    for (int64_t c = 0; c < n_contigs; c++) {
+        int64_t len = (int64_t)(contig_off[c + 1] - contig_off[c]);
+        if (len >= k) n += len - k + 1;
+    }
+    return n;
+}
+
+typedef struct { uint64_t key; uint64_t loc; } or_kl;
+
+/* Stable merge sort on key (equal keys keep input order). */
+static void merge_sort_kl(or_kl *a, or_kl *tmp, int64_t n) {
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n;
+            int64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) tmp[o++] = (a[j].key < a[i].key) ? a[j++] : a[i++];
+            while (i < mid) tmp[o++] = a[i++];
+            while (j < hi) tmp[o++] = a[j++];
+        }
+        memcpy(a, tmp, (size_t)n * sizeof(or_kl));
+    }
+}
+
+/* dense-v1 build over ASCII contig bases (already converted; case-folded to
+ * upper by the reader). kmers: sorted unique MSB-first 2-bit keys; offsets:
+ * CSR (n_keys+1); postings: u32 pairs (contig_id, pos0) ordered by (key, input
+ * order) = (key, cid, pos) because contigs are enumerated in order.
+ * Follows encode_kmers_contig + build_dense_pack, mojo_linear_pack.py:179-287
+ * (stable mergesort :239). Capacities: kmers/offsets-1/postings/2 >=
+ * or_count_windows(). Returns 0 or -1 on allocation failure. */
+int or_build_dense(const uint8_t *seq, const uint64_t *contig_off, int64_t n_contigs, int k,
+                   uint64_t *kmers_out, uint64_t *offsets_out, uint32_t *postings_out,
+                   int64_t *n_keys_out, int64_t *n_postings_out) {
+    int64_t cap = or_count_windows(contig_off, n_contigs, k);
+    or_kl *a = (or_kl *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(or_kl));
+    or_kl *t = (or_kl *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(or_kl));
+    if (!a || !t) { free(a); free(t); return -1; }
+    uint64_t kmask = (k >= 32) ? ~0ull : ((1ull << (2 * k)) - 1ull);
+    int64_t n = 0;
+    for (int64_t c = 0; c < n_contigs; c++) {
+        int64_t off = (int64_t)contig_off[c];
+        int64_t len = (int64_t)(contig_off[c + 1] - contig_off[c]);
+        if (len < k) continue;
+        for (int64_t p = 0; p + k <= len; p++) {
+            uint64_t key = 0;
+            int ok = 1;
+            for (int j = 0; j < k; j++) {
+                uint8_t cd = or_code_of(seq[off + p + j]);
+                if (cd > 3) { ok = 0; break; }
+                key = (key << 2) | cd;
+            }
+            if (!ok || key > kmask) continue;
+            a[n].key = key;
+            a[n].loc = ((uint64_t)c << 32) | (uint64_t)p;
+            n++;
+        }
+    }
+    merge_sort_kl(a, t, n);
+    int64_t nk = 0;
+    for (int64_t i = 0; i < n; i++) {
+        if (i == 0 || a[i].key != a[i - 1].key) {
+            kmers_out[nk] = a[i].key;
+            offsets_out[nk] = (uint64_t)i;
+            nk++;
+        }
+        postings_out[2 * i] = (uint32_t)(a[i].loc >> 32);
+        postings_out[2 * i + 1] = (uint32_t)(a[i].loc & 0xFFFFFFFFull);
+    }
+    offsets_out[nk] = (uint64_t)n;
+    *n_keys_out = nk;
+    *n_postings_out = n;
+    free(a);
+    free(t);
+    return 0;
+}
+
+/* ------------------------------------------------------------ A6 / A7 */
+
+typedef struct {
+    const uint64_t *kmers;
+    const uint64_t *offsets;
+    const uint32_t *postings; /* (cid, pos) pairs */
+    int64_t n_keys;
+    int64_t n_postings;
+    const uint8_t *ref_codes; /* pack_bases(sequences.bin) */
+    const uint64_t *contig_off;
+    int64_t n_contigs;
+} or_index;
+
+typedef struct {
+    int32_t k;
+    int32_t stride;
+    int32_t max_occ;
+    int32_t vote_occ;
+    int32_t vote_fast;
+    int32_t max_diff;
+    int32_t max_soft;
+    int32_t max_indel;
+    int32_t do_rc;
+} or_params;
+
+/* Output record arrays (SoA), one entry per read; the fields of the reference's
+ * vote_extend_kernel outputs, linear_gpu_locate.mojo:778-787. */
+typedef struct {
+    int32_t *cid;
+    uint32_t *pos;
+    uint32_t *mapq;
+    int32_t *flag;
+    uint32_t *sl;
+    uint32_t *sr;
+    uint32_t *nm;
+    uint32_t *iop;
+    uint32_t *iat;
+    uint32_t *ilen;
+} or_recs;
+
+/* Algorithmic-traffic counters (per batch). */
+typedef struct {
+    int64_t probes;    /* valid keys looked up */
+    int64_t postings;  /* postings read (vote + rescue + high-occ tally) */
+    int64_t windows;   /* reference windows compared (extend + rescue + tally) */
+    int64_t reads_rc;  /* reads that ran the RC pass */
+} or_counters;
+
+static void rec_reset(or_recs *o, int64_t r) {
+    o->cid[r] = -1;
+    o->pos[r] = 0;
+    o->mapq[r] = 0;
+    o->flag[r] = 4;
+    o->sl[r] = 0;
+    o->sr[r] = 0;
+    o->nm[r] = 0;
+    o->iop[r] = 0;
+    o->iat[r] = 0;
+    o->ilen[r] = 0;
+}
+
+static inline int mism(uint8_t a, uint8_t b) { return a > 3 || b > 3 || a != b; }
+
+#define OR_TOP 256
+#define OR_KEEP 16
+#define OR_MAX_SLOTS 128
+#define OR_HX 48
+
+/* Per-read vote -> KEEP -> gapless/softclip(/indel) extend -> rescue ->
+ * high-occ exact tally -> MAPQ. pass_mode 0 = vote + extend + low-occ
+ * rescue; pass_mode 1 = high-occ exact tally only (read must be unmapped).
+ * Follows vote_extend_kernel, linear_gpu_locate.mojo:749-1607 (pass_mode 2 is
+ * dead code there, :863-902, and is not restated).
+ * Counter increments are approximate for traffic accounting only. */
+static void vote_extend_read(const or_index *ix, const uint8_t *codes, const uint32_t *lens,
+                             int64_t max_len, int64_t slots_per, const uint64_t *occ_start,
+                             const uint64_t *occ_end, const uint32_t *q_offs,
+                             int64_t max_diff, int64_t max_soft, int64_t max_indel,
+                             int64_t vote_occ, int pass_mode, int32_t strand_flag, int64_t rid,
+                             or_recs *out, int64_t *c_post, int64_t *c_win) {
+    /* :800-802 pass-1 safety */
+    if (pass_mode == 1 && out->cid[rid] >= 0) return;
+    rec_reset(out, rid);
+    int64_t qlen = (int64_t)lens[rid];
+    if (qlen <= 0) return;
+
+    uint64_t cand_key[OR_TOP];
+    int32_t cand_n[OR_TOP];
+    int64_t n_cand = 0;
+
+    /* :819-853 valid slots ordered by ascending occ (stable insertion). */
+    int32_t ord_slot[OR_MAX_SLOTS];
+    int32_t ord_occ[OR_MAX_SLOTS];
+    int64_t n_ord = 0;
+    for (int64_t sg = 0; sg < slots_per && n_ord < OR_MAX_SLOTS; sg++) {
+        int64_t si = rid * slots_per + sg;
+        int64_t a = (int64_t)occ_start[si], b = (int64_t)occ_end[si];
+        int64_t occ = b - a;
+        if (occ > 0 && a >= 0 && b <= ix->n_postings) {
+            int64_t ins = n_ord;
+            for (int64_t t = 0; t < n_ord; t++)
+                if (occ < (int64_t)ord_occ[t]) { ins = t; break; }
+            for (int64_t s = n_ord; s > ins; s--) {
+                ord_slot[s] = ord_slot[s - 1];
+                ord_occ[s] = ord_occ[s - 1];
+            }
+            ord_slot[ins] = (int32_t)sg;
+            ord_occ[ins] = (int32_t)occ;
+            n_ord++;
+        }
+    }
+    int64_t vote_cap = vote_occ < 1 ? 256 : vote_occ;
+
+    /* :903-973 serial vote into TOP with first-min replacement and early stop. */
+    if (pass_mode == 0) {
+        int64_t last_i = -1;
+        for (int64_t oi = 0; oi < n_ord; oi++) {
+            int64_t occ_v = ord_occ[oi];
+            if (occ_v > vote_cap) continue;
+            int64_t si = rid * slots_per + ord_slot[oi];
+            int64_t a = (int64_t)occ_start[si], b = (int64_t)occ_end[si];
+            int64_t q_off = (int64_t)q_offs[si];
+            for (int64_t pi = a; pi < b; pi++) {
+                int64_t cid = (int64_t)ix->postings[pi * 2];
+                int64_t pos = (int64_t)ix->postings[pi * 2 + 1];
+                (*c_post)++;
+                if (!(cid >= 0 && cid < ix->n_contigs && pos >= q_off)) continue;
+                uint64_t vk = ((uint64_t)cid << 32) | (uint64_t)(pos - q_off);
+                int64_t found = -1;
+                if (last_i >= 0 && cand_key[last_i] == vk) {
+                    found = last_i;
+                } else {
+                    for (int64_t ci = 0; ci < n_cand; ci++)
+                        if (cand_key[ci] == vk) { found = ci; break; }
+                }
+                if (found >= 0) {
+                    cand_n[found]++;
+                    last_i = found;
+                } else if (n_cand < OR_TOP) {
+                    cand_key[n_cand] = vk;
+                    cand_n[n_cand] = 1;
+                    last_i = n_cand;
+                    n_cand++;
+                } else {
+                    int64_t min_i = 0;
+                    int32_t min_n = cand_n[0];
+                    for (int64_t z = 1; z < OR_TOP; z++)
+                        if (cand_n[z] < min_n) { min_n = cand_n[z]; min_i = z; }
+                    if (min_n <= 1) {
+                        cand_key[min_i] = vk;
+                        cand_n[min_i] = 1;
+                        last_i = min_i;
+                    }
+                }
+            }
+            if (n_cand > 0 && oi + 1 < n_ord) {
+                int64_t b1 = 0, b2 = 0;
+                for (int64_t z = 0; z < n_cand; z++) {
+                    int64_t zv = cand_n[z];
+                    if (zv > b1) { b2 = b1; b1 = zv; }
+                    else if (zv > b2) b2 = zv;
+                }
+                if (b1 >= 4 && b1 >= b2 + 2) break;
+            }
+        }
+    }
+
+    /* :975-1004 stable top-KEEP by votes (strict >). */
+    int32_t pick_i[OR_KEEP];
+    int32_t pick_n[OR_KEEP];
+    int64_t n_pick = 0;
+    if (pass_mode == 0) {
+        for (int64_t c = 0; c < n_cand; c++) {
+            int32_t votes = cand_n[c];
+            int inserted = 0;
+            for (int64_t p = 0; p < n_pick; p++) {
+                if (votes > pick_n[p]) {
+                    int64_t s = n_pick;
+                    if (s > OR_KEEP - 1) s = OR_KEEP - 1;
+                    for (; s > p; s--) {
+                        pick_i[s] = pick_i[s - 1];
+                        pick_n[s] = pick_n[s - 1];
+                    }
+                    pick_i[p] = (int32_t)c;
+                    pick_n[p] = votes;
+                    if (n_pick < OR_KEEP) n_pick++;
+                    inserted = 1;
+                    break;
+                }
+            }
+            if (!inserted && n_pick < OR_KEEP) {
+                pick_i[n_pick] = (int32_t)c;
+                pick_n[n_pick] = votes;
+                n_pick++;
+            }
+        }
+    }
+
+    /* :1006-1024 caps. */
+    const uint8_t *q = codes + rid * max_len;
+    const uint8_t *R = ix->ref_codes;
+    int64_t budget = max_diff;
+    if (budget < 2) budget = 2;
+    if (budget > qlen / 2) budget = qlen / 2;
+    int64_t clip_cap = max_soft;
+    if (clip_cap > qlen / 4) clip_cap = qlen / 4;
+    int64_t indel_cap = max_indel;
+    if (indel_cap < 0) indel_cap = 0;
+    if (indel_cap > 4) indel_cap = 4;
+    int64_t shift_cap = indel_cap;
+    if (shift_cap < 2) shift_cap = 2;
+    if (shift_cap > 6) shift_cap = 6;
+
+    int64_t best_cost = 9999;
+    int32_t best_votes = 0;
+    int64_t best_cid = -1, best_pos = 0, best_sl = 0, best_sr = 0;
+    int64_t best_iop = 0, best_iat = 0, best_ilen = 0;
+
+    /* :1037-1243 extend picks at adj 0, -1, +1, -2, +2, ... */
+    for (int64_t pk = 0; pk < n_pick && n_cand > 0; pk++) {
+        uint64_t bkey = cand_key[pick_i[pk]];
+        int32_t votes = pick_n[pk];
+        int64_t bcid = (int64_t)(bkey >> 32);
+        int64_t bstart0 = (int64_t)(bkey & 0xFFFFFFFFull);
+        if (bcid >= 0 && bcid < ix->n_contigs && bstart0 >= 0) {
+            int64_t off0 = (int64_t)ix->contig_off[bcid];
+            int64_t clen = (int64_t)ix->contig_off[bcid + 1] - off0;
+            for (int64_t adj_i = 0; adj_i <= shift_cap * 2; adj_i++) {
+                int64_t adj = 0;
+                if (adj_i > 0) {
+                    int64_t mag = (adj_i + 1) / 2;
+                    adj = (adj_i & 1) ? -mag : mag;
+                }
+                int64_t bstart = bstart0 + adj;
+                if (bstart < 0 || bstart + qlen > clen) continue;
+                const uint8_t *r = R + off0 + bstart;
+                (*c_win)++;
+                int64_t probe_nm = 0;
+                for (int64_t pkp = 1; pkp <= 6; pkp++) {
+                    int64_t jp = (pkp * (qlen - 1)) / 7;
+                    if (jp < clip_cap) jp = clip_cap;
+                    if (jp > qlen - 1 - clip_cap) jp = qlen - 1 - clip_cap;
+                    if (jp >= 0 && jp < qlen && mism(r[jp], q[jp])) probe_nm++;
+                }
+                if (probe_nm > budget + indel_cap) continue;
+                int64_t nm_all = 0;
+                for (int64_t j = 0; j < qlen && nm_all <= budget; j++)
+                    if (mism(r[j], q[j])) nm_all++;
+                int64_t sl = 0, sr = 0, nm = nm_all;
+                if (nm_all > budget) {
+                    while (sl < clip_cap) {
+                        if (!mism(r[sl], q[sl])) break;
+                        sl++;
+                    }
+                    while (sr < clip_cap) {
+                        int64_t jr = qlen - 1 - sr;
+                        if (jr <= sl) break;
+                        if (!mism(r[jr], q[jr])) break;
+                        sr++;
+                    }
+                    nm = 0;
+                    for (int64_t j = sl; j < qlen - sr && nm <= budget; j++)
+                        if (mism(r[j], q[j])) nm++;
+                }
+                int64_t alen = qlen - sl - sr;
+                if (nm <= budget && alen >= 24) {
+                    if (nm < best_cost || (nm == best_cost && votes > best_votes)) {
+                        best_cost = nm;
+                        best_votes = votes;
+                        best_cid = bcid;
+                        best_pos = bstart + sl;
+                        best_sl = sl;
+                        best_sr = sr;
+                        best_iop = best_iat = best_ilen = 0;
+                    }
+                }
+                /* :1142-1235 single indel at adj 0 on a coarse grid (step 2). */
+                if (indel_cap > 0 && adj == 0 && (nm > budget || alen < 24)) {
+                    for (int64_t dlen = 1; dlen <= indel_cap; dlen++) {
+                        if (bstart + qlen + dlen <= clen) {
+                            for (int64_t g = 0; g <= qlen; g += 2) {
+                                int64_t nd = dlen, t = 0;
+                                for (; t < g && nd <= budget; t++)
+                                    if (mism(r[t], q[t])) nd++;
+                                for (; t < qlen && nd <= budget; t++)
+                                    if (mism(r[t + dlen], q[t])) nd++;
+                                if (nd <= budget &&
+                                    (nd < best_cost || (nd == best_cost && votes > best_votes))) {
+                                    best_cost = nd;
+                                    best_votes = votes;
+                                    best_cid = bcid;
+                                    best_pos = bstart;
+                                    best_sl = best_sr = 0;
+                                    best_iop = 2;
+                                    best_iat = g;
+                                    best_ilen = dlen;
+                                }
+                            }
+                        }
+                        if (qlen > dlen && bstart + (qlen - dlen) <= clen) {
+                            for (int64_t gi = 0; gi <= qlen - dlen; gi += 2) {
+                                int64_t ni = dlen, ti = 0;
+                                for (; ti < gi && ni <= budget; ti++)
+                                    if (mism(r[ti], q[ti])) ni++;
+                                for (int64_t tj = gi + dlen, rj = gi; tj < qlen && ni <= budget; tj++, rj++)
+                                    if (mism(r[rj], q[tj])) ni++;
+                                if (ni <= budget &&
+                                    (ni < best_cost || (ni == best_cost && votes > best_votes))) {
+                                    best_cost = ni;
+                                    best_votes = votes;
+                                    best_cid = bcid;
+                                    best_pos = bstart;
+                                    best_sl = best_sr = 0;
+                                    best_iop = 1;
+                                    best_iat = gi;
+                                    best_ilen = dlen;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (best_cost == 0) break;
+            }
+            if (best_cost == 0) break;
+        }
+    }
+
+    /* :1245-1532 low-occ seed rescue (pass 0, only if unmapped) and high-occ
+     * exact tally (pass 1). */
+    if (pass_mode == 1 || best_cid < 0) {
+        uint64_t hx_key[OR_HX];
+        int32_t hx_n[OR_HX];
+        int32_t hx_min_occ[OR_HX];
+        int64_t n_hx = 0, n_high_tried = 0;
+        int64_t ri = 0;
+        while (ri < n_ord) {
+            int64_t occ_r = ord_occ[ri];
+            if (occ_r <= 0) { ri++; continue; }
+            int high = occ_r > vote_cap;
+            if (high && pass_mode == 0) { ri++; continue; }
+            if (!high && pass_mode == 1) { ri++; continue; }
+            if (high) {
+                n_high_tried++;
+                if (n_high_tried > 96) { ri++; continue; }
+            }
+            int64_t si = rid * slots_per + ord_slot[ri];
+            int64_t a_r = (int64_t)occ_start[si], b_r = (int64_t)occ_end[si];
+            int64_t q_off_r = (int64_t)q_offs[si];
+            if (high) {
+                for (int64_t ph = a_r; ph < b_r; ph++) {
+                    int64_t cid_h = (int64_t)ix->postings[ph * 2];
+                    int64_t pos_h = (int64_t)ix->postings[ph * 2 + 1];
+                    (*c_post)++;
+                    if (!(cid_h >= 0 && cid_h < ix->n_contigs && pos_h >= q_off_r)) continue;
+                    int64_t bs_h = pos_h - q_off_r;
+                    int64_t o0 = (int64_t)ix->contig_off[cid_h];
+                    int64_t o1 = (int64_t)ix->contig_off[cid_h + 1];
+                    if (!(bs_h >= 0 && bs_h + qlen <= o1 - o0)) continue;
+                    const uint8_t *r = R + o0 + bs_h;
+                    (*c_win)++;
+                    int exact = 1;
+                    for (int64_t j = 0; j < qlen; j++)
+                        if (mism(r[j], q[j])) { exact = 0; break; }
+                    if (!exact) continue;
+                    uint64_t vk = ((uint64_t)cid_h << 32) | (uint64_t)bs_h;
+                    int64_t f = -1;
+                    for (int64_t c = 0; c < n_hx; c++)
+                        if (hx_key[c] == vk) { f = c; break; }
+                    if (f >= 0) {
+                        hx_n[f]++;
+                        if ((int32_t)occ_r < hx_min_occ[f]) hx_min_occ[f] = (int32_t)occ_r;
+                    } else if (n_hx < OR_HX) {
+                        hx_key[n_hx] = vk;
+                        hx_n[n_hx] = 1;
+                        hx_min_occ[n_hx] = (int32_t)occ_r;
+                        n_hx++;
+                    }
+                }
+                if (n_hx > 0) {
+                    int64_t b1 = 0, b2 = 0;
+                    for (int64_t z = 0; z < n_hx; z++) {
+                        int64_t zv = hx_n[z];
+                        if (zv > b1) { b2 = b1; b1 = zv; }
+                        else if (zv > b2) b2 = zv;
+                    }
+                    if (b1 >= 3 && b1 > b2)
+                        while (ri + 1 < n_ord && (int64_t)ord_occ[ri + 1] > vote_cap) ri++;
+                }
+                ri++;
+                continue;
+            }
+            /* low-occ rescue: gapless -> softclip -> single indel near seed */
+            for (int64_t pr = a_r; pr < b_r; pr++) {
+                int64_t cid_r = (int64_t)ix->postings[pr * 2];
+                int64_t pos_r = (int64_t)ix->postings[pr * 2 + 1];
+                (*c_post)++;
+                if (!(cid_r >= 0 && cid_r < ix->n_contigs && pos_r >= q_off_r)) continue;
+                int64_t bstart_r = pos_r - q_off_r;
+                int64_t off0_r = (int64_t)ix->contig_off[cid_r];
+                int64_t clen_r = (int64_t)ix->contig_off[cid_r + 1] - off0_r;
+                if (!(bstart_r >= 0 && bstart_r + qlen <= clen_r)) continue;
+                const uint8_t *r = R + off0_r + bstart_r;
+                (*c_win)++;
+                int64_t nm_r = 0;
+                for (int64_t j = 0; j < qlen && nm_r <= budget; j++)
+                    if (mism(r[j], q[j])) nm_r++;
+                int64_t sl_r = 0, sr_r = 0, cost_r = nm_r;
+                int64_t iop_r = 0, iat_r = 0, ilen_r = 0, pos_out = bstart_r;
+                if (nm_r > budget && clip_cap > 0) {
+                    while (sl_r < clip_cap) {
+                        if (!mism(r[sl_r], q[sl_r])) break;
+                        sl_r++;
+                    }
+                    while (sr_r < clip_cap) {
+                        int64_t jr = qlen - 1 - sr_r;
+                        if (jr <= sl_r) break;
+                        if (!mism(r[jr], q[jr])) break;
+                        sr_r++;
+                    }
+                    cost_r = 0;
+                    for (int64_t j = sl_r; j < qlen - sr_r && cost_r <= budget; j++)
+                        if (mism(r[j], q[j])) cost_r++;
+                    pos_out = bstart_r + sl_r;
+                    if (qlen - sl_r - sr_r < 32) cost_r = 9999;
+                }
+                if (indel_cap > 0 && cost_r > budget) {
+                    for (int64_t dlen = 1; dlen <= indel_cap; dlen++) {
+                        if (bstart_r + qlen + dlen <= clen_r) {
+                            int64_t g = q_off_r;
+                            if (g > qlen) g = qlen;
+                            int64_t g0 = g - 8 < 0 ? 0 : g - 8;
+                            int64_t g1 = g + 8 > qlen ? qlen : g + 8;
+                            for (int64_t gg = g0; gg <= g1; gg++) {
+                                int64_t nd = dlen, t = 0;
+                                for (; t < gg && nd <= budget; t++)
+                                    if (mism(r[t], q[t])) nd++;
+                                for (; t < qlen && nd <= budget; t++)
+                                    if (mism(r[t + dlen], q[t])) nd++;
+                                if (nd < cost_r) {
+                                    cost_r = nd;
+                                    sl_r = sr_r = 0;
+                                    iop_r = 2;
+                                    iat_r = gg;
+                                    ilen_r = dlen;
+                                    pos_out = bstart_r;
+                                }
+                            }
+                        }
+                        if (qlen > dlen && bstart_r + (qlen - dlen) <= clen_r) {
+                            int64_t g2 = q_off_r;
+                            if (g2 > qlen - dlen) g2 = qlen - dlen;
+                            int64_t g0i = g2 - 8 < 0 ? 0 : g2 - 8;
+                            int64_t g1i = g2 + 8 > qlen - dlen ? qlen - dlen : g2 + 8;
+                            for (int64_t gi = g0i; gi <= g1i; gi++) {
+                                int64_t ni = dlen, ti = 0;
+                                for (; ti < gi && ni <= budget; ti++)
+                                    if (mism(r[ti], q[ti])) ni++;
+                                for (int64_t tj = gi + dlen, rj = gi; tj < qlen && ni <= budget; tj++, rj++)
+                                    if (mism(r[rj], q[tj])) ni++;
+                                if (ni < cost_r) {
+                                    cost_r = ni;
+                                    sl_r = sr_r = 0;
+                                    iop_r = 1;
+                                    iat_r = gi;
+                                    ilen_r = dlen;
+                                    pos_out = bstart_r;
+                                }
+                            }
+                        }
+                    }
+                }
+                if (cost_r <= budget && (cost_r < best_cost || best_cid < 0)) {
+                    best_cost = cost_r;
+                    best_votes = occ_r <= 64 ? 5 : 4;
+                    best_cid = cid_r;
+                    best_pos = pos_out;
+                    best_sl = sl_r;
+                    best_sr = sr_r;
+                    best_iop = iop_r;
+                    best_iat = iat_r;
+                    best_ilen = ilen_r;
+                    if (cost_r == 0 && iop_r == 0) {
+                        pr = b_r; /* loop increment leaves pr > b_r: ends the scan */
+                        ri = n_ord;
+                    }
+                }
+            }
+            if (best_cost == 0 && best_iop == 0 && best_cid >= 0) break;
+            ri++;
+        }
+
+        /* :1534-1585 resolve the high-occ exact tally. */
+        if (n_hx > 0) {
+            int64_t best_hi = 0, second_hi = 0, best_hx_i = -1, best_mocc = 0x7FFFFFFF;
+            for (int64_t h = 0; h < n_hx; h++) {
+                int64_t vn = hx_n[h], mo = hx_min_occ[h];
+                if (vn > best_hi) {
+                    second_hi = best_hi;
+                    best_hi = vn;
+                    best_mocc = mo;
+                    best_hx_i = h;
+                } else if (vn == best_hi && mo < best_mocc) {
+                    best_mocc = mo;
+                    best_hx_i = h;
+                } else if (vn > second_hi) {
+                    second_hi = vn;
+                }
+            }
+            if (best_hx_i >= 0 && (best_hi >= 2 || (best_hi >= 1 && second_hi == 0)) &&
+                (best_cid < 0 || best_cost > 0 || (int64_t)best_votes < best_hi)) {
+                uint64_t bk = hx_key[best_hx_i];
+                best_cost = 0;
+                if (best_hi > second_hi) {
+                    best_votes = (int32_t)(best_hi > 5 ? 5 : best_hi);
+                } else {
+                    best_votes = 1;
+                }
+                best_cid = (int64_t)(bk >> 32);
+                best_pos = (int64_t)(bk & 0xFFFFFFFFull);
+                best_sl = best_sr = 0;
+                best_iop = best_iat = best_ilen = 0;
+            }
+        }
+    }
+
+    /* :1587-1607 MAPQ and outputs. */
+    if (best_cid >= 0) {
+        uint32_t mq = 20;
+        if (best_votes >= 3) mq = 40;
+        if (best_votes >= 5) mq = 60;
+        if (best_cost > 2 && mq > 20) mq = 20;
+        if (best_iop != 0 && mq > 20) mq = 20;
+        out->cid[rid] = (int32_t)best_cid;
+        out->pos[rid] = (uint32_t)best_pos;
+        out->mapq[rid] = mq;
+        out->flag[rid] = strand_flag;
+        out->sl[rid] = (uint32_t)best_sl;
+        out->sr[rid] = (uint32_t)best_sr;
+        out->nm[rid] = (uint32_t)best_cost;
+        out->iop[rid] = (uint32_t)best_iop;
+        out->iat[rid] = (uint32_t)best_iat;
+        out->ilen[rid] = (uint32_t)best_ilen;
+    }
+}
+
+/* Kernel-level entry: run vote_extend over reads (n_active < 0: all n_reads;
+ * else the rid_map list). Exposed for branch-level tests. */
+void or_vote_extend(const or_index *ix, const uint8_t *codes, const uint32_t *lens,
+                    int64_t max_len, int64_t slots_per, int64_t n_reads,
+                    const uint64_t *occ_start, const uint64_t *occ_end, const uint32_t *q_offs,
+                    int64_t max_diff, int64_t max_soft, int64_t max_indel, int64_t vote_occ,
+                    int pass_mode, int64_t n_active, const int32_t *rid_map, int32_t strand_flag,
+                    or_recs *out) {
+    int64_t n = n_active >= 0 ? n_active : n_reads;
+    int64_t cp = 0, cw = 0;
+    for (int64_t t = 0; t < n; t++) {
+        int64_t rid = n_active >= 0 ? rid_map[t] : t;
+        vote_extend_read(ix, codes, lens, max_len, slots_per, occ_start, occ_end, q_offs,
+                         max_diff, max_soft, max_indel, vote_occ, pass_mode, strand_flag, rid,
+                         out, &cp, &cw);
+    }
+}
+
+/* ------------------------------------------------------ A5 + A8 driver */
+
+static int64_t count_valid_keys(const uint64_t *keys, int64_t from, int64_t n) {
+    int64_t c = 0;
+    for (int64_t i = from; i < from + n; i++) c += keys[i] != OR_EMPTY_KEY;
+    return c;
+}
+
+/* One read through the whole per-batch sequence of the reference's host
+ * driver: FW encode/locate/pass 0 (vote_fast) -> optional pass 0 at vote_occ ->
+ * pass 1 if unmapped -> RC (need_rc = FW unmapped or FW nm > 0) with the same
+ * passes -> FW/RC merge. Per-read results do not depend on the batch.
+ * Follows linear_gpu_locate.mojo:1936-2032 (FW), 2215-2342 (pass 1),
+ * 2344-2635 (RC), 2101-2113 (merge), 548-600 (RC codes), 217-240 (hit). */
+static void map_one_read(const or_index *ix, const or_params *P, const uint8_t *codes_all,
+                         const uint32_t *lens, int64_t max_len, int64_t slots_per, int64_t rid,
+                         uint8_t *rc_buf, uint64_t *keys, uint32_t *qoffs, uint64_t *st,
+                         uint64_t *en, or_recs *fw, or_recs *rc, or_recs *mg, or_counters *cnt) {
+    int64_t base = rid * slots_per;
+    int64_t cp = 0, cw = 0;
+    /* FW */
+    encode_one_read(codes_all, rid, lens[rid], max_len, P->k, P->stride, slots_per, keys, qoffs);
+    for (int64_t s = 0; s < slots_per; s++)
+        locate_one(keys[base + s], ix->kmers, ix->offsets, ix->n_keys, P->max_occ, &st[base + s],
+                   &en[base + s]);
+    cnt->probes += count_valid_keys(keys, base, slots_per);
+    vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
+                     P->max_soft, P->max_indel, P->vote_fast, 0, 0, rid, fw, &cp, &cw);
+    if (fw->cid[rid] < 0) {
+        if (P->vote_fast < P->vote_occ)
+            vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
+                             P->max_soft, P->max_indel, P->vote_occ, 0, 0, rid, fw, &cp, &cw);
+        if (fw->cid[rid] < 0)
+            vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
+                             P->max_soft, P->max_indel, P->vote_occ, 1, 0, rid, fw, &cp, &cw);
+    }
+    /* RC */
+    rec_reset(rc, rid);
+    if (P->do_rc && (fw->cid[rid] < 0 || fw->nm[rid] > 0)) {
+        cnt->reads_rc++;
+        int64_t L = (int64_t)lens[rid];
+        const uint8_t *src = codes_all + rid * max_len;
+        uint8_t *dst = rc_buf + rid * max_len;
+        for (int64_t i = 0; i < max_len; i++) dst[i] = 255;
+        for (int64_t j = 0; j < L; j++) {
+            uint8_t c = src[L - 1 - j];
+            dst[j] = c <= 3 ? (uint8_t)(c ^ 3) : 255;
+        }
+        encode_one_read(rc_buf, rid, lens[rid], max_len, P->k, P->stride, slots_per, keys, qoffs);
+        for (int64_t s = 0; s < slots_per; s++)
+            locate_one(keys[base + s], ix->kmers, ix->offsets, ix->n_keys, P->max_occ,
+                       &st[base + s], &en[base + s]);
+        cnt->probes += count_valid_keys(keys, base, slots_per);
+        vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
+                         P->max_soft, P->max_indel, P->vote_fast, 0, 16, rid, rc, &cp, &cw);
+        if (rc->cid[rid] < 0) {
+            if (P->vote_fast < P->vote_occ)
+                vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs,
+                                 P->max_diff, P->max_soft, P->max_indel, P->vote_occ, 0, 16, rid,
+                                 rc, &cp, &cw);
+            if (rc->cid[rid] < 0)
+                vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs,
+                                 P->max_diff, P->max_soft, P->max_indel, P->vote_occ, 1, 16, rid,
+                                 rc, &cp, &cw);
+        }
+    }
+    /* merge: RC if FW unmapped and RC mapped, or both mapped and RC has lower
+     * nm (tie -> higher mapq). */
+    int use_rc = 0;
+    if (fw->cid[rid] < 0 && rc->cid[rid] >= 0) use_rc = 1;
+    else if (fw->cid[rid] >= 0 && rc->cid[rid] >= 0) {
+        if (rc->nm[rid] < fw->nm[rid] || (rc->nm[rid] == fw->nm[rid] && rc->mapq[rid] > fw->mapq[rid]))
+            use_rc = 1;
+    }
+    const or_recs *s = use_rc ? rc : fw;
+    mg->cid[rid] = s->cid[rid];
+    mg->pos[rid] = s->pos[rid];
+    mg->mapq[rid] = s->mapq[rid];
+    mg->flag[rid] = s->flag[rid];
+    mg->sl[rid] = s->sl[rid];
+    mg->sr[rid] = s->sr[rid];
+    mg->nm[rid] = s->nm[rid];
+    mg->iop[rid] = s->iop[rid];
+    mg->iat[rid] = s->iat[rid];
+    mg->ilen[rid] = s->ilen[rid];
+    /* _hit_from_gpu (:229-234): unmapped, or no aligned middle -> unmapped. */
+    int64_t m = (int64_t)lens[rid] - (int64_t)mg->sl[rid] - (int64_t)mg->sr[rid];
+    if (mg->cid[rid] < 0 || m < 1) rec_reset(mg, rid);
+    cnt->postings += cp;
+    cnt->windows += cw;
+}
+
+/* Slots per read for a batch: S = (max(max_len, k) - k) / stride + 1
+ * (linear_gpu_locate.mojo:1853-1857). */
+int64_t or_slots_per(int64_t max_len, int64_t k, int64_t stride) {
+    if (max_len < k) max_len = k;
+    return (max_len - k) / stride + 1;
+}
+
+/* Full batch over reads in the reference's padded layout: bases[n x max_len]
+ * ASCII (already in-silico converted), padded with 'N'. Outputs FW, RC and
+ * merged records. Parallel over reads (OpenMP) when nthreads > 1. */
+int or_map_batch(const or_index *ix, const or_params *P, const uint8_t *bases,
+                 const uint32_t *lens, int64_t n, int64_t max_len, or_recs *fw, or_recs *rc,
+                 or_recs *mg, or_counters *cnt, int nthreads) {
+    if (max_len < P->k) return -1; /* caller pads to >= k like :1853 */
+    int64_t S = or_slots_per(max_len, P->k, P->stride);
+    int64_t nb = n * max_len;
+    uint8_t *codes = (uint8_t *)malloc((size_t)(nb > 0 ? nb : 1));
+    uint8_t *rcb = (uint8_t *)malloc((size_t)(nb > 0 ? nb : 1));
+    uint64_t *keys = (uint64_t *)malloc((size_t)(n * S > 0 ? n * S : 1) * 8);
+    uint32_t *qoffs = (uint32_t *)malloc((size_t)(n * S > 0 ? n * S : 1) * 4);
+    uint64_t *st = (uint64_t *)malloc((size_t)(n * S > 0 ? n * S : 1) * 8);
+    uint64_t *en = (uint64_t *)malloc((size_t)(n * S > 0 ? n * S : 1) * 8);
+    if (!codes || !rcb || !keys || !qoffs || !st || !en) {
+        free(codes); free(rcb); free(keys); free(qoffs); free(st); free(en);
+        return -2;
+    }
+    or_counters total = {0, 0, 0, 0};
+#ifdef _OPENMP
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+#endif
+    {
+        or_counters local = {0, 0, 0, 0};
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int64_t r = 0; r < n; r++) {
+            or_pack_bases(bases + r * max_len, codes + r * max_len, max_len);
+            map_one_read(ix, P, codes, lens, max_len, S, r, rcb, keys, qoffs, st, en, fw, rc, mg,
+                         &local);
+        }
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+        {
+            total.probes += local.probes;
+            total.postings += local.postings;
+            total.windows += local.windows;
+            total.reads_rc += local.reads_rc;
+        }
+    }
+    if (cnt) *cnt = total;
+    free(codes); free(rcb); free(keys); free(qoffs); free(st); free(en);
+    return 0;
+}
+
+/* ------------------------------------------------------------- A10 */
+
+/* Follows _unclipped5, linear_gpu_fm.mojo:371-386 (reverse reads use sl). */
+int64_t or_unclipped5(int64_t flag, int64_t pos0, int64_t sl, int64_t sr, int64_t qlen) {
+    if ((flag & 4) != 0 || qlen <= 0) return 0;
+    if (sl < 0) sl = 0;
+    if (sr < 0) sr = 0;
+    if (sl + sr >= qlen) { sl = 0; sr = 0; }
+    int64_t mid = qlen - sl - sr;
+    if ((flag & 16) != 0) return pos0 + mid + sl - 1;
+    return pos0 - sl;
+}
+
+/* Follows _coord_key, linear_gpu_fm.mojo:389-395. */
+uint64_t or_coord_key(int64_t tid, int64_t pos0, int64_t orig) {
+    if (tid < 0) return (0xFFFFFFFFull << 32) | ((uint64_t)orig & 0xFFFFFFFFull);
+    if (pos0 < 0) pos0 = 0;
+    return ((uint64_t)tid << 32) | (uint64_t)pos0;
+}
+
+/* Follows _pack_end, linear_gpu_fm.mojo:398-405. */
+uint64_t or_pack_end(int64_t tid, int64_t u5, int64_t strand) {
+    uint64_t t = tid >= 0 ? (uint64_t)(tid + 1) : 0;
+    int64_t u = u5 + 1073741824;
+    if (u < 0) u = 0;
+    return (t << 48) | ((uint64_t)(strand & 1) << 47) | ((uint64_t)u & 140737488355327ull);
+}
+
+/* Single-end sort keys per merged record. tid_of_cid maps index contig ids to
+ * canonical (f/r-stripped) target ids. pair = pair_base + record index.
+ * Follows the SE builder, linear_gpu_fm.mojo:800-823. */
+void or_sort_keys_se(int64_t n, const int32_t *cid, const uint32_t *pos, const int32_t *flag,
+                     const uint32_t *sl, const uint32_t *sr, const uint32_t *nm,
+                     const uint32_t *mapq, const uint32_t *lens, const int32_t *tid_of_cid,
+                     int64_t pair_base, uint64_t *coord, uint64_t *dhi, uint64_t *dlo,
+                     uint32_t *score, uint32_t *pair) {
+    for (int64_t i = 0; i < n; i++) {
+        int64_t tid = cid[i] >= 0 ? tid_of_cid[cid[i]] : -1;
+        int64_t fl = flag[i];
+        coord[i] = or_coord_key(tid, (int64_t)pos[i], i);
+        if (tid >= 0) {
+            dhi[i] = or_pack_end(tid, or_unclipped5(fl, pos[i], sl[i], sr[i], lens[i]), (fl >> 4) & 1);
+            dlo[i] = 0;
+        } else {
+            dhi[i] = ~0ull;
+            dlo[i] = (uint64_t)i;
+        }
+        int64_t nmc = nm[i];
+        if (nmc > 255) nmc = 255;
+        score[i] = (uint32_t)(((int64_t)mapq[i] << 8) | (255 - nmc));
+        pair[i] = (uint32_t)(pair_base + i);
+    }
+}
+
+/* ------------------------------------------------------------- A11 / A12 */
+
+typedef struct { uint64_t hi, lo; uint32_t idx; } or_k2;
+
+static void merge_sort_k2(or_k2 *a, or_k2 *tmp, int64_t n) {
+    for (int64_t w = 1; w < n; w *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * w) {
+            int64_t mid = lo + w < n ? lo + w : n;
+            int64_t hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int64_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) {
+                int take_j = a[j].hi < a[i].hi || (a[j].hi == a[i].hi && a[j].lo < a[i].lo);
+                tmp[o++] = take_j ? a[j++] : a[i++];
+            }
+            while (i < mid) tmp[o++] = a[i++];
+            while (j < hi) tmp[o++] = a[j++];
+        }
+        memcpy(a, tmp, (size_t)n * sizeof(or_k2));
+    }
+}
+
+/* perm = stable coord order; dup_perm = stable (dhi, dlo) order; is_dup in
+ * original order: over each run of equal (dhi, dlo) (dhi = ~0 never a dup),
+ * best = max score, ties -> min pair id; every record whose pair differs from
+ * the best pair is a duplicate.
+ * Follows gpu_sort_markdup, linear_gpu_sort.mojo:24-551 (one tile), and
+ * _markdup_runs / order_run_perms, bam_emit.py:577-601, 756-809. */
+int or_sort_markdup(int64_t n, const uint64_t *coord, const uint64_t *dhi, const uint64_t *dlo,
+                    const uint32_t *score, const uint32_t *pair, uint32_t *is_dup, uint32_t *perm,
+                    int do_markdup, uint32_t *dup_perm) {
+    if (n <= 0) return 0;
+    or_k2 *a = (or_k2 *)malloc((size_t)n * sizeof(or_k2));
+    or_k2 *t = (or_k2 *)malloc((size_t)n * sizeof(or_k2));
+    if (!a || !t) { free(a); free(t); return -1; }
+    for (int64_t i = 0; i < n; i++) is_dup[i] = 0;
+    if (do_markdup) {
+        for (int64_t i = 0; i < n; i++) { a[i].hi = dhi[i]; a[i].lo = dlo[i]; a[i].idx = (uint32_t)i; }
+        merge_sort_k2(a, t, n);
+        int64_t gi = 0;
+        while (gi < n) {
+            uint32_t o0 = a[gi].idx;
+            if (dhi[o0] == ~0ull) { is_dup[o0] = 0; gi++; continue; }
+            int64_t gj = gi + 1;
+            while (gj < n && dhi[a[gj].idx] == dhi[o0] && dlo[a[gj].idx] == dlo[o0]) gj++;
+            uint32_t best_pair = pair[o0], best_score = score[o0];
+            for (int64_t g = gi; g < gj; g++) {
+                uint32_t ok = a[g].idx;
+                if (score[ok] > best_score || (score[ok] == best_score && pair[ok] < best_pair)) {
+                    best_score = score[ok];
+                    best_pair = pair[ok];
+                }
+            }
+            for (int64_t g = gi; g < gj; g++) {
+                uint32_t ok = a[g].idx;
+                is_dup[ok] = pair[ok] != best_pair ? 1u : 0u;
+            }
+            gi = gj;
+        }
+        if (dup_perm)
+            for (int64_t i = 0; i < n; i++) dup_perm[i] = a[i].idx;
+    } else if (dup_perm) {
+        for (int64_t i = 0; i < n; i++) dup_perm[i] = (uint32_t)i;
+    }
+    for (int64_t i = 0; i < n; i++) { a[i].hi = coord[i]; a[i].lo = 0; a[i].idx = (uint32_t)i; }
+    merge_sort_k2(a, t, n);
+    for (int64_t i = 0; i < n; i++) perm[i] = a[i].idx;
+    free(a);
+    free(t);
+    return 0;
+}
+
+int or_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,217 @@
+"""ctypes wrapper of the oracle (oracle.c) — TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker; the product never imports it. See oracle.c for the reference
+file:line each function restates and for which functions are pinned by what.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+REC_FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm", "iop", "iat", "ilen")
+REC_DTYPES = {"cid": np.int32, "flag": np.int32}
+
+_lib = None
+
+
+def build() -> Path:
+    tmp = LIB.with_suffix(".so.tmp")
+    subprocess.run(
+        ["gcc", "-O3", "-march=x86-64-v3", "-fopenmp", "-fPIC", "-shared", "-Wall", "-o", str(tmp),
+         str(HERE / "oracle.c")],
+        check=True,
+    )
+    os.replace(tmp, LIB)
+    return LIB
+
+
+class Index(C.Structure):
+    _fields_ = [
+        ("kmers", C.c_void_p),
+        ("offsets", C.c_void_p),
+        ("postings", C.c_void_p),
+        ("n_keys", C.c_int64),
+        ("n_postings", C.c_int64),
+        ("ref_codes", C.c_void_p),
+        ("contig_off", C.c_void_p),
+        ("n_contigs", C.c_int64),
+    ]
+
+
+class Params(C.Structure):
+    _fields_ = [(f, C.c_int32) for f in
+                ("k", "stride", "max_occ", "vote_occ", "vote_fast", "max_diff", "max_soft",
+                 "max_indel", "do_rc")]
+
+
+class Recs(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in REC_FIELDS]
+
+
+class Counters(C.Structure):
+    _fields_ = [("probes", C.c_int64), ("postings", C.c_int64), ("windows", C.c_int64),
+                ("reads_rc", C.c_int64)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.is_file():
+            build()
+        L = C.CDLL(str(LIB))
+        P, I64 = C.c_void_p, C.c_int64
+        L.or_code_of.restype = C.c_uint8
+        L.or_code_of.argtypes = [C.c_uint8]
+        L.or_pack_bases.argtypes = [P, P, I64]
+        L.or_bs_convert.argtypes = [P, I64, C.c_int]
+        L.or_convert_fasta_c2t.argtypes = [P, I64]
+        L.or_wang_hash.restype = C.c_uint64
+        L.or_wang_hash.argtypes = [C.c_uint64]
+        L.or_encode_strided.argtypes = [P, P, I64, I64, I64, I64, I64, P, P]
+        L.or_locate_bsearch.argtypes = [P, I64, P, P, I64, I64, P, P]
+        L.or_count_windows.restype = I64
+        L.or_count_windows.argtypes = [P, I64, C.c_int]
+        L.or_build_dense.restype = C.c_int
+        L.or_build_dense.argtypes = [P, P, I64, C.c_int, P, P, P, C.POINTER(I64), C.POINTER(I64)]
+        L.or_vote_extend.argtypes = [C.POINTER(Index), P, P, I64, I64, I64, P, P, P, I64, I64, I64,
+                                     I64, C.c_int, I64, P, C.c_int32, C.POINTER(Recs)]
+        L.or_slots_per.restype = I64
+        L.or_slots_per.argtypes = [I64, I64, I64]
+        L.or_map_batch.restype = C.c_int
+        L.or_map_batch.argtypes = [C.POINTER(Index), C.POINTER(Params), P, P, I64, I64,
+                                   C.POINTER(Recs), C.POINTER(Recs), C.POINTER(Recs),
+                                   C.POINTER(Counters), C.c_int]
+        L.or_unclipped5.restype = I64
+        L.or_unclipped5.argtypes = [I64, I64, I64, I64, I64]
+        L.or_coord_key.restype = C.c_uint64
+        L.or_coord_key.argtypes = [I64, I64, I64]
+        L.or_pack_end.restype = C.c_uint64
+        L.or_pack_end.argtypes = [I64, I64, I64]
+        L.or_sort_keys_se.argtypes = [I64, P, P, P, P, P, P, P, P, P, I64, P, P, P, P, P]
+        L.or_sort_markdup.restype = C.c_int
+        L.or_sort_markdup.argtypes = [I64, P, P, P, P, P, P, P, C.c_int, P]
+        L.or_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def params(k=21, stride=5, max_occ=16384, vote_occ=256, vote_fast=None, max_diff=6, max_soft=8,
+           max_indel=0, do_rc=1) -> Params:
+    return Params(k, stride, max_occ, vote_occ, vote_occ if vote_fast is None else vote_fast,
+                  max_diff, max_soft, max_indel, do_rc)
+
+
+def new_recs(n: int) -> dict:
+    return {f: np.zeros(n, dtype=REC_DTYPES.get(f, np.uint32)) for f in REC_FIELDS}
+
+
+def recs_struct(d: dict) -> Recs:
+    r = Recs()
+    for f in REC_FIELDS:
+        setattr(r, f, _p(d[f]))
+    return r
+
+
+def build_dense(seq: np.ndarray, contig_off: np.ndarray, k: int):
+    """dense-v1 (kmers, offsets, postings) by stable sort (build_dense_pack)."""
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    contig_off = np.ascontiguousarray(contig_off, dtype=np.uint64)
+    nc = contig_off.size - 1
+    cap = int(lib().or_count_windows(_p(contig_off), nc, k))
+    kmers = np.empty(max(cap, 1), dtype=np.uint64)
+    offsets = np.empty(max(cap, 1) + 1, dtype=np.uint64)
+    postings = np.empty(2 * max(cap, 1), dtype=np.uint32)
+    nk, npost = C.c_int64(0), C.c_int64(0)
+    rc = lib().or_build_dense(_p(seq), _p(contig_off), nc, k, _p(kmers), _p(offsets), _p(postings),
+                              C.byref(nk), C.byref(npost))
+    if rc != 0:
+        raise MemoryError("or_build_dense failed")
+    return kmers[: nk.value].copy(), offsets[: nk.value + 1].copy(), postings[: 2 * npost.value].copy()
+
+
+class OracleIndex:
+    """Holds the arrays an or_index points at."""
+
+    def __init__(self, kmers, offsets, postings, seq, contig_off):
+        self.kmers = np.ascontiguousarray(kmers, dtype=np.uint64)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        self.postings = np.ascontiguousarray(postings, dtype=np.uint32)
+        seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        self.ref_codes = np.empty_like(seq)
+        lib().or_pack_bases(_p(seq), _p(self.ref_codes), seq.size)
+        self.contig_off = np.ascontiguousarray(contig_off, dtype=np.uint64)
+        self.struct = Index(_p(self.kmers), _p(self.offsets), _p(self.postings), self.kmers.size,
+                            self.postings.size // 2, _p(self.ref_codes), _p(self.contig_off),
+                            self.contig_off.size - 1)
+
+
+def convert_reads(bases: np.ndarray, mode: int) -> np.ndarray:
+    out = np.ascontiguousarray(bases, dtype=np.uint8).copy()
+    lib().or_bs_convert(_p(out), out.size, mode)
+    return out
+
+
+def map_batch(ix: OracleIndex, bases: np.ndarray, lens: np.ndarray, prm: Params | None = None,
+              nthreads: int = 1):
+    """Reads already in-silico converted, [n, max_len] ASCII padded 'N'.
+    Returns (merged, fw, rc, counters)."""
+    prm = prm or params()
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n, max_len = bases.shape
+    if max_len < prm.k:
+        pad = np.full((n, prm.k), ord("N"), dtype=np.uint8)
+        pad[:, :max_len] = bases
+        bases, max_len = pad, prm.k
+    mg, fw, rc = new_recs(n), new_recs(n), new_recs(n)
+    cnt = Counters()
+    r = lib().or_map_batch(C.byref(ix.struct), C.byref(prm), _p(bases), _p(lens), n, max_len,
+                           C.byref(recs_struct(fw)), C.byref(recs_struct(rc)),
+                           C.byref(recs_struct(mg)), C.byref(cnt), nthreads)
+    if r != 0:
+        raise RuntimeError(f"or_map_batch failed ({r})")
+    return mg, fw, rc, cnt
+
+
+def sort_keys_se(rec: dict, lens: np.ndarray, tid_of_cid: np.ndarray, pair_base: int = 0):
+    n = lens.size
+    coord = np.empty(n, np.uint64)
+    dhi = np.empty(n, np.uint64)
+    dlo = np.empty(n, np.uint64)
+    score = np.empty(n, np.uint32)
+    pair = np.empty(n, np.uint32)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    tid = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
+    lib().or_sort_keys_se(n, _p(rec["cid"]), _p(rec["pos"]), _p(rec["flag"]), _p(rec["sl"]),
+                          _p(rec["sr"]), _p(rec["nm"]), _p(rec["mapq"]), _p(lens), _p(tid),
+                          pair_base, _p(coord), _p(dhi), _p(dlo), _p(score), _p(pair))
+    return coord, dhi, dlo, score, pair
+
+
+def sort_markdup(coord, dhi, dlo, score, pair, do_markdup=True):
+    n = len(coord)
+    coord = np.ascontiguousarray(coord, dtype=np.uint64)
+    dhi = np.ascontiguousarray(dhi, dtype=np.uint64)
+    dlo = np.ascontiguousarray(dlo, dtype=np.uint64)
+    score = np.ascontiguousarray(score, dtype=np.uint32)
+    pair = np.ascontiguousarray(pair, dtype=np.uint32)
+    is_dup = np.zeros(n, np.uint32)
+    perm = np.zeros(n, np.uint32)
+    dup_perm = np.zeros(n, np.uint32)
+    r = lib().or_sort_markdup(n, _p(coord), _p(dhi), _p(dlo), _p(score), _p(pair), _p(is_dup),
+                              _p(perm), int(bool(do_markdup)), _p(dup_perm))
+    if r != 0:
+        raise MemoryError("or_sort_markdup failed")
+    return is_dup, perm, dup_perm

@@ -1,0 +1,249 @@
+"""GPU parity: every record the HIP path produces equals the oracle's, bit for
+bit, on seeded synthetic inputs (the reference's CPU science restated in
+oracle/oracle.c). Called through the C ABI (libmialign.so)."""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import mialign
+from mialign import synth
+from mialign._lib import MA_BS_C2T, MA_BS_G2A, MA_ENC_2BIT
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
+
+
+def say(msg: str) -> None:
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def assert_recs_equal(got, want, what):
+    for f in FIELDS:
+        bad = np.nonzero(got[f] != want[f])[0]
+        if bad.size:
+            i = int(bad[0])
+            detail = {g: (int(got[g][i]), int(want[g][i])) for g in FIELDS}
+            raise AssertionError(
+                f"{what}: field {f} differs on {bad.size} reads; first read {i} (got, want): {detail}"
+            )
+
+
+class World:
+    def __init__(self, ref, k=21):
+        self.ref = ref
+        self.dual = synth.dual_convert(ref)
+        self.k = k
+        km, of, po = orc.build_dense(self.dual.seq, self.dual.contig_off, k)
+        self.dense = (km, of, po)
+        self.oix = orc.OracleIndex(km, of, po, self.dual.seq, self.dual.contig_off)
+        self.gix = mialign.GpuIndex.build(self.dual.seq, self.dual.contig_off, k)
+        self.tid = synth.tid_of_cid(self.dual.names)
+
+    def oracle_map(self, reads, mode=1, prm=None):
+        conv = orc.convert_reads(reads.bases, mode)
+        return orc.map_batch(self.oix, conv, reads.lens, prm, nthreads=8)
+
+
+@pytest.fixture(scope="module")
+def w1():
+    say("fixture w1: 1 Mbp reference, k=21")
+    return World(synth.gen_reference(1, 1, 1_000_000))
+
+
+@pytest.fixture(scope="module")
+def wrep():
+    say("fixture wrep: 1 Mbp reference with segmental duplicates + tandem repeats")
+    ref = synth.gen_reference(5, 2, 1_000_000, repeat_frac=0.05)
+    # overlay two duplicate families: diverged (vote overflow) and exact (high-occ tally)
+    fam_a = synth.gen_reference(6, 1, 1_000_000, seg_families=1, seg_copies=300, seg_len=400,
+                                seg_div=0.02)
+    fam_b = synth.gen_reference(7, 1, 1_000_000, seg_families=1, seg_copies=400, seg_len=300,
+                                seg_div=0.0)
+    seq = ref.seq.copy()
+    seq[:500_000] = fam_a.seq[:500_000]
+    seq[500_000:] = fam_b.seq[500_000:]
+    ref.seq = seq
+    return World(ref)
+
+
+def test_index_build_matches_oracle(w1):
+    say("test_index_build_matches_oracle")
+    gk, go, gp = w1.gix.export_dense()
+    km, of, po = w1.dense
+    assert np.array_equal(gk, km)
+    assert np.array_equal(go, of)
+    assert np.array_equal(gp, po)
+
+
+def test_index_build_with_n_runs():
+    say("test_index_build_with_n_runs")
+    ref = synth.gen_reference(3, 3, 300_000)
+    seq = ref.seq.copy()
+    for s in (1000, 150_000, 299_990):
+        seq[s : s + 10] = ord("N")
+    seq[50_000:50_003] = ord("R")  # IUPAC code: not ACGT
+    ref.seq = seq
+    dual = synth.dual_convert(ref)
+    km, of, po = orc.build_dense(dual.seq, dual.contig_off, 21)
+    g = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21)
+    gk, go, gp = g.export_dense()
+    assert np.array_equal(gk, km) and np.array_equal(go, of) and np.array_equal(gp, po)
+
+
+def test_map_config1_ascii_all_records(w1):
+    say("test_map_config1_ascii_all_records: 10k x 150 bp, C2T")
+    reads = synth.gen_reads(2, w1.ref, 10_000)
+    got, gfw, grc = mialign.map_batch(w1.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T,
+                                      strands=True)
+    want, wfw, wrc, _ = w1.oracle_map(reads)
+    assert_recs_equal(gfw, wfw, "FW records")
+    assert_recs_equal(grc, wrc, "RC records")
+    assert_recs_equal(got, want, "merged records")
+    assert (got["cid"] >= 0).mean() > 0.99
+
+
+def test_map_config1_2bit_equals_ascii(w1):
+    say("test_map_config1_2bit_equals_ascii")
+    reads = synth.gen_reads(3, w1.ref, 5000)
+    packed = synth.pack_2bit(reads.bases, reads.lens)
+    got = mialign.map_batch(w1.gix, packed, reads.lens, 150, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
+    want, _, _, _ = w1.oracle_map(reads)
+    assert_recs_equal(got, want, "2-bit input")
+
+
+def test_map_uneven_lengths(w1):
+    say("test_map_uneven_lengths: 50-250 bp")
+    reads = synth.gen_reads(4, w1.ref, 8000, 50, 250)
+    got = mialign.map_batch(w1.gix, reads.bases, reads.lens, 250, bs_r1=MA_BS_C2T)
+    want, _, _, _ = w1.oracle_map(reads)
+    assert_recs_equal(got, want, "uneven lengths")
+
+
+def test_map_g2a_and_r1_r2_split(w1):
+    say("test_map_g2a_and_r1_r2_split")
+    reads = synth.gen_reads(8, w1.ref, 4000)
+    n1 = 2500
+    got = mialign.map_batch(w1.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T, bs_r2=MA_BS_G2A,
+                            n_r1=n1)
+    conv = np.concatenate([orc.convert_reads(reads.bases[:n1], 1),
+                           orc.convert_reads(reads.bases[n1:], 2)])
+    want, _, _, _ = orc.map_batch(w1.oix, conv, reads.lens, nthreads=8)
+    assert_recs_equal(got, want, "R1 C2T / R2 G2A")
+
+
+def test_map_repeats_overflow_and_high_occ(wrep):
+    say("test_map_repeats_overflow_and_high_occ")
+    reads = synth.gen_reads(9, wrep.ref, 8000, 60, 200, max_len=200)
+    got, gfw, grc = mialign.map_batch(wrep.gix, reads.bases, reads.lens, 200, bs_r1=MA_BS_C2T,
+                                      strands=True)
+    want, wfw, wrc, cnt = wrep.oracle_map(reads)
+    assert_recs_equal(gfw, wfw, "FW records (repeats)")
+    assert_recs_equal(grc, wrc, "RC records (repeats)")
+    assert_recs_equal(got, want, "merged records (repeats)")
+    # the exact family makes reads whose locus comes from the high-occ tally (votes 1 -> MAPQ 20)
+    assert ((want["mapq"] == 20) & (want["nm"] == 0)).sum() > 0
+
+
+def test_map_reads_with_n_and_short_reads(w1):
+    say("test_map_reads_with_n_and_short_reads")
+    reads = synth.gen_reads(10, w1.ref, 3000, 10, 150)
+    b = reads.bases.copy()
+    rng = np.random.default_rng(0)
+    for i in rng.choice(reads.n, 600, replace=False):
+        L = int(reads.lens[i])
+        for j in rng.choice(L, min(L, 3), replace=False):
+            b[i, j] = ord("N")
+    reads.bases = b
+    reads.lens[:5] = 0
+    got = mialign.map_batch(w1.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
+    want, _, _, _ = w1.oracle_map(reads)
+    assert_recs_equal(got, want, "reads with N / short reads")
+
+
+def test_map_knobs(w1):
+    say("test_map_knobs: max_diff 3, max_soft 4, vote_fast < vote_occ, no RC")
+    reads = synth.gen_reads(13, w1.ref, 3000, p_err=0.02)
+    for kw in ({"max_diff": 3, "max_soft": 4}, {"vote_fast": 2, "vote_occ": 256}, {"do_rc": 0}):
+        p = mialign.default_params(**kw)
+        op = orc.params(**{k: v for k, v in kw.items() if k != "vote_fast"},
+                        vote_fast=kw.get("vote_fast"))
+        got = mialign.map_batch(w1.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T, params=p)
+        want, _, _, _ = w1.oracle_map(reads, prm=op)
+        assert_recs_equal(got, want, f"knobs {kw}")
+
+
+def test_sort_markdup_parity_on_mapped_records(w1):
+    say("test_sort_markdup_parity_on_mapped_records")
+    reads = synth.gen_reads(14, w1.ref, 10_000)
+    want, _, _, _ = w1.oracle_map(reads)
+    keys = orc.sort_keys_se(want, reads.lens, w1.tid)
+    d, p, dp = mialign.sort_markdup(*keys)
+    wd, wp, wdp = orc.sort_markdup(*keys)
+    assert np.array_equal(p, wp)
+    assert np.array_equal(dp, wdp)
+    assert np.array_equal(d, wd)
+    assert d.sum() > 0
+
+
+def test_sort_markdup_stress_records():
+    say("test_sort_markdup_stress_records: 2M records")
+    key, rid, nm = synth.gen_sort_records(21, 2_000_000)
+    score = (255 - nm.astype(np.uint32)).astype(np.uint32)
+    dlo = np.zeros_like(key)
+    d, p, dp = mialign.sort_markdup(key, key, dlo, score, rid)
+    wd, wp, wdp = orc.sort_markdup(key, key, dlo, score, rid)
+    assert np.array_equal(p, wp)
+    assert np.array_equal(dp, wdp)
+    assert np.array_equal(d, wd)
+
+
+def test_sort_markdup_no_markdup_and_empty():
+    say("test_sort_markdup_no_markdup_and_empty")
+    key, rid, nm = synth.gen_sort_records(22, 10_000)
+    d, p, dp = mialign.sort_markdup(key, key, np.zeros_like(key), rid, rid, do_markdup=False)
+    assert not d.any()
+    assert np.array_equal(dp, np.arange(key.size, dtype=np.uint32))
+    assert np.array_equal(p, np.argsort(key, kind="stable").astype(np.uint32))
+    e = np.zeros(0, np.uint64)
+    d0, p0, _ = mialign.sort_markdup(e, e, e, np.zeros(0, np.uint32), np.zeros(0, np.uint32))
+    assert d0.size == 0 and p0.size == 0
+
+
+def test_seed_kmers_wang_hash():
+    say("test_seed_kmers_wang_hash")
+    rng = np.random.default_rng(1)
+    bases = np.frombuffer(b"ACGTN", dtype=np.uint8)[rng.integers(0, 5, size=(50, 40))]
+    bases = np.where(rng.random((50, 40)) < 0.9, bases, ord("A")).astype(np.uint8)
+    got = mialign.seed_kmers(bases, 11)
+    L = orc.lib()
+    for r in range(50):
+        for p in range(40):
+            w = bases[r, p : p + 11]
+            if p + 11 > 40 or any(c not in b"ACGT" for c in w.tobytes()):
+                assert got[r, p] == 0
+                continue
+            key = 0
+            for c in w.tobytes():
+                key = (key << 2) | b"ACGT".index(c)
+            assert got[r, p] == L.or_wang_hash(key)
+
+
+def test_index_open_dense_v1_dir(w1, tmp_path):
+    say("test_index_open_dense_v1_dir")
+    from mialign import mojo_linear_pack as pack
+
+    out = tmp_path / "pack"
+    pack.write_dense_dir(out, 21, *w1.dense, w1.dual.seq, w1.dual.contig_off, w1.dual.names)
+    g = mialign.GpuIndex.open(str(out))
+    assert g.n_keys == w1.dense[0].size
+    reads = synth.gen_reads(15, w1.ref, 1000)
+    got = mialign.map_batch(g, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
+    want, _, _, _ = w1.oracle_map(reads)
+    assert_recs_equal(got, want, "index opened from dense-v1 files")
