@@ -1,0 +1,218 @@
+"""Benchmark: reads/s of the seed -> extend -> sort path on one MI355X.
+
+Workload (default): synthetic 100 Mbp reference in 10 contigs (seeded ACGT,
+CpG depleted to ~1% of dinucleotides), dual C->T / G->A conversion, k=21
+dense-v1 index resident in HBM (hash table over u64 keys, u32 postings);
+10M single-end 150 bp bisulfite reads (non-CpG C->T p=0.99, CpG kept p=0.7,
+0.5% substitutions, 5% planted duplicates), 2-bit packed u8 input resident in
+HBM. One step = map every read (in-silico C->T, seeds, vote, extend, rescue,
+high-occ tally, FW + RC, merge) + SE sort keys + stable coordinate sort +
+duplicate marking, all on the device.
+
+Prints one JSON line. `roofline` is for the dominant kernel (the map kernel);
+`cpu_baseline` times the oracle (the C restatement of the reference's CPU
+science, OpenMP over reads + the stable-sort markdup) on a bounded sample of
+the same reads on this host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "mojo-align_amd" / "python"))
+sys.path.insert(0, str(ROOT))
+
+import numpy as np  # noqa: E402
+
+METRIC = "reads/s seed+extend+sort (bit-exact); served p50/p99 latency vs plain launch"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_read(cnt, n_sample: int, read_len: int) -> float:
+    """Bytes the algorithm must move per read: 2-bit read in, merged record out
+    (7 x u32), one 16-byte hash slot per seed probe, one 8-byte posting per
+    posting visited, one 2-bit reference window per window compared (counts
+    from the oracle's sequential walk of the same reads)."""
+    w = (read_len + 3) // 4
+    per = w + 28
+    per += (16 * cnt.probes + 8 * cnt.postings + w * cnt.windows) / n_sample
+    return per
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--genome", type=int, default=100_000_000)
+    ap.add_argument("--contigs", type=int, default=10)
+    ap.add_argument("--reads", type=int, default=10_000_000)
+    ap.add_argument("--read-len", type=int, default=150)
+    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(reads shard across GPUs with a replicated index; no multi-GPU run is made here)",
+              file=sys.stderr)
+        return 2
+
+    import mialign
+    from mialign import synth
+    from mialign._lib import MA_BS_C2T, MA_ENC_2BIT, check, lib
+
+    dev = 0
+    info = mialign.device_info(dev)
+    log(f"device {info['name']} cu={info['n_cu']} hbm={info['hbm_total'] / 2**30:.0f} GiB")
+
+    t0 = time.time()
+    ref = synth.gen_reference(1001, args.contigs, args.genome)
+    dual = synth.dual_convert(ref)
+    log(f"reference {args.genome} bp x2 conversions in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, dev)
+    log(f"GPU index build: {ix.n_keys} keys, {ix.n_postings} postings in {time.time() - t0:.1f}s")
+
+    t0 = time.time()
+    n, L = args.reads, args.read_len
+    reads = synth.gen_reads(1002, ref, n, L, L)
+    packed = synth.pack_2bit(reads.bases, reads.lens)
+    stride = packed.shape[1]
+    log(f"reads {n} x {L} bp generated + 2-bit packed in {time.time() - t0:.1f}s")
+
+    stream = mialign.Stream(dev)
+    d_bases = mialign.DeviceArray.from_host(packed, dev)
+    d_lens = mialign.DeviceArray.from_host(reads.lens, dev)
+    tid = synth.tid_of_cid(dual.names)
+    d_tid = mialign.DeviceArray.from_host(tid, dev)
+    recs = mialign.DeviceRecs(n, dev)
+    ws = mialign.Workspace(ix, n, L)
+    prm = mialign.default_params(k=21)
+    keys = {f: mialign.DeviceArray((n,), dt, dev) for f, dt in
+            (("coord", np.uint64), ("dhi", np.uint64), ("dlo", np.uint64), ("score", np.uint32),
+             ("pair", np.uint32))}
+    outs = {f: mialign.DeviceArray((n,), np.uint32, dev) for f in ("is_dup", "perm", "dup_perm")}
+    scratch_bytes = int(lib().ma_sort_scratch_bytes(n))
+    scratch = mialign.DeviceArray((scratch_bytes,), np.uint8, dev)
+    import ctypes as C
+
+    ev_a, ev_b = mialign.Event(), mialign.Event()
+
+    def step(timed_kernel: bool):
+        if timed_kernel:
+            ev_a.record(stream)
+        mialign.map_batch_dev(ix, ws, stream, d_bases, stride, d_lens, n, L, recs,
+                              encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T, params=prm)
+        if timed_kernel:
+            ev_b.record(stream)
+        check(lib().ma_sort_keys_se_dev(stream.handle, n, C.byref(recs.struct), d_lens.ptr, d_tid.ptr,
+                                        0, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
+                                        keys["score"].ptr, keys["pair"].ptr), "sort keys")
+        check(lib().ma_sort_markdup_dev(stream.handle, n, keys["coord"].ptr, keys["dhi"].ptr,
+                                        keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr,
+                                        outs["is_dup"].ptr, outs["perm"].ptr, 1, outs["dup_perm"].ptr,
+                                        scratch.ptr, scratch_bytes), "sort markdup")
+
+    for i in range(args.warmup):
+        step(False)
+        stream.sync()
+        log(f"warmup step {i + 1}/{args.warmup} done")
+    kernel_ms = []
+    stream.sync()
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        step(True)
+        ev_b.synchronize()
+        kernel_ms.append(ev_a.elapsed_ms(ev_b))
+    stream.sync()
+    elapsed = time.perf_counter() - t_start
+    ms_per_step = 1000.0 * elapsed / args.steps
+    value = n * args.steps / elapsed
+    map_ms = float(np.mean(kernel_ms))
+    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, map kernel {map_ms:.1f} ms, "
+        f"{value / 1e6:.2f} M reads/s")
+
+    merged = recs.download()
+    is_dup = outs["is_dup"].download()
+    mapped_frac = float((merged["cid"] >= 0).mean())
+
+    # ---- CPU baseline (oracle) on a bounded sample + sample parity + bytes/read
+    cpu = None
+    roofline = None
+    parity_sample = None
+    if not args.no_cpu:
+        from oracle import oracle as orc
+
+        t0 = time.time()
+        km, of, po = ix.export_dense()
+        oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+        del km, of, po
+        ns = min(args.cpu_sample, n)
+        conv = orc.convert_reads(reads.bases[:ns], 1)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"cpu baseline: oracle index ready in {time.time() - t0:.1f}s; mapping {ns} reads on {threads} threads")
+        t0 = time.perf_counter()
+        want, _, _, cnt = orc.map_batch(oix, conv, reads.lens[:ns], nthreads=threads)
+        kk = orc.sort_keys_se(want, reads.lens[:ns], tid)
+        wd, wp, _ = orc.sort_markdup(*kk)
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": ns / cpu_s, "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"first {ns} reads of the same workload (oracle map with OpenMP over reads "
+                         f"+ SE keys + stable-sort markdup), {cpu_s:.1f} s"}
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in
+                 ("cid", "pos", "mapq", "flag", "sl", "sr", "nm"))
+        parity_sample = {"reads": ns, "records_bit_exact": bool(ok)}
+        bpr = algorithmic_bytes_per_read(cnt, ns, L)
+        achieved = bpr * n / (map_ms / 1000.0) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "map_reads_kernel", "kernel_ms": round(map_ms, 3),
+                    "algorithmic_bytes_per_read": round(bpr, 1)}
+        log(f"cpu baseline {cpu['value']:.0f} reads/s; sample parity {ok}; {bpr:.0f} B/read algorithmic")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"single large batch: {args.genome // 1_000_000} Mbp reference "
+                        f"({args.contigs} contigs, 1% CpG) x2 conversions, {n} SE {L} bp bisulfite "
+                        "reads, 2-bit packed u8 input, k=21 index resident in HBM",
+            "reads": n,
+            "read_len": L,
+            "genome_bp": args.genome,
+            "k": 21,
+            "mapped_frac": round(mapped_frac, 4),
+            "duplicates_marked": int(is_dup.sum()),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "parity_sample": parity_sample,
+    }
+    print(json.dumps(line), flush=True)
+    ix.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

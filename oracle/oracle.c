@@ -280,6 +280,8 @@ typedef struct {
     int64_t postings;  /* postings read (vote + rescue + high-occ tally) */
     int64_t windows;   /* reference windows compared (extend + rescue + tally) */
     int64_t reads_rc;  /* reads that ran the RC pass */
+    int64_t overflow;  /* vote-table first-min replacements attempted (table full) */
+    int64_t tally;     /* reads placed by the high-occ exact tally */
 } or_counters;
 
 static void rec_reset(or_recs *o, int64_t r) {
@@ -313,7 +315,8 @@ static void vote_extend_read(const or_index *ix, const uint8_t *codes, const uin
                              const uint64_t *occ_end, const uint32_t *q_offs,
                              int64_t max_diff, int64_t max_soft, int64_t max_indel,
                              int64_t vote_occ, int pass_mode, int32_t strand_flag, int64_t rid,
-                             or_recs *out, int64_t *c_post, int64_t *c_win) {
+                             or_recs *out, int64_t *c_post, int64_t *c_win, int64_t *c_ovf,
+                             int64_t *c_tally) {
     /* :800-802 pass-1 safety */
     if (pass_mode == 1 && out->cid[rid] >= 0) return;
     rec_reset(out, rid);
@@ -378,6 +381,7 @@ static void vote_extend_read(const or_index *ix, const uint8_t *codes, const uin
                     last_i = n_cand;
                     n_cand++;
                 } else {
+                    (*c_ovf)++;
                     int64_t min_i = 0;
                     int32_t min_n = cand_n[0];
                     for (int64_t z = 1; z < OR_TOP; z++)
@@ -745,6 +749,7 @@ static void vote_extend_read(const or_index *ix, const uint8_t *codes, const uin
             if (best_hx_i >= 0 && (best_hi >= 2 || (best_hi >= 1 && second_hi == 0)) &&
                 (best_cid < 0 || best_cost > 0 || (int64_t)best_votes < best_hi)) {
                 uint64_t bk = hx_key[best_hx_i];
+                (*c_tally)++;
                 best_cost = 0;
                 if (best_hi > second_hi) {
                     best_votes = (int32_t)(best_hi > 5 ? 5 : best_hi);
@@ -788,12 +793,12 @@ void or_vote_extend(const or_index *ix, const uint8_t *codes, const uint32_t *le
                     int pass_mode, int64_t n_active, const int32_t *rid_map, int32_t strand_flag,
                     or_recs *out) {
     int64_t n = n_active >= 0 ? n_active : n_reads;
-    int64_t cp = 0, cw = 0;
+    int64_t cp = 0, cw = 0, co = 0, ct = 0;
     for (int64_t t = 0; t < n; t++) {
         int64_t rid = n_active >= 0 ? rid_map[t] : t;
         vote_extend_read(ix, codes, lens, max_len, slots_per, occ_start, occ_end, q_offs,
                          max_diff, max_soft, max_indel, vote_occ, pass_mode, strand_flag, rid,
-                         out, &cp, &cw);
+                         out, &cp, &cw, &co, &ct);
     }
 }
 
@@ -816,7 +821,7 @@ static void map_one_read(const or_index *ix, const or_params *P, const uint8_t *
                          uint8_t *rc_buf, uint64_t *keys, uint32_t *qoffs, uint64_t *st,
                          uint64_t *en, or_recs *fw, or_recs *rc, or_recs *mg, or_counters *cnt) {
     int64_t base = rid * slots_per;
-    int64_t cp = 0, cw = 0;
+    int64_t cp = 0, cw = 0, co = 0, ct = 0;
     /* FW */
     encode_one_read(codes_all, rid, lens[rid], max_len, P->k, P->stride, slots_per, keys, qoffs);
     for (int64_t s = 0; s < slots_per; s++)
@@ -824,14 +829,14 @@ static void map_one_read(const or_index *ix, const or_params *P, const uint8_t *
                    &en[base + s]);
     cnt->probes += count_valid_keys(keys, base, slots_per);
     vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
-                     P->max_soft, P->max_indel, P->vote_fast, 0, 0, rid, fw, &cp, &cw);
+                     P->max_soft, P->max_indel, P->vote_fast, 0, 0, rid, fw, &cp, &cw, &co, &ct);
     if (fw->cid[rid] < 0) {
         if (P->vote_fast < P->vote_occ)
             vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
-                             P->max_soft, P->max_indel, P->vote_occ, 0, 0, rid, fw, &cp, &cw);
+                             P->max_soft, P->max_indel, P->vote_occ, 0, 0, rid, fw, &cp, &cw, &co, &ct);
         if (fw->cid[rid] < 0)
             vote_extend_read(ix, codes_all, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
-                             P->max_soft, P->max_indel, P->vote_occ, 1, 0, rid, fw, &cp, &cw);
+                             P->max_soft, P->max_indel, P->vote_occ, 1, 0, rid, fw, &cp, &cw, &co, &ct);
     }
     /* RC */
     rec_reset(rc, rid);
@@ -851,16 +856,16 @@ static void map_one_read(const or_index *ix, const or_params *P, const uint8_t *
                        &st[base + s], &en[base + s]);
         cnt->probes += count_valid_keys(keys, base, slots_per);
         vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs, P->max_diff,
-                         P->max_soft, P->max_indel, P->vote_fast, 0, 16, rid, rc, &cp, &cw);
+                         P->max_soft, P->max_indel, P->vote_fast, 0, 16, rid, rc, &cp, &cw, &co, &ct);
         if (rc->cid[rid] < 0) {
             if (P->vote_fast < P->vote_occ)
                 vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs,
                                  P->max_diff, P->max_soft, P->max_indel, P->vote_occ, 0, 16, rid,
-                                 rc, &cp, &cw);
+                                 rc, &cp, &cw, &co, &ct);
             if (rc->cid[rid] < 0)
                 vote_extend_read(ix, rc_buf, lens, max_len, slots_per, st, en, qoffs,
                                  P->max_diff, P->max_soft, P->max_indel, P->vote_occ, 1, 16, rid,
-                                 rc, &cp, &cw);
+                                 rc, &cp, &cw, &co, &ct);
         }
     }
     /* merge: RC if FW unmapped and RC mapped, or both mapped and RC has lower
@@ -887,6 +892,8 @@ static void map_one_read(const or_index *ix, const or_params *P, const uint8_t *
     if (mg->cid[rid] < 0 || m < 1) rec_reset(mg, rid);
     cnt->postings += cp;
     cnt->windows += cw;
+    cnt->overflow += co;
+    cnt->tally += ct;
 }
 
 /* Slots per read for a batch: S = (max(max_len, k) - k) / stride + 1
@@ -915,13 +922,13 @@ int or_map_batch(const or_index *ix, const or_params *P, const uint8_t *bases,
         free(codes); free(rcb); free(keys); free(qoffs); free(st); free(en);
         return -2;
     }
-    or_counters total = {0, 0, 0, 0};
+    or_counters total = {0, 0, 0, 0, 0, 0};
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel num_threads(nthreads)
 #endif
     {
-        or_counters local = {0, 0, 0, 0};
+        or_counters local = {0, 0, 0, 0, 0, 0};
 #ifdef _OPENMP
 #pragma omp for schedule(static)
 #endif
@@ -938,6 +945,8 @@ int or_map_batch(const or_index *ix, const or_params *P, const uint8_t *bases,
             total.postings += local.postings;
             total.windows += local.windows;
             total.reads_rc += local.reads_rc;
+            total.overflow += local.overflow;
+            total.tally += local.tally;
         }
     }
     if (cnt) *cnt = total;

@@ -58,7 +58,7 @@ class Recs(C.Structure):
 
 class Counters(C.Structure):
     _fields_ = [("probes", C.c_int64), ("postings", C.c_int64), ("windows", C.c_int64),
-                ("reads_rc", C.c_int64)]
+                ("reads_rc", C.c_int64), ("overflow", C.c_int64), ("tally", C.c_int64)]
 
 
 def lib():

@@ -59,18 +59,14 @@ def w1():
 
 @pytest.fixture(scope="module")
 def wrep():
-    say("fixture wrep: 1 Mbp reference with segmental duplicates + tandem repeats")
-    ref = synth.gen_reference(5, 2, 1_000_000, repeat_frac=0.05)
-    # overlay two duplicate families: diverged (vote overflow) and exact (high-occ tally)
-    fam_a = synth.gen_reference(6, 1, 1_000_000, seg_families=1, seg_copies=300, seg_len=400,
-                                seg_div=0.02)
-    fam_b = synth.gen_reference(7, 1, 1_000_000, seg_families=1, seg_copies=400, seg_len=300,
+    say("fixture wrep: 2 contigs with diverged (300 x 400 bp) and exact (800 x 300 bp) duplicates")
+    fam_a = synth.gen_reference(6, 1, 1_000_000, repeat_frac=0.05, seg_families=1, seg_copies=300,
+                                seg_len=400, seg_div=0.02)
+    fam_b = synth.gen_reference(7, 1, 1_000_000, seg_families=1, seg_copies=800, seg_len=300,
                                 seg_div=0.0)
-    seq = ref.seq.copy()
-    seq[:500_000] = fam_a.seq[:500_000]
-    seq[500_000:] = fam_b.seq[500_000:]
-    ref.seq = seq
-    return World(ref)
+    seq = np.concatenate([fam_a.seq, fam_b.seq])
+    off = np.array([0, fam_a.seq.size, seq.size], dtype=np.uint64)
+    return World(synth.Reference(seq, off, ["chr1", "chr2"]))
 
 
 def test_index_build_matches_oracle(w1):
@@ -147,8 +143,8 @@ def test_map_repeats_overflow_and_high_occ(wrep):
     assert_recs_equal(gfw, wfw, "FW records (repeats)")
     assert_recs_equal(grc, wrc, "RC records (repeats)")
     assert_recs_equal(got, want, "merged records (repeats)")
-    # the exact family makes reads whose locus comes from the high-occ tally (votes 1 -> MAPQ 20)
-    assert ((want["mapq"] == 20) & (want["nm"] == 0)).sum() > 0
+    # both rare branches ran: vote-table overflow replacement and the high-occ exact tally
+    assert cnt.overflow > 0 and cnt.tally > 0, (cnt.overflow, cnt.tally)
 
 
 def test_map_reads_with_n_and_short_reads(w1):
