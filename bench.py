@@ -58,7 +58,7 @@ def main() -> int:
     ap.add_argument("--contigs", type=int, default=10)
     ap.add_argument("--reads", type=int, default=10_000_000)
     ap.add_argument("--read-len", type=int, default=150)
-    ap.add_argument("--cpu-sample", type=int, default=400_000)
+    ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
 

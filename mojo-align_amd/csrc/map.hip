@@ -33,8 +33,6 @@ using namespace ma;
 
 namespace {
 
-constexpr int MAXW = 8;  // read words: reads up to 256 bases
-constexpr int MAX_READ = MAXW * 32;
 constexpr int TOP = 256;  // vote table (TOP, linear_gpu_locate.mojo:812)
 constexpr int KEEP = 16;  // extend picks (KEEP, :813)
 constexpr int HX = 48;    // exact tally (HX, :1249)
@@ -43,20 +41,7 @@ constexpr int WPB = 4;    // waves per block
 constexpr int BLOCK = WPB * 64;
 constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
-struct WaveState {
-    uint64_t fw2[MAXW + 2];  // [0] = 0 pad, read words at [1..W], zero tail
-    uint64_t fwN[MAXW + 2];
-    uint64_t rd2[MAXW + 1];  // read of the current pass (FW or RC), zero tail
-    uint64_t rdN[MAXW + 1];
-    uint64_t ord[MAXS];  // occ << 46 | slot << 40 | start, ascending
-    uint64_t cand_key[TOP];
-    int32_t cand_n[TOP];
-    int32_t pick_i[KEEP];
-    int32_t pick_n[KEEP];
-    uint64_t hx_key[HX];
-    int32_t hx_n[HX];
-    int32_t hx_min[HX];
-};
+constexpr int PBUF = 256;  // postings prefetched into LDS per wave
 
 struct Knobs {
     int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc;
@@ -87,20 +72,42 @@ struct ReadsIn {
     int64_t n_r1;
 };
 
-__device__ inline Rec unmapped_rec() { return Rec{-1, 0u, 0u, 4, 0u, 0u, 0u}; }
+#define DEV __device__ __forceinline__
 
-__device__ inline int lane_id() { return (int)(threadIdx.x & 63); }
+DEV Rec unmapped_rec() { return Rec{-1, 0u, 0u, 4, 0u, 0u, 0u}; }
+
+DEV Rec sel_rec(bool c, const Rec &a, const Rec &b) {
+    Rec r;
+    r.cid = c ? a.cid : b.cid;
+    r.pos = c ? a.pos : b.pos;
+    r.mapq = c ? a.mapq : b.mapq;
+    r.flag = c ? a.flag : b.flag;
+    r.sl = c ? a.sl : b.sl;
+    r.sr = c ? a.sr : b.sr;
+    r.nm = c ? a.nm : b.nm;
+    return r;
+}
+
+DEV int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Wave-uniform values read from LDS live in scalar registers.
+DEV uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+DEV int unii(int v) { return (int)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+DEV uint64_t uni64(uint64_t v) {
+    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
+           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+}
 
 // Orders LDS traffic between lanes of one wave (lockstep lanes, in-order LDS
 // queue per wave): compiler barrier + wait for outstanding LDS operations.
-__device__ inline void wave_sync() {
+DEV void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline uint64_t wave_min_u64(uint64_t v) {
+DEV uint64_t wave_min_u64(uint64_t v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         uint64_t w = __shfl_xor(v, o, 64);
@@ -109,8 +116,15 @@ __device__ inline uint64_t wave_min_u64(uint64_t v) {
     return v;
 }
 
-// Multiset top-2 (b1 >= b2) across the wave.
-__device__ inline void wave_top2(int &b1, int &b2) {
+// Multiset top-2 (b1 >= b2), one value at a time, with selects only.
+DEV void top2_push(int v, int &b1, int &b2) {
+    int nb2 = v > b1 ? b1 : (v > b2 ? v : b2);
+    int nb1 = v > b1 ? v : b1;
+    b1 = nb1;
+    b2 = nb2;
+}
+
+DEV void wave_top2(int &b1, int &b2) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) {
         int c1 = __shfl_xor(b1, o, 64), c2 = __shfl_xor(b2, o, 64);
@@ -122,113 +136,25 @@ __device__ inline void wave_top2(int &b1, int &b2) {
     }
 }
 
-__device__ inline void top2_push(int v, int &b1, int &b2) {
-    if (v > b1) {
-        b2 = b1;
-        b1 = v;
-    } else if (v > b2) {
-        b2 = v;
-    }
-}
-
 // 32 bases at base offset p of a zero-padded LDS word array.
-__device__ inline uint64_t lds_window(const uint64_t *w, int p) {
+DEV uint64_t lds_window(const uint64_t *w, int p) {
     int wi = p >> 5, sh = (p & 31) * 2;
     uint64_t a = w[wi];
     if (sh == 0) return a;
     return (a << sh) | (w[wi + 1] >> (64 - sh));
 }
 
-__device__ inline uint64_t pick_word(const uint64_t (&m)[MAXW], int i) {
-    uint64_t v = 0;
-#pragma unroll
-    for (int t = 0; t < MAXW; t++) v = (t == i) ? m[t] : v;
-    return v;
-}
-
 // Both bits of the groups j < n.
-__device__ inline uint64_t groups_mask2(int n) {
+DEV uint64_t groups_mask2(int n) {
     uint64_t f = first_groups_mask(n);
     return f | (f << 1);
-}
-
-// Mismatch flags (low bit of each 2-bit group) of the current read against
-// the reference window starting at global base g; bases >= L read as matches.
-// A base mismatches when either side is N or the codes differ.
-__device__ inline int window_mm(const IndexView &ix, int64_t g, const WaveState &S, int W, int L,
-                                uint64_t (&mm)[MAXW]) {
-    int tot = 0;
-#pragma unroll
-    for (int t = 0; t < MAXW; t++) {
-        uint64_t m = 0;
-        if (t < W) {
-            uint64_t r = window32(ix.ref2, g + 32 * t);
-            uint64_t x = r ^ S.rd2[t];
-            m = (x | (x >> 1)) & LOW2;
-            if (ix.ref_has_n) m |= window32(ix.refN, g + 32 * t);
-            m |= S.rdN[t];
-            m &= first_groups_mask(L - 32 * t);
-        }
-        mm[t] = m;
-        tot += __popcll(m);
-    }
-    return tot;
-}
-
-__device__ inline bool mm_bit(const uint64_t (&mm)[MAXW], int j) {
-    return (pick_word(mm, j >> 5) >> (62 - 2 * (j & 31))) & 1ull;
-}
-
-// Index of the first matching base (L if none).
-__device__ inline int first_match(const uint64_t (&mm)[MAXW], int W, int L) {
-    int res = L;
-    bool found = false;
-#pragma unroll
-    for (int t = 0; t < MAXW; t++) {
-        if (!found && t < W) {
-            uint64_t inv = ~mm[t] & first_groups_mask(L - 32 * t);
-            if (inv) {
-                res = 32 * t + (__clzll(inv) - 1) / 2;
-                found = true;
-            }
-        }
-    }
-    return res;
-}
-
-// Number of consecutive mismatching bases ending at base L-1.
-__device__ inline int trailing_run(const uint64_t (&mm)[MAXW], int L) {
-    int run = 0;
-    int t = (L - 1) >> 5;
-    int gl = (L - 1) & 31;
-    while (t >= 0) {
-        uint64_t v = pick_word(mm, t) >> (62 - 2 * gl);
-        uint64_t span = (gl == 31) ? ~0ull : ((1ull << (2 * gl + 2)) - 1);
-        uint64_t inv = ~v & LOW2 & span;
-        if (inv) return run + (__ffsll((long long)inv) - 1) / 2;
-        run += gl + 1;
-        t--;
-        gl = 31;
-    }
-    return run;
-}
-
-// Mismatches among bases [a, b).
-__device__ inline int count_range(const uint64_t (&mm)[MAXW], int a, int b) {
-    int tot = 0;
-#pragma unroll
-    for (int t = 0; t < MAXW; t++) {
-        uint64_t m = mm[t] & first_groups_mask(b - 32 * t) & ~first_groups_mask(a - 32 * t);
-        tot += __popcll(m);
-    }
-    return tot;
 }
 
 struct Caps {
     int budget, clip_cap, shift_cap;
 };
 
-__device__ inline Caps make_caps(const Knobs &kn, int L) {
+DEV Caps make_caps(const Knobs &kn, int L) {
     Caps c;
     int budget = kn.max_diff;
     if (budget < 2) budget = 2;
@@ -241,81 +167,13 @@ __device__ inline Caps make_caps(const Knobs &kn, int L) {
     return c;
 }
 
-// Soft-clip both ends (sl, sr <= clip_cap, sr stops before sl) and recount
-// with the reference's early abort (the count saturates at budget + 1).
-__device__ inline int softclip(const uint64_t (&mm)[MAXW], int W, int L, const Caps &cp, int &sl,
-                               int &sr) {
-    int fm = first_match(mm, W, L);
-    sl = fm < cp.clip_cap ? fm : cp.clip_cap;
-    int tr = trailing_run(mm, L);
-    int lim = L - 1 - sl;
-    if (lim < 0) lim = 0;
-    sr = tr < cp.clip_cap ? tr : cp.clip_cap;
-    if (sr > lim) sr = lim;
-    int nm = count_range(mm, sl, L - sr);
-    return nm > cp.budget ? cp.budget + 1 : nm;
-}
-
-__device__ inline uint64_t load_posting(const IndexView &ix, int64_t pi) {
+DEV uint64_t load_posting(const IndexView &ix, int64_t pi) {
     // (cid, pos) u32 pair: cid in the low word
     return *(const uint64_t *)(ix.postings + 2 * pi);
 }
 
-// ------------------------------------------------------------------ seeds
-// Lane s encodes slot s (q_off = s * stride) and probes the hash bucket;
-// valid seeds (0 < occ <= max_occ) are ranked by (occ, slot) into S.ord.
-__device__ int seed_and_order(const IndexView &ix, const Knobs &kn, WaveState &S, int L) {
-    int lane = lane_id();
-    int k = kn.k;
-    int n_slots = L >= k ? (L - k) / kn.stride + 1 : 0;
-    uint64_t e = ~0ull;
-    bool valid = false;
-    if (lane < n_slots) {
-        int q = lane * kn.stride;
-        uint64_t nw = lds_window(S.rdN, q) >> (64 - 2 * k);
-        if (nw == 0) {
-            uint64_t key = lds_window(S.rd2, q) >> (64 - 2 * k);
-            uint64_t b = bucket_of(key, ix.n_buckets);
-            bool done = false;
-            for (int probe = 0; probe < ix.max_probe && !done; probe++) {
-                const HashSlot *bk = ix.table + b * BUCKET_SLOTS;
-#pragma unroll
-                for (int s = 0; s < BUCKET_SLOTS; s++) {
-                    if (done) continue;
-                    uint64_t kk = bk[s].key;
-                    if (kk == key) {
-                        uint64_t val = bk[s].val;
-                        uint64_t occ = val >> 40;
-                        if (occ >= 1 && occ <= (uint64_t)kn.max_occ) {
-                            e = (occ << 46) | ((uint64_t)lane << 40) | (val & START_MASK);
-                            valid = true;
-                        }
-                        done = true;
-                    } else if (kk == EMPTY_KEY) {
-                        done = true;
-                    }
-                }
-                b = (b + 1 == ix.n_buckets) ? 0 : b + 1;
-            }
-        }
-    }
-    uint64_t vm = __ballot(valid);
-    int n_ord = __popcll(vm);
-    int rank = 0;
-    uint64_t m = vm;
-    while (m) {
-        int j = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        uint64_t ej = __shfl(e, j, 64);
-        rank += (ej < e) ? 1 : 0;
-    }
-    if (valid) S.ord[rank] = e;
-    wave_sync();
-    return n_ord;
-}
-
-__device__ inline Rec finish_rec(int64_t cid, int64_t pos, int votes, int cost, int sl, int sr,
-                                 int32_t strand_flag) {
+DEV Rec finish_rec(int64_t cid, int64_t pos, int votes, int cost, int sl, int sr,
+                   int32_t strand_flag) {
     Rec r;
     uint32_t mq = 20;
     if (votes >= 3) mq = 40;
@@ -331,353 +189,7 @@ __device__ inline Rec finish_rec(int64_t cid, int64_t pos, int votes, int cost, 
     return r;
 }
 
-// ----------------------------------------------------------------- pass 0
-// Vote over seeds with occ <= vote_cap, KEEP, extend at shifts 0,-1,+1,-2,+2,
-// then (if nothing accepted) the low-occ rescue.
-__device__ Rec pass0(const IndexView &ix, const Knobs &kn, WaveState &S, int n_ord, int W, int L,
-                     int vote_cap_in, int32_t strand_flag) {
-    const int lane = lane_id();
-    const uint64_t lt = lanemask_lt(lane);
-    const int vote_cap = vote_cap_in < 1 ? 256 : vote_cap_in;
-    int n_cand = 0;
-
-    for (int oi = 0; oi < n_ord; oi++) {
-        uint64_t e = S.ord[oi];
-        int occ = (int)(e >> 46);
-        if (occ > vote_cap) break;  // ord is ascending: no later seed votes
-        int q_off = (int)((e >> 40) & 63) * kn.stride;
-        int64_t start = (int64_t)(e & START_MASK);
-        for (int c0 = 0; c0 < occ; c0 += 64) {
-            bool valid = false;
-            uint64_t vk = 0;
-            if (c0 + lane < occ) {
-                uint64_t p = load_posting(ix, start + c0 + lane);
-                uint32_t cid = (uint32_t)p, pos = (uint32_t)(p >> 32);
-                if ((int64_t)cid < ix.n_contigs && (int64_t)pos >= q_off) {
-                    valid = true;
-                    vk = ((uint64_t)cid << 32) | (uint64_t)(pos - (uint32_t)q_off);
-                }
-            }
-            uint64_t vm = __ballot(valid);
-            if (!vm) continue;
-            int found = -1;
-            for (int c = 0; c < n_cand; c++) {
-                uint64_t ck = S.cand_key[c];
-                if (valid && ck == vk) found = c;
-            }
-            bool isnew = valid && found < 0;
-            uint64_t nwm = __ballot(isnew);
-            int n_new = __popcll(nwm);
-            if (n_cand + n_new <= TOP) {
-                // keys of one seed are distinct: counts and appends never collide
-                if (valid && found >= 0) S.cand_n[found] += 1;
-                if (isnew) {
-                    int r = __popcll(nwm & lt);
-                    S.cand_key[n_cand + r] = vk;
-                    S.cand_n[n_cand + r] = 1;
-                }
-                n_cand += n_new;
-                wave_sync();
-            } else {
-                // table would overflow: postings one at a time, in order
-                uint64_t m = vm;
-                while (m) {
-                    int j = __ffsll((long long)m) - 1;
-                    m &= m - 1;
-                    uint64_t key = __shfl(vk, j, 64);
-                    int f = -1;
-                    for (int base = 0; base < n_cand; base += 64) {
-                        int c = base + lane;
-                        bool hit = c < n_cand && S.cand_key[c] == key;
-                        uint64_t hm = __ballot(hit);
-                        if (hm) {
-                            f = base + __ffsll((long long)hm) - 1;
-                            break;
-                        }
-                    }
-                    if (f >= 0) {
-                        if (lane == 0) S.cand_n[f] += 1;
-                    } else if (n_cand < TOP) {
-                        if (lane == 0) {
-                            S.cand_key[n_cand] = key;
-                            S.cand_n[n_cand] = 1;
-                        }
-                        n_cand++;
-                    } else {
-                        uint64_t best = ~0ull;
-                        for (int base = 0; base < TOP; base += 64) {
-                            int c = base + lane;
-                            uint64_t v = ((uint64_t)(uint32_t)S.cand_n[c] << 32) | (uint64_t)c;
-                            best = v < best ? v : best;
-                        }
-                        best = wave_min_u64(best);
-                        int min_n = (int)(best >> 32);
-                        int min_i = (int)(best & 0xFFFFFFFFull);
-                        if (min_n <= 1 && lane == 0) {
-                            S.cand_key[min_i] = key;
-                            S.cand_n[min_i] = 1;
-                        }
-                    }
-                    wave_sync();
-                }
-            }
-        }
-        if (n_cand > 0 && oi + 1 < n_ord) {
-            int b1 = 0, b2 = 0;
-            for (int c = lane; c < n_cand; c += 64) top2_push(S.cand_n[c], b1, b2);
-            wave_top2(b1, b2);
-            if (b1 >= 4 && b1 >= b2 + 2) break;
-        }
-    }
-
-    // KEEP: stable top-16 by votes
-    int n_pick = n_cand < KEEP ? n_cand : KEEP;
-    for (int base = 0; base < n_cand; base += 64) {
-        int c = base + lane;
-        if (c < n_cand) {
-            int v = S.cand_n[c];
-            int rank = 0;
-            for (int c2 = 0; c2 < n_cand; c2++) {
-                int v2 = S.cand_n[c2];
-                rank += (v2 > v || (v2 == v && c2 < c)) ? 1 : 0;
-            }
-            if (rank < KEEP) {
-                S.pick_i[rank] = c;
-                S.pick_n[rank] = v;
-            }
-        }
-    }
-    wave_sync();
-
-    const Caps cp = make_caps(kn, L);
-    const int NA = 2 * cp.shift_cap + 1;
-    const int NE = n_pick * NA;
-    uint64_t best = ~0ull;  // cost << 48 | order << 16 | sl << 8 | sr
-    int e_begin = 0, width = 1;
-    while (e_begin < NE) {
-        int ev = e_begin + lane;
-        uint64_t cand = ~0ull;
-        if (lane < width && ev < NE) {
-            int pk = ev / NA, ai = ev - pk * NA;
-            int mag = (ai + 1) >> 1;
-            int adj = ai == 0 ? 0 : ((ai & 1) ? -mag : mag);
-            uint64_t key = S.cand_key[S.pick_i[pk]];
-            int64_t bcid = (int64_t)(key >> 32);
-            int64_t bstart = (int64_t)(key & 0xFFFFFFFFull) + adj;
-            if (bcid < ix.n_contigs) {
-                int64_t off0 = (int64_t)ix.contig_off[bcid];
-                int64_t clen = (int64_t)ix.contig_off[bcid + 1] - off0;
-                if (bstart >= 0 && bstart + L <= clen) {
-                    uint64_t mm[MAXW];
-                    int tot = window_mm(ix, off0 + bstart, S, W, L, mm);
-                    int probe_nm = 0;
-#pragma unroll
-                    for (int p = 1; p <= 6; p++) {
-                        int jp = (p * (L - 1)) / 7;
-                        if (jp < cp.clip_cap) jp = cp.clip_cap;
-                        if (jp > L - 1 - cp.clip_cap) jp = L - 1 - cp.clip_cap;
-                        if (jp >= 0 && jp < L && mm_bit(mm, jp)) probe_nm++;
-                    }
-                    if (probe_nm <= cp.budget) {
-                        int nm = tot > cp.budget ? cp.budget + 1 : tot;
-                        int sl = 0, sr = 0;
-                        if (nm > cp.budget) nm = softclip(mm, W, L, cp, sl, sr);
-                        if (nm <= cp.budget && L - sl - sr >= 24)
-                            cand = ((uint64_t)nm << 48) | ((uint64_t)ev << 16) | ((uint64_t)sl << 8) |
-                                   (uint64_t)sr;
-                    }
-                }
-            }
-        }
-        uint64_t mn = wave_min_u64(cand);
-        if (mn < best) best = mn;
-        if (best != ~0ull && (best >> 48) == 0) break;
-        e_begin += width;
-        width = 64;
-    }
-    if (best != ~0ull) {
-        int cost = (int)(best >> 48);
-        int ev = (int)((best >> 16) & 0xFFFF);
-        int sl = (int)((best >> 8) & 0xFF), sr = (int)(best & 0xFF);
-        int pk = ev / NA, ai = ev - pk * NA;
-        int mag = (ai + 1) >> 1;
-        int adj = ai == 0 ? 0 : ((ai & 1) ? -mag : mag);
-        uint64_t key = S.cand_key[S.pick_i[pk]];
-        int64_t pos = (int64_t)(key & 0xFFFFFFFFull) + adj + sl;
-        return finish_rec((int64_t)(key >> 32), pos, S.pick_n[pk], cost, sl, sr, strand_flag);
-    }
-
-    // low-occ rescue: every posting of every seed with occ <= vote_cap, first
-    // minimum cost wins, an exact hit ends the scan
-    int best_cost = 1 << 30;
-    int64_t b_cid = -1, b_pos = 0;
-    int b_sl = 0, b_sr = 0, b_votes = 0;
-    for (int oi = 0; oi < n_ord; oi++) {
-        uint64_t e = S.ord[oi];
-        int occ = (int)(e >> 46);
-        if (occ > vote_cap) break;
-        int q_off = (int)((e >> 40) & 63) * kn.stride;
-        int64_t start = (int64_t)(e & START_MASK);
-        bool stop = false;
-        for (int c0 = 0; c0 < occ; c0 += 64) {
-            uint64_t cand = ~0ull;
-            int64_t my_cid = 0, my_pos = 0;
-            if (c0 + lane < occ) {
-                uint64_t p = load_posting(ix, start + c0 + lane);
-                uint32_t cid = (uint32_t)p, pos = (uint32_t)(p >> 32);
-                if ((int64_t)cid < ix.n_contigs && (int64_t)pos >= q_off) {
-                    int64_t bstart = (int64_t)pos - q_off;
-                    int64_t off0 = (int64_t)ix.contig_off[cid];
-                    int64_t clen = (int64_t)ix.contig_off[cid + 1] - off0;
-                    if (bstart + L <= clen) {
-                        uint64_t mm[MAXW];
-                        int tot = window_mm(ix, off0 + bstart, S, W, L, mm);
-                        int cost = tot > cp.budget ? cp.budget + 1 : tot;
-                        int sl = 0, sr = 0;
-                        int64_t pos_out = bstart;
-                        if (cost > cp.budget && cp.clip_cap > 0) {
-                            cost = softclip(mm, W, L, cp, sl, sr);
-                            pos_out = bstart + sl;
-                            if (L - sl - sr < 32) cost = 9999;
-                        }
-                        if (cost <= cp.budget) {
-                            cand = ((uint64_t)cost << 48) | ((uint64_t)lane << 16) |
-                                   ((uint64_t)sl << 8) | (uint64_t)sr;
-                            my_cid = cid;
-                            my_pos = pos_out;
-                        }
-                    }
-                }
-            }
-            uint64_t mn = wave_min_u64(cand);
-            if (mn != ~0ull) {
-                int cost = (int)(mn >> 48);
-                int wl = (int)((mn >> 16) & 0xFFFF);
-                int64_t wc = __shfl(my_cid, wl, 64);
-                int64_t wp = __shfl(my_pos, wl, 64);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    b_cid = wc;
-                    b_pos = wp;
-                    b_sl = (int)((mn >> 8) & 0xFF);
-                    b_sr = (int)(mn & 0xFF);
-                    b_votes = occ <= 64 ? 5 : 4;
-                }
-                if (best_cost == 0) {
-                    stop = true;
-                    break;
-                }
-            }
-        }
-        if (stop) break;
-    }
-    if (b_cid >= 0) return finish_rec(b_cid, b_pos, b_votes, best_cost, b_sl, b_sr, strand_flag);
-    return unmapped_rec();
-}
-
-// ----------------------------------------------------------------- pass 1
-// High-occ exact tally for a read left unmapped by pass 0.
-__device__ Rec pass1(const IndexView &ix, const Knobs &kn, WaveState &S, int n_ord, int W, int L,
-                     int vote_cap_in, int32_t strand_flag) {
-    const int lane = lane_id();
-    const uint64_t lt = lanemask_lt(lane);
-    const int vote_cap = vote_cap_in < 1 ? 256 : vote_cap_in;
-    int n_hx = 0, tried = 0;
-    for (int oi = 0; oi < n_ord; oi++) {
-        uint64_t e = S.ord[oi];
-        int occ = (int)(e >> 46);
-        if (occ <= vote_cap) continue;
-        tried++;
-        if (tried > MAX_HIGH_TRIED) break;
-        int q_off = (int)((e >> 40) & 63) * kn.stride;
-        int64_t start = (int64_t)(e & START_MASK);
-        for (int c0 = 0; c0 < occ; c0 += 64) {
-            bool exact = false;
-            uint64_t vk = 0;
-            if (c0 + lane < occ) {
-                uint64_t p = load_posting(ix, start + c0 + lane);
-                uint32_t cid = (uint32_t)p, pos = (uint32_t)(p >> 32);
-                if ((int64_t)cid < ix.n_contigs && (int64_t)pos >= q_off) {
-                    int64_t bs = (int64_t)pos - q_off;
-                    int64_t o0 = (int64_t)ix.contig_off[cid];
-                    int64_t clen = (int64_t)ix.contig_off[cid + 1] - o0;
-                    if (bs + L <= clen) {
-                        uint64_t mm[MAXW];
-                        if (window_mm(ix, o0 + bs, S, W, L, mm) == 0) {
-                            exact = true;
-                            vk = ((uint64_t)cid << 32) | (uint64_t)bs;
-                        }
-                    }
-                }
-            }
-            uint64_t xm = __ballot(exact);
-            if (!xm) continue;
-            int found = -1;
-            for (int h = 0; h < n_hx; h++) {
-                uint64_t hk = S.hx_key[h];
-                if (exact && hk == vk) found = h;
-            }
-            bool isnew = exact && found < 0;
-            uint64_t nwm = __ballot(isnew);
-            if (exact && found >= 0) {
-                S.hx_n[found] += 1;
-                if (occ < S.hx_min[found]) S.hx_min[found] = occ;
-            }
-            if (isnew) {
-                int r = n_hx + __popcll(nwm & lt);
-                if (r < HX) {
-                    S.hx_key[r] = vk;
-                    S.hx_n[r] = 1;
-                    S.hx_min[r] = occ;
-                }
-            }
-            n_hx += __popcll(nwm);
-            if (n_hx > HX) n_hx = HX;
-            wave_sync();
-        }
-        if (n_hx > 0) {
-            int b1 = 0, b2 = 0;
-            for (int h = lane; h < n_hx; h += 64) top2_push(S.hx_n[h], b1, b2);
-            wave_top2(b1, b2);
-            if (b1 >= 3 && b1 > b2) break;
-        }
-    }
-    if (n_hx == 0) return unmapped_rec();
-    int best_hi = 0, second_hi = 0, best_i = -1, best_mocc = 0x7FFFFFFF;
-    for (int h = 0; h < n_hx; h++) {
-        int vn = S.hx_n[h], mo = S.hx_min[h];
-        if (vn > best_hi) {
-            second_hi = best_hi;
-            best_hi = vn;
-            best_mocc = mo;
-            best_i = h;
-        } else if (vn == best_hi && mo < best_mocc) {
-            best_mocc = mo;
-            best_i = h;
-        } else if (vn > second_hi) {
-            second_hi = vn;
-        }
-    }
-    if (best_i >= 0 && (best_hi >= 2 || (best_hi >= 1 && second_hi == 0))) {
-        uint64_t bk = S.hx_key[best_i];
-        int votes = best_hi > second_hi ? (best_hi > 5 ? 5 : best_hi) : 1;
-        return finish_rec((int64_t)(bk >> 32), (int64_t)(bk & 0xFFFFFFFFull), votes, 0, 0, 0,
-                          strand_flag);
-    }
-    return unmapped_rec();
-}
-
-__device__ Rec map_strand(const IndexView &ix, const Knobs &kn, WaveState &S, int W, int L,
-                          int32_t strand_flag) {
-    int n_ord = seed_and_order(ix, kn, S, L);
-    Rec r = pass0(ix, kn, S, n_ord, W, L, kn.vote_fast, strand_flag);
-    if (r.cid < 0 && kn.vote_fast < kn.vote_occ)
-        r = pass0(ix, kn, S, n_ord, W, L, kn.vote_occ, strand_flag);
-    if (r.cid < 0) r = pass1(ix, kn, S, n_ord, W, L, kn.vote_occ, strand_flag);
-    return r;
-}
-
-__device__ inline uint8_t ascii_code(uint8_t b) {
+DEV uint8_t ascii_code(uint8_t b) {
     switch (b) {
     case 'A': case 'a': return 0;
     case 'C': case 'c': return 1;
@@ -687,72 +199,7 @@ __device__ inline uint8_t ascii_code(uint8_t b) {
     }
 }
 
-// Lane t < W builds read word t (+ N word), applying the in-silico conversion.
-__device__ void load_read(const ReadsIn &in, int64_t r, int L, int W, WaveState &S) {
-    int lane = lane_id();
-    int mode = r < in.n_r1 ? in.bs_r1 : in.bs_r2;
-    const uint8_t *src = in.bases + r * in.stride;
-    uint64_t w = 0, nf = 0;
-    if (lane < W) {
-        if (in.encoding == MA_ENC_ASCII) {
-            for (int j = 0; j < 32; j++) {
-                int i = 32 * lane + j;
-                if (i >= L) break;
-                uint8_t c = ascii_code(src[i]);
-                if (c == 1 && mode == MA_BS_C2T) c = 3;
-                if (c == 2 && mode == MA_BS_G2A) c = 0;
-                if (c > 3) nf |= 1ull << (62 - 2 * j);
-                else w |= (uint64_t)c << (62 - 2 * j);
-            }
-        } else {
-            int nbytes = (L + 3) >> 2;
-            for (int j = 0; j < 8; j++) {
-                int bi = 8 * lane + j;
-                if (bi < nbytes) w |= (uint64_t)src[bi] << (56 - 8 * j);
-            }
-            if (mode == MA_BS_C2T) w |= (w & LOW2) << 1;
-            if (mode == MA_BS_G2A) w &= ~((~w & LOW2) << 1);
-            w &= groups_mask2(L - 32 * lane);
-        }
-    }
-    if (lane < MAXW + 2) {
-        S.fw2[lane] = 0;
-        S.fwN[lane] = 0;
-    }
-    wave_sync();
-    if (lane < W) {
-        S.fw2[lane + 1] = w;
-        S.fwN[lane + 1] = nf;
-    }
-    if (lane < MAXW + 1) {
-        S.rd2[lane] = lane < W ? w : 0;
-        S.rdN[lane] = lane < W ? nf : 0;
-    }
-    wave_sync();
-}
-
-// Reverse complement of the FW read into rd2/rdN (N stays N, codes c -> 3-c).
-__device__ void make_rc(WaveState &S, int L, int W) {
-    int lane = lane_id();
-    uint64_t w = 0, nf = 0;
-    if (lane < W) {
-        int p = L - 32 * (lane + 1) + 32;  // position in the padded array
-        uint64_t mask2 = groups_mask2(L - 32 * lane);
-        w = ~reverse_groups(lds_window(S.fw2, p)) & mask2;
-        nf = reverse_groups(lds_window(S.fwN, p)) & first_groups_mask(L - 32 * lane);
-        // an N base carries code 0 in fw2; its complement bits are irrelevant
-        // (the N flag makes it a mismatch) but keep them zero like FW
-        w &= ~(nf | (nf << 1));
-    }
-    wave_sync();
-    if (lane < MAXW + 1) {
-        S.rd2[lane] = lane < W ? w : 0;
-        S.rdN[lane] = lane < W ? nf : 0;
-    }
-    wave_sync();
-}
-
-__device__ inline void store_rec(const RecOut &o, int64_t r, const Rec &x) {
+DEV void store_rec(const RecOut &o, int64_t r, const Rec &x) {
     if (!o.cid) return;
     o.cid[r] = x.cid;
     o.pos[r] = x.pos;
@@ -766,46 +213,36 @@ __device__ inline void store_rec(const RecOut &o, int64_t r, const Rec &x) {
     if (o.ilen) o.ilen[r] = 0;
 }
 
-__global__ void __launch_bounds__(BLOCK) map_reads_kernel(IndexView ix, Knobs kn, ReadsIn in,
-                                                           RecOut merged, RecOut fw_out,
-                                                           RecOut rc_out,
-                                                           unsigned int *__restrict__ counter) {
-    __shared__ WaveState states[WPB];
-    WaveState &S = states[threadIdx.x >> 6];
-    const int lane = lane_id();
-    while (true) {
-        unsigned int rr = 0;
-        if (lane == 0) rr = atomicAdd(counter, 1u);
-        int64_t r = (int64_t)__shfl((int)rr, 0, 64);
-        r = (int64_t)(uint32_t)r;
-        if (r >= in.n) break;
-        int L = (int)in.lens[r];
-        if (L > in.max_len) L = 0;  // malformed row: reported unmapped, never read past its row
-        Rec fw = unmapped_rec(), rc = unmapped_rec();
-        if (L > 0 && L <= MAX_READ) {
-            int W = (L + 31) >> 5;
-            load_read(in, r, L, W, S);
-            fw = map_strand(ix, kn, S, W, L, 0);
-            bool need_rc = kn.do_rc && (fw.cid < 0 || fw.nm > 0);
-            if (need_rc) {
-                make_rc(S, L, W);
-                rc = map_strand(ix, kn, S, W, L, 16);
-            }
-        }
-        bool use_rc = false;
-        if (fw.cid < 0 && rc.cid >= 0) use_rc = true;
-        else if (fw.cid >= 0 && rc.cid >= 0 &&
-                 (rc.nm < fw.nm || (rc.nm == fw.nm && rc.mapq > fw.mapq)))
-            use_rc = true;
-        Rec m = use_rc ? rc : fw;
-        if (m.cid < 0 || L - (int)m.sl - (int)m.sr < 1) m = unmapped_rec();
-        if (lane == 0) {
-            store_rec(merged, r, m);
-            store_rec(fw_out, r, fw);
-            store_rec(rc_out, r, rc);
-        }
-    }
-}
+}  // namespace
+
+namespace w4 {
+namespace {
+constexpr int MAXW = 4;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 4;
+#include "map_body.inc"
+}  // namespace
+}  // namespace w4
+
+namespace w5 {
+namespace {
+constexpr int MAXW = 5;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 4;
+#include "map_body.inc"
+}  // namespace
+}  // namespace w5
+
+namespace w8 {
+namespace {
+constexpr int MAXW = 8;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 3;
+#include "map_body.inc"
+}  // namespace
+}  // namespace w8
+
+namespace {
 
 __global__ void zero_u32_kernel(unsigned int *p) { *p = 0u; }
 
@@ -860,7 +297,7 @@ int check_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
     MA_REQUIRE(p->max_indel == 0,
                "map: max_indel must be 0 (gapless extension; indel branches are not built)");
     MA_REQUIRE(p->max_occ >= 1 && p->max_occ < (1 << 17), "map: max_occ must be in 1..131071");
-    MA_REQUIRE(max_len >= 0 && max_len <= MAX_READ, "map: reads longer than 256 bases are not supported");
+    MA_REQUIRE(max_len >= 0 && max_len <= 256, "map: reads longer than 256 bases are not supported");
     int32_t ml = max_len < p->k ? p->k : max_len;
     MA_REQUIRE((ml - p->k) / p->stride + 1 <= MAXS, "map: more than 64 seed slots per read");
     return MA_OK;
@@ -922,8 +359,16 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     int64_t blocks = (n + WPB - 1) / WPB;
     int64_t cap = (int64_t)(n_cu > 0 ? n_cu : 256) * 8;
     if (blocks > cap) blocks = cap;
-    map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(index_view(ix), kn, in, mo, rec_out(d_fw),
-                                                          rec_out(d_rc), ws->counter.as<unsigned int>());
+    const IndexView v = index_view(ix);
+    const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
+    unsigned int *ctr = ws->counter.as<unsigned int>();
+    // the read-word class of the batch's longest read picks the kernel
+    if (max_len <= 4 * 32)
+        w4::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
+    else if (max_len <= 5 * 32)
+        w5::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
+    else
+        w8::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
     MA_LAUNCHED();
     return MA_OK;
 }

@@ -243,3 +243,14 @@ def test_index_open_dense_v1_dir(w1, tmp_path):
     got = mialign.map_batch(g, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
     want, _, _, _ = w1.oracle_map(reads)
     assert_recs_equal(got, want, "index opened from dense-v1 files")
+
+
+def test_map_short_read_class(w1):
+    say("test_map_short_read_class: 40-128 bp (4-word kernel)")
+    reads = synth.gen_reads(16, w1.ref, 6000, 40, 128, max_len=128)
+    got, gfw, grc = mialign.map_batch(w1.gix, reads.bases, reads.lens, 128, bs_r1=MA_BS_C2T,
+                                      strands=True)
+    want, wfw, wrc, _ = w1.oracle_map(reads)
+    assert_recs_equal(gfw, wfw, "FW records (short)")
+    assert_recs_equal(grc, wrc, "RC records (short)")
+    assert_recs_equal(got, want, "merged records (short)")
