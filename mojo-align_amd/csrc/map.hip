@@ -42,6 +42,7 @@ constexpr int BLOCK = WPB * 64;
 constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
 constexpr int PBUF = 256;  // postings prefetched into LDS per wave
+constexpr int CLAIM = 16;  // reads claimed per work-counter atomic
 
 struct Knobs {
     int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc;
@@ -114,6 +115,20 @@ DEV uint64_t wave_min_u64(uint64_t v) {
         v = w < v ? w : v;
     }
     return v;
+}
+
+DEV uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+DEV uint64_t readlane64(uint64_t v, int lane) {
+    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), lane) << 32) |
+           (uint64_t)__builtin_amdgcn_readlane((uint32_t)v, lane);
 }
 
 // Multiset top-2 (b1 >= b2), one value at a time, with selects only.
