@@ -49,6 +49,51 @@ def algorithmic_bytes_per_read(cnt, n_sample: int, read_len: int) -> float:
     return per
 
 
+def serve_section(ix, packed, lens, L, n_req: int, slots: int) -> dict:
+    """Request loop: n_req batches (sizes lognormal, median 50k, clipped to
+    [1k, 2M] reads) drawn from the resident workload's reads, Poisson arrivals
+    offered at 80% of the plain loop's throughput. Plain = the same batches,
+    one per launch, one after another."""
+    from mialign import serve
+    from mialign._lib import MA_BS_C2T, MA_ENC_2BIT
+
+    rng = np.random.default_rng(1003)
+    n_pool = lens.size
+    sizes = np.clip(np.exp(rng.normal(np.log(50_000), 1.0, n_req)), 1_000, min(2_000_000, n_pool)).astype(int)
+    reqs = []
+    for sz in sizes:
+        a = int(rng.integers(0, n_pool - sz + 1))
+        reqs.append(serve.Request(serve.pinned_copy(packed[a : a + sz]), serve.pinned_copy(lens[a : a + sz]),
+                                  L, MA_ENC_2BIT, bs_r1=MA_BS_C2T, out=serve.pinned_recs(int(sz))))
+    budget = 8 << 30
+    serve.run_plain(ix, reqs[:4], hbm_budget=budget)  # warm the allocator pool
+    plain = serve.run_plain(ix, reqs, hbm_budget=budget)
+    rate = 0.8 * plain.reads_per_s / float(sizes.mean())  # requests per second
+    t = 0.0
+    for r in reqs:
+        t += float(rng.exponential(1.0 / rate))
+        r.arrival = t
+    srv = serve.Server(ix, slots=slots, hbm_budget=budget)
+    served = serve.run_request_loop(srv, reqs)
+    srv.close()
+    return {
+        "requests": n_req,
+        "reads": int(sizes.sum()),
+        "batch_reads_median": int(np.median(sizes)),
+        "offered_load": "80% of plain throughput, Poisson arrivals",
+        "served_reads_per_s": round(served.reads_per_s, 1),
+        "served_p50_ms": round(served.pct(50), 3),
+        "served_p99_ms": round(served.pct(99), 3),
+        "plain_reads_per_s": round(plain.reads_per_s, 1),
+        "plain_p50_ms": round(plain.pct(50), 3),
+        "plain_p99_ms": round(plain.pct(99), 3),
+        "p99_ratio": round(served.pct(99) / plain.pct(99), 3) if plain.pct(99) else None,
+        "throughput_ratio": round(served.reads_per_s / plain.reads_per_s, 3),
+        "slots": slots,
+        "hbm_budget_bytes": budget,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +105,12 @@ def main() -> int:
     ap.add_argument("--read-len", type=int, default=150)
     ap.add_argument("--cpu-sample", type=int, default=5_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-only", action="store_true", help="skip the timed batch steps")
+    ap.add_argument("--launch", choices=("persistent", "batch"), default="persistent",
+                    help="map-kernel grid: persistent (8 blocks/CU + work counter) or sized to the batch")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -97,7 +148,7 @@ def main() -> int:
     tid = synth.tid_of_cid(dual.names)
     d_tid = mialign.DeviceArray.from_host(tid, dev)
     recs = mialign.DeviceRecs(n, dev)
-    ws = mialign.Workspace(ix, n, L)
+    ws = mialign.Workspace(ix, n, L, persistent=args.launch == "persistent")
     prm = mialign.default_params(k=21)
     keys = {f: mialign.DeviceArray((n,), dt, dev) for f, dt in
             (("coord", np.uint64), ("dhi", np.uint64), ("dlo", np.uint64), ("score", np.uint32),
@@ -124,6 +175,8 @@ def main() -> int:
                                         outs["is_dup"].ptr, outs["perm"].ptr, 1, outs["dup_perm"].ptr,
                                         scratch.ptr, scratch_bytes), "sort markdup")
 
+    if args.serve_only:
+        args.warmup, args.steps = 0, 1
     for i in range(args.warmup):
         step(False)
         stream.sync()
@@ -146,6 +199,12 @@ def main() -> int:
     merged = recs.download()
     is_dup = outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
+
+    # ---- served request loop vs plain one-batch-per-launch (same batches)
+    serving = None
+    if not args.no_serve:
+        serving = serve_section(ix, packed, reads.lens, L, args.serve_requests, args.serve_slots)
+        log(f"serving: {serving}")
 
     # ---- CPU baseline (oracle) on a bounded sample + sample parity + bytes/read
     cpu = None
@@ -208,6 +267,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity_sample": parity_sample,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     ix.close()

@@ -132,6 +132,11 @@ int ma_index_close(ma_index *ix);
 /* Workspace for up to max_reads reads of at most max_len bases. */
 int ma_workspace_create(ma_index *ix, int64_t max_reads, int32_t max_len, ma_workspace **out);
 int ma_workspace_destroy(ma_workspace *ws);
+/* Launch policy of the map kernel for this workspace: 1 (default) = a
+ * persistent grid (8 blocks per CU) taking read chunks from a counter; 0 = a
+ * grid sized to the batch, one chunk per wave, so concurrent launches on other
+ * streams interleave (what the served loop uses). Results are identical. */
+int ma_workspace_set_persistent(ma_workspace *ws, int32_t persistent);
 /* Bytes a workspace of this shape holds on the device (for admission). */
 int64_t ma_workspace_bytes(int64_t max_reads, int32_t max_len);
 
@@ -179,18 +184,26 @@ int ma_seed_kmers(int device, const uint8_t *bases, int64_t n, int32_t max_len, 
                   uint64_t *out_hash);
 
 /* ---- served request loop ---- */
-/* A server owns `slots` concurrent in-flight batches on their own streams and
- * admits requests while the free-HBM budget (bytes) allows. */
+/* A server owns `slots` streams and an HBM budget (bytes). Requests queue on
+ * submit; whenever a slot is free, the compatible requests waiting at the head
+ * of the queue (same row layout, conversion and knobs, single-end) are merged
+ * into one launch while the merged bytes fit the free budget. */
 int ma_serve_open(ma_index *ix, int32_t slots, int64_t hbm_budget, int32_t max_len,
                   ma_server **out);
-/* Submit host reads (same layout as ma_map_batch); returns a ticket >= 0, or
- * MA_ERR_NOMEM when the request cannot be admitted now (retry after poll). */
+/* Queue host reads (same layout as ma_map_batch); the host arrays must stay
+ * valid until the ticket completes (pinned memory lets the copies overlap).
+ * Returns a ticket >= 0; a request larger than the whole budget is an error. */
 int64_t ma_serve_submit(ma_server *s, const uint8_t *bases, int64_t stride, const uint32_t *lens,
                         int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
                         int64_t n_r1, const ma_params *p, ma_recs *merged_out);
-/* Poll: returns 1 and fills ms when the ticket is complete (outputs written),
- * 0 while pending. wait != 0 blocks until complete. */
+/* Returns 1 when the ticket is complete (records written to its arrays), 0
+ * while it is queued or running; wait != 0 blocks until complete. latency_ms
+ * is reserved (written 0; callers time requests themselves). */
 int ma_serve_poll(ma_server *s, int64_t ticket, int32_t wait, double *latency_ms);
+/* Progress the server and report finished tickets: writes up to `cap` of them
+ * to `tickets` and returns how many (each is reported once, by this call or by
+ * ma_serve_poll). wait != 0 blocks until one finishes or nothing is in flight. */
+int ma_serve_reap(ma_server *s, int64_t *tickets, int32_t cap, int32_t wait);
 int ma_serve_close(ma_server *s);
 
 #ifdef __cplusplus

@@ -225,6 +225,9 @@ int index_alloc(int device, ma_index **out) {
     MA_HIP(hipSetDevice(device));
     ma_index *ix = new ma_index();
     ix->device = device;
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) n_cu = 0;
+    ix->n_cu = n_cu > 0 ? n_cu : 256;
     hipError_t e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete ix;

@@ -13,6 +13,7 @@ struct ma_index {
     int64_t n_contigs = 0;
     int32_t ref_has_n = 0;
     int32_t max_probe = 1;
+    int32_t n_cu = 0;  // compute units of the device (launch sizing)
     uint64_t n_buckets = 0;
     ma::DevBuf kmers;     // u64[n_keys]       (dense-v1)
     ma::DevBuf offsets;   // u64[n_keys + 1]   (dense-v1)
@@ -28,6 +29,7 @@ struct ma_workspace {
     ma_index *ix = nullptr;
     int64_t max_reads = 0;  // 0 = no limit
     int32_t max_len = 0;
+    int32_t persistent = 1;  // 1: persistent grid + work counter; 0: grid sized to the batch
     ma::DevBuf counter;
 };
 

@@ -42,7 +42,8 @@ constexpr int BLOCK = WPB * 64;
 constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
 constexpr int PBUF = 256;  // postings prefetched into LDS per wave
-constexpr int CLAIM = 16;  // reads claimed per work-counter atomic
+constexpr int MAX_CLAIM = 16;  // most reads claimed per work-counter atomic
+constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kernels' register budget
 
 struct Knobs {
     int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc;
@@ -344,6 +345,12 @@ int ma_workspace_create(ma_index *ix, int64_t max_reads, int32_t max_len, ma_wor
     return MA_OK;
 }
 
+int ma_workspace_set_persistent(ma_workspace *ws, int32_t persistent) {
+    MA_REQUIRE(ws, "ma_workspace_set_persistent: null workspace");
+    ws->persistent = persistent ? 1 : 0;
+    return MA_OK;
+}
+
 int ma_workspace_destroy(ma_workspace *ws) {
     delete ws;
     return MA_OK;
@@ -366,24 +373,33 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     hipStream_t st = (hipStream_t)stream;
     Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc};
     ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
-    zero_u32_kernel<<<1, 1, 0, st>>>(ws->counter.as<unsigned int>());
-    MA_LAUNCHED();
-    int dev = ix->device;
-    int n_cu = 0;
-    MA_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-    int64_t blocks = (n + WPB - 1) / WPB;
-    int64_t cap = (int64_t)(n_cu > 0 ? n_cu : 256) * 8;
-    if (blocks > cap) blocks = cap;
+    // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
+    // the resident waves all get work. Persistent mode: at most 8 blocks per
+    // CU loop over chunks taken from a counter. Batch-sized mode: one chunk per
+    // wave, blocks retire as they finish (concurrent launches interleave).
+    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU;
+    int64_t claim = n / waves;
+    if (claim < 1) claim = 1;
+    if (claim > MAX_CLAIM) claim = MAX_CLAIM;
+    int64_t blocks = (n + claim * WPB - 1) / (claim * WPB);
+    unsigned int *ctr = nullptr;
+    if (ws->persistent) {
+        int64_t cap = (int64_t)ix->n_cu * 8;
+        if (blocks > cap) blocks = cap;
+        ctr = ws->counter.as<unsigned int>();
+        zero_u32_kernel<<<1, 1, 0, st>>>(ctr);
+        MA_LAUNCHED();
+    }
+    MA_REQUIRE(blocks < (1ll << 31), "ma_map_batch_dev: batch too large for one launch");
     const IndexView v = index_view(ix);
     const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
-    unsigned int *ctr = ws->counter.as<unsigned int>();
     // the read-word class of the batch's longest read picks the kernel
     if (max_len <= 4 * 32)
-        w4::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
+        w4::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
     else if (max_len <= 5 * 32)
-        w5::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
+        w5::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
     else
-        w8::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr);
+        w8::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
     MA_LAUNCHED();
     return MA_OK;
 }

@@ -110,6 +110,7 @@ SIGNATURES = {
     "ma_index_close": (C.c_int, [P]),
     "ma_workspace_create": (C.c_int, [P, I64, I32, C.POINTER(P)]),
     "ma_workspace_destroy": (C.c_int, [P]),
+    "ma_workspace_set_persistent": (C.c_int, [P, I32]),
     "ma_workspace_bytes": (I64, [I64, I32]),
     "ma_map_batch": (C.c_int, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                                C.POINTER(Recs), C.POINTER(Recs), C.POINTER(Recs)]),
@@ -126,6 +127,7 @@ SIGNATURES = {
     "ma_serve_submit": (I64, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                               C.POINTER(Recs)]),
     "ma_serve_poll": (C.c_int, [P, I64, I32, C.POINTER(C.c_double)]),
+    "ma_serve_reap": (C.c_int, [P, C.POINTER(I64), I32, I32]),
     "ma_serve_close": (C.c_int, [P]),
 }
 

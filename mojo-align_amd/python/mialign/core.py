@@ -279,11 +279,13 @@ class DeviceRecs:
 
 
 class Workspace:
-    def __init__(self, index: GpuIndex, max_reads: int, max_len: int):
+    def __init__(self, index: GpuIndex, max_reads: int, max_len: int, persistent: bool = True):
         h = C.c_void_p()
         check(lib().ma_workspace_create(index.handle, max_reads, max_len, C.byref(h)),
               "ma_workspace_create")
         self.handle = h.value
+        check(lib().ma_workspace_set_persistent(self.handle, int(persistent)),
+              "ma_workspace_set_persistent")
 
     def __del__(self):
         try:

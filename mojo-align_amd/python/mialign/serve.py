@@ -1,0 +1,203 @@
+"""Served request loop over a resident index (continuous batching).
+
+The C side (csrc/serve.hip) owns `slots` streams and a free-HBM budget;
+`Server.submit` admits a read batch when a slot and the budget allow and
+enqueues its copy-in, map kernel and copy-out on that slot's stream, so
+requests arriving while others run overlap with them on the device.
+`run_request_loop` drives Poisson-like arrivals through it and reports
+throughput and p50/p99 batch latency; `run_plain` is the baseline of one
+batch per launch, strictly one after another.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+import weakref
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from ._lib import MA_ERR_NOMEM, Params, check, default_params, host_recs, lib, ptr, recs_struct
+from .core import GpuIndex
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """numpy array in page-locked host memory (freed with the array), so
+    copies to and from the device run asynchronously."""
+    dtype = np.dtype(dtype)
+    nbytes = max(int(np.prod(shape, dtype=np.int64)) * dtype.itemsize, 1)
+    p = C.c_void_p()
+    check(lib().ma_host_alloc(nbytes, C.byref(p)), "ma_host_alloc")
+    buf = (C.c_uint8 * nbytes).from_address(p.value)
+    weakref.finalize(buf, lib().ma_host_free, p.value)
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape, dtype=np.int64))).reshape(shape)
+
+
+def pinned_copy(a: np.ndarray) -> np.ndarray:
+    out = pinned_empty(a.shape, a.dtype)
+    out[...] = a
+    return out
+
+
+def pinned_recs(n: int) -> Dict[str, np.ndarray]:
+    """Merged-record arrays as consecutive slices of one pinned block (one
+    device-to-host copy per request)."""
+    proto = host_recs(0)
+    block = pinned_empty((len(proto) * max(n, 1) * 4,), np.uint8)
+    return {f: block[i * n * 4 : (i + 1) * n * 4].view(a.dtype) for i, (f, a) in enumerate(proto.items())}
+
+
+@dataclass
+class Request:
+    bases: np.ndarray  # [n, stride] uint8
+    lens: np.ndarray  # [n] uint32
+    max_len: int
+    encoding: int
+    bs_r1: int = 0
+    bs_r2: int = 0
+    n_r1: Optional[int] = None
+    arrival: float = 0.0  # seconds from the start of the run
+    out: Optional[Dict[str, np.ndarray]] = None  # preallocated (pinned) merged records
+
+    @property
+    def n(self) -> int:
+        return int(self.lens.size)
+
+
+@dataclass
+class _Flight:
+    req: Request
+    out: Dict[str, np.ndarray]
+    struct: object
+    t_submit: float
+
+
+class Server:
+    def __init__(self, index: GpuIndex, slots: int = 4, hbm_budget: int = 8 << 30, max_len: int = 256):
+        h = C.c_void_p()
+        check(lib().ma_serve_open(index.handle, slots, int(hbm_budget), max_len, C.byref(h)),
+              "ma_serve_open")
+        self.handle = h.value
+        self.index = index
+        self.inflight: Dict[int, _Flight] = {}
+
+    def submit(self, req: Request, params: Optional[Params] = None) -> Optional[int]:
+        """Ticket, or None when the request is not admitted now."""
+        params = params or default_params(k=self.index.k)
+        out = req.out if req.out is not None else pinned_recs(req.n)
+        st = recs_struct(out)
+        t = lib().ma_serve_submit(self.handle, ptr(req.bases), req.bases.shape[1], ptr(req.lens), req.n,
+                                  req.max_len, req.encoding, req.bs_r1, req.bs_r2,
+                                  req.n if req.n_r1 is None else req.n_r1, C.byref(params), C.byref(st))
+        if t == MA_ERR_NOMEM:
+            return None
+        check(int(t), "ma_serve_submit")
+        self.inflight[int(t)] = _Flight(req, out, st, time.perf_counter())
+        return int(t)
+
+    def poll(self, ticket: int, wait: bool = False):
+        """(done, records) — records is the merged record dict once done."""
+        lat = C.c_double(0)
+        r = check(lib().ma_serve_poll(self.handle, ticket, int(wait), C.byref(lat)), "ma_serve_poll")
+        if r == 0:
+            return False, None
+        fl = self.inflight.pop(ticket)
+        return True, fl.out
+
+    def reap(self, wait: bool = False) -> Dict[int, Dict[str, np.ndarray]]:
+        """Finished tickets since the last call -> their merged records."""
+        buf = (C.c_int64 * 256)()
+        out: Dict[int, Dict[str, np.ndarray]] = {}
+        while True:
+            n = check(lib().ma_serve_reap(self.handle, buf, 256, int(wait and not out)), "ma_serve_reap")
+            for i in range(n):
+                fl = self.inflight.pop(int(buf[i]))
+                out[int(buf[i])] = fl.out
+            if n < 256:
+                return out
+
+    def close(self) -> None:
+        if self.handle:
+            lib().ma_serve_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class LoopStats:
+    reads: int
+    wall_s: float
+    latencies_ms: List[float] = field(default_factory=list)
+
+    @property
+    def reads_per_s(self) -> float:
+        return self.reads / self.wall_s if self.wall_s > 0 else 0.0
+
+    def pct(self, q: float) -> float:
+        return float(np.percentile(np.array(self.latencies_ms), q)) if self.latencies_ms else 0.0
+
+
+def run_request_loop(server: Server, requests: Sequence[Request], params: Optional[Params] = None,
+                     keep_outputs: bool = False):
+    """Submit requests at their arrival times (the server queues and merges
+    them as slots and HBM free up); latency of a request = completion time -
+    arrival time, queueing included."""
+    order = sorted(range(len(requests)), key=lambda i: requests[i].arrival)
+    active: Dict[int, int] = {}  # ticket -> request index
+    lat = [0.0] * len(requests)
+    outs: Dict[int, Dict[str, np.ndarray]] = {}
+    t0 = time.perf_counter()
+    pi = 0
+    done = 0
+    while done < len(requests):
+        now = time.perf_counter() - t0
+        while pi < len(order) and requests[order[pi]].arrival <= now:
+            i = order[pi]
+            t = server.submit(requests[i], params)
+            if t is None:
+                raise RuntimeError("request not admitted")
+            active[t] = i
+            pi += 1
+        finished = server.reap(wait=False)
+        tnow = time.perf_counter() - t0
+        for t, rec in finished.items():
+            i = active.pop(t)
+            lat[i] = (tnow - requests[i].arrival) * 1000.0
+            if keep_outputs:
+                outs[i] = rec
+            done += 1
+        if not active and pi < len(order):
+            dt = requests[order[pi]].arrival - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(min(dt, 0.0005))
+    wall = time.perf_counter() - t0
+    st = LoopStats(sum(r.n for r in requests), wall, lat)
+    return (st, outs) if keep_outputs else st
+
+
+def run_plain(index: GpuIndex, requests: Sequence[Request], params: Optional[Params] = None,
+              hbm_budget: int = 8 << 30) -> LoopStats:
+    """Baseline: one batch per launch, strictly one after another (a one-slot
+    server, waiting for each batch before the next); latency = that batch's
+    own copy-in + map + copy-out time."""
+    srv = Server(index, slots=1, hbm_budget=hbm_budget)
+    lat = []
+    t0 = time.perf_counter()
+    try:
+        for r in requests:
+            t = time.perf_counter()
+            tk = srv.submit(r, params)
+            if tk is None:
+                raise RuntimeError("plain launch: request larger than the HBM budget")
+            srv.poll(tk, wait=True)
+            lat.append((time.perf_counter() - t) * 1000.0)
+    finally:
+        srv.close()
+    return LoopStats(sum(r.n for r in requests), time.perf_counter() - t0, lat)

@@ -254,3 +254,24 @@ def test_map_short_read_class(w1):
     assert_recs_equal(gfw, wfw, "FW records (short)")
     assert_recs_equal(grc, wrc, "RC records (short)")
     assert_recs_equal(got, want, "merged records (short)")
+
+
+def test_served_loop_results_same_alone_and_in_the_mix(w1):
+    say("test_served_loop_results_same_alone_and_in_the_mix")
+    from mialign import serve
+
+    rng = np.random.default_rng(7)
+    reqs = []
+    for i, n in enumerate((1, 700, 5000, 120, 3000, 64, 9000, 2500, 17, 4000)):
+        rd = synth.gen_reads(100 + i, w1.ref, n, 50, 150, max_len=150)
+        reqs.append(serve.Request(serve.pinned_copy(rd.bases), serve.pinned_copy(rd.lens), 150, 0,
+                                  bs_r1=MA_BS_C2T, arrival=float(rng.uniform(0, 0.02))))
+    srv = serve.Server(w1.gix, slots=4, hbm_budget=4 << 20)  # small budget: admission must wait
+    st, outs = serve.run_request_loop(srv, reqs, keep_outputs=True)
+    srv.close()
+    assert len(outs) == len(reqs) and len(st.latencies_ms) == len(reqs)
+    for i, r in enumerate(reqs):
+        alone = mialign.map_batch(w1.gix, r.bases, r.lens, 150, bs_r1=MA_BS_C2T)
+        assert_recs_equal(outs[i], alone, f"request {i} served vs alone")
+        want, _, _, _ = w1.oracle_map(synth.Reads(r.bases, r.lens, None, None, None, None))
+        assert_recs_equal(outs[i], want, f"request {i} served vs oracle")
