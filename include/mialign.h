@@ -137,6 +137,11 @@ int ma_workspace_destroy(ma_workspace *ws);
  * grid sized to the batch, one chunk per wave, so concurrent launches on other
  * streams interleave (what the served loop uses). Results are identical. */
 int ma_workspace_set_persistent(ma_workspace *ws, int32_t persistent);
+/* Launch shape: persistent as above; chunks_per_wave (1..64, default 1) =
+ * read chunks per resident wave, so claim = clamp(n / (resident waves x
+ * chunks_per_wave), 1, 16) reads. More, smaller chunks shorten the tail of a
+ * small batch. Results are identical for every setting. */
+int ma_workspace_tune(ma_workspace *ws, int32_t persistent, int32_t chunks_per_wave);
 /* Bytes a workspace of this shape holds on the device (for admission). */
 int64_t ma_workspace_bytes(int64_t max_reads, int32_t max_len);
 
@@ -184,21 +189,31 @@ int ma_seed_kmers(int device, const uint8_t *bases, int64_t n, int32_t max_len, 
                   uint64_t *out_hash);
 
 /* ---- served request loop ---- */
-/* A server owns `slots` streams and an HBM budget (bytes). Requests queue on
- * submit; whenever a slot is free, the compatible requests waiting at the head
- * of the queue (same row layout, conversion and knobs, single-end) are merged
- * into one launch while the merged bytes fit the free budget. */
+/* A server owns `slots` streams and an HBM budget (bytes); each slot's launch
+ * buffer is reserved at open. Requests queue on submit and are cut into pieces
+ * of at most `launch_reads` reads (default 131072; a paired request is also
+ * split where its conversion changes). Whenever a slot is free, one launch is
+ * assembled from waiting pieces in deadline order (see ma_serve_tune), merging compatible pieces (same row layout, conversion
+ * and knobs) while the launch's bytes fit the free budget. */
 int ma_serve_open(ma_index *ix, int32_t slots, int64_t hbm_budget, int32_t max_len,
                   ma_server **out);
-/* Queue host reads (same layout as ma_map_batch); the host arrays must stay
- * valid until the ticket completes (pinned memory lets the copies overlap).
- * Returns a ticket >= 0; a request larger than the whole budget is an error. */
+/* launch_reads >= 1024 caps the reads of one launch; waiting work runs in
+ * order of arrival + size_weight x its estimated stand-alone time (0 = arrival
+ * order; default 0.5); persistent / chunks_per_wave set every slot's launch
+ * shape as in ma_workspace_tune (defaults 0 and 4). Only while nothing is in
+ * flight. */
+int ma_serve_tune(ma_server *s, int64_t launch_reads, double size_weight, int32_t persistent,
+                  int32_t chunks_per_wave);
+/* Queue host reads (same layout and record outputs as ma_map_batch's merged
+ * records); the host arrays must stay valid until the ticket completes (pinned
+ * memory lets the copies overlap; seven record arrays at one constant pitch in
+ * one block are copied back with one 2D copy per piece). Returns a ticket >= 0. */
 int64_t ma_serve_submit(ma_server *s, const uint8_t *bases, int64_t stride, const uint32_t *lens,
                         int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
                         int64_t n_r1, const ma_params *p, ma_recs *merged_out);
-/* Returns 1 when the ticket is complete (records written to its arrays), 0
- * while it is queued or running; wait != 0 blocks until complete. latency_ms
- * is reserved (written 0; callers time requests themselves). */
+/* Returns 1 when the ticket is complete (records written to its arrays) and
+ * writes its latency (submit to observed completion, ms), 0 while it is queued
+ * or running; wait != 0 blocks until complete. */
 int ma_serve_poll(ma_server *s, int64_t ticket, int32_t wait, double *latency_ms);
 /* Progress the server and report finished tickets: writes up to `cap` of them
  * to `tickets` and returns how many (each is reported once, by this call or by

@@ -30,9 +30,12 @@ struct ma_workspace {
     int64_t max_reads = 0;  // 0 = no limit
     int32_t max_len = 0;
     int32_t persistent = 1;  // 1: persistent grid + work counter; 0: grid sized to the batch
+    int32_t chunks_per_wave = 1;  // work chunks per resident wave (sets the claim size)
     ma::DevBuf counter;
 };
 
 namespace ma {
 IndexView index_view(const ma_index *ix);
+// Validate map knobs against the index and the longest read (MA_ERR_ARG + message).
+int check_map_params(const ma_index *ix, const ma_params *p, int32_t max_len);
 }

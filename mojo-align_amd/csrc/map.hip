@@ -306,7 +306,11 @@ RecOut rec_out(const ma_recs *r) {
     return o;
 }
 
-int check_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
+}  // namespace
+
+namespace ma {
+
+int check_map_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
     MA_REQUIRE(p, "map: null params");
     MA_REQUIRE(p->k == ix->k, "map: params.k differs from the index k");
     MA_REQUIRE(p->stride >= 1, "map: stride must be >= 1");
@@ -319,7 +323,7 @@ int check_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
     return MA_OK;
 }
 
-}  // namespace
+}  // namespace ma
 
 extern "C" {
 
@@ -351,6 +355,14 @@ int ma_workspace_set_persistent(ma_workspace *ws, int32_t persistent) {
     return MA_OK;
 }
 
+int ma_workspace_tune(ma_workspace *ws, int32_t persistent, int32_t chunks_per_wave) {
+    MA_REQUIRE(ws && (persistent == 0 || persistent == 1) && chunks_per_wave >= 1 && chunks_per_wave <= 64,
+               "ma_workspace_tune: bad arguments");
+    ws->persistent = persistent;
+    ws->chunks_per_wave = chunks_per_wave;
+    return MA_OK;
+}
+
 int ma_workspace_destroy(ma_workspace *ws) {
     delete ws;
     return MA_OK;
@@ -366,7 +378,7 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     MA_REQUIRE(encoding == MA_ENC_ASCII || encoding == MA_ENC_2BIT, "ma_map_batch_dev: bad encoding");
     MA_REQUIRE(encoding == MA_ENC_ASCII ? stride >= max_len : stride * 4 >= max_len,
                "ma_map_batch_dev: stride smaller than a read row");
-    MA_TRY(check_params(ix, p, max_len));
+    MA_TRY(check_map_params(ix, p, max_len));
     if (n == 0) return MA_OK;
     RecOut mo = rec_out(d_merged);
     MA_REQUIRE(mo.cid, "ma_map_batch_dev: merged record arrays must be non-null");
@@ -374,10 +386,11 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc};
     ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
     // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
-    // the resident waves all get work. Persistent mode: at most 8 blocks per
+    // the resident waves all get `chunks_per_wave` chunks (more, smaller chunks
+    // shorten the tail of a small batch). Persistent mode: at most 8 blocks per
     // CU loop over chunks taken from a counter. Batch-sized mode: one chunk per
     // wave, blocks retire as they finish (concurrent launches interleave).
-    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU;
+    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU * ws->chunks_per_wave;
     int64_t claim = n / waves;
     if (claim < 1) claim = 1;
     if (claim > MAX_CLAIM) claim = MAX_CLAIM;
@@ -409,7 +422,7 @@ int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint3
                  int64_t n_r1, const ma_params *p, ma_recs *merged, ma_recs *fw, ma_recs *rc) {
     MA_REQUIRE(ix && merged, "ma_map_batch: null index/output");
     MA_REQUIRE(n >= 0, "ma_map_batch: n < 0");
-    MA_TRY(check_params(ix, p, max_len));
+    MA_TRY(check_map_params(ix, p, max_len));
     if (n == 0) return MA_OK;
     MA_REQUIRE(bases && lens, "ma_map_batch: null input");
     for (int64_t i = 0; i < n; i++)

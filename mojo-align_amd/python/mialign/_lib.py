@@ -111,6 +111,7 @@ SIGNATURES = {
     "ma_workspace_create": (C.c_int, [P, I64, I32, C.POINTER(P)]),
     "ma_workspace_destroy": (C.c_int, [P]),
     "ma_workspace_set_persistent": (C.c_int, [P, I32]),
+    "ma_workspace_tune": (C.c_int, [P, I32, I32]),
     "ma_workspace_bytes": (I64, [I64, I32]),
     "ma_map_batch": (C.c_int, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                                C.POINTER(Recs), C.POINTER(Recs), C.POINTER(Recs)]),
@@ -124,6 +125,7 @@ SIGNATURES = {
     "ma_radix_sort_u64_dev": (C.c_int, [P, I64, P, P, P, I64]),
     "ma_seed_kmers": (C.c_int, [C.c_int, P, I64, I32, I32, P]),
     "ma_serve_open": (C.c_int, [P, I32, I64, I32, C.POINTER(P)]),
+    "ma_serve_tune": (C.c_int, [P, I64, C.c_double, I32, I32]),
     "ma_serve_submit": (I64, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                               C.POINTER(Recs)]),
     "ma_serve_poll": (C.c_int, [P, I64, I32, C.POINTER(C.c_double)]),

@@ -1,9 +1,11 @@
 """Served request loop over a resident index (continuous batching).
 
 The C side (csrc/serve.hip) owns `slots` streams and a free-HBM budget;
-`Server.submit` admits a read batch when a slot and the budget allow and
-enqueues its copy-in, map kernel and copy-out on that slot's stream, so
-requests arriving while others run overlap with them on the device.
+`Server.submit` queues a read batch; whenever a slot is free the server
+assembles one launch from waiting work in deadline order (large batches cut
+into pieces, small ones merged) and enqueues its copy-in, map kernel and
+copy-out on that slot's stream, so requests arriving while others run
+overlap with them on the device.
 `run_request_loop` drives Poisson-like arrivals through it and reports
 throughput and p50/p99 batch latency; `run_plain` is the baseline of one
 batch per launch, strictly one after another.
@@ -12,6 +14,7 @@ batch per launch, strictly one after another.
 from __future__ import annotations
 
 import ctypes as C
+import gc
 import time
 import weakref
 from dataclasses import dataclass, field
@@ -21,6 +24,9 @@ import numpy as np
 
 from ._lib import MA_ERR_NOMEM, Params, check, default_params, host_recs, lib, ptr, recs_struct
 from .core import GpuIndex
+
+DEFAULT_LAUNCH_READS = 128 << 10
+DEFAULT_SIZE_WEIGHT = 0.5
 
 
 def pinned_empty(shape, dtype) -> np.ndarray:
@@ -75,13 +81,18 @@ class _Flight:
 
 
 class Server:
-    def __init__(self, index: GpuIndex, slots: int = 4, hbm_budget: int = 8 << 30, max_len: int = 256):
+    def __init__(self, index: GpuIndex, slots: int = 2, hbm_budget: int = 8 << 30, max_len: int = 256,
+                 launch_reads: Optional[int] = None, size_weight: float = DEFAULT_SIZE_WEIGHT,
+                 persistent: bool = False, chunks_per_wave: int = 4):
         h = C.c_void_p()
         check(lib().ma_serve_open(index.handle, slots, int(hbm_budget), max_len, C.byref(h)),
               "ma_serve_open")
         self.handle = h.value
         self.index = index
         self.inflight: Dict[int, _Flight] = {}
+        self._buf = (C.c_int64 * 256)()
+        check(lib().ma_serve_tune(self.handle, int(launch_reads or DEFAULT_LAUNCH_READS), float(size_weight),
+                                  int(persistent), int(chunks_per_wave)), "ma_serve_tune")
 
     def submit(self, req: Request, params: Optional[Params] = None) -> Optional[int]:
         """Ticket, or None when the request is not admitted now."""
@@ -104,11 +115,12 @@ class Server:
         if r == 0:
             return False, None
         fl = self.inflight.pop(ticket)
+        self.last_latency_ms = lat.value
         return True, fl.out
 
     def reap(self, wait: bool = False) -> Dict[int, Dict[str, np.ndarray]]:
         """Finished tickets since the last call -> their merged records."""
-        buf = (C.c_int64 * 256)()
+        buf = self._buf
         out: Dict[int, Dict[str, np.ndarray]] = {}
         while True:
             n = check(lib().ma_serve_reap(self.handle, buf, 256, int(wait and not out)), "ma_serve_reap")
@@ -153,7 +165,21 @@ def run_request_loop(server: Server, requests: Sequence[Request], params: Option
     active: Dict[int, int] = {}  # ticket -> request index
     lat = [0.0] * len(requests)
     outs: Dict[int, Dict[str, np.ndarray]] = {}
+    gc_was = gc.isenabled()
+    gc.disable()  # no collector pauses inside the timed loop
     t0 = time.perf_counter()
+    try:
+        _drive(server, requests, params, order, active, lat, outs, keep_outputs, t0)
+    finally:
+        if gc_was:
+            gc.enable()
+    wall = time.perf_counter() - t0
+    st = LoopStats(sum(r.n for r in requests), wall, lat)
+    return (st, outs) if keep_outputs else st
+
+
+def _drive(server, requests, params, order, active, lat, outs, keep_outputs, t0) -> None:
+    """Submit due requests, collect finished ones, sleep briefly when idle."""
     pi = 0
     done = 0
     while done < len(requests):
@@ -177,20 +203,22 @@ def run_request_loop(server: Server, requests: Sequence[Request], params: Option
             dt = requests[order[pi]].arrival - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(min(dt, 0.0005))
-    wall = time.perf_counter() - t0
-    st = LoopStats(sum(r.n for r in requests), wall, lat)
-    return (st, outs) if keep_outputs else st
 
 
 def run_plain(index: GpuIndex, requests: Sequence[Request], params: Optional[Params] = None,
-              hbm_budget: int = 8 << 30) -> LoopStats:
+              hbm_budget: int = 8 << 30, **shape) -> LoopStats:
     """Baseline: one batch per launch, strictly one after another (a one-slot
     server, waiting for each batch before the next); latency = that batch's
     own copy-in + map + copy-out time."""
-    srv = Server(index, slots=1, hbm_budget=hbm_budget)
+    srv = Server(index, slots=1, hbm_budget=hbm_budget, launch_reads=1 << 30, **shape)
     lat = []
-    t0 = time.perf_counter()
+    gc_was = gc.isenabled()
+    gc.disable()
     try:
+        big = max(requests, key=lambda r: r.n)  # sizes the launch buffer before timing
+        srv.poll(srv.submit(Request(big.bases, big.lens, big.max_len, big.encoding, big.bs_r1, big.bs_r2,
+                                    big.n_r1, out=big.out), params), wait=True)
+        t0 = time.perf_counter()
         for r in requests:
             t = time.perf_counter()
             tk = srv.submit(r, params)
@@ -199,5 +227,7 @@ def run_plain(index: GpuIndex, requests: Sequence[Request], params: Optional[Par
             srv.poll(tk, wait=True)
             lat.append((time.perf_counter() - t) * 1000.0)
     finally:
+        if gc_was:
+            gc.enable()
         srv.close()
     return LoopStats(sum(r.n for r in requests), time.perf_counter() - t0, lat)

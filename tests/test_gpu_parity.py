@@ -256,22 +256,42 @@ def test_map_short_read_class(w1):
     assert_recs_equal(got, want, "merged records (short)")
 
 
-def test_served_loop_results_same_alone_and_in_the_mix(w1):
-    say("test_served_loop_results_same_alone_and_in_the_mix")
+@pytest.mark.parametrize("size_weight,launch_reads", [(2.0, 2048), (0.0, 131072)])
+def test_served_loop_results_same_alone_and_in_the_mix(w1, size_weight, launch_reads):
+    """Requests of mixed sizes arriving over 20 ms, merged and (with 2048-read
+    launches) cut into pieces; one paired request (R1 C2T, R2 G2A) is split at
+    its conversion boundary; some requests take records into one pinned block
+    (one 2D copy per piece), others into separate pageable arrays (one copy per
+    field). Every request's records equal the same batch mapped alone and the
+    oracle's."""
+    say(f"test_served_loop_results_same_alone_and_in_the_mix[{size_weight}-{launch_reads}]")
     from mialign import serve
+    from mialign._lib import host_recs
 
     rng = np.random.default_rng(7)
-    reqs = []
-    for i, n in enumerate((1, 700, 5000, 120, 3000, 64, 9000, 2500, 17, 4000)):
+    reqs, modes = [], []
+    for i, n in enumerate((1, 700, 5000, 120, 3000, 64, 9000, 2500, 17, 4000, 6000)):
         rd = synth.gen_reads(100 + i, w1.ref, n, 50, 150, max_len=150)
+        paired = i == 10
+        n_r1 = 3500 if paired else None
+        out = host_recs(n) if i % 3 == 1 else None
         reqs.append(serve.Request(serve.pinned_copy(rd.bases), serve.pinned_copy(rd.lens), 150, 0,
-                                  bs_r1=MA_BS_C2T, arrival=float(rng.uniform(0, 0.02))))
-    srv = serve.Server(w1.gix, slots=4, hbm_budget=4 << 20)  # small budget: admission must wait
+                                  bs_r1=MA_BS_C2T, bs_r2=MA_BS_G2A if paired else 0, n_r1=n_r1,
+                                  arrival=float(rng.uniform(0, 0.02)), out=out))
+        modes.append(n_r1)
+    srv = serve.Server(w1.gix, slots=4, hbm_budget=4 << 20, launch_reads=launch_reads,
+                       size_weight=size_weight)  # small budget: admission must wait
     st, outs = serve.run_request_loop(srv, reqs, keep_outputs=True)
     srv.close()
     assert len(outs) == len(reqs) and len(st.latencies_ms) == len(reqs)
     for i, r in enumerate(reqs):
-        alone = mialign.map_batch(w1.gix, r.bases, r.lens, 150, bs_r1=MA_BS_C2T)
+        n_r1 = modes[i]
+        alone = mialign.map_batch(w1.gix, r.bases, r.lens, 150, bs_r1=MA_BS_C2T,
+                                  bs_r2=MA_BS_G2A if n_r1 is not None else 0, n_r1=n_r1)
         assert_recs_equal(outs[i], alone, f"request {i} served vs alone")
-        want, _, _, _ = w1.oracle_map(synth.Reads(r.bases, r.lens, None, None, None, None))
+        if n_r1 is None:
+            conv = orc.convert_reads(r.bases, 1)
+        else:
+            conv = np.concatenate([orc.convert_reads(r.bases[:n_r1], 1), orc.convert_reads(r.bases[n_r1:], 2)])
+        want, _, _, _ = orc.map_batch(w1.oix, conv, r.lens, nthreads=8)
         assert_recs_equal(outs[i], want, f"request {i} served vs oracle")
