@@ -125,6 +125,9 @@ int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int
                    int64_t *n_contigs, int64_t *device_bytes, int32_t *k);
 /* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
  * postings[2*n_postings]); any pointer may be NULL. */
+/* Device hash table shape: buckets (4 slots of 16 bytes each; every key sits
+ * in one of its two buckets) and how many keys the build's cuckoo pass placed. */
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cuckoo);
 int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings);
 int ma_index_close(ma_index *ix);
 

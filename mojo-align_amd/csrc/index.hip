@@ -7,10 +7,11 @@
 // by (key, contig, pos). The build here runs on the GPU: one key per window,
 // a stable radix sort by key, head flags + scan for the CSR. Lookups do not
 // binary-search kmers (locate_bsearch_kernel, linear_gpu_locate.mojo:453-490);
-// they probe one 64-byte bucket of a hash table that maps key -> (start, occ),
-// so the resulting [start, end) ranges are the same.
+// they load a key's two 64-byte buckets of a two-choice hash table that maps
+// key -> (start, occ), so the resulting [start, end) ranges are the same.
 #include <fcntl.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <sys/mman.h>
 #include <sys/stat.h>
@@ -114,36 +115,97 @@ __global__ void csr_kernel(const uint64_t *__restrict__ keys, const uint64_t *__
 
 __global__ void set_u64_kernel(uint64_t *p, uint64_t v) { *p = v; }
 
-// Insert every unique key; claims a slot with a 64-bit CAS on the key word.
+// Claim an empty slot of bucket `bk` for (key, val); false when it is full.
+__device__ bool claim_slot(HashSlot *bk, uint64_t key, uint64_t val) {
+    for (int s = 0; s < BUCKET_SLOTS; s++) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&bk[s].key,
+                                            (unsigned long long)EMPTY_KEY, (unsigned long long)key);
+        if (prev == EMPTY_KEY) {
+            bk[s].val = val;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ uint64_t slot_value(const uint64_t *offsets, int64_t i) {
+    uint64_t start = offsets[i];
+    uint64_t occ = offsets[i + 1] - start;
+    if (occ > OCC_SAT) occ = OCC_SAT;
+    return (occ << 40) | (start & START_MASK);
+}
+
+// Parallel two-choice insert: the less occupied of a key's two buckets first
+// (a racy hint, exactness does not depend on it), then the other. Keys that
+// find both full are listed for the cuckoo pass.
 __global__ void hash_insert_kernel(const uint64_t *__restrict__ kmers,
                                    const uint64_t *__restrict__ offsets, int64_t n_keys,
                                    HashSlot *__restrict__ table, uint64_t n_buckets,
-                                   int *__restrict__ max_probe, int *__restrict__ fail) {
+                                   int64_t *__restrict__ overflow, unsigned int *__restrict__ n_overflow,
+                                   unsigned int overflow_cap) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_keys;
          i += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t key = kmers[i];
-        uint64_t start = offsets[i];
-        uint64_t occ = offsets[i + 1] - start;
-        if (occ > OCC_SAT) occ = OCC_SAT;
-        uint64_t val = (occ << 40) | (start & START_MASK);
+        const uint64_t key = kmers[i];
+        const uint64_t val = slot_value(offsets, i);
+        const uint64_t b1 = bucket_of(key, n_buckets);
+        const uint64_t b2 = bucket2_of(key, n_buckets, b1);
+        int o1 = 0, o2 = 0;
+        for (int s = 0; s < BUCKET_SLOTS; s++) {
+            o1 += table[b1 * BUCKET_SLOTS + s].key != EMPTY_KEY;
+            o2 += table[b2 * BUCKET_SLOTS + s].key != EMPTY_KEY;
+        }
+        const uint64_t first = o2 < o1 ? b2 : b1, second = o2 < o1 ? b1 : b2;
+        if (claim_slot(table + first * BUCKET_SLOTS, key, val)) continue;
+        if (claim_slot(table + second * BUCKET_SLOTS, key, val)) continue;
+        unsigned int j = atomicAdd(n_overflow, 1u);
+        if (j < overflow_cap) overflow[j] = i;
+    }
+}
+
+// Cuckoo placement of the listed keys, one thread, after the parallel pass:
+// a key goes into a free slot of its current bucket or displaces a resident
+// (slot chosen by a step hash), which then moves to its own other bucket.
+// Every loop is bounded by `step_budget` in total; running out sets *fail and
+// the host rebuilds with more buckets.
+__global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ kmers, const uint64_t *__restrict__ offsets,
+                                   HashSlot *__restrict__ table, uint64_t n_buckets,
+                                   const int64_t *__restrict__ overflow, const unsigned int *__restrict__ n_overflow,
+                                   unsigned int overflow_cap, int64_t step_budget, int *__restrict__ fail) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned int n = *n_overflow;
+    if (n > overflow_cap) {
+        *fail = 1;
+        return;
+    }
+    int64_t steps = 0;
+    for (unsigned int j = 0; j < n; j++) {
+        uint64_t key = kmers[overflow[j]];
+        uint64_t val = slot_value(offsets, overflow[j]);
         uint64_t b = bucket_of(key, n_buckets);
-        int done = 0;
-        for (uint64_t probe = 0; probe < n_buckets && !done; probe++) {
+        bool placed = false;
+        while (!placed) {
+            if (++steps > step_budget) {
+                *fail = 1;
+                return;
+            }
             HashSlot *bk = table + b * BUCKET_SLOTS;
-            for (int s = 0; s < BUCKET_SLOTS; s++) {
-                unsigned long long prev = atomicCAS((unsigned long long *)&bk[s].key,
-                                                    (unsigned long long)EMPTY_KEY,
-                                                    (unsigned long long)key);
-                if (prev == EMPTY_KEY) {
+            for (int s = 0; s < BUCKET_SLOTS && !placed; s++) {
+                if (bk[s].key == EMPTY_KEY) {
+                    bk[s].key = key;
                     bk[s].val = val;
-                    atomicMax(max_probe, (int)probe + 1);
-                    done = 1;
-                    break;
+                    placed = true;
                 }
             }
-            b = (b + 1 == n_buckets) ? 0 : b + 1;
+            if (placed) break;
+            const int victim = (int)(splitmix_mix((uint64_t)steps ^ key) & (BUCKET_SLOTS - 1));
+            const uint64_t vk = bk[victim].key, vv = bk[victim].val;
+            bk[victim].key = key;
+            bk[victim].val = val;
+            key = vk;
+            val = vv;
+            const uint64_t c1 = bucket_of(key, n_buckets);
+            b = (b == c1) ? bucket2_of(key, n_buckets, c1) : c1;
         }
-        if (!done) atomicOr(fail, 1);
     }
 }
 
@@ -189,35 +251,52 @@ int pack_reference(ma_index *ix, const uint8_t *seq, int64_t n_bases, const uint
     return MA_OK;
 }
 
-// Hash table over the resident kmers/offsets.
+// Two-choice hash table over the resident kmers/offsets at load 0.5 (0.4,
+// then 0.3, if the cuckoo pass runs out of steps). MIALIGN_HASH_LOAD in
+// (0, 0.95] sets the first load (a tuning and test knob: high loads send many
+// keys through the cuckoo pass).
 int build_hash(ma_index *ix, hipStream_t st) {
-    int64_t nk = ix->n_keys;
-    uint64_t n_buckets = (uint64_t)((double)nk / (BUCKET_SLOTS * 0.5)) + 1;
-    ix->n_buckets = n_buckets;
-    MA_TRY(ix->table.alloc((size_t)n_buckets * BUCKET_SLOTS * sizeof(HashSlot)));
-    int64_t n_slots = (int64_t)n_buckets * BUCKET_SLOTS;
-    fill_slots_kernel<<<grid_for(n_slots), 256, 0, st>>>(ix->table.as<HashSlot>(), n_slots);
-    MA_LAUNCHED();
-    DevBuf d_misc;
+    const int64_t nk = ix->n_keys;
+    constexpr unsigned int OVERFLOW_CAP = 1u << 16;
+    constexpr int64_t STEP_BUDGET = 1ll << 22;
+    DevBuf d_misc, d_over;
     MA_TRY(d_misc.alloc(256));
-    MA_TRY(fill_u64(st, d_misc.as<uint64_t>(), 2, 0));
-    int *d_probe = d_misc.as<int>(), *d_fail = d_misc.as<int>() + 1;
-    if (nk > 0) {
-        hash_insert_kernel<<<grid_for(nk), 256, 0, st>>>(ix->kmers.as<uint64_t>(),
-                                                         ix->offsets.as<uint64_t>(), nk,
-                                                         ix->table.as<HashSlot>(), n_buckets,
-                                                         d_probe, d_fail);
+    MA_TRY(d_over.alloc((size_t)OVERFLOW_CAP * 8));
+    unsigned int *d_nover = d_misc.as<unsigned int>();
+    int *d_fail = d_misc.as<int>() + 1;
+    double first = 0.5;
+    if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
+        const double v = atof(env);
+        MA_REQUIRE(v > 0 && v <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
+        first = v;
+    }
+    for (double load : {first, 0.4, 0.3}) {
+        if (load > first) continue;
+        const uint64_t n_buckets = (uint64_t)((double)nk / (BUCKET_SLOTS * load)) + 1;
+        ix->n_buckets = n_buckets;
+        MA_TRY(ix->table.alloc((size_t)n_buckets * BUCKET_SLOTS * sizeof(HashSlot)));
+        const int64_t n_slots = (int64_t)n_buckets * BUCKET_SLOTS;
+        fill_slots_kernel<<<grid_for(n_slots), 256, 0, st>>>(ix->table.as<HashSlot>(), n_slots);
         MA_LAUNCHED();
+        MA_TRY(fill_u64(st, d_misc.as<uint64_t>(), 1, 0));
+        if (nk > 0) {
+            hash_insert_kernel<<<grid_for(nk), 256, 0, st>>>(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(),
+                                                             nk, ix->table.as<HashSlot>(), n_buckets,
+                                                             d_over.as<int64_t>(), d_nover, OVERFLOW_CAP);
+            MA_LAUNCHED();
+            hash_cuckoo_kernel<<<1, 64, 0, st>>>(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(),
+                                                 ix->table.as<HashSlot>(), n_buckets, d_over.as<int64_t>(),
+                                                 d_nover, OVERFLOW_CAP, STEP_BUDGET, d_fail);
+            MA_LAUNCHED();
+        }
+        int h[2] = {0, 0};
+        MA_HIP(hipMemcpyAsync(h, d_misc.p, 8, hipMemcpyDeviceToHost, st));
+        MA_HIP(hipStreamSynchronize(st));
+        ix->n_cuckoo = (int64_t)h[0];
+        if (!h[1]) return MA_OK;
     }
-    int h[2] = {0, 0};
-    MA_HIP(hipMemcpyAsync(h, d_misc.p, 8, hipMemcpyDeviceToHost, st));
-    MA_HIP(hipStreamSynchronize(st));
-    if (h[1]) {
-        set_error("hash index build: table full");
-        return MA_ERR_NOMEM;
-    }
-    ix->max_probe = h[0] < 1 ? 1 : h[0];
-    return MA_OK;
+    set_error("hash index build: cuckoo placement did not converge");
+    return MA_ERR_NOMEM;
 }
 
 int index_alloc(int device, ma_index **out) {
@@ -295,7 +374,6 @@ IndexView index_view(const ma_index *ix) {
     IndexView v;
     v.table = ix->table.as<HashSlot>();
     v.n_buckets = ix->n_buckets;
-    v.max_probe = ix->max_probe;
     v.k = ix->k;
     v.postings = ix->postings.as<uint32_t>();
     v.n_postings = ix->n_postings;
@@ -486,6 +564,13 @@ int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int
     if (device_bytes)
         *device_bytes = (int64_t)(ix->kmers.bytes + ix->offsets.bytes + ix->postings.bytes +
                                   ix->ref2.bytes + ix->refN.bytes + ix->coff.bytes + ix->table.bytes);
+    return MA_OK;
+}
+
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cuckoo) {
+    MA_REQUIRE(ix, "ma_index_hash_stats: null index");
+    if (n_buckets) *n_buckets = (int64_t)ix->n_buckets;
+    if (n_cuckoo) *n_cuckoo = ix->n_cuckoo;
     return MA_OK;
 }
 

@@ -78,6 +78,9 @@ constexpr uint64_t LOW2 = 0x5555555555555555ull;  // low bit of each 2-bit group
 constexpr uint64_t EMPTY_KEY = ~0ull;
 
 // Hash index slot: key (MSB-first 2-bit k-mer) and value = occ(24) << 40 | start(40).
+// Two-choice bucketed table: every key sits in one of its two 64-byte buckets
+// (bucket_of / bucket2_of), so a lookup loads both at once and is done in one
+// memory round trip, present or absent.
 struct HashSlot {
     uint64_t key;
     uint64_t val;
@@ -90,7 +93,6 @@ constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
 struct IndexView {
     const HashSlot *table;
     uint64_t n_buckets;
-    int32_t max_probe;
     int32_t k;
     const uint32_t *postings;  // (cid, pos) pairs
     int64_t n_postings;
@@ -115,12 +117,32 @@ __host__ __device__ inline uint64_t wang_hash(uint64_t key) {
     return key;
 }
 
-__host__ __device__ inline uint64_t bucket_of(uint64_t key, uint64_t n_buckets) {
+__host__ __device__ inline uint64_t mulhi64(uint64_t a, uint64_t b) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return __umul64hi(wang_hash(key), n_buckets);
+    return __umul64hi(a, b);
 #else
-    return (uint64_t)(((unsigned __int128)wang_hash(key) * n_buckets) >> 64);
+    return (uint64_t)(((unsigned __int128)a * b) >> 64);
 #endif
+}
+
+// SplitMix64 output mix (Steele, Lea and Flood, OOPSLA 2014): the second,
+// independent bucket choice.
+__host__ __device__ inline uint64_t splitmix_mix(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint64_t bucket_of(uint64_t key, uint64_t n_buckets) {
+    return mulhi64(wang_hash(key), n_buckets);
+}
+
+// Second bucket of a key (the next bucket when both choices coincide).
+__host__ __device__ inline uint64_t bucket2_of(uint64_t key, uint64_t n_buckets, uint64_t b1) {
+    uint64_t b = mulhi64(splitmix_mix(key), n_buckets);
+    if (b == b1) b = (b + 1 == n_buckets) ? 0 : b + 1;
+    return b;
 }
 
 // ------------------------------------------------------ 2-bit word helpers

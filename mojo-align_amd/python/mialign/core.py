@@ -196,6 +196,11 @@ class GpuIndex:
             "k": k.value,
         }
 
+    def hash_stats(self) -> dict:
+        nb, nc = C.c_int64(0), C.c_int64(0)
+        check(lib().ma_index_hash_stats(self.handle, C.byref(nb), C.byref(nc)), "ma_index_hash_stats")
+        return {"n_buckets": nb.value, "n_cuckoo": nc.value}
+
     def export_dense(self):
         kmers = np.empty(self.n_keys, dtype=np.uint64)
         offsets = np.empty(self.n_keys + 1, dtype=np.uint64)
