@@ -229,16 +229,16 @@ int pack_reference(ma_index *ix, const uint8_t *seq, int64_t n_bases, const uint
     ix->n_bases = n_bases;
     ix->n_contigs = n_contigs;
     int64_t n_words = (n_bases + 31) / 32;
-    MA_TRY(ix->ref2.alloc((size_t)(n_words + 4) * 8));
-    MA_TRY(ix->refN.alloc((size_t)(n_words + 4) * 8));
+    MA_TRY(ix->ref2.alloc((size_t)(n_words + REF_PAD_WORDS) * 8));
+    MA_TRY(ix->refN.alloc((size_t)(n_words + REF_PAD_WORDS) * 8));
     MA_TRY(ix->coff.alloc((size_t)(n_contigs + 1) * 8));
     MA_TRY(upload(ix->coff.p, contig_off, (size_t)(n_contigs + 1) * 8, st));
     DevBuf d_seq, d_flag;
     MA_TRY(d_seq.alloc((size_t)n_bases));
     MA_TRY(d_flag.alloc(256));
     MA_TRY(upload(d_seq.p, seq, (size_t)n_bases, st));
-    MA_TRY(fill_u64(st, ix->ref2.as<uint64_t>() + n_words, 4, 0));
-    MA_TRY(fill_u64(st, ix->refN.as<uint64_t>() + n_words, 4, 0));
+    MA_TRY(fill_u64(st, ix->ref2.as<uint64_t>() + n_words, REF_PAD_WORDS, 0));
+    MA_TRY(fill_u64(st, ix->refN.as<uint64_t>() + n_words, REF_PAD_WORDS, 0));
     MA_TRY(fill_u64(st, d_flag.as<uint64_t>(), 1, 0));
     pack_ref_kernel<<<grid_for(n_words), 256, 0, st>>>(d_seq.as<uint8_t>(), n_bases,
                                                        ix->ref2.as<uint64_t>(), ix->refN.as<uint64_t>(),

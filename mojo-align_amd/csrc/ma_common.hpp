@@ -86,6 +86,9 @@ struct HashSlot {
     uint64_t val;
 };
 constexpr int BUCKET_SLOTS = 4;  // 64-byte buckets
+// Zero words after the packed reference: a window's words are loaded as one
+// batch of up to 9 words from its first word, wherever it ends.
+constexpr int64_t REF_PAD_WORDS = 16;
 constexpr uint64_t START_MASK = (1ull << 40) - 1;
 constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
 
@@ -96,7 +99,7 @@ struct IndexView {
     int32_t k;
     const uint32_t *postings;  // (cid, pos) pairs
     int64_t n_postings;
-    const uint64_t *ref2;  // 2-bit MSB-first, 32 bases / word (+2 pad words)
+    const uint64_t *ref2;  // 2-bit MSB-first, 32 bases / word (+REF_PAD_WORDS zero words)
     const uint64_t *refN;  // N flags in the same geometry (low bit of the group)
     int32_t ref_has_n;
     const uint64_t *contig_off;  // n_contigs + 1
@@ -153,6 +156,12 @@ __device__ inline uint64_t window32(const uint64_t *w, int64_t g) {
     uint64_t a = w[wi];
     if (sh == 0) return a;
     return (a << sh) | (w[wi + 1] >> (64 - sh));
+}
+
+// Groups of word a shifted left by sh bits (even, 0..62), filled from word b.
+__device__ inline uint64_t funnel64(uint64_t a, uint64_t b, int sh) {
+    const uint64_t lo = b >> ((64 - sh) & 63);
+    return (a << sh) | (sh ? lo : 0ull);
 }
 
 // Reverse the order of the 32 two-bit groups of a word.

@@ -43,6 +43,7 @@ constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
 constexpr int PBUF = 256;  // postings prefetched into LDS per wave
 constexpr int MAX_CLAIM = 16;  // most reads claimed per work-counter atomic
+constexpr int COFF_LDS = 256;  // contig offsets cached in LDS below this contig count
 constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kernels' register budget
 
 struct Knobs {
@@ -232,30 +233,75 @@ DEV void store_rec(const RecOut &o, int64_t r, const Rec &x) {
 }  // namespace
 
 namespace w4 {
+namespace lds {
 namespace {
 constexpr int MAXW = 4;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
+constexpr bool COFF_IN_LDS = true;
 #include "map_body.inc"
 }  // namespace
+}  // namespace lds
+}  // namespace w4
+
+namespace w4 {
+namespace glb {
+namespace {
+constexpr int MAXW = 4;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 4;
+constexpr bool COFF_IN_LDS = false;
+#include "map_body.inc"
+}  // namespace
+}  // namespace glb
 }  // namespace w4
 
 namespace w5 {
+namespace lds {
 namespace {
 constexpr int MAXW = 5;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
+constexpr bool COFF_IN_LDS = true;
 #include "map_body.inc"
 }  // namespace
+}  // namespace lds
+}  // namespace w5
+
+namespace w5 {
+namespace glb {
+namespace {
+constexpr int MAXW = 5;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 4;
+constexpr bool COFF_IN_LDS = false;
+#include "map_body.inc"
+}  // namespace
+}  // namespace glb
 }  // namespace w5
 
 namespace w8 {
+namespace lds {
 namespace {
 constexpr int MAXW = 8;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 3;
+constexpr bool COFF_IN_LDS = true;
 #include "map_body.inc"
 }  // namespace
+}  // namespace lds
+}  // namespace w8
+
+namespace w8 {
+namespace glb {
+namespace {
+constexpr int MAXW = 8;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 3;
+constexpr bool COFF_IN_LDS = false;
+#include "map_body.inc"
+}  // namespace
+}  // namespace glb
 }  // namespace w8
 
 namespace {
@@ -406,13 +452,20 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     MA_REQUIRE(blocks < (1ll << 31), "ma_map_batch_dev: batch too large for one launch");
     const IndexView v = index_view(ix);
     const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
-    // the read-word class of the batch's longest read picks the kernel
-    if (max_len <= 4 * 32)
-        w4::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    else if (max_len <= 5 * 32)
-        w5::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    else
-        w8::map_reads_kernel<<<(unsigned)blocks, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    // the read-word class of the batch's longest read picks the kernel, the
+    // contig count where the contig offsets live
+    const unsigned g = (unsigned)blocks;
+    const bool lds = ix->n_contigs < COFF_LDS;
+    if (max_len <= 4 * 32) {
+        if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    } else if (max_len <= 5 * 32) {
+        if (lds) w5::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w5::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    } else {
+        if (lds) w8::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w8::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    }
     MA_LAUNCHED();
     return MA_OK;
 }
