@@ -220,6 +220,38 @@ def test_sort_markdup_stress_records():
     assert np.array_equal(d, wd)
 
 
+def test_sort_markdup_full_size_properties():
+    """200M sort-stress records (the full sort + markdup workload): perm is the
+    stable key order, every duplicate group keeps exactly one record, the first
+    of its highest score (pair = record index), checked with vectorised numpy."""
+    say("test_sort_markdup_full_size_properties: 200M records")
+    n = 200_000_000
+    key, rid, nm = synth.gen_sort_records(23, n)
+    score = (255 - nm.astype(np.uint32)).astype(np.uint32)
+    del nm
+    d, p, _ = mialign.sort_markdup(key, key, np.zeros(n, np.uint64), score, rid)
+    k = key[p]
+    assert (k[1:] >= k[:-1]).all(), "perm is not the key order"
+    eq = k[1:] == k[:-1]
+    assert (p[1:][eq] > p[:-1][eq]).all(), "perm is not stable"
+    del key
+    starts = np.flatnonzero(np.r_[True, ~eq])
+    del eq, k
+    s = score[p]
+    gmax = np.maximum.reduceat(s, starts)
+    gid = np.repeat(np.arange(starts.size), np.diff(np.r_[starts, n]))
+    at_max = s == gmax[gid]
+    del s, gmax
+    pos = np.arange(n, dtype=np.int64)
+    first_max = np.minimum.reduceat(np.where(at_max, pos, n), starts)
+    del at_max
+    keep = d[p] == 0
+    assert np.array_equal(np.bincount(gid, weights=keep, minlength=starts.size), np.ones(starts.size)), \
+        "a duplicate group does not keep exactly one record"
+    assert np.array_equal(pos[keep], first_max), "the kept record is not the first of the best score"
+    assert n - starts.size == int(d.sum()) > 0.05 * n
+
+
 def test_sort_markdup_no_markdup_and_empty():
     say("test_sort_markdup_no_markdup_and_empty")
     key, rid, nm = synth.gen_sort_records(22, 10_000)
