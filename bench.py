@@ -116,6 +116,12 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
     }
 
 
+# HBM bytes one map launch moves at the default workload (100 Mbp x2 reference,
+# 10M x 150 bp reads): rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
+# factors read off copy_u64_kernel (a copy of known size) in the same runs.
+MAP_TRAFFIC_PROFILED = {"genome_bp": 100_000_000, "reads": 10_000_000, "read_len": 150,
+                        "bytes": 179.4e9, "kernel_ms": 58.43,
+                        "source": "profiles/map_reads_kernel_pmc_100Mbp_10Mx150.md"}
 SORT_BYTES_PER_RECORD = 170  # 2 x 13 B x 6 digit passes (45-bit key) + 13 B markdup read + 1 B flag
 
 
@@ -328,8 +334,15 @@ def main() -> int:
         parity_sample = {"reads": ns, "records_bit_exact": bool(ok)}
         bpr = algorithmic_bytes_per_read(cnt, ns, L)
         achieved = bpr * n / (map_ms / 1000.0) / 1e9
+        tp = MAP_TRAFFIC_PROFILED
+        traffic, traffic_detail = None, None
+        if (tp["genome_bp"], tp["reads"], tp["read_len"]) == (args.genome, n, L):
+            traffic = tp["bytes"]
+            traffic_detail = {"per_read": round(tp["bytes"] / n, 1), "x_algorithmic": round(tp["bytes"] / (bpr * n), 2),
+                              "profiled_kernel_ms": tp["kernel_ms"], "source": tp["source"]}
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                    "traffic_detail": traffic_detail,
                     "kernel": "map_reads_kernel", "kernel_ms": round(map_ms, 3),
                     "algorithmic_bytes_per_read": round(bpr, 1)}
         log(f"cpu baseline {cpu['value']:.0f} reads/s; sample parity {ok}; {bpr:.0f} B/read algorithmic")
