@@ -118,16 +118,29 @@ int ma_index_from_dense(int device, int32_t k, const uint64_t *kmers, int64_t n_
                         const uint8_t *seq, int64_t n_bases, const uint64_t *contig_off,
                         int64_t n_contigs, ma_index **out);
 /* Build the dense-v1 index on the GPU from (already converted) ASCII contig
- * bases; byte-identical to build_dense_pack's kmers/offsets/postings. */
+ * bases; byte-identical to build_dense_pack's kmers/offsets/postings. The
+ * windows are sorted one key range at a time, sized to the free HBM, so a
+ * 3.1 Gbp x 2 reference builds within one GPU's memory. */
 int ma_index_build(int device, int32_t k, const uint8_t *seq, int64_t n_bases,
                    const uint64_t *contig_off, int64_t n_contigs, ma_index **out);
+typedef struct ma_build_opts {
+    int32_t keep_dense;     /* 1 (default): kmers/offsets stay resident (export, table rebuild);
+                               0: only the hash table, postings and reference (export_dense of
+                               kmers/offsets then fails) */
+    int32_t partition_bits; /* key-prefix partitions 2^bits the chunks are made of (1..10, default 10) */
+    int64_t chunk_windows;  /* windows sorted at once (0 = from the free HBM) */
+} ma_build_opts;
+void ma_build_opts_default(ma_build_opts *opts);
+int ma_index_build_opts(int device, int32_t k, const uint8_t *seq, int64_t n_bases,
+                        const uint64_t *contig_off, int64_t n_contigs, const ma_build_opts *opts,
+                        ma_index **out);
 int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int64_t *n_bases,
                    int64_t *n_contigs, int64_t *device_bytes, int32_t *k);
-/* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
- * postings[2*n_postings]); any pointer may be NULL. */
 /* Device hash table shape: buckets (4 slots of 16 bytes each; every key sits
  * in one of its two buckets) and how many keys the build's cuckoo pass placed. */
 int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cuckoo);
+/* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
+ * postings[2*n_postings]); any pointer may be NULL. */
 int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings);
 int ma_index_close(ma_index *ix);
 

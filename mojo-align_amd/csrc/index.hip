@@ -58,38 +58,6 @@ __global__ void pack_ref_kernel(const uint8_t *__restrict__ seq, int64_t n_bases
     }
 }
 
-// One key per base position; windows that leave the contig or hold an N get
-// the sentinel key 1 << 2k (sorts after every valid key).
-__global__ void window_keys_kernel(const uint64_t *__restrict__ ref2, const uint64_t *__restrict__ refN,
-                                   const uint64_t *__restrict__ contig_off, int64_t n_contigs,
-                                   int64_t n_bases, int k, uint64_t *__restrict__ keys,
-                                   uint64_t *__restrict__ locs, unsigned long long *__restrict__ n_valid) {
-    uint64_t invalid = 1ull << (2 * k);
-    unsigned long long local = 0;
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < n_bases;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        int64_t lo = 0, hi = n_contigs;  // last contig whose start <= g
-        while (hi - lo > 1) {
-            int64_t mid = (lo + hi) >> 1;
-            if ((int64_t)contig_off[mid] <= g) lo = mid;
-            else hi = mid;
-        }
-        int64_t p = g - (int64_t)contig_off[lo];
-        int64_t clen = (int64_t)(contig_off[lo + 1] - contig_off[lo]);
-        uint64_t key = invalid;
-        if (p + k <= clen) {
-            uint64_t nw = window32(refN, g) >> (64 - 2 * k);
-            if (nw == 0) {
-                key = window32(ref2, g) >> (64 - 2 * k);
-                local++;
-            }
-        }
-        keys[g] = key;
-        locs[g] = ((uint64_t)lo << 32) | (uint64_t)p;
-    }
-    if (local) atomicAdd(n_valid, local);
-}
-
 __global__ void key_heads_kernel(const uint64_t *__restrict__ keys, int64_t n,
                                  uint32_t *__restrict__ heads) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -115,6 +83,127 @@ __global__ void csr_kernel(const uint64_t *__restrict__ keys, const uint64_t *__
 
 __global__ void set_u64_kernel(uint64_t *p, uint64_t v) { *p = v; }
 
+// ------------------------------------------------ key-range partitioned build
+// The build walks the reference's windows several times, each time keeping
+// only the windows whose key lies in one key range (a "chunk"), so no more
+// than one chunk's (key, location) pairs and sort buffers are ever resident.
+// Chunks are unions of consecutive 2^bits key-prefix partitions sized from an
+// exact histogram. Within a chunk, windows are written in position order (per
+// block counts, a scan, then ordered writes), so the stable key sort yields
+// postings ordered (key, contig, pos) exactly as a whole-reference sort does.
+constexpr int PART_ITEMS = 16;                    // consecutive positions per thread
+constexpr int64_t PART_TILE = 256 * PART_ITEMS;   // positions per block
+
+struct WindowScan {
+    const uint64_t *ref2, *refN, *coff;
+    int64_t n_contigs, n_bases;
+    int k;
+};
+
+// Key of the window at g (~0 when it leaves its contig or holds an N); c is a
+// contig cursor that only moves forward.
+__device__ inline uint64_t window_key_at(const WindowScan &w, int64_t g, int64_t &c, int64_t *loc) {
+    while (c + 1 < w.n_contigs && (int64_t)w.coff[c + 1] <= g) c++;
+    const int64_t p = g - (int64_t)w.coff[c];
+    const int64_t clen = (int64_t)(w.coff[c + 1] - w.coff[c]);
+    *loc = (c << 32) | p;
+    if (p + w.k > clen) return ~0ull;
+    if ((window32(w.refN, g) >> (64 - 2 * w.k)) != 0) return ~0ull;
+    return window32(w.ref2, g) >> (64 - 2 * w.k);
+}
+
+__device__ inline int64_t first_contig(const WindowScan &w, int64_t g) {
+    int64_t lo = 0, hi = w.n_contigs;  // last contig whose start <= g
+    while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)w.coff[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Histogram of valid windows by key >> shift (nbins <= 1024).
+__global__ void __launch_bounds__(256) part_hist_kernel(WindowScan w, int shift, int nbins,
+                                                        unsigned long long *__restrict__ hist) {
+    __shared__ unsigned int h[1024];
+    for (int i = threadIdx.x; i < nbins; i += 256) h[i] = 0;
+    __syncthreads();
+    const int64_t g0 = (int64_t)blockIdx.x * PART_TILE + (int64_t)threadIdx.x * PART_ITEMS;
+    if (g0 < w.n_bases) {
+        int64_t c = first_contig(w, g0), loc;
+        for (int j = 0; j < PART_ITEMS; j++) {
+            const int64_t g = g0 + j;
+            if (g >= w.n_bases) break;
+            const uint64_t key = window_key_at(w, g, c, &loc);
+            if (key != ~0ull) atomicAdd(&h[key >> shift], 1u);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbins; i += 256)
+        if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
+}
+
+// Per block: windows with key in [klo, khi).
+__global__ void __launch_bounds__(256) part_count_kernel(WindowScan w, uint64_t klo, uint64_t khi,
+                                                         uint32_t *__restrict__ counts) {
+    __shared__ uint64_t sh[4];
+    const int64_t g0 = (int64_t)blockIdx.x * PART_TILE + (int64_t)threadIdx.x * PART_ITEMS;
+    uint64_t cnt = 0;
+    if (g0 < w.n_bases) {
+        int64_t c = first_contig(w, g0), loc;
+        for (int j = 0; j < PART_ITEMS; j++) {
+            const int64_t g = g0 + j;
+            if (g >= w.n_bases) break;
+            const uint64_t key = window_key_at(w, g, c, &loc);
+            cnt += (key >= klo && key < khi) ? 1 : 0;
+        }
+    }
+    uint64_t tot;
+    (void)block_excl_scan_256(cnt, sh, tot);
+    if (threadIdx.x == 0) counts[blockIdx.x] = (uint32_t)tot;
+}
+
+// The same windows, written in position order at the block's scanned offset.
+__global__ void __launch_bounds__(256) part_write_kernel(WindowScan w, uint64_t klo, uint64_t khi,
+                                                         const uint64_t *__restrict__ block_off,
+                                                         uint64_t *__restrict__ keys, uint64_t *__restrict__ locs) {
+    __shared__ uint64_t sh[4];
+    const int64_t g0 = (int64_t)blockIdx.x * PART_TILE + (int64_t)threadIdx.x * PART_ITEMS;
+    uint64_t cnt = 0;
+    int64_t c0 = 0;
+    if (g0 < w.n_bases) {
+        int64_t c = first_contig(w, g0), loc;
+        c0 = c;
+        for (int j = 0; j < PART_ITEMS; j++) {
+            const int64_t g = g0 + j;
+            if (g >= w.n_bases) break;
+            const uint64_t key = window_key_at(w, g, c, &loc);
+            cnt += (key >= klo && key < khi) ? 1 : 0;
+        }
+    }
+    uint64_t tot;
+    uint64_t off = block_off[blockIdx.x] + block_excl_scan_256(cnt, sh, tot);
+    if (g0 < w.n_bases && cnt) {
+        int64_t c = c0, loc;
+        for (int j = 0; j < PART_ITEMS; j++) {
+            const int64_t g = g0 + j;
+            if (g >= w.n_bases) break;
+            const uint64_t key = window_key_at(w, g, c, &loc);
+            if (key >= klo && key < khi) {
+                keys[off] = key;
+                locs[off] = (uint64_t)loc;
+                off++;
+            }
+        }
+    }
+}
+
+__global__ void add_u64_kernel(uint64_t *__restrict__ dst, const uint64_t *__restrict__ src, int64_t n,
+                               uint64_t add) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i] + add;
+}
+
 // Claim an empty slot of bucket `bk` for (key, val); false when it is full.
 __device__ bool claim_slot(HashSlot *bk, uint64_t key, uint64_t val) {
     for (int s = 0; s < BUCKET_SLOTS; s++) {
@@ -128,25 +217,26 @@ __device__ bool claim_slot(HashSlot *bk, uint64_t key, uint64_t val) {
     return false;
 }
 
-__device__ uint64_t slot_value(const uint64_t *offsets, int64_t i) {
-    uint64_t start = offsets[i];
+__device__ uint64_t slot_value(const uint64_t *offsets, int64_t i, uint64_t start_add) {
+    const uint64_t start = offsets[i];
     uint64_t occ = offsets[i + 1] - start;
     if (occ > OCC_SAT) occ = OCC_SAT;
-    return (occ << 40) | (start & START_MASK);
+    return (occ << 40) | ((start + start_add) & START_MASK);
 }
 
-// Parallel two-choice insert: the less occupied of a key's two buckets first
-// (a racy hint, exactness does not depend on it), then the other. Keys that
-// find both full are listed for the cuckoo pass.
+// Parallel two-choice insert of keys [0, n) (posting starts offsets[i] +
+// start_add): the less occupied of a key's two buckets first (a racy hint,
+// exactness does not depend on it), then the other. Keys that find both full
+// are listed, with their values, for the cuckoo pass.
 __global__ void hash_insert_kernel(const uint64_t *__restrict__ kmers,
-                                   const uint64_t *__restrict__ offsets, int64_t n_keys,
+                                   const uint64_t *__restrict__ offsets, int64_t n_keys, uint64_t start_add,
                                    HashSlot *__restrict__ table, uint64_t n_buckets,
-                                   int64_t *__restrict__ overflow, unsigned int *__restrict__ n_overflow,
-                                   unsigned int overflow_cap) {
+                                   uint64_t *__restrict__ ov_keys, uint64_t *__restrict__ ov_vals,
+                                   unsigned int *__restrict__ n_overflow, unsigned int overflow_cap) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_keys;
          i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = kmers[i];
-        const uint64_t val = slot_value(offsets, i);
+        const uint64_t val = slot_value(offsets, i, start_add);
         const uint64_t b1 = bucket_of(key, n_buckets);
         const uint64_t b2 = bucket2_of(key, n_buckets, b1);
         int o1 = 0, o2 = 0;
@@ -157,20 +247,23 @@ __global__ void hash_insert_kernel(const uint64_t *__restrict__ kmers,
         const uint64_t first = o2 < o1 ? b2 : b1, second = o2 < o1 ? b1 : b2;
         if (claim_slot(table + first * BUCKET_SLOTS, key, val)) continue;
         if (claim_slot(table + second * BUCKET_SLOTS, key, val)) continue;
-        unsigned int j = atomicAdd(n_overflow, 1u);
-        if (j < overflow_cap) overflow[j] = i;
+        const unsigned int j = atomicAdd(n_overflow, 1u);
+        if (j < overflow_cap) {
+            ov_keys[j] = key;
+            ov_vals[j] = val;
+        }
     }
 }
 
-// Cuckoo placement of the listed keys, one thread, after the parallel pass:
+// Cuckoo placement of the listed keys, one thread, after the parallel passes:
 // a key goes into a free slot of its current bucket or displaces a resident
 // (slot chosen by a step hash), which then moves to its own other bucket.
 // Every loop is bounded by `step_budget` in total; running out sets *fail and
-// the host rebuilds with more buckets.
-__global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ kmers, const uint64_t *__restrict__ offsets,
+// the host rebuilds at a lower load.
+__global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ ov_keys, const uint64_t *__restrict__ ov_vals,
                                    HashSlot *__restrict__ table, uint64_t n_buckets,
-                                   const int64_t *__restrict__ overflow, const unsigned int *__restrict__ n_overflow,
-                                   unsigned int overflow_cap, int64_t step_budget, int *__restrict__ fail) {
+                                   const unsigned int *__restrict__ n_overflow, unsigned int overflow_cap,
+                                   int64_t step_budget, int *__restrict__ fail) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     const unsigned int n = *n_overflow;
     if (n > overflow_cap) {
@@ -179,8 +272,8 @@ __global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ kmers, const uin
     }
     int64_t steps = 0;
     for (unsigned int j = 0; j < n; j++) {
-        uint64_t key = kmers[overflow[j]];
-        uint64_t val = slot_value(offsets, overflow[j]);
+        uint64_t key = ov_keys[j];
+        uint64_t val = ov_vals[j];
         uint64_t b = bucket_of(key, n_buckets);
         bool placed = false;
         while (!placed) {
@@ -251,49 +344,81 @@ int pack_reference(ma_index *ix, const uint8_t *seq, int64_t n_bases, const uint
     return MA_OK;
 }
 
-// Two-choice hash table over the resident kmers/offsets at load 0.5 (0.4,
-// then 0.3, if the cuckoo pass runs out of steps). MIALIGN_HASH_LOAD in
-// (0, 0.95] sets the first load (a tuning and test knob: high loads send many
-// keys through the cuckoo pass).
-int build_hash(ma_index *ix, hipStream_t st) {
-    const int64_t nk = ix->n_keys;
-    constexpr unsigned int OVERFLOW_CAP = 1u << 16;
-    constexpr int64_t STEP_BUDGET = 1ll << 22;
-    DevBuf d_misc, d_over;
-    MA_TRY(d_misc.alloc(256));
-    MA_TRY(d_over.alloc((size_t)OVERFLOW_CAP * 8));
-    unsigned int *d_nover = d_misc.as<unsigned int>();
-    int *d_fail = d_misc.as<int>() + 1;
-    double first = 0.5;
-    if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
-        const double v = atof(env);
-        MA_REQUIRE(v > 0 && v <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
-        first = v;
-    }
-    for (double load : {first, 0.4, 0.3}) {
-        if (load > first) continue;
-        const uint64_t n_buckets = (uint64_t)((double)nk / (BUCKET_SLOTS * load)) + 1;
+// Two-choice hash table, built in three steps: begin (size and clear the
+// table for n_keys at a load), insert (any number of key ranges, each with its
+// posting-start shift), finish (cuckoo pass; false when it ran out of steps).
+struct HashBuilder {
+    static constexpr unsigned int OVERFLOW_CAP = 1u << 20;
+    static constexpr int64_t STEP_BUDGET = 1ll << 22;
+    ma_index *ix = nullptr;
+    hipStream_t st = nullptr;
+    DevBuf misc, ov_keys, ov_vals;
+
+    int begin(ma_index *index, hipStream_t stream, int64_t n_keys, double load) {
+        ix = index;
+        st = stream;
+        if (!misc.p) {
+            MA_TRY(misc.alloc(256));
+            MA_TRY(ov_keys.alloc((size_t)OVERFLOW_CAP * 8));
+            MA_TRY(ov_vals.alloc((size_t)OVERFLOW_CAP * 8));
+        }
+        const uint64_t n_buckets = (uint64_t)((double)n_keys / (BUCKET_SLOTS * load)) + 1;
         ix->n_buckets = n_buckets;
+        ix->table.release();
         MA_TRY(ix->table.alloc((size_t)n_buckets * BUCKET_SLOTS * sizeof(HashSlot)));
         const int64_t n_slots = (int64_t)n_buckets * BUCKET_SLOTS;
         fill_slots_kernel<<<grid_for(n_slots), 256, 0, st>>>(ix->table.as<HashSlot>(), n_slots);
         MA_LAUNCHED();
-        MA_TRY(fill_u64(st, d_misc.as<uint64_t>(), 1, 0));
-        if (nk > 0) {
-            hash_insert_kernel<<<grid_for(nk), 256, 0, st>>>(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(),
-                                                             nk, ix->table.as<HashSlot>(), n_buckets,
-                                                             d_over.as<int64_t>(), d_nover, OVERFLOW_CAP);
-            MA_LAUNCHED();
-            hash_cuckoo_kernel<<<1, 64, 0, st>>>(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(),
-                                                 ix->table.as<HashSlot>(), n_buckets, d_over.as<int64_t>(),
-                                                 d_nover, OVERFLOW_CAP, STEP_BUDGET, d_fail);
-            MA_LAUNCHED();
-        }
+        return fill_u64(st, misc.as<uint64_t>(), 1, 0);
+    }
+    int insert(const uint64_t *kmers, const uint64_t *offsets, int64_t n, uint64_t start_add) {
+        if (n <= 0) return MA_OK;
+        hash_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<HashSlot>(),
+                                                        ix->n_buckets, ov_keys.as<uint64_t>(),
+                                                        ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
+                                                        OVERFLOW_CAP);
+        MA_LAUNCHED();
+        return MA_OK;
+    }
+    int finish(bool *ok) {
+        hash_cuckoo_kernel<<<1, 64, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(), ix->table.as<HashSlot>(),
+                                             ix->n_buckets, misc.as<unsigned int>(), OVERFLOW_CAP, STEP_BUDGET,
+                                             misc.as<int>() + 1);
+        MA_LAUNCHED();
         int h[2] = {0, 0};
-        MA_HIP(hipMemcpyAsync(h, d_misc.p, 8, hipMemcpyDeviceToHost, st));
+        MA_HIP(hipMemcpyAsync(h, misc.p, 8, hipMemcpyDeviceToHost, st));
         MA_HIP(hipStreamSynchronize(st));
         ix->n_cuckoo = (int64_t)h[0];
-        if (!h[1]) return MA_OK;
+        *ok = h[1] == 0;
+        return MA_OK;
+    }
+};
+
+// First table load: 0.5, or MIALIGN_HASH_LOAD in (0, 0.95] (a tuning and test
+// knob: high loads send many keys through the cuckoo pass).
+int first_hash_load(double *load) {
+    *load = 0.5;
+    if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
+        const double v = atof(env);
+        MA_REQUIRE(v > 0 && v <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
+        *load = v;
+    }
+    return MA_OK;
+}
+
+// Table over the resident kmers/offsets: the first load, then 0.4 and 0.3 if
+// the cuckoo pass runs out of steps.
+int build_hash(ma_index *ix, hipStream_t st) {
+    double first;
+    MA_TRY(first_hash_load(&first));
+    HashBuilder hb;
+    for (double load : {first, 0.4, 0.3}) {
+        if (load > first) continue;
+        MA_TRY(hb.begin(ix, st, ix->n_keys, load));
+        MA_TRY(hb.insert(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(), ix->n_keys, 0));
+        bool ok = false;
+        MA_TRY(hb.finish(&ok));
+        if (ok) return MA_OK;
     }
     set_error("hash index build: cuckoo placement did not converge");
     return MA_ERR_NOMEM;
@@ -423,90 +548,249 @@ int ma_index_from_dense(int device, int32_t k, const uint64_t *kmers, int64_t n_
     return MA_OK;
 }
 
+void ma_build_opts_default(ma_build_opts *o) {
+    if (!o) return;
+    o->keep_dense = 1;
+    o->partition_bits = 10;
+    o->chunk_windows = 0;
+}
+
 int ma_index_build(int device, int32_t k, const uint8_t *seq, int64_t n_bases,
                    const uint64_t *contig_off, int64_t n_contigs, ma_index **out) {
+    ma_build_opts o;
+    ma_build_opts_default(&o);
+    return ma_index_build_opts(device, k, seq, n_bases, contig_off, n_contigs, &o, out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Consecutive partitions grouped into chunks of at most `cap` windows (a
+// partition larger than cap is a chunk of its own).
+std::vector<std::pair<int, int>> plan_chunks(const std::vector<uint64_t> &hist, uint64_t cap) {
+    std::vector<std::pair<int, int>> chunks;
+    int b = 0;
+    const int nb = (int)hist.size();
+    while (b < nb) {
+        uint64_t sum = hist[b];
+        int e = b + 1;
+        while (e < nb && sum + hist[e] <= cap) sum += hist[e++];
+        chunks.push_back({b, e});
+        b = e;
+    }
+    return chunks;
+}
+
+// Resident buffers of one chunk: collected pairs, sort alternates, radix and
+// scan scratch, and the chunk's CSR (kmers / offsets relative to the chunk).
+struct ChunkBufs {
+    DevBuf K0, K1, V0, V1, heads, excl, part, bh, bo, h8, km, of;
+    int reserve(int64_t n) {
+        const int64_t m = n > 0 ? n : 1;
+        const int64_t nb = rs_nblocks(m);
+        MA_TRY(K0.reserve((size_t)m * 8));
+        MA_TRY(K1.reserve((size_t)m * 8));
+        MA_TRY(V0.reserve((size_t)m * 8));
+        MA_TRY(V1.reserve((size_t)m * 8));
+        MA_TRY(heads.reserve((size_t)m * 4));
+        MA_TRY(excl.reserve((size_t)m * 8));
+        MA_TRY(part.reserve(scan_scratch_bytes(m)));
+        MA_TRY(bh.reserve((size_t)(256 * nb * 4)));
+        MA_TRY(bo.reserve((size_t)(256 * nb * 8)));
+        MA_TRY(h8.reserve(8 * 256 * 8));
+        MA_TRY(km.reserve((size_t)m * 8));
+        MA_TRY(of.reserve((size_t)(m + 1) * 8));
+        return MA_OK;
+    }
+    void release() {
+        for (DevBuf *b : {&K0, &K1, &V0, &V1, &heads, &excl, &part, &bh, &bo, &h8, &km, &of}) b->release();
+    }
+};
+constexpr int64_t CHUNK_BYTES_PER_WINDOW = 8 * 4 + 4 + 8 + 8 + 16 + 1;
+
+// Collect the windows of key range [klo, khi) in position order and sort
+// them by key (stable). Returns the window count through *nc; sorted keys
+// and locations end in (*kp, *vp).
+int collect_sorted(const WindowScan &w, uint64_t klo, uint64_t khi, int64_t nc, ChunkBufs &cb, DevBuf &blk,
+                   DevBuf &blk_off, DevBuf &blk_part, int64_t nblk, hipStream_t st, uint64_t **kp, uint64_t **vp) {
+    part_count_kernel<<<(unsigned)nblk, 256, 0, st>>>(w, klo, khi, blk.as<uint32_t>());
+    MA_LAUNCHED();
+    MA_TRY(exclusive_scan_u32(st, nblk, blk.as<uint32_t>(), blk_off.as<uint64_t>(), blk_part.as<uint64_t>(),
+                              nullptr));
+    part_write_kernel<<<(unsigned)nblk, 256, 0, st>>>(w, klo, khi, blk_off.as<uint64_t>(), cb.K0.as<uint64_t>(),
+                                                      cb.V0.as<uint64_t>());
+    MA_LAUNCHED();
+    uint64_t *k0 = cb.K0.as<uint64_t>(), *k1 = cb.K1.as<uint64_t>();
+    uint64_t *v0 = cb.V0.as<uint64_t>(), *v1 = cb.V1.as<uint64_t>();
+    RadixScratch rs{cb.bh.as<uint32_t>(), cb.bo.as<uint64_t>(), cb.h8.as<unsigned long long>()};
+    MA_TRY(radix_sort_pairs<uint64_t>(st, nc, k0, v0, k1, v1, rs, 2 * w.k));
+    *kp = k0;
+    *vp = v0;
+    return MA_OK;
+}
+
+// Heads + exclusive scan over nc sorted keys; returns the unique-key count.
+int count_keys(int64_t nc, const uint64_t *kp, ChunkBufs &cb, hipStream_t st, int64_t *n_keys) {
+    *n_keys = 0;
+    if (nc <= 0) return MA_OK;
+    key_heads_kernel<<<grid_for(nc), 256, 0, st>>>(kp, nc, cb.heads.as<uint32_t>());
+    MA_LAUNCHED();
+    uint64_t *partial = cb.part.as<uint64_t>();
+    MA_TRY(exclusive_scan_u32(st, nc, cb.heads.as<uint32_t>(), cb.excl.as<uint64_t>(), partial,
+                              partial + rs_nblocks(nc)));
+    uint64_t nk = 0;
+    MA_HIP(hipMemcpyAsync(&nk, partial + rs_nblocks(nc), 8, hipMemcpyDeviceToHost, st));
+    MA_HIP(hipStreamSynchronize(st));
+    *n_keys = (int64_t)nk;
+    return MA_OK;
+}
+
+int64_t free_hbm() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return (int64_t)fr;
+}
+
+// The dense-v1 arrays and the hash table of the packed reference in ix, built
+// one key-range chunk at a time. Sweep 1 counts the unique keys (to size the
+// table); sweep 2 writes each chunk's postings slice and inserts its keys.
+int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
+    const WindowScan w{ix->ref2.as<uint64_t>(), ix->refN.as<uint64_t>(), ix->coff.as<uint64_t>(), ix->n_contigs,
+                       ix->n_bases, k};
+    const int bits = o.partition_bits < 2 * k ? o.partition_bits : 2 * k;
+    const int shift = 2 * k - bits;
+    const int nbins = 1 << bits;
+    const int64_t nblk = (ix->n_bases + PART_TILE - 1) / PART_TILE;
+    MA_REQUIRE(nblk < (1ll << 31), "index build: reference too long");
+    DevBuf d_hist, blk, blk_off, blk_part;
+    MA_TRY(d_hist.alloc((size_t)nbins * 8));
+    MA_TRY(blk.alloc((size_t)nblk * 4));
+    MA_TRY(blk_off.alloc((size_t)nblk * 8));
+    MA_TRY(blk_part.alloc(scan_scratch_bytes(nblk)));
+    MA_TRY(fill_u64(st, d_hist.as<uint64_t>(), nbins, 0));
+    part_hist_kernel<<<(unsigned)nblk, 256, 0, st>>>(w, shift, nbins, d_hist.as<unsigned long long>());
+    MA_LAUNCHED();
+    std::vector<uint64_t> hist((size_t)nbins);
+    MA_HIP(hipMemcpyAsync(hist.data(), d_hist.p, (size_t)nbins * 8, hipMemcpyDeviceToHost, st));
+    MA_HIP(hipStreamSynchronize(st));
+    uint64_t n_valid = 0, max_bin = 0;
+    for (uint64_t h : hist) {
+        n_valid += h;
+        max_bin = h > max_bin ? h : max_bin;
+    }
+    ix->n_postings = (int64_t)n_valid;
+    auto cap_for = [&](int64_t reserve_bytes) -> uint64_t {
+        if (o.chunk_windows > 0) return (uint64_t)o.chunk_windows;
+        int64_t room = (int64_t)(0.85 * (double)free_hbm()) - reserve_bytes;
+        int64_t c = room / CHUNK_BYTES_PER_WINDOW;
+        return c > 0 ? (uint64_t)c : 0;
+    };
+    auto key_lo = [&](int b) -> uint64_t { return (uint64_t)b << shift; };
+
+    // sweep 1: unique keys per chunk
+    ChunkBufs cb;
+    int64_t n_keys = 0;
+    {
+        uint64_t cap = cap_for(0);
+        if (cap < max_bin) cap = max_bin;  // a partition never splits (fails below if it cannot fit)
+        const auto chunks = plan_chunks(hist, cap);
+        for (const auto &ch : chunks) {
+            uint64_t nc = 0;
+            for (int b = ch.first; b < ch.second; b++) nc += hist[b];
+            if (nc == 0) continue;
+            MA_TRY(cb.reserve((int64_t)nc));
+            uint64_t *kp, *vp;
+            MA_TRY(collect_sorted(w, key_lo(ch.first), key_lo(ch.second), (int64_t)nc, cb, blk, blk_off, blk_part,
+                                  nblk, st, &kp, &vp));
+            int64_t nk = 0;
+            MA_TRY(count_keys((int64_t)nc, kp, cb, st, &nk));
+            n_keys += nk;
+        }
+    }
+    ix->n_keys = n_keys;
+    cb.release();  // sweep-1 buffers go before the resident arrays are allocated
+    MA_TRY(ix->postings.alloc((size_t)(n_valid > 0 ? n_valid : 1) * 8));
+    if (o.keep_dense) {
+        MA_TRY(ix->kmers.alloc((size_t)(n_keys > 0 ? n_keys : 1) * 8));
+        MA_TRY(ix->offsets.alloc((size_t)(n_keys + 1) * 8));
+    }
+    double first;
+    MA_TRY(first_hash_load(&first));
+    HashBuilder hb;
+    for (double load : {first, 0.4, 0.3}) {
+        if (load > first) continue;
+        MA_TRY(hb.begin(ix, st, n_keys, load));
+        // sweep 2: postings slices, table inserts, dense-v1 slices
+        uint64_t cap = cap_for(0);
+        if (cap < max_bin) cap = max_bin;
+        const auto chunks = plan_chunks(hist, cap);
+        uint64_t post_base = 0, key_base = 0;
+        for (const auto &ch : chunks) {
+            uint64_t nc = 0;
+            for (int b = ch.first; b < ch.second; b++) nc += hist[b];
+            if (nc == 0) continue;
+            MA_TRY(cb.reserve((int64_t)nc));
+            uint64_t *kp, *vp;
+            MA_TRY(collect_sorted(w, key_lo(ch.first), key_lo(ch.second), (int64_t)nc, cb, blk, blk_off, blk_part,
+                                  nblk, st, &kp, &vp));
+            int64_t nk = 0;
+            MA_TRY(count_keys((int64_t)nc, kp, cb, st, &nk));
+            csr_kernel<<<grid_for((int64_t)nc), 256, 0, st>>>(kp, vp, cb.heads.as<uint32_t>(), cb.excl.as<uint64_t>(),
+                                                              (int64_t)nc, cb.km.as<uint64_t>(), cb.of.as<uint64_t>(),
+                                                              ix->postings.as<uint32_t>() + 2 * post_base);
+            MA_LAUNCHED();
+            set_u64_kernel<<<1, 1, 0, st>>>(cb.of.as<uint64_t>() + nk, nc);
+            MA_LAUNCHED();
+            MA_TRY(hb.insert(cb.km.as<uint64_t>(), cb.of.as<uint64_t>(), nk, post_base));
+            if (o.keep_dense) {
+                MA_HIP(hipMemcpyAsync(ix->kmers.as<uint64_t>() + key_base, cb.km.p, (size_t)nk * 8,
+                                      hipMemcpyDeviceToDevice, st));
+                add_u64_kernel<<<grid_for(nk), 256, 0, st>>>(ix->offsets.as<uint64_t>() + key_base,
+                                                             cb.of.as<uint64_t>(), nk, post_base);
+                MA_LAUNCHED();
+            }
+            post_base += nc;
+            key_base += (uint64_t)nk;
+        }
+        if (o.keep_dense) {
+            set_u64_kernel<<<1, 1, 0, st>>>(ix->offsets.as<uint64_t>() + n_keys, n_valid);
+            MA_LAUNCHED();
+        }
+        bool ok = false;
+        MA_TRY(hb.finish(&ok));
+        if (ok) return MA_OK;
+    }
+    set_error("hash index build: cuckoo placement did not converge");
+    return MA_ERR_NOMEM;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ma_index_build_opts(int device, int32_t k, const uint8_t *seq, int64_t n_bases, const uint64_t *contig_off,
+                        int64_t n_contigs, const ma_build_opts *opts, ma_index **out) {
     MA_REQUIRE(k >= 1 && k <= 31, "ma_index_build: k must be in 1..31");
-    MA_REQUIRE(n_bases > 0 && n_contigs >= 1 && seq && contig_off, "ma_index_build: bad arguments");
+    MA_REQUIRE(n_bases > 0 && n_contigs >= 1 && seq && contig_off && out, "ma_index_build: bad arguments");
     MA_REQUIRE((int64_t)contig_off[n_contigs] == n_bases, "ma_index_build: contig_off[n] != n_bases");
     MA_REQUIRE(n_contigs < (1ll << 32), "ma_index_build: too many contigs");
+    ma_build_opts o;
+    ma_build_opts_default(&o);
+    if (opts) o = *opts;
+    MA_REQUIRE(o.partition_bits >= 1 && o.partition_bits <= 10 && o.chunk_windows >= 0,
+               "ma_index_build: partition_bits must be in 1..10, chunk_windows >= 0");
     ma_index *ix = nullptr;
     MA_TRY(index_alloc(device, &ix));
     hipStream_t st = ix->stream;
     ix->k = k;
-    int rc = MA_OK;
-    do {
-        if ((rc = pack_reference(ix, seq, n_bases, contig_off, n_contigs, st)) != MA_OK) break;
-        int64_t n = n_bases;
-        DevBuf K0, K1, V0, V1, heads, excl, partial, bh, bo, h8, cnt;
-        int64_t nb = rs_nblocks(n);
-        if ((rc = K0.alloc((size_t)n * 8)) || (rc = K1.alloc((size_t)n * 8)) ||
-            (rc = V0.alloc((size_t)n * 8)) || (rc = V1.alloc((size_t)n * 8)) ||
-            (rc = bh.alloc((size_t)(256 * nb * 4))) || (rc = bo.alloc((size_t)(256 * nb * 8))) ||
-            (rc = h8.alloc(8 * 256 * 8)) || (rc = cnt.alloc(256)))
-            break;
-        if ((rc = fill_u64(st, cnt.as<uint64_t>(), 1, 0)) != MA_OK) break;
-        window_keys_kernel<<<grid_for(n), 256, 0, st>>>(
-            ix->ref2.as<uint64_t>(), ix->refN.as<uint64_t>(), ix->coff.as<uint64_t>(), n_contigs,
-            n_bases, k, K0.as<uint64_t>(), V0.as<uint64_t>(), cnt.as<unsigned long long>());
-        if (hipGetLastError() != hipSuccess) {
-            set_error("window_keys_kernel launch failed");
-            rc = MA_ERR_HIP;
-            break;
-        }
-        unsigned long long n_valid = 0;
-        if (hipMemcpyAsync(&n_valid, cnt.p, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-            hipStreamSynchronize(st) != hipSuccess) {
-            set_error("index build: count readback failed");
-            rc = MA_ERR_HIP;
-            break;
-        }
-        RadixScratch rs{bh.as<uint32_t>(), bo.as<uint64_t>(), h8.as<unsigned long long>()};
-        uint64_t *kp = K0.as<uint64_t>(), *ka = K1.as<uint64_t>();
-        uint64_t *vp = V0.as<uint64_t>(), *va = V1.as<uint64_t>();
-        if ((rc = radix_sort_pairs<uint64_t>(st, n, kp, vp, ka, va, rs, 2 * k + 1)) != MA_OK) break;
-        bh.release();
-        bo.release();
-        int64_t nv = (int64_t)n_valid;
-        ix->n_postings = nv;
-        if ((rc = heads.alloc((size_t)(nv > 0 ? nv : 1) * 4)) || (rc = excl.alloc((size_t)(nv > 0 ? nv : 1) * 8)) ||
-            (rc = partial.alloc(scan_scratch_bytes(nv > 0 ? nv : 1))))
-            break;
-        uint64_t n_keys = 0;
-        if (nv > 0) {
-            key_heads_kernel<<<grid_for(nv), 256, 0, st>>>(kp, nv, heads.as<uint32_t>());
-            if ((rc = exclusive_scan_u32(st, nv, heads.as<uint32_t>(), excl.as<uint64_t>(),
-                                         partial.as<uint64_t>(), partial.as<uint64_t>() + rs_nblocks(nv))) != MA_OK)
-                break;
-            if (hipMemcpyAsync(&n_keys, partial.as<uint64_t>() + rs_nblocks(nv), 8,
-                               hipMemcpyDeviceToHost, st) != hipSuccess ||
-                hipStreamSynchronize(st) != hipSuccess) {
-                set_error("index build: key count readback failed");
-                rc = MA_ERR_HIP;
-                break;
-            }
-        }
-        ix->n_keys = (int64_t)n_keys;
-        if ((rc = ix->kmers.alloc((size_t)n_keys * 8)) || (rc = ix->offsets.alloc((size_t)(n_keys + 1) * 8)) ||
-            (rc = ix->postings.alloc((size_t)nv * 8)))
-            break;
-        if (nv > 0) {
-            csr_kernel<<<grid_for(nv), 256, 0, st>>>(kp, vp, heads.as<uint32_t>(), excl.as<uint64_t>(), nv,
-                                                     ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(),
-                                                     ix->postings.as<uint32_t>());
-        }
-        set_u64_kernel<<<1, 1, 0, st>>>(ix->offsets.as<uint64_t>() + n_keys, (uint64_t)nv);
-        if (hipGetLastError() != hipSuccess) {
-            set_error("csr_kernel launch failed");
-            rc = MA_ERR_HIP;
-            break;
-        }
-        if (hipStreamSynchronize(st) != hipSuccess) {
-            set_error("index build failed on device");
-            rc = MA_ERR_HIP;
-            break;
-        }
-    } while (0);
-    if (rc == MA_OK) rc = build_hash(ix, st);
+    ix->keep_dense = o.keep_dense ? 1 : 0;
+    int rc = pack_reference(ix, seq, n_bases, contig_off, n_contigs, st);
+    if (rc == MA_OK) rc = build_chunked(ix, k, o, st);
+    if (rc == MA_OK && hipStreamSynchronize(st) != hipSuccess) {
+        set_error("index build failed on device");
+        rc = MA_ERR_HIP;
+    }
     if (rc != MA_OK) {
         ma_index_close(ix);
         return rc;
@@ -576,6 +860,10 @@ int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cucko
 
 int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings) {
     MA_REQUIRE(ix, "ma_index_export_dense: null index");
+    if (!ix->keep_dense && (kmers || offsets)) {
+        set_error("ma_index_export_dense: index built without resident kmers/offsets (keep_dense = 0)");
+        return MA_ERR_UNSUPPORTED;
+    }
     MA_HIP(hipSetDevice(ix->device));
     hipStream_t st = ix->stream;
     if (kmers && ix->n_keys)

@@ -44,6 +44,10 @@ class Params(C.Structure):
     ]
 
 
+class BuildOpts(C.Structure):
+    _fields_ = [("keep_dense", C.c_int32), ("partition_bits", C.c_int32), ("chunk_windows", C.c_int64)]
+
+
 class Recs(C.Structure):
     _fields_ = [
         ("cid", C.c_void_p),
@@ -104,6 +108,8 @@ SIGNATURES = {
     "ma_index_open": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(P)]),
     "ma_index_from_dense": (C.c_int, [C.c_int, I32, P, I64, P, P, I64, P, I64, P, I64, C.POINTER(P)]),
     "ma_index_build": (C.c_int, [C.c_int, I32, P, I64, P, I64, C.POINTER(P)]),
+    "ma_build_opts_default": (None, [C.POINTER(BuildOpts)]),
+    "ma_index_build_opts": (C.c_int, [C.c_int, I32, P, I64, P, I64, C.POINTER(BuildOpts), C.POINTER(P)]),
     "ma_index_stats": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
                                  C.POINTER(I64), C.POINTER(I32)]),
     "ma_index_hash_stats": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64)]),

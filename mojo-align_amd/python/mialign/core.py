@@ -9,6 +9,7 @@ from typing import Optional
 import numpy as np
 
 from ._lib import (
+    BuildOpts,
     MA_ENC_2BIT,
     MA_ENC_ASCII,
     REC_DTYPES,
@@ -156,13 +157,18 @@ class GpuIndex:
         return cls(h.value, device)
 
     @classmethod
-    def build(cls, seq: np.ndarray, contig_off: np.ndarray, k: int, device: int = 0) -> "GpuIndex":
+    def build(cls, seq: np.ndarray, contig_off: np.ndarray, k: int, device: int = 0, *,
+              keep_dense: bool = True, partition_bits: int = 10, chunk_windows: int = 0) -> "GpuIndex":
+        """GPU dense-v1 build (one key-range chunk of windows at a time) + hash table."""
         seq = np.ascontiguousarray(seq, dtype=np.uint8)
         contig_off = np.ascontiguousarray(contig_off, dtype=np.uint64)
+        opts = BuildOpts()
+        lib().ma_build_opts_default(C.byref(opts))
+        opts.keep_dense, opts.partition_bits, opts.chunk_windows = int(keep_dense), int(partition_bits), int(chunk_windows)
         h = C.c_void_p()
         check(
-            lib().ma_index_build(device, k, ptr(seq), seq.size, ptr(contig_off), contig_off.size - 1,
-                                 C.byref(h)),
+            lib().ma_index_build_opts(device, k, ptr(seq), seq.size, ptr(contig_off), contig_off.size - 1,
+                                      C.byref(opts), C.byref(h)),
             "ma_index_build",
         )
         return cls(h.value, device)

@@ -78,6 +78,26 @@ def test_index_build_matches_oracle(w1):
     assert np.array_equal(gp, po)
 
 
+def test_index_build_in_key_range_chunks(w1):
+    """The build sorts one key-range chunk of windows at a time: many small
+    chunks, a coarse 2-partition split and one chunk all give the oracle's
+    dense-v1 bytes; a lean index (no resident kmers/offsets) maps the same."""
+    say("test_index_build_in_key_range_chunks")
+    km, of, po = w1.dense
+    for bits, cw in ((10, 50_000), (3, 300_000), (1, 0)):
+        g = mialign.GpuIndex.build(w1.dual.seq, w1.dual.contig_off, 21, partition_bits=bits, chunk_windows=cw)
+        gk, go, gp = g.export_dense()
+        assert np.array_equal(gk, km) and np.array_equal(go, of) and np.array_equal(gp, po), (bits, cw)
+    lean = mialign.GpuIndex.build(w1.dual.seq, w1.dual.contig_off, 21, keep_dense=False, chunk_windows=100_000)
+    assert lean.n_keys == km.size and lean.n_postings == po.size // 2
+    with pytest.raises(mialign.MialignError):
+        lean.export_dense()
+    reads = synth.gen_reads(22, w1.ref, 5000)
+    got = mialign.map_batch(lean, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
+    want, _, _, _ = w1.oracle_map(reads)
+    assert_recs_equal(got, want, "lean chunked index vs oracle")
+
+
 def test_index_build_with_n_runs():
     say("test_index_build_with_n_runs")
     ref = synth.gen_reference(3, 3, 300_000)
