@@ -11,7 +11,7 @@
  * vote_occ = vote_fast = 256, max_diff 6, max_soft 8, RC on; indel_cap is a
  * parameter (0 = gapless contract).
  *
- * Parity status per function (how each is pinned by tests/test_oracle_*.py):
+ * Parity status per function (how each is pinned by tests/test_oracle.py):
  *   or_code_of / or_pack_bases / or_bs_convert ..... reference known answers +
  *                                                    closed form (per-base table)
  *   or_wang_hash ..................................... published integer hash;
@@ -25,6 +25,8 @@
  *   or_sort_keys_se .................................. closed-form key formulas
  *   or_sort_markdup .................................. reference known answers
  *                                                    (test_bam_emit.py) + brute force
+ *   or_recount_nm (a checker, no reference counterpart) ... an independent Python
+ *                                                    recount + planted one-base shifts
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -1077,6 +1079,41 @@ int or_sort_markdup(int64_t n, const uint64_t *coord, const uint64_t *dhi, const
     free(a);
     free(t);
     return 0;
+}
+
+/* Checker for reported records (no reference counterpart): mismatches between
+ * each converted read, reverse-complemented when its flag has 16, and the
+ * dual reference at (cid, pos) over the aligned bases [sl, L - sr); a base
+ * mismatches when either side is not ACGT or the codes differ (the rule of
+ * linear_gpu_locate.mojo's gapless extend). -1 for unmapped records, -2 for a
+ * window outside its contig. */
+void or_recount_nm(const uint8_t *seq, const uint64_t *contig_off, int64_t n_contigs, const uint8_t *reads,
+                   int64_t stride, const uint32_t *lens, int64_t n, const int32_t *cid, const uint32_t *pos,
+                   const int32_t *flag, const uint32_t *sl, const uint32_t *sr, int32_t *nm_out) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        if (cid[i] < 0 || cid[i] >= n_contigs) {
+            nm_out[i] = -1;
+            continue;
+        }
+        const int64_t L = lens[i], a = sl[i], b = L - (int64_t)sr[i];
+        const int64_t c0 = (int64_t)contig_off[cid[i]], clen = (int64_t)contig_off[cid[i] + 1] - c0;
+        if (b < a || (int64_t)pos[i] + (b - a) > clen) {
+            nm_out[i] = -2;
+            continue;
+        }
+        const uint8_t *r = reads + i * stride;
+        const int rev = (flag[i] & 16) != 0;
+        int32_t nm = 0;
+        for (int64_t j = a; j < b; j++) {
+            uint8_t q = rev ? r[L - 1 - j] : r[j];
+            uint8_t cq = or_code_of(q);
+            if (rev && cq <= 3) cq = (uint8_t)(3 - cq);
+            const uint8_t cr = or_code_of(seq[c0 + (int64_t)pos[i] + (j - a)]);
+            nm += (cq > 3 || cr > 3 || cq != cr) ? 1 : 0;
+        }
+        nm_out[i] = nm;
+    }
 }
 
 int or_num_threads(void) {

@@ -99,6 +99,7 @@ def lib():
         L.or_sort_markdup.restype = C.c_int
         L.or_sort_markdup.argtypes = [I64, P, P, P, P, P, P, P, C.c_int, P]
         L.or_num_threads.restype = C.c_int
+        L.or_recount_nm.argtypes = [P, P, I64, P, I64, P, I64, P, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -155,6 +156,19 @@ class OracleIndex:
         self.struct = Index(_p(self.kmers), _p(self.offsets), _p(self.postings), self.kmers.size,
                             self.postings.size // 2, _p(self.ref_codes), _p(self.contig_off),
                             self.contig_off.size - 1)
+
+
+def recount_nm(seq, contig_off, conv_reads, lens, recs) -> np.ndarray:
+    """Mismatches of each record at its reported locus (-1 unmapped, -2 out of contig)."""
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    contig_off = np.ascontiguousarray(contig_off, dtype=np.uint64)
+    conv_reads = np.ascontiguousarray(conv_reads, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    f = {k: np.ascontiguousarray(recs[k]) for k in ("cid", "pos", "flag", "sl", "sr")}
+    out = np.empty(lens.size, dtype=np.int32)
+    lib().or_recount_nm(_p(seq), _p(contig_off), contig_off.size - 1, _p(conv_reads), conv_reads.shape[1], _p(lens),
+                        lens.size, _p(f["cid"]), _p(f["pos"]), _p(f["flag"]), _p(f["sl"]), _p(f["sr"]), _p(out))
+    return out
 
 
 def convert_reads(bases: np.ndarray, mode: int) -> np.ndarray:

@@ -190,6 +190,34 @@ def test_reported_nm_equals_recount(uniq):
             assert mg["mapq"][i] == 20
 
 
+def test_recount_nm_checker(uniq):
+    """or_recount_nm (the checker used at full size) equals the Python loop
+    above and every reported nm; a record moved by one base no longer does."""
+    ref, dual, ix = uniq
+    reads = synth.gen_reads(34, ref, 1500, 60, 150, p_err=0.02, max_len=150)
+    convd = orc.convert_reads(reads.bases, 1)
+    mg, _, _, _ = orc.map_batch(ix, convd, reads.lens)
+    got = orc.recount_nm(dual.seq, dual.contig_off, convd, reads.lens, mg)
+    seq = dual.seq.tobytes()
+    for i in range(0, reads.n, 7):
+        if mg["cid"][i] < 0:
+            assert got[i] == -1
+            continue
+        Lr = int(reads.lens[i])
+        q = convd[i, :Lr].tobytes()
+        if mg["flag"][i] == 16:
+            q = revcomp(q)
+        sl, sr = int(mg["sl"][i]), int(mg["sr"][i])
+        g = int(dual.contig_off[mg["cid"][i]]) + int(mg["pos"][i])
+        assert got[i] == sum(a != b for a, b in zip(q[sl : Lr - sr], seq[g : g + Lr - sl - sr])), i
+    m = mg["cid"] >= 0
+    assert m.mean() > 0.9 and np.array_equal(got[m], mg["nm"][m].astype(np.int32))
+    moved = dict(mg)
+    moved["pos"] = mg["pos"] + 1
+    off = orc.recount_nm(dual.seq, dual.contig_off, convd, reads.lens, moved)
+    assert (off[m] != got[m]).mean() > 0.9
+
+
 def _tiny_world(seq: bytes, k=21):
     ref = synth.Reference(np.frombuffer(seq, dtype=np.uint8).copy(),
                           np.array([0, len(seq)], dtype=np.uint64), ["chr1"])
