@@ -225,9 +225,11 @@ __device__ uint64_t slot_value(const uint64_t *offsets, int64_t i, uint64_t star
 }
 
 // Parallel two-choice insert of keys [0, n) (posting starts offsets[i] +
-// start_add): the less occupied of a key's two buckets first (a racy hint,
-// exactness does not depend on it), then the other. Keys that find both full
-// are listed, with their values, for the cuckoo pass.
+// start_add): the first bucket, then the second. A key is therefore in its
+// second bucket only while its first is full (slots are never emptied, and a
+// cuckoo move refills the slot it takes), which lets a lookup stop after the
+// first bucket unless that bucket is full. Keys that find both full are
+// listed, with their values, for the cuckoo pass.
 __global__ void hash_insert_kernel(const uint64_t *__restrict__ kmers,
                                    const uint64_t *__restrict__ offsets, int64_t n_keys, uint64_t start_add,
                                    HashSlot *__restrict__ table, uint64_t n_buckets,
@@ -239,14 +241,8 @@ __global__ void hash_insert_kernel(const uint64_t *__restrict__ kmers,
         const uint64_t val = slot_value(offsets, i, start_add);
         const uint64_t b1 = bucket_of(key, n_buckets);
         const uint64_t b2 = bucket2_of(key, n_buckets, b1);
-        int o1 = 0, o2 = 0;
-        for (int s = 0; s < BUCKET_SLOTS; s++) {
-            o1 += table[b1 * BUCKET_SLOTS + s].key != EMPTY_KEY;
-            o2 += table[b2 * BUCKET_SLOTS + s].key != EMPTY_KEY;
-        }
-        const uint64_t first = o2 < o1 ? b2 : b1, second = o2 < o1 ? b1 : b2;
-        if (claim_slot(table + first * BUCKET_SLOTS, key, val)) continue;
-        if (claim_slot(table + second * BUCKET_SLOTS, key, val)) continue;
+        if (claim_slot(table + b1 * BUCKET_SLOTS, key, val)) continue;
+        if (claim_slot(table + b2 * BUCKET_SLOTS, key, val)) continue;
         const unsigned int j = atomicAdd(n_overflow, 1u);
         if (j < overflow_cap) {
             ov_keys[j] = key;
@@ -364,6 +360,7 @@ struct HashBuilder {
         }
         const uint64_t n_buckets = (uint64_t)((double)n_keys / (BUCKET_SLOTS * load)) + 1;
         ix->n_buckets = n_buckets;
+        ix->first_bucket_first = load <= 0.25 ? 1 : 0;
         ix->table.release();
         MA_TRY(ix->table.alloc((size_t)n_buckets * BUCKET_SLOTS * sizeof(HashSlot)));
         const int64_t n_slots = (int64_t)n_buckets * BUCKET_SLOTS;
@@ -394,10 +391,16 @@ struct HashBuilder {
     }
 };
 
-// First table load: 0.5, or MIALIGN_HASH_LOAD in (0, 0.95] (a tuning and test
-// knob: high loads send many keys through the cuckoo pass).
-int first_hash_load(double *load) {
+// First table load: 0.15 when that table takes at most a tenth of the free
+// HBM (almost every lookup then reads one bucket), else 0.5 (lookups read both
+// buckets at once); MIALIGN_HASH_LOAD in (0, 0.95] overrides it (a tuning and
+// test knob: high loads send many keys through the cuckoo pass).
+int first_hash_load(int64_t n_keys, double *load) {
     *load = 0.5;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
+        (double)n_keys / 0.15 * (double)sizeof(HashSlot) <= 0.1 * (double)fr)
+        *load = 0.15;
     if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
         const double v = atof(env);
         MA_REQUIRE(v > 0 && v <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
@@ -410,7 +413,7 @@ int first_hash_load(double *load) {
 // the cuckoo pass runs out of steps.
 int build_hash(ma_index *ix, hipStream_t st) {
     double first;
-    MA_TRY(first_hash_load(&first));
+    MA_TRY(first_hash_load(ix->n_keys, &first));
     HashBuilder hb;
     for (double load : {first, 0.4, 0.3}) {
         if (load > first) continue;
@@ -499,6 +502,7 @@ IndexView index_view(const ma_index *ix) {
     IndexView v;
     v.table = ix->table.as<HashSlot>();
     v.n_buckets = ix->n_buckets;
+    v.first_bucket_first = ix->first_bucket_first;
     v.k = ix->k;
     v.postings = ix->postings.as<uint32_t>();
     v.n_postings = ix->n_postings;
@@ -716,7 +720,7 @@ int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
         MA_TRY(ix->offsets.alloc((size_t)(n_keys + 1) * 8));
     }
     double first;
-    MA_TRY(first_hash_load(&first));
+    MA_TRY(first_hash_load(n_keys, &first));
     HashBuilder hb;
     for (double load : {first, 0.4, 0.3}) {
         if (load > first) continue;

@@ -79,8 +79,9 @@ constexpr uint64_t EMPTY_KEY = ~0ull;
 
 // Hash index slot: key (MSB-first 2-bit k-mer) and value = occ(24) << 40 | start(40).
 // Two-choice bucketed table: every key sits in one of its two 64-byte buckets
-// (bucket_of / bucket2_of), so a lookup loads both at once and is done in one
-// memory round trip, present or absent.
+// (bucket_of / bucket2_of), and in the second only while the first is full.
+// At low load a lookup reads the first bucket and, rarely, the second; at
+// high load it loads both at once (one memory round trip either way).
 struct HashSlot {
     uint64_t key;
     uint64_t val;
@@ -96,6 +97,7 @@ constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
 struct IndexView {
     const HashSlot *table;
     uint64_t n_buckets;
+    int32_t first_bucket_first;  // 1: second bucket only when the first is full and misses
     int32_t k;
     const uint32_t *postings;  // (cid, pos) pairs
     int64_t n_postings;

@@ -14,6 +14,7 @@ struct ma_index {
     int32_t ref_has_n = 0;
     int64_t n_cuckoo = 0;  // keys placed by the cuckoo pass of the hash build
     int32_t keep_dense = 1;  // kmers/offsets resident (0: table + postings only)
+    int32_t first_bucket_first = 0;  // lookups read the second bucket only when the first is full
     int32_t n_cu = 0;  // compute units of the device (launch sizing)
     uint64_t n_buckets = 0;
     ma::DevBuf kmers;     // u64[n_keys]       (dense-v1)
