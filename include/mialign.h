@@ -136,9 +136,12 @@ int ma_index_build_opts(int device, int32_t k, const uint8_t *seq, int64_t n_bas
                         ma_index **out);
 int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int64_t *n_bases,
                    int64_t *n_contigs, int64_t *device_bytes, int32_t *k);
-/* Device hash table shape: buckets (4 slots of 16 bytes each; every key sits
- * in one of its two buckets) and how many keys the build's cuckoo pass placed. */
-int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cuckoo);
+/* Device hash table shape. slot_bytes 8: the compact table (64-byte buckets
+ * of 8 slots, each a quotiented key tag + occ + start; n_overflow keys found
+ * both of their buckets full and sit in a small stash); slot_bytes 16: 64-byte
+ * buckets of 4 {key, occ << 40 | start} slots (n_overflow keys were placed by
+ * the cuckoo pass). Every key sits in one of its two buckets (or the stash). */
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes);
 /* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
  * postings[2*n_postings]); any pointer may be NULL. */
 int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings);

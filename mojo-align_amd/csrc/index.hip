@@ -298,6 +298,59 @@ __global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ ov_keys, const u
     }
 }
 
+// Compact insert of keys [0, n) (posting starts offsets[i] + start_add):
+// first choice, then second, each an 8-slot bucket claimed by a 64-bit CAS of
+// the whole slot. Keys finding both full go, with their 16-byte-table value,
+// to the overflow list (the stash is built from it).
+__global__ void compact_insert_kernel(const uint64_t *__restrict__ kmers, const uint64_t *__restrict__ offsets,
+                                      int64_t n, uint64_t start_add, uint64_t *__restrict__ ctable, int m, int r,
+                                      int sb, uint64_t *__restrict__ ov_keys, uint64_t *__restrict__ ov_vals,
+                                      unsigned int *__restrict__ n_overflow, unsigned int overflow_cap) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = kmers[i];
+        const uint64_t start = offsets[i] + start_add;
+        const uint64_t occ_all = offsets[i + 1] - offsets[i];
+        const uint64_t occ = occ_all < COCC_SAT ? occ_all : COCC_SAT;
+        bool done = false;
+        for (int c = 0; c < 2 && !done; c++) {
+            const uint64_t h = key_perm(key, m, c);
+            const uint64_t b = h >> r;
+            const uint64_t rem = h & ((1ull << r) - 1);
+            const uint64_t slot = ((uint64_t)c << 63) | (rem << (63 - r)) | (occ << sb) | start;
+            unsigned long long *bk = (unsigned long long *)(ctable + b * CBUCKET_SLOTS);
+            for (int s = 0; s < CBUCKET_SLOTS && !done; s++)
+                done = atomicCAS(&bk[s], 0ull, (unsigned long long)slot) == 0ull;
+        }
+        if (!done) {
+            const unsigned int j = atomicAdd(n_overflow, 1u);
+            if (j < overflow_cap) {
+                ov_keys[j] = key;
+                ov_vals[j] = ((occ_all < OCC_SAT ? occ_all : OCC_SAT) << 40) | (start & START_MASK);
+            }
+        }
+    }
+}
+
+// Stash: the overflow keys in a 16-byte-slot two-choice table; keys finding
+// both of their stash buckets full are listed for a cuckoo pass on the stash.
+__global__ void stash_insert_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ vals,
+                                    int64_t n, HashSlot *__restrict__ stash, uint64_t nb,
+                                    uint64_t *__restrict__ ov_keys, uint64_t *__restrict__ ov_vals,
+                                    unsigned int *__restrict__ n_overflow, unsigned int overflow_cap) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = keys[i];
+        const uint64_t b1 = bucket_of(key, nb);
+        const uint64_t b2 = bucket2_of(key, nb, b1);
+        if (claim_slot(stash + b1 * BUCKET_SLOTS, key, vals[i])) continue;
+        if (claim_slot(stash + b2 * BUCKET_SLOTS, key, vals[i])) continue;
+        const unsigned int j = atomicAdd(n_overflow, 1u);
+        if (j < overflow_cap) {
+            ov_keys[j] = key;
+            ov_vals[j] = vals[i];
+        }
+    }
+}
+
 __global__ void fill_slots_kernel(HashSlot *t, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
@@ -340,52 +393,137 @@ int pack_reference(ma_index *ix, const uint8_t *seq, int64_t n_bases, const uint
     return MA_OK;
 }
 
-// Two-choice hash table, built in three steps: begin (size and clear the
-// table for n_keys at a load), insert (any number of key ranges, each with its
-// posting-start shift), finish (cuckoo pass; false when it ran out of steps).
+int64_t free_hbm() {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    return (int64_t)fr;
+}
+
+// Hash table builder. begin_* sizes and clears a table (compact 8-byte slots,
+// or 16-byte slots at a load), insert adds key ranges (each with its
+// posting-start shift), finish places the overflow keys (compact: the stash;
+// 16-byte: the cuckoo pass) and reports whether that succeeded.
 struct HashBuilder {
     static constexpr unsigned int OVERFLOW_CAP = 1u << 20;
     static constexpr int64_t STEP_BUDGET = 1ll << 22;
+    static constexpr unsigned int STASH_OVERFLOW_CAP = 1u << 16;
     ma_index *ix = nullptr;
     hipStream_t st = nullptr;
-    DevBuf misc, ov_keys, ov_vals;
+    DevBuf misc, ov_keys, ov_vals, ov2_keys, ov2_vals;
 
-    int begin(ma_index *index, hipStream_t stream, int64_t n_keys, double load) {
+    int prepare(ma_index *index, hipStream_t stream) {
         ix = index;
         st = stream;
         if (!misc.p) {
             MA_TRY(misc.alloc(256));
             MA_TRY(ov_keys.alloc((size_t)OVERFLOW_CAP * 8));
             MA_TRY(ov_vals.alloc((size_t)OVERFLOW_CAP * 8));
+            MA_TRY(ov2_keys.alloc((size_t)STASH_OVERFLOW_CAP * 8));
+            MA_TRY(ov2_vals.alloc((size_t)STASH_OVERFLOW_CAP * 8));
         }
+        ix->table.release();
+        ix->stash.release();
+        ix->stash_nb = 0;
+        return fill_u64(st, misc.as<uint64_t>(), 2, 0);  // overflow counts + fail flag
+    }
+    int begin_wide(ma_index *index, hipStream_t stream, int64_t n_keys, double load) {
+        MA_TRY(prepare(index, stream));
         const uint64_t n_buckets = (uint64_t)((double)n_keys / (BUCKET_SLOTS * load)) + 1;
+        ix->compact = 0;
         ix->n_buckets = n_buckets;
         ix->first_bucket_first = load <= 0.25 ? 1 : 0;
-        ix->table.release();
         MA_TRY(ix->table.alloc((size_t)n_buckets * BUCKET_SLOTS * sizeof(HashSlot)));
         const int64_t n_slots = (int64_t)n_buckets * BUCKET_SLOTS;
         fill_slots_kernel<<<grid_for(n_slots), 256, 0, st>>>(ix->table.as<HashSlot>(), n_slots);
         MA_LAUNCHED();
-        return fill_u64(st, misc.as<uint64_t>(), 1, 0);
+        return MA_OK;
+    }
+    // *fits = false when the slot fields do not fit 64 bits within a table of
+    // at most half the free HBM (then the 16-byte table is used).
+    int begin_compact(ma_index *index, hipStream_t stream, int64_t n_keys, int64_t n_postings, int k, double load,
+                      bool *fits) {
+        *fits = false;
+        const int m = 2 * k;
+        int sb = 1;
+        while (sb < 40 && (1ll << sb) <= n_postings) sb++;
+        int nbits = 1;
+        while (nbits < m && (double)n_keys > (double)CBUCKET_SLOTS * load * (double)(1ll << nbits)) nbits++;
+        if (const char *env = getenv("MIALIGN_HASH_BUCKET_BITS")) {  // test knob: 2^bits buckets
+            const int v = atoi(env);
+            MA_REQUIRE(v >= 1 && v <= m && (int64_t)CBUCKET_SLOTS << v >= n_keys,
+                       "MIALIGN_HASH_BUCKET_BITS out of range for this index");
+            nbits = v;
+        }
+        while (nbits < m && (m - nbits) + 1 + COCC_BITS + sb > 64) nbits++;
+        const int r = m - nbits;
+        if (r + 1 + COCC_BITS + sb > 64 || nbits > 40) return MA_OK;
+        const int64_t bytes = (int64_t)CBUCKET_SLOTS * 8 << nbits;
+        if ((double)bytes > 0.5 * (double)free_hbm()) return MA_OK;
+        MA_TRY(prepare(index, stream));
+        ix->compact = 1;
+        ix->c_m = m;
+        ix->c_r = r;
+        ix->c_sb = sb;
+        ix->n_buckets = 1ull << nbits;
+        ix->first_bucket_first = 1;
+        MA_TRY(ix->table.alloc((size_t)bytes));
+        MA_TRY(fill_u64(st, ix->table.as<uint64_t>(), bytes / 8, 0));
+        *fits = true;
+        return MA_OK;
     }
     int insert(const uint64_t *kmers, const uint64_t *offsets, int64_t n, uint64_t start_add) {
         if (n <= 0) return MA_OK;
-        hash_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<HashSlot>(),
-                                                        ix->n_buckets, ov_keys.as<uint64_t>(),
-                                                        ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
-                                                        OVERFLOW_CAP);
+        if (ix->compact) {
+            compact_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<uint64_t>(),
+                                                               ix->c_m, ix->c_r, ix->c_sb, ov_keys.as<uint64_t>(),
+                                                               ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
+                                                               OVERFLOW_CAP);
+        } else {
+            hash_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<HashSlot>(),
+                                                            ix->n_buckets, ov_keys.as<uint64_t>(),
+                                                            ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
+                                                            OVERFLOW_CAP);
+        }
         MA_LAUNCHED();
         return MA_OK;
     }
     int finish(bool *ok) {
-        hash_cuckoo_kernel<<<1, 64, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(), ix->table.as<HashSlot>(),
-                                             ix->n_buckets, misc.as<unsigned int>(), OVERFLOW_CAP, STEP_BUDGET,
-                                             misc.as<int>() + 1);
-        MA_LAUNCHED();
+        *ok = false;
+        if (ix->compact) {
+            unsigned int n_ov = 0;
+            MA_HIP(hipMemcpyAsync(&n_ov, misc.p, 4, hipMemcpyDeviceToHost, st));
+            MA_HIP(hipStreamSynchronize(st));
+            ix->n_cuckoo = (int64_t)n_ov;
+            if (n_ov > OVERFLOW_CAP) return MA_OK;
+            if (n_ov > 0) {
+                const uint64_t nb = (uint64_t)((double)n_ov / (BUCKET_SLOTS * 0.25)) + 1;
+                MA_TRY(ix->stash.alloc((size_t)nb * BUCKET_SLOTS * sizeof(HashSlot)));
+                fill_slots_kernel<<<grid_for((int64_t)nb * BUCKET_SLOTS), 256, 0, st>>>(ix->stash.as<HashSlot>(),
+                                                                                        (int64_t)nb * BUCKET_SLOTS);
+                MA_LAUNCHED();
+                // misc: [0] overflow count, [1] fail flag, [2] stash overflow count
+                unsigned int *n_ov2 = misc.as<unsigned int>() + 2;
+                stash_insert_kernel<<<grid_for(n_ov), 256, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(),
+                                                                    n_ov, ix->stash.as<HashSlot>(), nb,
+                                                                    ov2_keys.as<uint64_t>(), ov2_vals.as<uint64_t>(),
+                                                                    n_ov2, STASH_OVERFLOW_CAP);
+                MA_LAUNCHED();
+                hash_cuckoo_kernel<<<1, 64, 0, st>>>(ov2_keys.as<uint64_t>(), ov2_vals.as<uint64_t>(),
+                                                     ix->stash.as<HashSlot>(), nb, n_ov2, STASH_OVERFLOW_CAP,
+                                                     STEP_BUDGET, misc.as<int>() + 1);
+                MA_LAUNCHED();
+                ix->stash_nb = nb;
+            }
+        } else {
+            hash_cuckoo_kernel<<<1, 64, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(),
+                                                 ix->table.as<HashSlot>(), ix->n_buckets, misc.as<unsigned int>(),
+                                                 OVERFLOW_CAP, STEP_BUDGET, misc.as<int>() + 1);
+            MA_LAUNCHED();
+        }
         int h[2] = {0, 0};
         MA_HIP(hipMemcpyAsync(h, misc.p, 8, hipMemcpyDeviceToHost, st));
         MA_HIP(hipStreamSynchronize(st));
-        ix->n_cuckoo = (int64_t)h[0];
+        if (!ix->compact) ix->n_cuckoo = (int64_t)h[0];
         *ok = h[1] == 0;
         return MA_OK;
     }
@@ -409,22 +547,49 @@ int first_hash_load(int64_t n_keys, double *load) {
     return MA_OK;
 }
 
-// Table over the resident kmers/offsets: the first load, then 0.4 and 0.3 if
-// the cuckoo pass runs out of steps.
-int build_hash(ma_index *ix, hipStream_t st) {
-    double first;
-    MA_TRY(first_hash_load(ix->n_keys, &first));
+// The hash table, with insert_all(hb) adding every key: the compact format
+// first (unless MIALIGN_HASH_FORMAT=wide, or its fields do not fit), else
+// 16-byte slots at the first load, then 0.4 and 0.3. MIALIGN_HASH_LOAD sets
+// the load of whichever format is built.
+template <class F>
+int build_table(ma_index *ix, hipStream_t st, int64_t n_keys, int64_t n_postings, F insert_all) {
     HashBuilder hb;
+    const char *fmt = getenv("MIALIGN_HASH_FORMAT");
+    const bool wide_only = fmt && strcmp(fmt, "wide") == 0;
+    double load_env = 0;
+    if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
+        load_env = atof(env);
+        MA_REQUIRE(load_env > 0 && load_env <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
+    }
+    if (!wide_only) {
+        bool fits = false;
+        MA_TRY(hb.begin_compact(ix, st, n_keys, n_postings, ix->k, load_env > 0 ? load_env : 0.35, &fits));
+        if (fits) {
+            MA_TRY(insert_all(hb));
+            bool ok = false;
+            MA_TRY(hb.finish(&ok));
+            if (ok) return MA_OK;
+        }
+    }
+    double first;
+    MA_TRY(first_hash_load(n_keys, &first));
     for (double load : {first, 0.4, 0.3}) {
         if (load > first) continue;
-        MA_TRY(hb.begin(ix, st, ix->n_keys, load));
-        MA_TRY(hb.insert(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(), ix->n_keys, 0));
+        MA_TRY(hb.begin_wide(ix, st, n_keys, load));
+        MA_TRY(insert_all(hb));
         bool ok = false;
         MA_TRY(hb.finish(&ok));
         if (ok) return MA_OK;
     }
-    set_error("hash index build: cuckoo placement did not converge");
+    set_error("hash index build: overflow placement did not converge");
     return MA_ERR_NOMEM;
+}
+
+// Table over the resident kmers/offsets.
+int build_hash(ma_index *ix, hipStream_t st) {
+    return build_table(ix, st, ix->n_keys, ix->n_postings, [&](HashBuilder &hb) {
+        return hb.insert(ix->kmers.as<uint64_t>(), ix->offsets.as<uint64_t>(), ix->n_keys, 0);
+    });
 }
 
 int index_alloc(int device, ma_index **out) {
@@ -503,6 +668,13 @@ IndexView index_view(const ma_index *ix) {
     v.table = ix->table.as<HashSlot>();
     v.n_buckets = ix->n_buckets;
     v.first_bucket_first = ix->first_bucket_first;
+    v.compact = ix->compact;
+    v.ctable = ix->compact ? ix->table.as<uint64_t>() : nullptr;
+    v.c_m = ix->c_m;
+    v.c_r = ix->c_r;
+    v.c_sb = ix->c_sb;
+    v.stash = ix->stash.as<HashSlot>();
+    v.stash_nb = ix->stash_nb;
     v.k = ix->k;
     v.postings = ix->postings.as<uint32_t>();
     v.n_postings = ix->n_postings;
@@ -650,12 +822,6 @@ int count_keys(int64_t nc, const uint64_t *kp, ChunkBufs &cb, hipStream_t st, in
     return MA_OK;
 }
 
-int64_t free_hbm() {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
-    return (int64_t)fr;
-}
-
 // The dense-v1 arrays and the hash table of the packed reference in ix, built
 // one key-range chunk at a time. Sweep 1 counts the unique keys (to size the
 // table); sweep 2 writes each chunk's postings slice and inserts its keys.
@@ -719,13 +885,8 @@ int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
         MA_TRY(ix->kmers.alloc((size_t)(n_keys > 0 ? n_keys : 1) * 8));
         MA_TRY(ix->offsets.alloc((size_t)(n_keys + 1) * 8));
     }
-    double first;
-    MA_TRY(first_hash_load(n_keys, &first));
-    HashBuilder hb;
-    for (double load : {first, 0.4, 0.3}) {
-        if (load > first) continue;
-        MA_TRY(hb.begin(ix, st, n_keys, load));
-        // sweep 2: postings slices, table inserts, dense-v1 slices
+    // sweep 2 (per table attempt): postings slices, table inserts, dense-v1 slices
+    return build_table(ix, st, n_keys, (int64_t)n_valid, [&](HashBuilder &hb) -> int {
         uint64_t cap = cap_for(0);
         if (cap < max_bin) cap = max_bin;
         const auto chunks = plan_chunks(hist, cap);
@@ -761,12 +922,8 @@ int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
             set_u64_kernel<<<1, 1, 0, st>>>(ix->offsets.as<uint64_t>() + n_keys, n_valid);
             MA_LAUNCHED();
         }
-        bool ok = false;
-        MA_TRY(hb.finish(&ok));
-        if (ok) return MA_OK;
-    }
-    set_error("hash index build: cuckoo placement did not converge");
-    return MA_ERR_NOMEM;
+        return MA_OK;
+    });
 }
 
 }  // namespace
@@ -850,15 +1007,16 @@ int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int
     if (n_contigs) *n_contigs = ix->n_contigs;
     if (k) *k = ix->k;
     if (device_bytes)
-        *device_bytes = (int64_t)(ix->kmers.bytes + ix->offsets.bytes + ix->postings.bytes +
-                                  ix->ref2.bytes + ix->refN.bytes + ix->coff.bytes + ix->table.bytes);
+        *device_bytes = (int64_t)(ix->kmers.bytes + ix->offsets.bytes + ix->postings.bytes + ix->ref2.bytes +
+                                  ix->refN.bytes + ix->coff.bytes + ix->table.bytes + ix->stash.bytes);
     return MA_OK;
 }
 
-int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_cuckoo) {
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes) {
     MA_REQUIRE(ix, "ma_index_hash_stats: null index");
     if (n_buckets) *n_buckets = (int64_t)ix->n_buckets;
-    if (n_cuckoo) *n_cuckoo = ix->n_cuckoo;
+    if (n_overflow) *n_overflow = ix->n_cuckoo;
+    if (slot_bytes) *slot_bytes = ix->compact ? 8 : (int32_t)sizeof(HashSlot);
     return MA_OK;
 }
 

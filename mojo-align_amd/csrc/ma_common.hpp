@@ -93,11 +93,38 @@ constexpr int64_t REF_PAD_WORDS = 16;
 constexpr uint64_t START_MASK = (1ull << 40) - 1;
 constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
 
+// Compact table: 64-byte buckets of eight 8-byte slots. A key's slot is
+// tag | occ << sb | start, the tag being the choice bit and the low r bits of
+// the permuted key (the bucket index holds the rest), so equal (bucket, tag)
+// means equal key. Empty slots are 0 (a key has occ >= 1).
+constexpr int CBUCKET_SLOTS = 8;
+constexpr int COCC_BITS = 18;  // saturates above any max_occ (< 2^17)
+constexpr uint64_t COCC_SAT = (1ull << COCC_BITS) - 1;
+
+// A bijection on m-bit keys (1 <= m <= 62): two multiply (odd constant) +
+// xorshift rounds modulo 2^m; choice 0 and 1 use different multipliers.
+__host__ __device__ inline uint64_t key_perm(uint64_t x, int m, int c) {
+    const uint64_t M = (1ull << m) - 1;
+    const uint64_t c1 = c ? 0xD6E8FEB86659FD93ull : 0x9E3779B97F4A7C15ull;
+    const uint64_t c2 = c ? 0xA0761D6478BD642Full : 0xBF58476D1CE4E5B9ull;
+    const int s1 = (m + 1) / 2, s2 = m / 3 + 1;
+    x = (x * c1) & M;
+    x ^= x >> s1;
+    x = (x * c2) & M;
+    x ^= x >> s2;
+    return x;
+}
+
 // Resident index (device pointers) — passed by value to kernels.
 struct IndexView {
-    const HashSlot *table;
+    const HashSlot *table;   // 16-byte-slot table (compact == 0)
     uint64_t n_buckets;
     int32_t first_bucket_first;  // 1: second bucket only when the first is full and misses
+    int32_t compact;             // 1: 8-byte-slot table in ctable (+ stash)
+    const uint64_t *ctable;
+    int32_t c_m, c_r, c_sb;      // key bits, remainder bits, start bits
+    const HashSlot *stash;       // keys whose two compact buckets were full (16-byte slots)
+    uint64_t stash_nb;           // its buckets (0 = empty)
     int32_t k;
     const uint32_t *postings;  // (cid, pos) pairs
     int64_t n_postings;

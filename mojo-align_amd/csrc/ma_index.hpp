@@ -15,6 +15,9 @@ struct ma_index {
     int64_t n_cuckoo = 0;  // keys placed by the cuckoo pass of the hash build
     int32_t keep_dense = 1;  // kmers/offsets resident (0: table + postings only)
     int32_t first_bucket_first = 0;  // lookups read the second bucket only when the first is full
+    int32_t compact = 0;             // table holds 8-byte slots (see IndexView)
+    int32_t c_m = 0, c_r = 0, c_sb = 0;
+    uint64_t stash_nb = 0;
     int32_t n_cu = 0;  // compute units of the device (launch sizing)
     uint64_t n_buckets = 0;
     ma::DevBuf kmers;     // u64[n_keys]       (dense-v1)
@@ -23,7 +26,8 @@ struct ma_index {
     ma::DevBuf ref2;      // 2-bit reference words
     ma::DevBuf refN;      // N flags, same geometry
     ma::DevBuf coff;      // u64[n_contigs + 1]
-    ma::DevBuf table;     // HashSlot[n_buckets * BUCKET_SLOTS]
+    ma::DevBuf table;     // HashSlot[n_buckets * BUCKET_SLOTS], or u64[n_buckets * CBUCKET_SLOTS] compact
+    ma::DevBuf stash;     // compact only: HashSlot table of the overflow keys
 };
 
 // Per-launch state of the map kernel (its work counter).

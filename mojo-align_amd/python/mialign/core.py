@@ -203,9 +203,11 @@ class GpuIndex:
         }
 
     def hash_stats(self) -> dict:
-        nb, nc = C.c_int64(0), C.c_int64(0)
-        check(lib().ma_index_hash_stats(self.handle, C.byref(nb), C.byref(nc)), "ma_index_hash_stats")
-        return {"n_buckets": nb.value, "n_cuckoo": nc.value}
+        nb, no, sbytes = C.c_int64(0), C.c_int64(0), C.c_int32(0)
+        check(lib().ma_index_hash_stats(self.handle, C.byref(nb), C.byref(no), C.byref(sbytes)),
+              "ma_index_hash_stats")
+        return {"n_buckets": nb.value, "n_overflow": no.value, "slot_bytes": sbytes.value,
+                "slots_per_bucket": 64 // sbytes.value}
 
     def export_dense(self):
         kmers = np.empty(self.n_keys, dtype=np.uint64)

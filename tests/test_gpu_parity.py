@@ -113,26 +113,36 @@ def test_index_build_with_n_runs():
     assert np.array_equal(gk, km) and np.array_equal(go, of) and np.array_equal(gp, po)
 
 
-def test_hash_table_cuckoo_placements_bit_exact(w1, monkeypatch):
-    """At load 0.8 many keys find both buckets full and go through the cuckoo
-    pass, and lookups read both buckets at once; the default table (low load,
-    first bucket first) reads one. Both return every key's range: records
-    equal the oracle's and each other's."""
-    say("test_hash_table_cuckoo_placements_bit_exact: load 0.8")
-    monkeypatch.setenv("MIALIGN_HASH_LOAD", "0.8")
+@pytest.mark.parametrize("fmt", ["wide", "compact"])
+def test_hash_table_overflow_placements_bit_exact(w1, monkeypatch, fmt):
+    """Tables built nearly full: 16-byte slots at load 0.8 (many keys placed by
+    the cuckoo pass, lookups read both buckets at once) and compact 8-byte
+    slots at ~0.95 (2^18 buckets for the 2M keys: many keys find both buckets
+    full and go to the stash). Lookups must still return every key's range:
+    records equal the oracle's and those of the default table (compact, low
+    load, first bucket first)."""
+    say(f"test_hash_table_overflow_placements_bit_exact[{fmt}]")
+    monkeypatch.setenv("MIALIGN_HASH_FORMAT", fmt)
+    if fmt == "wide":
+        monkeypatch.setenv("MIALIGN_HASH_LOAD", "0.8")
+    else:
+        monkeypatch.setenv("MIALIGN_HASH_BUCKET_BITS", "18")
     full_ix = mialign.GpuIndex.build(w1.dual.seq, w1.dual.contig_off, 21)
-    monkeypatch.delenv("MIALIGN_HASH_LOAD")
+    for v in ("MIALIGN_HASH_FORMAT", "MIALIGN_HASH_LOAD", "MIALIGN_HASH_BUCKET_BITS"):
+        monkeypatch.delenv(v, raising=False)
     hs, hs0 = full_ix.hash_stats(), w1.gix.hash_stats()
-    assert hs["n_buckets"] * 4 * 0.75 < full_ix.n_keys <= hs["n_buckets"] * 4 * 0.8
-    assert hs["n_cuckoo"] > 1000, hs
-    # the default table: load 0.15 (first bucket first) when HBM allows, else 0.5 (both buckets)
-    assert w1.gix.n_keys <= hs0["n_buckets"] * 4 * 0.15 or w1.gix.n_keys <= hs0["n_buckets"] * 4 * 0.5
+    say(f"full table {hs}, default table {hs0}")
+    assert hs["slot_bytes"] == (16 if fmt == "wide" else 8)
+    load = full_ix.n_keys / (hs["n_buckets"] * hs["slots_per_bucket"])
+    assert (0.72 < load <= 0.8) if fmt == "wide" else (0.9 < load <= 1.0), load
+    assert hs["n_overflow"] > 1000, hs
+    assert hs0["slot_bytes"] == 8 and w1.gix.n_keys <= hs0["n_buckets"] * 8 * 0.35
     reads = synth.gen_reads(21, w1.ref, 20_000)
     got = mialign.map_batch(full_ix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
     want, _, _, _ = w1.oracle_map(reads)
-    assert_recs_equal(got, want, "load-0.8 table vs oracle")
+    assert_recs_equal(got, want, f"full {fmt} table vs oracle")
     base = mialign.map_batch(w1.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T)
-    assert_recs_equal(got, base, "load-0.8 table vs the default table")
+    assert_recs_equal(got, base, f"full {fmt} table vs the default table")
 
 
 def test_map_config1_ascii_all_records(w1):
