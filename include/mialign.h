@@ -140,8 +140,13 @@ int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int
  * of 8 slots, each a quotiented key tag + occ + start; n_overflow keys found
  * both of their buckets full and sit in a small stash); slot_bytes 16: 64-byte
  * buckets of 4 {key, occ << 40 | start} slots (n_overflow keys were placed by
- * the cuckoo pass). Every key sits in one of its two buckets (or the stash). */
-int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes);
+ * the cuckoo pass). Every key sits in one of its two buckets (or the stash).
+ * inline_postings 1: compact slots of one-posting keys hold that posting's
+ * global base position instead of its start (a lookup needs no postings load;
+ * indexes of fewer than 256 contigs whose positions fit the start field).
+ * Any output pointer may be NULL. */
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes,
+                        int32_t *inline_postings);
 /* Copy the dense-v1 arrays back (kmers[n_keys], offsets[n_keys+1],
  * postings[2*n_postings]); any pointer may be NULL. */
 int ma_index_export_dense(ma_index *ix, uint64_t *kmers, uint64_t *offsets, uint32_t *postings);

@@ -301,22 +301,41 @@ __global__ void hash_cuckoo_kernel(const uint64_t *__restrict__ ov_keys, const u
 // Compact insert of keys [0, n) (posting starts offsets[i] + start_add):
 // first choice, then second, each an 8-slot bucket claimed by a 64-bit CAS of
 // the whole slot. Keys finding both full go, with their 16-byte-table value,
-// to the overflow list (the stash is built from it).
+// to the overflow list (the stash is built from it). With `inl`, a key with
+// one posting stores that posting's global base position instead of its
+// start, so a lookup returns the posting itself (no dependent load): position
+// coff[cid] + pos, or n_bases for a posting with cid >= n_contigs (decoded as
+// an out-of-range cid, which every reader treats alike). A posting past its
+// contig's end cannot be stored so and sets *inl_fail (the table is then
+// rebuilt without inline postings).
 __global__ void compact_insert_kernel(const uint64_t *__restrict__ kmers, const uint64_t *__restrict__ offsets,
                                       int64_t n, uint64_t start_add, uint64_t *__restrict__ ctable, int m, int r,
-                                      int sb, uint64_t *__restrict__ ov_keys, uint64_t *__restrict__ ov_vals,
-                                      unsigned int *__restrict__ n_overflow, unsigned int overflow_cap) {
+                                      int sb, int inl, const uint32_t *__restrict__ postings,
+                                      const uint64_t *__restrict__ coff, int64_t n_contigs,
+                                      uint64_t *__restrict__ ov_keys, uint64_t *__restrict__ ov_vals,
+                                      unsigned int *__restrict__ n_overflow, unsigned int overflow_cap,
+                                      unsigned int *__restrict__ inl_fail) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t key = kmers[i];
         const uint64_t start = offsets[i] + start_add;
         const uint64_t occ_all = offsets[i + 1] - offsets[i];
         const uint64_t occ = occ_all < COCC_SAT ? occ_all : COCC_SAT;
+        uint64_t field = start;
+        if (inl && occ_all == 1) {
+            const uint32_t cid = postings[2 * start], pos = postings[2 * start + 1];
+            if ((int64_t)cid >= n_contigs) {
+                field = coff[n_contigs];
+            } else {
+                field = coff[cid] + pos;
+                if (field >= coff[cid + 1]) atomicOr(inl_fail, 1u);
+            }
+        }
         bool done = false;
         for (int c = 0; c < 2 && !done; c++) {
             const uint64_t h = key_perm(key, m, c);
             const uint64_t b = h >> r;
             const uint64_t rem = h & ((1ull << r) - 1);
-            const uint64_t slot = ((uint64_t)c << 63) | (rem << (63 - r)) | (occ << sb) | start;
+            const uint64_t slot = ((uint64_t)c << 63) | (rem << (63 - r)) | (occ << sb) | field;
             unsigned long long *bk = (unsigned long long *)(ctable + b * CBUCKET_SLOTS);
             for (int s = 0; s < CBUCKET_SLOTS && !done; s++)
                 done = atomicCAS(&bk[s], 0ull, (unsigned long long)slot) == 0ull;
@@ -440,12 +459,17 @@ struct HashBuilder {
     }
     // *fits = false when the slot fields do not fit 64 bits within a table of
     // at most half the free HBM (then the 16-byte table is used).
+    // With try_inline, one-posting keys hold their position (compact_insert_kernel)
+    // when the contig offsets fit the map kernel's LDS cache and the wider
+    // start field needs no more buckets than the plain one.
     int begin_compact(ma_index *index, hipStream_t stream, int64_t n_keys, int64_t n_postings, int k, double load,
-                      bool *fits) {
+                      bool try_inline, bool *fits) {
         *fits = false;
         const int m = 2 * k;
         int sb = 1;
         while (sb < 40 && (1ll << sb) <= n_postings) sb++;
+        int sb_pos = 1;  // global positions 0..n_bases (n_bases marks an out-of-range cid)
+        while (sb_pos < 40 && (1ll << sb_pos) <= index->n_bases) sb_pos++;
         int nbits = 1;
         while (nbits < m && (double)n_keys > (double)CBUCKET_SLOTS * load * (double)(1ll << nbits)) nbits++;
         if (const char *env = getenv("MIALIGN_HASH_BUCKET_BITS")) {  // test knob: 2^bits buckets
@@ -457,6 +481,10 @@ struct HashBuilder {
         while (nbits < m && (m - nbits) + 1 + COCC_BITS + sb > 64) nbits++;
         const int r = m - nbits;
         if (r + 1 + COCC_BITS + sb > 64 || nbits > 40) return MA_OK;
+        const int sb_inl = sb_pos > sb ? sb_pos : sb;
+        const bool inl = try_inline && index->n_contigs < COFF_LDS && r + 1 + COCC_BITS + sb_inl <= 64 &&
+                         sb_inl <= 40;
+        if (inl) sb = sb_inl;
         const int64_t bytes = (int64_t)CBUCKET_SLOTS * 8 << nbits;
         if ((double)bytes > 0.5 * (double)free_hbm()) return MA_OK;
         MA_TRY(prepare(index, stream));
@@ -464,6 +492,7 @@ struct HashBuilder {
         ix->c_m = m;
         ix->c_r = r;
         ix->c_sb = sb;
+        ix->c_inline = inl ? 1 : 0;
         ix->n_buckets = 1ull << nbits;
         ix->first_bucket_first = 1;
         MA_TRY(ix->table.alloc((size_t)bytes));
@@ -474,10 +503,10 @@ struct HashBuilder {
     int insert(const uint64_t *kmers, const uint64_t *offsets, int64_t n, uint64_t start_add) {
         if (n <= 0) return MA_OK;
         if (ix->compact) {
-            compact_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<uint64_t>(),
-                                                               ix->c_m, ix->c_r, ix->c_sb, ov_keys.as<uint64_t>(),
-                                                               ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
-                                                               OVERFLOW_CAP);
+            compact_insert_kernel<<<grid_for(n), 256, 0, st>>>(
+                kmers, offsets, n, start_add, ix->table.as<uint64_t>(), ix->c_m, ix->c_r, ix->c_sb, ix->c_inline,
+                ix->postings.as<uint32_t>(), ix->coff.as<uint64_t>(), ix->n_contigs, ov_keys.as<uint64_t>(),
+                ov_vals.as<uint64_t>(), misc.as<unsigned int>(), OVERFLOW_CAP, misc.as<unsigned int>() + 3);
         } else {
             hash_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<HashSlot>(),
                                                             ix->n_buckets, ov_keys.as<uint64_t>(),
@@ -487,13 +516,21 @@ struct HashBuilder {
         MA_LAUNCHED();
         return MA_OK;
     }
-    int finish(bool *ok) {
+    // *inl_fail: a one-posting key's posting lies past its contig (compact
+    // table with inline postings only); the table must be rebuilt without.
+    int finish(bool *ok, bool *inl_fail = nullptr) {
         *ok = false;
+        if (inl_fail) *inl_fail = false;
         if (ix->compact) {
-            unsigned int n_ov = 0;
-            MA_HIP(hipMemcpyAsync(&n_ov, misc.p, 4, hipMemcpyDeviceToHost, st));
+            unsigned int hd[4] = {0, 0, 0, 0};  // overflow count, fail flag, stash overflow, inline failure
+            MA_HIP(hipMemcpyAsync(hd, misc.p, sizeof(hd), hipMemcpyDeviceToHost, st));
             MA_HIP(hipStreamSynchronize(st));
+            const unsigned int n_ov = hd[0];
             ix->n_cuckoo = (int64_t)n_ov;
+            if (hd[3]) {
+                if (inl_fail) *inl_fail = true;
+                return MA_OK;
+            }
             if (n_ov > OVERFLOW_CAP) return MA_OK;
             if (n_ov > 0) {
                 const uint64_t nb = (uint64_t)((double)n_ov / (BUCKET_SLOTS * 0.25)) + 1;
@@ -501,7 +538,7 @@ struct HashBuilder {
                 fill_slots_kernel<<<grid_for((int64_t)nb * BUCKET_SLOTS), 256, 0, st>>>(ix->stash.as<HashSlot>(),
                                                                                         (int64_t)nb * BUCKET_SLOTS);
                 MA_LAUNCHED();
-                // misc: [0] overflow count, [1] fail flag, [2] stash overflow count
+                // misc: [0] overflow count, [1] fail flag, [2] stash overflow count, [3] inline failure
                 unsigned int *n_ov2 = misc.as<unsigned int>() + 2;
                 stash_insert_kernel<<<grid_for(n_ov), 256, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(),
                                                                     n_ov, ix->stash.as<HashSlot>(), nb,
@@ -561,15 +598,19 @@ int build_table(ma_index *ix, hipStream_t st, int64_t n_keys, int64_t n_postings
         load_env = atof(env);
         MA_REQUIRE(load_env > 0 && load_env <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
     }
-    if (!wide_only) {
+    const char *inl_env = getenv("MIALIGN_HASH_INLINE");  // test knob: "0" keeps every key's start
+    bool try_inline = !(inl_env && strcmp(inl_env, "0") == 0);
+    while (!wide_only) {
         bool fits = false;
-        MA_TRY(hb.begin_compact(ix, st, n_keys, n_postings, ix->k, load_env > 0 ? load_env : 0.35, &fits));
-        if (fits) {
-            MA_TRY(insert_all(hb));
-            bool ok = false;
-            MA_TRY(hb.finish(&ok));
-            if (ok) return MA_OK;
-        }
+        MA_TRY(hb.begin_compact(ix, st, n_keys, n_postings, ix->k, load_env > 0 ? load_env : 0.35, try_inline,
+                                &fits));
+        if (!fits) break;
+        MA_TRY(insert_all(hb));
+        bool ok = false, inl_fail = false;
+        MA_TRY(hb.finish(&ok, &inl_fail));
+        if (ok) return MA_OK;
+        if (!inl_fail || !try_inline) break;
+        try_inline = false;
     }
     double first;
     MA_TRY(first_hash_load(n_keys, &first));
@@ -673,6 +714,7 @@ IndexView index_view(const ma_index *ix) {
     v.c_m = ix->c_m;
     v.c_r = ix->c_r;
     v.c_sb = ix->c_sb;
+    v.c_inline = ix->compact ? ix->c_inline : 0;
     v.stash = ix->stash.as<HashSlot>();
     v.stash_nb = ix->stash_nb;
     v.k = ix->k;
@@ -1012,11 +1054,13 @@ int ma_index_stats(const ma_index *ix, int64_t *n_keys, int64_t *n_postings, int
     return MA_OK;
 }
 
-int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes) {
+int ma_index_hash_stats(const ma_index *ix, int64_t *n_buckets, int64_t *n_overflow, int32_t *slot_bytes,
+                        int32_t *inline_postings) {
     MA_REQUIRE(ix, "ma_index_hash_stats: null index");
     if (n_buckets) *n_buckets = (int64_t)ix->n_buckets;
     if (n_overflow) *n_overflow = ix->n_cuckoo;
     if (slot_bytes) *slot_bytes = ix->compact ? 8 : (int32_t)sizeof(HashSlot);
+    if (inline_postings) *inline_postings = ix->compact ? ix->c_inline : 0;
     return MA_OK;
 }
 

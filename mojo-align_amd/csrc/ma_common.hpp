@@ -100,6 +100,10 @@ constexpr uint64_t OCC_SAT = (1ull << 24) - 1;
 constexpr int CBUCKET_SLOTS = 8;
 constexpr int COCC_BITS = 18;  // saturates above any max_occ (< 2^17)
 constexpr uint64_t COCC_SAT = (1ull << COCC_BITS) - 1;
+// Contig offsets fit the map kernel's LDS cache below this contig count; the
+// compact table holds one-posting keys inline only then (their decode is a
+// binary search over those offsets).
+constexpr int COFF_LDS = 256;
 
 // A bijection on m-bit keys (1 <= m <= 62): two multiply (odd constant) +
 // xorshift rounds modulo 2^m; choice 0 and 1 use different multipliers.
@@ -123,6 +127,8 @@ struct IndexView {
     int32_t compact;             // 1: 8-byte-slot table in ctable (+ stash)
     const uint64_t *ctable;
     int32_t c_m, c_r, c_sb;      // key bits, remainder bits, start bits
+    int32_t c_inline;            // 1: a compact slot with occ 1 holds its posting's global base
+                                 // position contig_off[cid] + pos (n_bases for a cid >= n_contigs)
     const HashSlot *stash;       // keys whose two compact buckets were full (16-byte slots)
     uint64_t stash_nb;           // its buckets (0 = empty)
     int32_t k;

@@ -17,6 +17,7 @@ struct ma_index {
     int32_t first_bucket_first = 0;  // lookups read the second bucket only when the first is full
     int32_t compact = 0;             // table holds 8-byte slots (see IndexView)
     int32_t c_m = 0, c_r = 0, c_sb = 0;
+    int32_t c_inline = 0;            // compact: one-posting keys hold their position (IndexView)
     uint64_t stash_nb = 0;
     int32_t n_cu = 0;  // compute units of the device (launch sizing)
     uint64_t n_buckets = 0;

@@ -43,7 +43,6 @@ constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
 constexpr int PBUF = 256;  // postings prefetched into LDS per wave
 constexpr int MAX_CLAIM = 16;  // most reads claimed per work-counter atomic
-constexpr int COFF_LDS = 256;  // contig offsets cached in LDS below this contig count
 constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kernels' register budget
 
 struct Knobs {
@@ -93,12 +92,16 @@ DEV Rec sel_rec(bool c, const Rec &a, const Rec &b) {
 
 DEV int lane_id() { return (int)(threadIdx.x & 63); }
 
-// Wave-uniform values read from LDS live in scalar registers.
-DEV uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// Wave-uniform values read from LDS live in scalar registers. The lane-read
+// builtins return int: each 32-bit half goes through uint32_t before it is
+// widened, or a low word with bit 31 set would sign-extend over the high word
+// (a posting start >= 2^31, the inline-posting marker).
+DEV uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane(v); }
 DEV int unii(int v) { return (int)__builtin_amdgcn_readfirstlane((uint32_t)v); }
 DEV uint64_t uni64(uint64_t v) {
-    return ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32)) << 32) |
-           (uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
 }
 
 // Orders LDS traffic between lanes of one wave (lockstep lanes, in-order LDS
@@ -129,8 +132,9 @@ DEV uint32_t wave_min_u32(uint32_t v) {
 }
 
 DEV uint64_t readlane64(uint64_t v, int lane) {
-    return ((uint64_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), lane) << 32) |
-           (uint64_t)__builtin_amdgcn_readlane((uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, lane);
+    return ((uint64_t)hi << 32) | (uint64_t)lo;
 }
 
 // Multiset top-2 (b1 >= b2), one value at a time, with selects only.
@@ -216,18 +220,29 @@ DEV uint8_t ascii_code(uint8_t b) {
     }
 }
 
-DEV void store_rec(const RecOut &o, int64_t r, const Rec &x) {
-    if (!o.cid) return;
-    o.cid[r] = x.cid;
-    o.pos[r] = x.pos;
-    o.mapq[r] = x.mapq;
-    o.flag[r] = x.flag;
-    o.sl[r] = x.sl;
-    o.sr[r] = x.sr;
-    o.nm[r] = x.nm;
-    if (o.iop) o.iop[r] = 0;
-    if (o.iat) o.iat[r] = 0;
-    if (o.ilen) o.ilen[r] = 0;
+// Record outputs as one table of field arrays: entry 10 * o + f is field f
+// (cid pos mapq flag sl sr nm iop iat ilen) of output o (merged, FW, RC), null
+// when that output or field is absent. The map kernel keeps the table in LDS
+// and stores a read's records with one lane per field, so the 30 pointers hold
+// no scalar registers across the kernel.
+constexpr int REC_FIELDS = 10;
+constexpr int REC_WORDS = 7;  // fields kept in Rec; iop, iat and ilen are 0 (gapless)
+constexpr int N_REC_PTRS = 3 * REC_FIELDS;
+static_assert(sizeof(Rec) == 4 * REC_WORDS, "Rec is seven 32-bit fields");
+
+DEV uint32_t *rec_field(const RecOut &o, int f) {
+    if (!o.cid) return nullptr;
+    uint32_t *p = (uint32_t *)o.cid;
+    p = f == 1 ? o.pos : p;
+    p = f == 2 ? o.mapq : p;
+    p = f == 3 ? (uint32_t *)o.flag : p;
+    p = f == 4 ? o.sl : p;
+    p = f == 5 ? o.sr : p;
+    p = f == 6 ? o.nm : p;
+    p = f == 7 ? o.iop : p;
+    p = f == 8 ? o.iat : p;
+    p = f == 9 ? o.ilen : p;
+    return p;
 }
 
 }  // namespace

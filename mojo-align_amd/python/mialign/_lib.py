@@ -112,7 +112,7 @@ SIGNATURES = {
     "ma_index_build_opts": (C.c_int, [C.c_int, I32, P, I64, P, I64, C.POINTER(BuildOpts), C.POINTER(P)]),
     "ma_index_stats": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64), C.POINTER(I64),
                                  C.POINTER(I64), C.POINTER(I32)]),
-    "ma_index_hash_stats": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I32)]),
+    "ma_index_hash_stats": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I32), C.POINTER(I32)]),
     "ma_index_export_dense": (C.c_int, [P, P, P, P]),
     "ma_index_close": (C.c_int, [P]),
     "ma_workspace_create": (C.c_int, [P, I64, I32, C.POINTER(P)]),

@@ -203,11 +203,11 @@ class GpuIndex:
         }
 
     def hash_stats(self) -> dict:
-        nb, no, sbytes = C.c_int64(0), C.c_int64(0), C.c_int32(0)
-        check(lib().ma_index_hash_stats(self.handle, C.byref(nb), C.byref(no), C.byref(sbytes)),
+        nb, no, sbytes, inl = C.c_int64(0), C.c_int64(0), C.c_int32(0), C.c_int32(0)
+        check(lib().ma_index_hash_stats(self.handle, C.byref(nb), C.byref(no), C.byref(sbytes), C.byref(inl)),
               "ma_index_hash_stats")
         return {"n_buckets": nb.value, "n_overflow": no.value, "slot_bytes": sbytes.value,
-                "slots_per_bucket": 64 // sbytes.value}
+                "slots_per_bucket": 64 // sbytes.value, "inline_postings": bool(inl.value)}
 
     def export_dense(self):
         kmers = np.empty(self.n_keys, dtype=np.uint64)

@@ -113,26 +113,30 @@ def test_index_build_with_n_runs():
     assert np.array_equal(gk, km) and np.array_equal(go, of) and np.array_equal(gp, po)
 
 
-@pytest.mark.parametrize("fmt", ["wide", "compact"])
+@pytest.mark.parametrize("fmt", ["wide", "compact", "compact-plain"])
 def test_hash_table_overflow_placements_bit_exact(w1, monkeypatch, fmt):
     """Tables built nearly full: 16-byte slots at load 0.8 (many keys placed by
     the cuckoo pass, lookups read both buckets at once) and compact 8-byte
     slots at ~0.95 (2^18 buckets for the 2M keys: many keys find both buckets
-    full and go to the stash). Lookups must still return every key's range:
-    records equal the oracle's and those of the default table (compact, low
-    load, first bucket first)."""
+    full and go to the stash), with one-posting keys held inline or (plain)
+    as posting starts. Lookups must still return every key's range: records
+    equal the oracle's and those of the default table (compact, low load,
+    first bucket first, inline postings)."""
     say(f"test_hash_table_overflow_placements_bit_exact[{fmt}]")
-    monkeypatch.setenv("MIALIGN_HASH_FORMAT", fmt)
+    monkeypatch.setenv("MIALIGN_HASH_FORMAT", fmt.split("-")[0])
     if fmt == "wide":
         monkeypatch.setenv("MIALIGN_HASH_LOAD", "0.8")
     else:
         monkeypatch.setenv("MIALIGN_HASH_BUCKET_BITS", "18")
+    if fmt == "compact-plain":
+        monkeypatch.setenv("MIALIGN_HASH_INLINE", "0")
     full_ix = mialign.GpuIndex.build(w1.dual.seq, w1.dual.contig_off, 21)
-    for v in ("MIALIGN_HASH_FORMAT", "MIALIGN_HASH_LOAD", "MIALIGN_HASH_BUCKET_BITS"):
+    for v in ("MIALIGN_HASH_FORMAT", "MIALIGN_HASH_LOAD", "MIALIGN_HASH_BUCKET_BITS", "MIALIGN_HASH_INLINE"):
         monkeypatch.delenv(v, raising=False)
     hs, hs0 = full_ix.hash_stats(), w1.gix.hash_stats()
     say(f"full table {hs}, default table {hs0}")
     assert hs["slot_bytes"] == (16 if fmt == "wide" else 8)
+    assert hs["inline_postings"] == (fmt == "compact") and hs0["inline_postings"]
     load = full_ix.n_keys / (hs["n_buckets"] * hs["slots_per_bucket"])
     assert (0.72 < load <= 0.8) if fmt == "wide" else (0.9 < load <= 1.0), load
     assert hs["n_overflow"] > 1000, hs
@@ -184,6 +188,25 @@ def test_map_g2a_and_r1_r2_split(w1):
                            orc.convert_reads(reads.bases[n1:], 2)])
     want, _, _, _ = orc.map_batch(w1.oix, conv, reads.lens, nthreads=8)
     assert_recs_equal(got, want, "R1 C2T / R2 G2A")
+
+
+def test_map_many_contigs_offsets_in_hbm():
+    """600 contigs (300 x 2 conversions, 2.5 kbp each): the contig offsets no
+    longer fit the kernel's LDS cache, so the kernel reads them from HBM and
+    the table holds every key's posting start (no inline postings). Reads
+    near contig ends exercise the window bounds checks."""
+    say("test_map_many_contigs_offsets_in_hbm: 300 contigs, 750 kbp")
+    w = World(synth.gen_reference(12, 300, 750_000))
+    assert w.dual.contig_off.size - 1 >= 256
+    hs = w.gix.hash_stats()
+    assert hs["slot_bytes"] == 8 and not hs["inline_postings"], hs
+    reads = synth.gen_reads(13, w.ref, 6000, 60, 150)
+    got, gfw, grc = mialign.map_batch(w.gix, reads.bases, reads.lens, 150, bs_r1=MA_BS_C2T, strands=True)
+    want, wfw, wrc, _ = w.oracle_map(reads)
+    assert_recs_equal(gfw, wfw, "FW records (600 contigs)")
+    assert_recs_equal(grc, wrc, "RC records (600 contigs)")
+    assert_recs_equal(got, want, "merged records (600 contigs)")
+    assert (got["cid"] >= 0).mean() > 0.5
 
 
 def test_map_repeats_overflow_and_high_occ(wrep):
