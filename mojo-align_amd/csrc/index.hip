@@ -10,6 +10,7 @@
 // they load a key's two 64-byte buckets of a two-choice hash table that maps
 // key -> (start, occ), so the resulting [start, end) ranges are the same.
 #include <fcntl.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -412,10 +413,41 @@ int pack_reference(ma_index *ix, const uint8_t *seq, int64_t n_bases, const uint
     return MA_OK;
 }
 
+// Free device memory once it has settled. Memory of released buffers comes
+// back to the free count some milliseconds after the release returns (the
+// driver frees it asynchronously): read right after the build's 200 GB of
+// chunk buffers go, the count showed a third of what it reached moments later.
+// So read until two readings 10 ms apart no longer grow (at most ~1 s).
 int64_t free_hbm() {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    if (hipDeviceSynchronize() != hipSuccess) return 0;
+    size_t fr = 0, tot = 0, prev = 0;
+    for (int i = 0; i < 100; i++) {
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+        if (i > 0 && fr <= prev) break;
+        prev = fr;
+        usleep(10000);
+    }
     return (int64_t)fr;
+}
+
+// Build decisions on stderr when MIALIGN_VERBOSE is set (table format, size,
+// load, overflow counts), for sizing an index against the device's HBM.
+// VLOG evaluates its arguments only then.
+bool verbose() {
+    static const bool on = getenv("MIALIGN_VERBOSE") != nullptr;
+    return on;
+}
+#define VLOG(...)                          \
+    do {                                   \
+        if (verbose()) vlog(__VA_ARGS__); \
+    } while (0)
+void vlog(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[mialign index] ");
+    vfprintf(stderr, fmt, ap);
+    fputc('\n', stderr);
+    va_end(ap);
 }
 
 // Hash table builder. begin_* sizes and clears a table (compact 8-byte slots,
@@ -426,20 +458,27 @@ struct HashBuilder {
     static constexpr unsigned int OVERFLOW_CAP = 1u << 20;
     static constexpr int64_t STEP_BUDGET = 1ll << 22;
     static constexpr unsigned int STASH_OVERFLOW_CAP = 1u << 16;
+    // Compact table: target load, and the highest load it may take (fewer
+    // buckets) when the table at the target would exceed half the free HBM;
+    // overflow lists are then sized for n_keys / 200 keys (8-slot buckets
+    // with first-choice-first inserts overflow ~2e-3 of keys at 0.62).
+    static constexpr double COMPACT_MAX_LOAD = 0.65;
     ma_index *ix = nullptr;
     hipStream_t st = nullptr;
+    unsigned int ov_cap = OVERFLOW_CAP;
     DevBuf misc, ov_keys, ov_vals, ov2_keys, ov2_vals;
 
-    int prepare(ma_index *index, hipStream_t stream) {
+    int prepare(ma_index *index, hipStream_t stream, unsigned int cap = OVERFLOW_CAP) {
         ix = index;
         st = stream;
+        ov_cap = cap;
         if (!misc.p) {
             MA_TRY(misc.alloc(256));
-            MA_TRY(ov_keys.alloc((size_t)OVERFLOW_CAP * 8));
-            MA_TRY(ov_vals.alloc((size_t)OVERFLOW_CAP * 8));
             MA_TRY(ov2_keys.alloc((size_t)STASH_OVERFLOW_CAP * 8));
             MA_TRY(ov2_vals.alloc((size_t)STASH_OVERFLOW_CAP * 8));
         }
+        MA_TRY(ov_keys.reserve((size_t)ov_cap * 8));
+        MA_TRY(ov_vals.reserve((size_t)ov_cap * 8));
         ix->table.release();
         ix->stash.release();
         ix->stash_nb = 0;
@@ -448,6 +487,8 @@ struct HashBuilder {
     int begin_wide(ma_index *index, hipStream_t stream, int64_t n_keys, double load) {
         MA_TRY(prepare(index, stream));
         const uint64_t n_buckets = (uint64_t)((double)n_keys / (BUCKET_SLOTS * load)) + 1;
+        VLOG("16-byte table: %lld keys at load %.2f (%.1f GB), free HBM %.1f GB", (long long)n_keys, load,
+             (double)n_buckets * BUCKET_SLOTS * sizeof(HashSlot) / 1e9, free_hbm() / 1e9);
         ix->compact = 0;
         ix->n_buckets = n_buckets;
         ix->first_bucket_first = load <= 0.25 ? 1 : 0;
@@ -479,6 +520,17 @@ struct HashBuilder {
             nbits = v;
         }
         while (nbits < m && (m - nbits) + 1 + COCC_BITS + sb > 64) nbits++;
+        // fewer buckets (a higher load) when the table would take more than
+        // half the free HBM, down to COMPACT_MAX_LOAD and the field fit
+        const double budget = 0.5 * (double)free_hbm();
+        unsigned int cap = OVERFLOW_CAP;
+        while (nbits > 1 && (double)((int64_t)CBUCKET_SLOTS * 8 << nbits) > budget &&
+               (double)n_keys <= COMPACT_MAX_LOAD * (double)CBUCKET_SLOTS * (double)(1ll << (nbits - 1)) &&
+               (m - nbits + 1) + 1 + COCC_BITS + sb <= 64) {
+            nbits--;
+            const int64_t c = n_keys / 200;
+            cap = c > (int64_t)cap ? (unsigned int)(c < (1ll << 31) ? c : (1ll << 31)) : cap;
+        }
         const int r = m - nbits;
         if (r + 1 + COCC_BITS + sb > 64 || nbits > 40) return MA_OK;
         const int sb_inl = sb_pos > sb ? sb_pos : sb;
@@ -486,8 +538,12 @@ struct HashBuilder {
                          sb_inl <= 40;
         if (inl) sb = sb_inl;
         const int64_t bytes = (int64_t)CBUCKET_SLOTS * 8 << nbits;
-        if ((double)bytes > 0.5 * (double)free_hbm()) return MA_OK;
-        MA_TRY(prepare(index, stream));
+        VLOG("compact table: %lld keys, 2^%d buckets (%.1f GB, load %.3f), start bits %d, inline %d, "
+             "overflow cap %u, half of free HBM %.1f GB%s", (long long)n_keys, nbits, bytes / 1e9,
+             (double)n_keys / ((double)CBUCKET_SLOTS * (double)(1ll << nbits)), sb, (int)inl, cap, budget / 1e9,
+             (double)bytes > budget ? ": does not fit" : "");
+        if ((double)bytes > budget) return MA_OK;
+        MA_TRY(prepare(index, stream, cap));
         ix->compact = 1;
         ix->c_m = m;
         ix->c_r = r;
@@ -506,12 +562,12 @@ struct HashBuilder {
             compact_insert_kernel<<<grid_for(n), 256, 0, st>>>(
                 kmers, offsets, n, start_add, ix->table.as<uint64_t>(), ix->c_m, ix->c_r, ix->c_sb, ix->c_inline,
                 ix->postings.as<uint32_t>(), ix->coff.as<uint64_t>(), ix->n_contigs, ov_keys.as<uint64_t>(),
-                ov_vals.as<uint64_t>(), misc.as<unsigned int>(), OVERFLOW_CAP, misc.as<unsigned int>() + 3);
+                ov_vals.as<uint64_t>(), misc.as<unsigned int>(), ov_cap, misc.as<unsigned int>() + 3);
         } else {
             hash_insert_kernel<<<grid_for(n), 256, 0, st>>>(kmers, offsets, n, start_add, ix->table.as<HashSlot>(),
                                                             ix->n_buckets, ov_keys.as<uint64_t>(),
                                                             ov_vals.as<uint64_t>(), misc.as<unsigned int>(),
-                                                            OVERFLOW_CAP);
+                                                            ov_cap);
         }
         MA_LAUNCHED();
         return MA_OK;
@@ -527,11 +583,12 @@ struct HashBuilder {
             MA_HIP(hipStreamSynchronize(st));
             const unsigned int n_ov = hd[0];
             ix->n_cuckoo = (int64_t)n_ov;
+            VLOG("compact table: %u keys found both buckets full (cap %u), inline failure %u", n_ov, ov_cap, hd[3]);
             if (hd[3]) {
                 if (inl_fail) *inl_fail = true;
                 return MA_OK;
             }
-            if (n_ov > OVERFLOW_CAP) return MA_OK;
+            if (n_ov > ov_cap) return MA_OK;
             if (n_ov > 0) {
                 const uint64_t nb = (uint64_t)((double)n_ov / (BUCKET_SLOTS * 0.25)) + 1;
                 MA_TRY(ix->stash.alloc((size_t)nb * BUCKET_SLOTS * sizeof(HashSlot)));
@@ -554,7 +611,7 @@ struct HashBuilder {
         } else {
             hash_cuckoo_kernel<<<1, 64, 0, st>>>(ov_keys.as<uint64_t>(), ov_vals.as<uint64_t>(),
                                                  ix->table.as<HashSlot>(), ix->n_buckets, misc.as<unsigned int>(),
-                                                 OVERFLOW_CAP, STEP_BUDGET, misc.as<int>() + 1);
+                                                 ov_cap, STEP_BUDGET, misc.as<int>() + 1);
             MA_LAUNCHED();
         }
         int h[2] = {0, 0};
@@ -572,10 +629,7 @@ struct HashBuilder {
 // test knob: high loads send many keys through the cuckoo pass).
 int first_hash_load(int64_t n_keys, double *load) {
     *load = 0.5;
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) == hipSuccess &&
-        (double)n_keys / 0.15 * (double)sizeof(HashSlot) <= 0.1 * (double)fr)
-        *load = 0.15;
+    if ((double)n_keys / 0.15 * (double)sizeof(HashSlot) <= 0.1 * (double)free_hbm()) *load = 0.15;
     if (const char *env = getenv("MIALIGN_HASH_LOAD")) {
         const double v = atof(env);
         MA_REQUIRE(v > 0 && v <= 0.95, "MIALIGN_HASH_LOAD must be in (0, 0.95]");
@@ -900,6 +954,8 @@ int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
     };
     auto key_lo = [&](int b) -> uint64_t { return (uint64_t)b << shift; };
 
+    VLOG("chunked build: %lld valid windows, largest partition %llu, free HBM %.1f GB", (long long)n_valid,
+         (unsigned long long)max_bin, free_hbm() / 1e9);
     // sweep 1: unique keys per chunk
     ChunkBufs cb;
     int64_t n_keys = 0;
@@ -921,8 +977,10 @@ int build_chunked(ma_index *ix, int k, const ma_build_opts &o, hipStream_t st) {
         }
     }
     ix->n_keys = n_keys;
+    VLOG("sweep 1: %lld keys, free HBM %.1f GB with its chunk buffers", (long long)n_keys, free_hbm() / 1e9);
     cb.release();  // sweep-1 buffers go before the resident arrays are allocated
     MA_TRY(ix->postings.alloc((size_t)(n_valid > 0 ? n_valid : 1) * 8));
+    VLOG("postings allocated (%.1f GB), free HBM %.1f GB", (double)n_valid * 8 / 1e9, free_hbm() / 1e9);
     if (o.keep_dense) {
         MA_TRY(ix->kmers.alloc((size_t)(n_keys > 0 ? n_keys : 1) * 8));
         MA_TRY(ix->offsets.alloc((size_t)(n_keys + 1) * 8));
