@@ -197,6 +197,182 @@ def sort_section(n: int, cpu_sample: int, with_cpu: bool) -> dict:
     return out
 
 
+class StepRunner:
+    """One bench step over resident reads: map (FW + RC + merge), SE sort keys,
+    stable sorts + duplicate marking, all on one stream; the map kernel's time
+    from HIP events on that stream."""
+
+    def __init__(self, ix, packed, lens, L, tid, persistent=True):
+        import ctypes as C
+        import mialign
+        from mialign._lib import lib
+
+        self.C, self.mialign, self.lib = C, mialign, lib
+        self.ix, self.L, self.n, self.stride = ix, L, lens.size, packed.shape[1]
+        dev = 0
+        self.stream = mialign.Stream(dev)
+        self.d_bases = mialign.DeviceArray.from_host(packed, dev)
+        self.d_lens = mialign.DeviceArray.from_host(lens, dev)
+        self.d_tid = mialign.DeviceArray.from_host(tid, dev)
+        self.recs = mialign.DeviceRecs(self.n, dev)
+        self.ws = mialign.Workspace(ix, self.n, L, persistent=persistent)
+        self.prm = mialign.default_params(k=21)
+        self.keys = {f: mialign.DeviceArray((self.n,), dt, dev) for f, dt in
+                     (("coord", np.uint64), ("dhi", np.uint64), ("dlo", np.uint64), ("score", np.uint32),
+                      ("pair", np.uint32))}
+        self.outs = {f: mialign.DeviceArray((self.n,), np.uint32, dev) for f in ("is_dup", "perm", "dup_perm")}
+        self.scratch_bytes = int(lib().ma_sort_scratch_bytes(self.n))
+        self.scratch = mialign.DeviceArray((self.scratch_bytes,), np.uint8, dev)
+        self.ev_a, self.ev_b = mialign.Event(), mialign.Event()
+
+    def step(self, timed_kernel: bool) -> None:
+        from mialign._lib import MA_BS_C2T, MA_ENC_2BIT, check
+
+        C, lib, n, st, keys, outs = self.C, self.lib, self.n, self.stream, self.keys, self.outs
+        if timed_kernel:
+            self.ev_a.record(st)
+        self.mialign.map_batch_dev(self.ix, self.ws, st, self.d_bases, self.stride, self.d_lens, n, self.L,
+                                   self.recs, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T, params=self.prm)
+        if timed_kernel:
+            self.ev_b.record(st)
+        check(lib().ma_sort_keys_se_dev(st.handle, n, C.byref(self.recs.struct), self.d_lens.ptr, self.d_tid.ptr,
+                                        0, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
+                                        keys["score"].ptr, keys["pair"].ptr), "sort keys")
+        check(lib().ma_sort_markdup_dev(st.handle, n, keys["coord"].ptr, keys["dhi"].ptr,
+                                        keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr,
+                                        outs["is_dup"].ptr, outs["perm"].ptr, 1, outs["dup_perm"].ptr,
+                                        self.scratch.ptr, self.scratch_bytes), "sort markdup")
+
+    def run(self, steps: int, warmup: int, label: str):
+        """Returns (reads/s over the timed steps, ms per step, mean map-kernel ms)."""
+        for i in range(warmup):
+            self.step(False)
+            self.stream.sync()
+            log(f"{label}: warmup step {i + 1}/{warmup} done")
+        kernel_ms = []
+        self.stream.sync()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            self.step(True)
+            self.ev_b.synchronize()
+            kernel_ms.append(self.ev_a.elapsed_ms(self.ev_b))
+        self.stream.sync()
+        elapsed = time.perf_counter() - t_start
+        return self.n * steps / elapsed, 1000.0 * elapsed / steps, float(np.mean(kernel_ms))
+
+    def close(self) -> None:
+        for a in ("d_bases", "d_lens", "d_tid", "recs", "ws", "keys", "outs", "scratch"):
+            setattr(self, a, None)
+
+
+def cpu_map_baseline(ix, dual, bases, lens, tid, ns, what):
+    """Oracle on the first ns reads (OpenMP over reads + SE keys + stable-sort
+    markdup) on the host's threads; returns (baseline dict, oracle records,
+    oracle counters)."""
+    from oracle import oracle as orc
+
+    t0 = time.time()
+    km, of, po = ix.export_dense()
+    oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+    del km, of, po
+    conv = orc.convert_reads(bases[:ns], 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline ({what}): oracle index ready in {time.time() - t0:.1f}s; mapping {ns} reads on {threads} threads")
+    t0 = time.perf_counter()
+    want, _, _, cnt = orc.map_batch(oix, conv, lens[:ns], nthreads=threads)
+    kk = orc.sort_keys_se(want, lens[:ns], tid)
+    orc.sort_markdup(*kk)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": ns / cpu_s, "unit": "reads/s", "cores": threads, "kind": "port",
+           "sample": f"first {ns} reads of {what} (oracle map with OpenMP over reads + SE keys + "
+                     f"stable-sort markdup), {cpu_s:.1f} s"}
+    return cpu, want, cnt
+
+
+FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
+
+
+def genome_scale_section(args) -> dict:
+    """The 3.1 Gbp reference of the uneven-length and served configurations
+    (24 contigs, 5% low-complexity repeat blocks, both conversions indexed:
+    6.2 Gbp): (1) 150 bp reads, map + sort + markdup, next to the oracle on the
+    host's threads on a sample of the same reads (records compared); (2) reads
+    of 50-250 bp in one persistent launch per step; (3) the served loop over
+    those uneven reads with the index resident."""
+    import mialign
+    from mialign import synth
+
+    res = {"reference": "3.1 Gbp synthetic (24 contigs, 1% CpG, 5% low-complexity repeat blocks) x2 conversions"}
+    t0 = time.time()
+    ref = synth.gen_reference(31, 24, 3_100_000_000, repeat_frac=0.05)
+    dual = synth.dual_convert(ref)
+    tid = synth.tid_of_cid(dual.names)
+    log(f"genome scale: reference 3.1 Gbp x2 in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, 0, keep_dense=not args.no_cpu)
+    tb = time.time() - t0
+    res["index"] = {"keys": ix.n_keys, "postings": ix.n_postings, "build_s": round(tb, 1),
+                    "table": ix.hash_stats()}
+    log(f"genome scale: index {ix.n_keys} keys, {ix.n_postings} postings in {tb:.1f}s, {res['index']['table']}")
+
+    # (1) 150 bp reads
+    n, L = args.gs_reads, 150
+    reads = synth.gen_reads(1102, ref, n, L, L)
+    packed = synth.pack_2bit(reads.bases, reads.lens)
+    sr = StepRunner(ix, packed, reads.lens, L, tid)
+    v, ms, map_ms = sr.run(args.steps, args.warmup, "genome scale 150 bp")
+    merged = sr.recs.download()
+    sr.close()
+    del packed
+    r150 = {"reads": n, "read_len": L, "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3),
+            "map_kernel_ms": round(map_ms, 3), "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4)}
+    log(f"genome scale 150 bp: {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
+    if not args.no_cpu:
+        ns = min(args.gs_cpu_sample, n)
+        cpu, want, cnt = cpu_map_baseline(ix, dual, reads.bases, reads.lens, tid, ns,
+                                          "the 3.1 Gbp 150 bp workload")
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
+        bpr = algorithmic_bytes_per_read(cnt, ns, L)
+        achieved = bpr * n / (map_ms / 1000.0) / 1e9
+        r150["cpu_baseline"] = cpu
+        r150["x_cpu"] = round(v / cpu["value"], 1)
+        r150["parity_sample"] = {"reads": ns, "records_bit_exact": bool(ok)}
+        r150["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "map_reads_kernel",
+                            "kernel_ms": round(map_ms, 3), "algorithmic_bytes_per_read": round(bpr, 1)}
+        log(f"genome scale 150 bp: cpu {cpu['value']:.0f} reads/s ({r150['x_cpu']}x), sample parity {ok}, "
+            f"{bpr:.0f} B/read algorithmic")
+        del want
+    del reads, merged
+    res["reads_150bp"] = r150
+
+    # (2) uneven lengths, one persistent launch per step
+    n2 = args.gs_uneven_reads
+    ur = synth.gen_reads(1103, ref, n2, 50, 250, max_len=250)
+    upacked = synth.pack_2bit(ur.bases, ur.lens)
+    ulens = ur.lens
+    del ur
+    sr = StepRunner(ix, upacked, ulens, 250, tid)
+    v, ms, map_ms = sr.run(max(1, args.steps // 2), 1, "genome scale 50-250 bp")
+    merged = sr.recs.download()
+    sr.close()
+    res["reads_50_250bp"] = {"reads": n2, "read_len": "uniform 50-250", "mean_len": round(float(ulens.mean()), 1),
+                             "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3),
+                             "map_kernel_ms": round(map_ms, 3),
+                             "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
+                             "launch": "one persistent map launch over all reads per step"}
+    log(f"genome scale 50-250 bp: {n2} reads, {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
+    del merged
+
+    # (3) served loop over the uneven reads, index resident
+    if not args.no_serve:
+        res["serving"] = serve_section(ix, upacked, ulens, 250, args.serve_requests, args.serve_slots,
+                                       args.serve_launch_reads, args.serve_size_weight)
+        log(f"genome scale serving: {res['serving']}")
+    ix.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +396,12 @@ def main() -> int:
     ap.add_argument("--serve-only", action="store_true", help="skip the timed batch steps")
     ap.add_argument("--launch", choices=("persistent", "batch"), default="persistent",
                     help="map-kernel grid: persistent (8 blocks/CU + work counter) or sized to the batch")
+    ap.add_argument("--no-genome-scale", action="store_true",
+                    help="skip the 3.1 Gbp reference section (150 bp + 50-250 bp reads, served loop)")
+    ap.add_argument("--gs-reads", type=int, default=10_000_000, help="150 bp reads on the 3.1 Gbp index")
+    ap.add_argument("--gs-uneven-reads", type=int, default=50_000_000,
+                    help="50-250 bp reads on the 3.1 Gbp index (one launch)")
+    ap.add_argument("--gs-cpu-sample", type=int, default=2_000_000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -230,7 +412,6 @@ def main() -> int:
 
     import mialign
     from mialign import synth
-    from mialign._lib import MA_BS_C2T, MA_ENC_2BIT, check, lib
 
     dev = 0
     info = mialign.device_info(dev)
@@ -248,66 +429,20 @@ def main() -> int:
     n, L = args.reads, args.read_len
     reads = synth.gen_reads(1002, ref, n, L, L)
     packed = synth.pack_2bit(reads.bases, reads.lens)
-    stride = packed.shape[1]
     log(f"reads {n} x {L} bp generated + 2-bit packed in {time.time() - t0:.1f}s")
 
-    stream = mialign.Stream(dev)
-    d_bases = mialign.DeviceArray.from_host(packed, dev)
-    d_lens = mialign.DeviceArray.from_host(reads.lens, dev)
     tid = synth.tid_of_cid(dual.names)
-    d_tid = mialign.DeviceArray.from_host(tid, dev)
-    recs = mialign.DeviceRecs(n, dev)
-    ws = mialign.Workspace(ix, n, L, persistent=args.launch == "persistent")
-    prm = mialign.default_params(k=21)
-    keys = {f: mialign.DeviceArray((n,), dt, dev) for f, dt in
-            (("coord", np.uint64), ("dhi", np.uint64), ("dlo", np.uint64), ("score", np.uint32),
-             ("pair", np.uint32))}
-    outs = {f: mialign.DeviceArray((n,), np.uint32, dev) for f in ("is_dup", "perm", "dup_perm")}
-    scratch_bytes = int(lib().ma_sort_scratch_bytes(n))
-    scratch = mialign.DeviceArray((scratch_bytes,), np.uint8, dev)
-    import ctypes as C
-
-    ev_a, ev_b = mialign.Event(), mialign.Event()
-
-    def step(timed_kernel: bool):
-        if timed_kernel:
-            ev_a.record(stream)
-        mialign.map_batch_dev(ix, ws, stream, d_bases, stride, d_lens, n, L, recs,
-                              encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T, params=prm)
-        if timed_kernel:
-            ev_b.record(stream)
-        check(lib().ma_sort_keys_se_dev(stream.handle, n, C.byref(recs.struct), d_lens.ptr, d_tid.ptr,
-                                        0, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
-                                        keys["score"].ptr, keys["pair"].ptr), "sort keys")
-        check(lib().ma_sort_markdup_dev(stream.handle, n, keys["coord"].ptr, keys["dhi"].ptr,
-                                        keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr,
-                                        outs["is_dup"].ptr, outs["perm"].ptr, 1, outs["dup_perm"].ptr,
-                                        scratch.ptr, scratch_bytes), "sort markdup")
-
+    sr = StepRunner(ix, packed, reads.lens, L, tid, persistent=args.launch == "persistent")
     if args.serve_only:
         args.warmup, args.steps = 0, 1
-    for i in range(args.warmup):
-        step(False)
-        stream.sync()
-        log(f"warmup step {i + 1}/{args.warmup} done")
-    kernel_ms = []
-    stream.sync()
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        step(True)
-        ev_b.synchronize()
-        kernel_ms.append(ev_a.elapsed_ms(ev_b))
-    stream.sync()
-    elapsed = time.perf_counter() - t_start
-    ms_per_step = 1000.0 * elapsed / args.steps
-    value = n * args.steps / elapsed
-    map_ms = float(np.mean(kernel_ms))
+    value, ms_per_step, map_ms = sr.run(args.steps, args.warmup, "config 2")
     log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, map kernel {map_ms:.1f} ms, "
         f"{value / 1e6:.2f} M reads/s")
 
-    merged = recs.download()
-    is_dup = outs["is_dup"].download()
+    merged = sr.recs.download()
+    is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
+    sr.close()
 
     # ---- served request loop vs plain one-batch-per-launch (same batches)
     serving = None
@@ -321,26 +456,9 @@ def main() -> int:
     roofline = None
     parity_sample = None
     if not args.no_cpu:
-        from oracle import oracle as orc
-
-        t0 = time.time()
-        km, of, po = ix.export_dense()
-        oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
-        del km, of, po
         ns = min(args.cpu_sample, n)
-        conv = orc.convert_reads(reads.bases[:ns], 1)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        log(f"cpu baseline: oracle index ready in {time.time() - t0:.1f}s; mapping {ns} reads on {threads} threads")
-        t0 = time.perf_counter()
-        want, _, _, cnt = orc.map_batch(oix, conv, reads.lens[:ns], nthreads=threads)
-        kk = orc.sort_keys_se(want, reads.lens[:ns], tid)
-        wd, wp, _ = orc.sort_markdup(*kk)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": ns / cpu_s, "unit": "reads/s", "cores": threads, "kind": "port",
-               "sample": f"first {ns} reads of the same workload (oracle map with OpenMP over reads "
-                         f"+ SE keys + stable-sort markdup), {cpu_s:.1f} s"}
-        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in
-                 ("cid", "pos", "mapq", "flag", "sl", "sr", "nm"))
+        cpu, want, cnt = cpu_map_baseline(ix, dual, reads.bases, reads.lens, tid, ns, "the same workload")
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
         parity_sample = {"reads": ns, "records_bit_exact": bool(ok)}
         bpr = algorithmic_bytes_per_read(cnt, ns, L)
         achieved = bpr * n / (map_ms / 1000.0) / 1e9
@@ -356,11 +474,15 @@ def main() -> int:
                     "kernel": "map_reads_kernel", "kernel_ms": round(map_ms, 3),
                     "algorithmic_bytes_per_read": round(bpr, 1)}
         log(f"cpu baseline {cpu['value']:.0f} reads/s; sample parity {ok}; {bpr:.0f} B/read algorithmic")
+        del want
+    ix.close()
+    del reads, packed, dual, ref
 
     sort_stress = None
     if args.sort_records > 0:
-        del d_bases, d_lens, recs, keys, outs, scratch
         sort_stress = sort_section(args.sort_records, args.sort_cpu_sample, not args.no_cpu)
+
+    genome_scale = None if args.no_genome_scale else genome_scale_section(args)
 
     line = {
         "metric": METRIC,
@@ -391,9 +513,9 @@ def main() -> int:
         "parity_sample": parity_sample,
         "serving": serving,
         "sort_stress": sort_stress,
+        "genome_scale": genome_scale,
     }
     print(json.dumps(line), flush=True)
-    ix.close()
     return 0
 
 

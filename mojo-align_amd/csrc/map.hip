@@ -131,6 +131,15 @@ DEV uint32_t wave_min_u32(uint32_t v) {
     return v;
 }
 
+DEV uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
 DEV uint64_t readlane64(uint64_t v, int lane) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((uint32_t)(v >> 32), lane);
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((uint32_t)v, lane);
