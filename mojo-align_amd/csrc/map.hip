@@ -35,6 +35,8 @@ namespace {
 
 constexpr int TOP = 256;  // vote table (TOP, linear_gpu_locate.mojo:812)
 constexpr int KEEP = 16;  // extend picks (KEEP, :813)
+constexpr int VHASH_BITS = 9;  // vote hash: 512 slots for at most TOP keys
+constexpr int VHASH = 1 << VHASH_BITS;
 constexpr int HX = 48;    // exact tally (HX, :1249)
 constexpr int MAXS = 64;  // seeds per read (one per lane)
 constexpr int WPB = 4;    // waves per block
