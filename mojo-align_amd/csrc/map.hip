@@ -38,6 +38,8 @@ constexpr int KEEP = 16;  // extend picks (KEEP, :813)
 constexpr int VHASH_BITS = 9;  // vote hash: 512 slots for at most TOP keys
 constexpr int VHASH = 1 << VHASH_BITS;
 constexpr int HX = 48;    // exact tally (HX, :1249)
+constexpr int HXHASH_BITS = 7;  // tally hash: 128 slots for HX keys
+constexpr int HXHASH = 1 << HXHASH_BITS;
 constexpr int MAXS = 64;  // seeds per read (one per lane)
 constexpr int WPB = 4;    // waves per block
 constexpr int BLOCK = WPB * 64;
