@@ -130,7 +130,7 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
 # 10M x 150 bp reads): rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
 # factors read off copy_u64_kernel (a copy of known size) in the same runs.
 MAP_TRAFFIC_PROFILED = {"genome_bp": 100_000_000, "reads": 10_000_000, "read_len": 150,
-                        "bytes": 82.6e9, "kernel_ms": 41.98,
+                        "bytes": 85.2e9, "kernel_ms": 40.5,
                         "source": "profiles/map_reads_kernel_pmc_100Mbp_10Mx150.md"}
 SORT_BYTES_PER_RECORD = 170  # 2 x 13 B x 6 digit passes (45-bit key) + 13 B markdup read + 1 B flag
 
