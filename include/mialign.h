@@ -23,6 +23,12 @@
  *   ma_device_info .......... select_device / probe_device_context + hbm_info,
  *                             gpu-common/src/gpu_device.mojo:19-63, gpu_kernels.mojo:42-65,
  *                             gpu-common/python/gpu_mem.py:88-121
+ *   ma_bam_pack_se_dev ...... _pack_bam_aln over the SE branch of _emit_bam_from_batch
+ *                             + _gather_perm_to_writer / _or_dup_flag,
+ *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 766-800, 177-225, 408-420
+ *   ma_bam_pack_dev ......... _pack_bam_aln from explicit per-record fields (:497-665)
+ *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw,
+ *                             fq2bam-meth/python/bam_emit.py:77-185, 467-471
  *   ma_serve_* .............. no single reference counterpart: the request loop
  *                             generalises the batch loop of map_fastq_dense_gpu_locate
  *                             with admission by free HBM (gpu_mem.py:203-377)
@@ -205,6 +211,64 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
 /* Stable LSD radix sort of u64 keys carrying u32 values (device pointers). */
 int ma_radix_sort_u64_dev(void *stream, int64_t n, uint64_t *d_keys, uint32_t *d_vals,
                           void *d_scratch, int64_t scratch_bytes);
+
+/* ---- BAM records and BGZF ---- */
+/* Per-record BAM fields (device int32 arrays); ntid / npos / tlen may be NULL
+ * (mate tid -1, mate pos -1, tlen 0). Meaning as _pack_bam_aln's arguments. */
+typedef struct {
+    const int32_t *flag, *tid, *pos, *mapq, *sl, *sr, *nm, *ntid, *npos, *tlen;
+} ma_bam_fields;
+/* The reads of a batch (device pointers): read i's name is
+ * names[name_off[i] .. name_off[i+1]) (no terminator), its original
+ * (pre-conversion) ASCII bases seq + i*stride (lens[i] of them) and its FASTQ
+ * quality qual + i*stride (qual_lens[i] bytes, qual_lens NULL = lens; qual
+ * NULL = absent, QUAL 0xFF). */
+typedef struct {
+    const uint8_t *names;
+    const int64_t *name_off;
+    const uint8_t *seq;
+    const uint8_t *qual;
+    const uint32_t *lens;
+    const uint32_t *qual_lens;
+    int64_t stride;
+} ma_bam_reads;
+/* Device scratch for n records; bytes one record takes at most. */
+int64_t ma_bam_scratch_bytes(int64_t n);
+int64_t ma_bam_record_bound(int32_t name_len, int32_t len, int32_t rg_n);
+/* Uncompressed BAM alignment records (block_size first, little endian) of n
+ * single-end merged map records, written back to back into d_out in output
+ * order: output record j is read perm[j] (d_perm NULL = identity), with flag
+ * 0x400 set when apply_dup and d_is_dup[perm[j]] != 0. Fields as the
+ * reference's SE batch emit: flag = record flag | 0x41, tid = d_tid_of_cid[cid]
+ * (cid outside [0, n_tid) -> unmapped, flag | 4), CIGAR slS mM srS (clips
+ * swapped for flag 16), SEQ / QUAL reversed (and complemented) for flag 16,
+ * tags RG:Z:<rg> NM:C. d_rec_off[n+1] receives each record's byte offset and
+ * the total in d_rec_off[n]; records that would pass out_cap are not written
+ * (the caller compares d_rec_off[n] with out_cap). d_out 4-byte aligned;
+ * rg_n <= 240. Enqueued on stream. */
+int ma_bam_pack_se_dev(void *stream, int64_t n, const ma_recs *d_merged, const int32_t *d_tid_of_cid,
+                       int32_t n_tid, const ma_bam_reads *reads, const char *rg, int32_t rg_n,
+                       const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
+                       uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,
+                       int64_t scratch_bytes);
+/* The same from explicit fields (the generic _pack_bam_aln batch). */
+int ma_bam_pack_dev(void *stream, int64_t n, const ma_bam_fields *fields, const ma_bam_reads *reads,
+                    const char *rg, int32_t rg_n, const uint32_t *d_perm, const uint32_t *d_is_dup,
+                    int32_t apply_dup, uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off,
+                    void *d_scratch, int64_t scratch_bytes);
+/* BGZF: consecutive 65280-byte members (raw deflate at `level`, zlib) of a
+ * host stream, compressed by `threads` workers, optionally followed by the
+ * 28-byte EOF member; ma_bgzf_bound(n) bytes always suffice. */
+int64_t ma_bgzf_bound(int64_t n);
+int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threads, int32_t add_eof,
+                     uint8_t *dst, int64_t dst_cap, int64_t *out_n);
+/* A BGZF file writer: writes accumulate into 65280-byte members exactly as one
+ * stream (a member never waits on a record boundary); close flushes the last
+ * partial member and the EOF member. */
+typedef struct ma_bgzf ma_bgzf;
+int ma_bgzf_open(const char *path, int32_t level, int32_t threads, ma_bgzf **out);
+int ma_bgzf_write(ma_bgzf *w, const void *data, int64_t n);
+int ma_bgzf_close(ma_bgzf *w, int64_t *bytes_written);
 
 /* ---- gpu-common seed kernels ---- */
 /* ASCII reads [n x max_len] -> per-position Wang hash of the MSB-first k-mer

@@ -1,0 +1,598 @@
+// BAM alignment records on the device (gather by the sort's permutation, the
+// duplicate flag applied in the same pass) and BGZF framing on the host.
+//
+// Record format: _pack_bam_aln, fq2bam-meth/src/linear_gpu_fm.mojo:497-665;
+// gather + 0x400: _gather_perm_to_writer (:177-225) and _or_dup_flag (:408-420);
+// SE field derivation: _emit_bam_from_batch (:766-800); BGZF members:
+// _bgzf_block / _BgzfWriter, fq2bam-meth/python/bam_emit.py:77-185.
+//
+// Design: two kernels and a scan. bam_size_kernel writes every output record's
+// length (output order j, source record perm[j]); an exclusive scan turns them
+// into byte offsets (the total lands in rec_off[n]); bam_pack_kernel then
+// gives each record one wave, whose lanes each produce a 4-byte-aligned
+// word of the output stream (byte stores only where a word is shared with a
+// neighbouring record), so the stream is written once, in order, with no host
+// gather and no staging copy.
+#include <zlib.h>
+
+#include <algorithm>
+#include <thread>
+#include <vector>
+
+#include "ma_sort.hpp"
+
+using namespace ma;
+
+namespace {
+
+constexpr int BAM_RG_MAX = 240;   // read-group id bytes carried in the kernel arguments
+constexpr int BAM_WPB = 4;        // waves (records in flight) per 256-thread block
+constexpr int64_t BGZF_BLOCK = 65280;  // uncompressed bytes per BGZF member (bam_emit.py:25)
+
+struct RgArg {
+    int32_t n;
+    uint8_t b[BAM_RG_MAX];
+};
+
+// Where a record's fields come from: SE mode derives them from merged map
+// records (+ contig -> target table); otherwise ten int32 arrays (the last
+// three may be null: mate tid -1, mate pos -1, tlen 0).
+struct FieldSrc {
+    int32_t se;
+    int32_t n_tid;
+    const int32_t *cid;
+    const uint32_t *pos, *mapq;
+    const int32_t *rflag;
+    const uint32_t *sl, *sr, *nm;
+    const int32_t *tid_of_cid;
+    const int32_t *f[10];  // flag tid pos mapq sl sr nm ntid npos tlen
+};
+
+struct ReadSrc {
+    const uint8_t *names;
+    const int64_t *name_off;
+    const uint8_t *seq;
+    const uint8_t *qual;
+    const uint32_t *lens;
+    const uint32_t *qual_lens;
+    int64_t stride;
+};
+
+struct RecF {
+    int64_t flag, tid, pos, mapq, sl, sr, nm, ntid, npos, tlen;
+};
+
+__device__ __forceinline__ RecF load_fields(const FieldSrc &s, int64_t i) {
+    RecF r;
+    if (s.se) {
+        // _emit_bam_from_batch SE: flag | 1 | 64; unknown contig -> tid -1, flag | 4.
+        const int32_t c = s.cid[i];
+        r.flag = (int64_t)s.rflag[i] | 1 | 64;
+        r.tid = -1;
+        if (c >= 0 && c < s.n_tid)
+            r.tid = s.tid_of_cid[c];
+        else
+            r.flag |= 4;
+        r.pos = s.pos[i];
+        r.mapq = s.mapq[i];
+        r.sl = s.sl[i];
+        r.sr = s.sr[i];
+        r.nm = s.nm[i];
+        r.ntid = -1;
+        r.npos = -1;
+        r.tlen = 0;
+    } else {
+        r.flag = s.f[0][i];
+        r.tid = s.f[1][i];
+        r.pos = s.f[2][i];
+        r.mapq = s.f[3][i];
+        r.sl = s.f[4][i];
+        r.sr = s.f[5][i];
+        r.nm = s.f[6][i];
+        r.ntid = s.f[7] ? s.f[7][i] : -1;
+        r.npos = s.f[8] ? s.f[8][i] : -1;
+        r.tlen = s.f[9] ? s.f[9][i] : 0;
+    }
+    return r;
+}
+
+__host__ __device__ __forceinline__ int64_t reg2bin(int64_t beg, int64_t end) {
+    if (end <= beg) end = beg + 1;
+    end -= 1;
+    if ((beg >> 14) == (end >> 14)) return 4681 + (beg >> 14);
+    if ((beg >> 17) == (end >> 17)) return 585 + (beg >> 17);
+    if ((beg >> 20) == (end >> 20)) return 73 + (beg >> 20);
+    if ((beg >> 23) == (end >> 23)) return 9 + (beg >> 23);
+    if ((beg >> 26) == (end >> 26)) return 1 + (beg >> 26);
+    return 0;
+}
+
+// A C G T (either case) -> 1 2 4 8, else 15 (the 4-bit BAM codes of _nt16).
+__device__ __forceinline__ uint32_t nt16(uint32_t b) {
+    const uint32_t u = b & 0xDF;  // upper case
+    return u == 'A' ? 1u : u == 'C' ? 2u : u == 'G' ? 4u : u == 'T' ? 8u : 15u;
+}
+
+// 1<->8, 2<->4, else 15 (_nt16_comp): a 4-bit reverse of a one-hot code.
+__device__ __forceinline__ uint32_t nt16_comp(uint32_t c) {
+    return (c == 1 || c == 2 || c == 4 || c == 8) ? (((c & 1) << 3) | ((c & 2) << 1) | ((c & 4) >> 1) | (c >> 3))
+                                                  : 15u;
+}
+
+// Byte layout of one record (offsets from the record's block_size field).
+struct RecLay {
+    uint32_t hdr[9];  // block_size, refID, pos, bin_mq_nl, flag_nc, l_seq, next_refID, next_pos, tlen
+    uint32_t cig[3];
+    int32_t name_len, qlen, o_cig, o_seq, o_qual, o_tags, total;
+    int32_t rev, has_qual, nm8;
+};
+
+__device__ __forceinline__ RecLay lay_out(const RecF &f, int64_t name_len, int64_t qlen,
+                                          int64_t qual_len, bool qual_ptr, int rg_n) {
+    RecLay L;
+    const bool rev = (f.flag & 16) != 0;
+    int64_t slv = f.sl < 0 ? 0 : f.sl, srv = f.sr < 0 ? 0 : f.sr;
+    if (slv + srv >= qlen) slv = srv = 0;
+    const int64_t mid = qlen - slv - srv;
+    if (rev) {
+        const int64_t t = slv;
+        slv = srv;
+        srv = t;
+    }
+    const bool mapped = f.tid >= 0 && (f.flag & 4) == 0;
+    // CIGAR slS mM srS, the empty operations left out (selects, not a
+    // dynamically indexed array, which would live in scratch)
+    const bool a = mapped && slv > 0, b = mapped && mid > 0, c = mapped && srv > 0;
+    const uint32_t opA = (uint32_t)((slv << 4) | 4), opB = (uint32_t)(mid << 4), opC = (uint32_t)((srv << 4) | 4);
+    const uint32_t tailBC = b ? opB : (c ? opC : 0u), tailC = c ? opC : 0u;
+    L.cig[0] = a ? opA : tailBC;
+    L.cig[1] = a ? tailBC : (b ? tailC : 0u);
+    L.cig[2] = (a && b) ? tailC : 0u;
+    const int nc = (int)a + (int)b + (int)c;
+    const int64_t l_qname = name_len + 1;
+    const int64_t block = 32 + l_qname + 4 * nc + (qlen + 1) / 2 + qlen + (3 + rg_n + 1 + 4);
+    int64_t bin = 4680, tid_w = -1, pos_w = -1;
+    if (mapped) {
+        tid_w = f.tid;
+        pos_w = f.pos;
+        int64_t end = f.pos + mid;
+        if (end <= f.pos) end = f.pos + 1;
+        bin = reg2bin(f.pos, end);
+    }
+    int64_t ntid = f.ntid, npos = f.npos;
+    if (ntid < 0) ntid = npos = -1;
+    const int64_t mq = f.mapq < 0 ? 0 : (f.mapq > 255 ? 255 : f.mapq);
+    L.hdr[0] = (uint32_t)block;
+    L.hdr[1] = (uint32_t)tid_w;
+    L.hdr[2] = (uint32_t)pos_w;
+    L.hdr[3] = (uint32_t)((bin << 16) | (mq << 8) | l_qname);
+    L.hdr[4] = (uint32_t)((f.flag << 16) | nc);
+    L.hdr[5] = (uint32_t)qlen;
+    L.hdr[6] = (uint32_t)ntid;
+    L.hdr[7] = (uint32_t)npos;
+    L.hdr[8] = (uint32_t)f.tlen;
+    L.name_len = (int32_t)name_len;
+    L.qlen = (int32_t)qlen;
+    L.o_cig = (int32_t)(36 + l_qname);
+    L.o_seq = L.o_cig + 4 * nc;
+    L.o_qual = L.o_seq + (int32_t)((qlen + 1) / 2);
+    L.o_tags = L.o_qual + (int32_t)qlen;
+    L.total = (int32_t)(4 + block);
+    L.rev = rev;
+    L.has_qual = qual_ptr && qual_len == qlen;
+    L.nm8 = (int32_t)(f.nm < 0 ? 0 : (f.nm > 255 ? 255 : f.nm));
+    return L;
+}
+
+__device__ __forceinline__ uint32_t word_byte(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
+
+// Byte b of a record (0 <= b < L.total); `words` = the record's 9 header words
+// and 3 CIGAR words (LDS: a lane-indexed array would live in scratch).
+__device__ __forceinline__ uint32_t rec_byte(const RecLay &L, const uint32_t *words, int b,
+                                             const uint8_t *__restrict__ name,
+                                             const uint8_t *__restrict__ seq,
+                                             const uint8_t *__restrict__ qual,
+                                             const uint8_t *__restrict__ rg, int rg_n) {
+    if (b < 36) return word_byte(words[b >> 2], b & 3);
+    if (b < L.o_cig) {
+        const int t = b - 36;
+        return t < L.name_len ? (uint32_t)name[t] : 0u;
+    }
+    if (b < L.o_seq) {
+        const int t = b - L.o_cig;
+        return word_byte(words[9 + (t >> 2)], t & 3);
+    }
+    if (b < L.o_qual) {
+        const int i = 2 * (b - L.o_seq);
+        uint32_t a, c = 0;
+        if (L.rev) {
+            a = nt16_comp(nt16(seq[L.qlen - 1 - i]));
+            if (i + 1 < L.qlen) c = nt16_comp(nt16(seq[L.qlen - 2 - i]));
+        } else {
+            a = nt16(seq[i]);
+            if (i + 1 < L.qlen) c = nt16(seq[i + 1]);
+        }
+        return (a << 4) | c;
+    }
+    if (b < L.o_tags) {
+        if (!L.has_qual) return 255u;
+        const int t = b - L.o_qual;
+        const uint32_t q = qual[L.rev ? L.qlen - 1 - t : t];
+        return q >= 33 ? q - 33 : 255u;
+    }
+    const int t = b - L.o_tags;
+    if (t < 3) return t == 0 ? 'R' : (t == 1 ? 'G' : 'Z');
+    if (t < 3 + rg_n) return rg[t - 3];
+    if (t == 3 + rg_n) return 0u;
+    const int u = t - 4 - rg_n;
+    return u == 0 ? 'N' : (u == 1 ? 'M' : (u == 2 ? 'C' : (uint32_t)L.nm8));
+}
+
+__device__ __forceinline__ int64_t src_of(const uint32_t *perm, int64_t j) {
+    return perm ? (int64_t)perm[j] : j;
+}
+
+__global__ void bam_size_kernel(int64_t n, FieldSrc fs, ReadSrc rs, int rg_n,
+                                const uint32_t *__restrict__ perm, uint32_t *__restrict__ sizes) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = src_of(perm, j);
+        const RecF f = load_fields(fs, i);
+        const int64_t qlen = rs.lens[i];
+        const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
+        const RecLay L = lay_out(f, rs.name_off[i + 1] - rs.name_off[i], qlen, ql, rs.qual != nullptr, rg_n);
+        sizes[j] = (uint32_t)L.total;
+    }
+}
+
+// The read-group id bytes, from the kernel arguments into device memory (one
+// wave), so the pack kernel reads them with ordinary loads.
+__global__ void bam_rg_kernel(RgArg rg, uint8_t *__restrict__ dst) {
+    for (int t = threadIdx.x; t < BAM_RG_MAX; t += blockDim.x) dst[t] = rg.b[t];
+}
+
+__global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, ReadSrc rs,
+                                                       const uint8_t *__restrict__ rg, int rg_n,
+                                                       const uint32_t *__restrict__ perm,
+                                                       const uint32_t *__restrict__ is_dup, int apply_dup,
+                                                       const uint64_t *__restrict__ rec_off,
+                                                       uint8_t *__restrict__ out, int64_t out_cap) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * BAM_WPB + (threadIdx.x >> 6));
+    const int64_t n_waves = (int64_t)gridDim.x * BAM_WPB;
+    uint32_t *const out32 = reinterpret_cast<uint32_t *>(out);
+    __shared__ uint32_t s_words[BAM_WPB][12];
+    uint32_t *const wl = s_words[threadIdx.x >> 6];
+    for (int64_t j = wave; j < n; j += n_waves) {
+        const int64_t i = src_of(perm, j);
+        RecF f = load_fields(fs, i);
+        if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
+        const int64_t qlen = rs.lens[i];
+        const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
+        const int64_t nb = rs.name_off[i];
+        const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
+        const int64_t base = (int64_t)rec_off[j];
+        if (base + L.total > out_cap) continue;  // the caller sees rec_off[n] > out_cap
+        __builtin_amdgcn_wave_barrier();  // the previous record's reads of wl are done
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            if (lane == k) wl[k] = k < 9 ? L.hdr[k] : L.cig[k - 9];
+        __builtin_amdgcn_wave_barrier();
+        const uint8_t *name = rs.names + nb;
+        const uint8_t *seq = rs.seq + i * rs.stride;
+        const uint8_t *qual = rs.qual ? rs.qual + i * rs.stride : nullptr;
+        const int64_t d0 = base >> 2, d1 = (base + L.total - 1) >> 2;
+        for (int64_t d = d0 + lane; d <= d1; d += 64) {
+            const int b0 = (int)(4 * d - base);
+            if (b0 >= 0 && b0 + 4 <= L.total) {
+                uint32_t w = rec_byte(L, wl, b0, name, seq, qual, rg, rg_n);
+                w |= rec_byte(L, wl, b0 + 1, name, seq, qual, rg, rg_n) << 8;
+                w |= rec_byte(L, wl, b0 + 2, name, seq, qual, rg, rg_n) << 16;
+                w |= rec_byte(L, wl, b0 + 3, name, seq, qual, rg, rg_n) << 24;
+                out32[d] = w;
+            } else {
+                // a word shared with the neighbouring record: this record's bytes only
+                for (int k = 0; k < 4; k++) {
+                    const int b = b0 + k;
+                    if (b >= 0 && b < L.total) out[base + b] = (uint8_t)rec_byte(L, wl, b, name, seq, qual, rg, rg_n);
+                }
+            }
+        }
+    }
+}
+
+struct BamScratch {
+    size_t sizes, partial, rg, total;
+};
+
+BamScratch bam_layout(int64_t n) {
+    BamScratch s;
+    size_t off = 0;
+    auto take = [&](size_t b) {
+        size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    s.sizes = take((size_t)(n < 1 ? 1 : n) * 4);
+    s.partial = take(scan_scratch_bytes(n < 1 ? 1 : n));
+    s.rg = take(BAM_RG_MAX);
+    s.total = off;
+    return s;
+}
+
+int bam_pack(void *stream, int64_t n, const FieldSrc &fs, const ma_bam_reads *r, const char *rg,
+             int32_t rg_n, const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
+             uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,
+             int64_t scratch_bytes, const char *who) {
+    MA_REQUIRE(n >= 0 && r && d_rec_off, std::string(who) + ": bad arguments");
+    MA_REQUIRE(n <= 0xFFFFFFFFll, std::string(who) + ": n exceeds u32 record ids");
+    MA_REQUIRE(rg_n >= 0 && rg_n <= BAM_RG_MAX && (rg_n == 0 || rg),
+               std::string(who) + ": read-group id longer than 240 bytes");
+    MA_REQUIRE(!apply_dup || d_is_dup, std::string(who) + ": apply_dup without is_dup");
+    MA_REQUIRE(((uintptr_t)d_out & 3) == 0, std::string(who) + ": output not 4-byte aligned");
+    BamScratch S = bam_layout(n);
+    MA_REQUIRE(scratch_bytes >= (int64_t)S.total, std::string(who) + ": scratch too small");
+    hipStream_t st = (hipStream_t)stream;
+    if (n == 0) return fill_u64(st, d_rec_off, 1, 0);
+    MA_REQUIRE(r->names && r->name_off && r->seq && r->lens, std::string(who) + ": null read array");
+    ReadSrc rs{r->names, r->name_off, r->seq, r->qual, r->lens, r->qual_lens, r->stride};
+    RgArg ra;
+    ra.n = rg_n;
+    for (int k = 0; k < BAM_RG_MAX; k++) ra.b[k] = k < rg_n ? (uint8_t)rg[k] : 0;
+    char *s = (char *)d_scratch;
+    uint32_t *sizes = (uint32_t *)(s + S.sizes);
+    bam_size_kernel<<<grid_for(n), 256, 0, st>>>(n, fs, rs, rg_n, d_perm, sizes);
+    MA_LAUNCHED();
+    MA_TRY(exclusive_scan_u32(st, n, sizes, d_rec_off, (uint64_t *)(s + S.partial), d_rec_off + n));
+    uint8_t *d_rg = (uint8_t *)(s + S.rg);
+    bam_rg_kernel<<<1, 64, 0, st>>>(ra, d_rg);
+    MA_LAUNCHED();
+    const int64_t blocks = std::min<int64_t>((n + BAM_WPB - 1) / BAM_WPB, 256 * 64);
+    bam_pack_kernel<<<(unsigned)blocks, 256, 0, st>>>(n, fs, rs, d_rg, rg_n, d_perm, d_is_dup, apply_dup,
+                                                      d_rec_off, d_out, out_cap);
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
+// ------------------------------------------------------------------ BGZF (host)
+constexpr uint8_t BGZF_EOF[28] = {0x1f, 0x8b, 0x08, 0x04, 0x00, 0x00, 0x00, 0x00, 0x00, 0xff,
+                                  0x06, 0x00, 0x42, 0x43, 0x02, 0x00, 0x1b, 0x00, 0x03, 0x00,
+                                  0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00, 0x00};
+constexpr int64_t BGZF_MEMBER_MAX = 65536;
+
+void put_le16(uint8_t *p, uint32_t v) {
+    p[0] = (uint8_t)v;
+    p[1] = (uint8_t)(v >> 8);
+}
+void put_le32(uint8_t *p, uint32_t v) {
+    for (int k = 0; k < 4; k++) p[k] = (uint8_t)(v >> (8 * k));
+}
+
+// One member (header + raw deflate + CRC32 + ISIZE) of src[0..n) into dst
+// (BGZF_MEMBER_MAX bytes); returns its length or -1.
+int64_t bgzf_member(const uint8_t *src, int64_t n, int level, uint8_t *dst) {
+    z_stream zs{};
+    if (deflateInit2(&zs, level, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY) != Z_OK) return -1;
+    // The reference compresses with Python's zlib object: compress() (Z_NO_FLUSH)
+    // then flush() (Z_FINISH), each into an output buffer that starts at 32 KiB
+    // and grows by 64 KiB while deflate fills it. Stored blocks (level 0) are cut
+    // at those buffer ends, so the same call pattern gives the same bytes.
+    zs.next_in = const_cast<Bytef *>(src);
+    zs.avail_in = (uInt)n;
+    uint8_t *const out0 = dst + 18;
+    const int64_t room = BGZF_MEMBER_MAX - 18 - 8;
+    int rc = Z_OK;
+    for (int phase = 0; phase < 2 && rc != Z_STREAM_ERROR; phase++) {
+        int64_t grant = 32768;
+        do {
+            const int64_t used = (int64_t)zs.total_out;
+            const int64_t give = std::min(grant, room - used);
+            if (give <= 0) {
+                rc = Z_STREAM_ERROR;  // the member would pass 64 KiB
+                break;
+            }
+            zs.next_out = out0 + used;
+            zs.avail_out = (uInt)give;
+            rc = deflate(&zs, phase == 0 ? Z_NO_FLUSH : Z_FINISH);
+            grant = 65536;
+        } while (zs.avail_out == 0 && rc != Z_STREAM_ERROR && rc != Z_STREAM_END);
+    }
+    const int64_t clen = (int64_t)zs.total_out;
+    deflateEnd(&zs);
+    if (rc != Z_STREAM_END) return -1;
+    static const uint8_t head[16] = {31, 139, 8, 4, 0, 0, 0, 0, 0, 255, 6, 0, 66, 67, 2, 0};
+    for (int k = 0; k < 16; k++) dst[k] = head[k];
+    const int64_t total = 18 + clen + 8;
+    put_le16(dst + 16, (uint32_t)(total - 1));
+    put_le32(dst + 18 + clen, (uint32_t)crc32(0L, src, (uInt)n));
+    put_le32(dst + 18 + clen + 4, (uint32_t)n);
+    return total;
+}
+
+// Compresses src into consecutive 65280-byte members with up to `threads`
+// workers; worker t's members, in order, end up in parts[t] (so parts[0],
+// parts[1], ... concatenated is the BGZF stream). Returns false on a zlib error.
+bool bgzf_compress_parts(const uint8_t *src, int64_t n, int level, int threads,
+                         std::vector<std::vector<uint8_t>> &parts) {
+    const int64_t nb = (n + BGZF_BLOCK - 1) / BGZF_BLOCK;
+    parts.clear();
+    if (nb == 0) return true;
+    const int T = std::max(1, std::min<int>(threads, (int)std::min<int64_t>(nb, 256)));
+    parts.resize(T);
+    std::vector<int> ok(T, 1);
+    auto work = [&](int t) {
+        const int64_t b0 = nb * t / T, b1 = nb * (t + 1) / T;
+        parts[t].resize((size_t)(b1 - b0) * BGZF_MEMBER_MAX);
+        int64_t off = 0;
+        for (int64_t b = b0; b < b1; b++) {
+            const int64_t s = b * BGZF_BLOCK, len = std::min(BGZF_BLOCK, n - s);
+            const int64_t m = bgzf_member(src + s, len, level, parts[t].data() + off);
+            if (m < 0) {
+                ok[t] = 0;
+                return;
+            }
+            off += m;
+        }
+        parts[t].resize((size_t)off);
+    };
+    if (T == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < T; t++) pool.emplace_back(work, t);
+        for (auto &th : pool) th.join();
+    }
+    for (int t = 0; t < T; t++)
+        if (!ok[t]) return false;
+    return true;
+}
+
+}  // namespace
+
+struct ma_bgzf {
+    FILE *fh = nullptr;
+    int level = 1, threads = 1;
+    std::vector<uint8_t> tail;  // < 65280 bytes waiting for the next member
+    std::vector<std::vector<uint8_t>> parts;  // compressed members of one write, per worker
+    int64_t written = 0;
+};
+
+extern "C" {
+
+int64_t ma_bam_scratch_bytes(int64_t n) { return (int64_t)bam_layout(n).total; }
+
+int64_t ma_bam_record_bound(int32_t name_len, int32_t len, int32_t rg_n) {
+    return 36 + (int64_t)name_len + 1 + 12 + ((int64_t)len + 1) / 2 + len + 3 + rg_n + 1 + 4;
+}
+
+int ma_bam_pack_dev(void *stream, int64_t n, const ma_bam_fields *f, const ma_bam_reads *r,
+                    const char *rg, int32_t rg_n, const uint32_t *d_perm, const uint32_t *d_is_dup,
+                    int32_t apply_dup, uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off,
+                    void *d_scratch, int64_t scratch_bytes) {
+    MA_REQUIRE(f, "ma_bam_pack_dev: null fields");
+    MA_REQUIRE(n == 0 || (f->flag && f->tid && f->pos && f->mapq && f->sl && f->sr && f->nm),
+               "ma_bam_pack_dev: null field array");
+    FieldSrc fs{};
+    fs.se = 0;
+    const int32_t *arr[10] = {f->flag, f->tid, f->pos, f->mapq, f->sl, f->sr, f->nm, f->ntid, f->npos, f->tlen};
+    for (int k = 0; k < 10; k++) fs.f[k] = arr[k];
+    return bam_pack(stream, n, fs, r, rg, rg_n, d_perm, d_is_dup, apply_dup, d_out, out_cap, d_rec_off,
+                    d_scratch, scratch_bytes, "ma_bam_pack_dev");
+}
+
+int ma_bam_pack_se_dev(void *stream, int64_t n, const ma_recs *m, const int32_t *d_tid_of_cid,
+                       int32_t n_tid, const ma_bam_reads *r, const char *rg, int32_t rg_n,
+                       const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
+                       uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,
+                       int64_t scratch_bytes) {
+    MA_REQUIRE(m && (n_tid == 0 || d_tid_of_cid) && n_tid >= 0, "ma_bam_pack_se_dev: bad arguments");
+    FieldSrc fs{};
+    fs.se = 1;
+    fs.n_tid = n_tid;
+    fs.cid = m->cid;
+    fs.pos = m->pos;
+    fs.mapq = m->mapq;
+    fs.rflag = m->flag;
+    fs.sl = m->sl;
+    fs.sr = m->sr;
+    fs.nm = m->nm;
+    fs.tid_of_cid = d_tid_of_cid;
+    return bam_pack(stream, n, fs, r, rg, rg_n, d_perm, d_is_dup, apply_dup, d_out, out_cap, d_rec_off,
+                    d_scratch, scratch_bytes, "ma_bam_pack_se_dev");
+}
+
+int64_t ma_bgzf_bound(int64_t n) {
+    const int64_t nb = (n + BGZF_BLOCK - 1) / BGZF_BLOCK;
+    return nb * BGZF_MEMBER_MAX + 28;
+}
+
+int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threads, int32_t add_eof,
+                     uint8_t *dst, int64_t dst_cap, int64_t *out_n) {
+    MA_REQUIRE(n >= 0 && (n == 0 || src) && dst && out_n, "ma_bgzf_compress: bad arguments");
+    MA_REQUIRE(level >= 0 && level <= 9, "ma_bgzf_compress: level outside 0..9");
+    std::vector<std::vector<uint8_t>> parts;
+    MA_REQUIRE(bgzf_compress_parts(src, n, level, threads, parts), "ma_bgzf_compress: deflate failed");
+    int64_t total = 0;
+    for (auto &p : parts) total += (int64_t)p.size();
+    MA_REQUIRE(total + (add_eof ? 28 : 0) <= dst_cap, "ma_bgzf_compress: output buffer too small");
+    total = 0;
+    for (auto &p : parts) {
+        std::copy(p.begin(), p.end(), dst + total);
+        total += (int64_t)p.size();
+    }
+    if (add_eof) {
+        std::copy(BGZF_EOF, BGZF_EOF + 28, dst + total);
+        total += 28;
+    }
+    *out_n = total;
+    return MA_OK;
+}
+
+int ma_bgzf_open(const char *path, int32_t level, int32_t threads, ma_bgzf **out) {
+    MA_REQUIRE(path && out, "ma_bgzf_open: bad arguments");
+    FILE *fh = fopen(path, "wb");
+    if (!fh) {
+        set_error(std::string("ma_bgzf_open: cannot open ") + path);
+        return MA_ERR_IO;
+    }
+    ma_bgzf *w = new ma_bgzf;
+    w->fh = fh;
+    w->level = std::max(0, std::min(9, (int)level));
+    w->threads = std::max(1, (int)threads);
+    *out = w;
+    return MA_OK;
+}
+
+namespace {
+int bgzf_emit(ma_bgzf *w, const uint8_t *src, int64_t n) {
+    MA_REQUIRE(bgzf_compress_parts(src, n, w->level, w->threads, w->parts), "ma_bgzf_write: deflate failed");
+    for (auto &p : w->parts) {
+        if (!p.empty() && fwrite(p.data(), 1, p.size(), w->fh) != p.size()) {
+            set_error("ma_bgzf_write: write failed");
+            return MA_ERR_IO;
+        }
+        w->written += (int64_t)p.size();
+    }
+    return MA_OK;
+}
+}  // namespace
+
+int ma_bgzf_write(ma_bgzf *w, const void *data, int64_t n) {
+    MA_REQUIRE(w && w->fh && n >= 0 && (n == 0 || data), "ma_bgzf_write: bad arguments");
+    const uint8_t *p = (const uint8_t *)data;
+    if (!w->tail.empty()) {
+        const int64_t take = std::min<int64_t>(n, BGZF_BLOCK - (int64_t)w->tail.size());
+        w->tail.insert(w->tail.end(), p, p + take);
+        p += take;
+        n -= take;
+        if ((int64_t)w->tail.size() < BGZF_BLOCK) return MA_OK;
+        MA_TRY(bgzf_emit(w, w->tail.data(), BGZF_BLOCK));
+        w->tail.clear();
+    }
+    const int64_t full = n / BGZF_BLOCK * BGZF_BLOCK;
+    if (full) MA_TRY(bgzf_emit(w, p, full));
+    w->tail.assign(p + full, p + n);
+    return MA_OK;
+}
+
+int ma_bgzf_close(ma_bgzf *w, int64_t *bytes_written) {
+    MA_REQUIRE(w, "ma_bgzf_close: null writer");
+    int rc = MA_OK;
+    if (w->fh) {
+        if (!w->tail.empty()) rc = bgzf_emit(w, w->tail.data(), (int64_t)w->tail.size());
+        if (rc == MA_OK && fwrite(BGZF_EOF, 1, 28, w->fh) != 28) {
+            set_error("ma_bgzf_close: write failed");
+            rc = MA_ERR_IO;
+        }
+        w->written += 28;
+        if (fclose(w->fh) != 0 && rc == MA_OK) {
+            set_error("ma_bgzf_close: close failed");
+            rc = MA_ERR_IO;
+        }
+    }
+    if (bytes_written) *bytes_written = w->written;
+    delete w;
+    return rc;
+}
+
+}  // extern "C"

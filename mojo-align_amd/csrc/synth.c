@@ -12,6 +12,7 @@
  * reads are exact copies of an earlier read (planted PCR duplicates).
  */
 #include <stdint.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -273,4 +274,43 @@ void ms_gen_sort_records(uint64_t seed, int64_t n, int64_t n_contigs, int64_t ge
             i++;
         }
     }
+}
+
+/* FASTQ-like read names and qualities for n reads: name i is an
+ * Illumina-style "SIM:<run>:HMI355X:<lane>:<tile>:<x>:<y>" (names back to back,
+ * name_off[n+1]); qualities are phred+33 bytes into [n x max_len] rows, Q2..Q41
+ * with a slow decline along the read and occasional dips. Returns the bytes of
+ * names written, or -1 when name_cap is too small. */
+int64_t ms_gen_names_quals(uint64_t seed, int64_t n, const uint32_t *lens, int64_t max_len,
+                           uint8_t *qual, char *names, int64_t name_cap, int64_t *name_off) {
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++) {
+        uint64_t r = rnd(seed, 40, (uint64_t)i);
+        int len = 0;
+        char buf[64];
+        unsigned lane = 1 + (unsigned)(r % 4), tile = 1101 + (unsigned)((r >> 8) % 24);
+        unsigned x = (unsigned)((r >> 16) % 30000), y = (unsigned)((r >> 32) % 30000);
+        len = snprintf(buf, sizeof buf, "SIM:%u:HMI355X:%u:%u:%u:%u", 1 + (unsigned)(i / 100000000), lane, tile, x, y);
+        if (off + len > name_cap) return -1;
+        name_off[i] = off;
+        memcpy(names + off, buf, (size_t)len);
+        off += len;
+    }
+    name_off[n] = off;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        uint8_t *q = qual + i * max_len;
+        int64_t L = lens[i];
+        for (int64_t j = 0; j < max_len; j++) {
+            if (j >= L) { q[j] = 0; continue; }
+            uint64_t r = rnd(seed, 41 + (uint64_t)i, (uint64_t)j);
+            int base = 40 - (int)(j * 8 / (L > 0 ? L : 1));
+            int v = base - (int)(r % 6);
+            if ((r >> 20) % 50 == 0) v = 2 + (int)((r >> 28) % 12);
+            if (v < 2) v = 2;
+            if (v > 41) v = 41;
+            q[j] = (uint8_t)(33 + v);
+        }
+    }
+    return off;
 }

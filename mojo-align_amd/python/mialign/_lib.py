@@ -63,6 +63,25 @@ class Recs(C.Structure):
     ]
 
 
+BAM_FIELDS = ("flag", "tid", "pos", "mapq", "sl", "sr", "nm", "ntid", "npos", "tlen")
+
+
+class BamFields(C.Structure):
+    _fields_ = [(f, C.c_void_p) for f in BAM_FIELDS]
+
+
+class BamReads(C.Structure):
+    _fields_ = [
+        ("names", C.c_void_p),
+        ("name_off", C.c_void_p),
+        ("seq", C.c_void_p),
+        ("qual", C.c_void_p),
+        ("lens", C.c_void_p),
+        ("qual_lens", C.c_void_p),
+        ("stride", C.c_int64),
+    ]
+
+
 REC_FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm", "iop", "iat", "ilen")
 REC_DTYPES = {
     "cid": np.int32,
@@ -131,6 +150,17 @@ SIGNATURES = {
     "ma_sort_markdup_dev": (C.c_int, [P, I64, P, P, P, P, P, P, P, I32, P, P, I64]),
     "ma_radix_sort_u64_dev": (C.c_int, [P, I64, P, P, P, I64]),
     "ma_seed_kmers": (C.c_int, [C.c_int, P, I64, I32, I32, P]),
+    "ma_bam_scratch_bytes": (I64, [I64]),
+    "ma_bam_record_bound": (I64, [I32, I32, I32]),
+    "ma_bam_pack_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, I32, C.POINTER(BamReads), C.c_char_p,
+                                     I32, P, P, I32, P, I64, P, P, I64]),
+    "ma_bam_pack_dev": (C.c_int, [P, I64, C.POINTER(BamFields), C.POINTER(BamReads), C.c_char_p, I32, P,
+                                  P, I32, P, I64, P, P, I64]),
+    "ma_bgzf_bound": (I64, [I64]),
+    "ma_bgzf_compress": (C.c_int, [P, I64, I32, I32, I32, P, I64, C.POINTER(I64)]),
+    "ma_bgzf_open": (C.c_int, [C.c_char_p, I32, I32, C.POINTER(P)]),
+    "ma_bgzf_write": (C.c_int, [P, P, I64]),
+    "ma_bgzf_close": (C.c_int, [P, C.POINTER(I64)]),
     "ma_serve_open": (C.c_int, [P, I32, I64, I32, C.POINTER(P)]),
     "ma_serve_tune": (C.c_int, [P, I64, C.c_double, I32, I32]),
     "ma_serve_submit": (I64, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),

@@ -20,7 +20,7 @@ CSRC = PKG_ROOT / "csrc"
 INCLUDE = PKG_ROOT.parent / "include"
 BUILD_DIR = PKG_ROOT / "_build"
 
-HIP_SOURCES = ["runtime.hip", "index.hip", "sort.hip", "map.hip", "serve.hip"]
+HIP_SOURCES = ["runtime.hip", "index.hip", "sort.hip", "map.hip", "serve.hip", "bam.hip"]
 HIP_DEPS = ["ma_common.hpp", "ma_index.hpp", "ma_sort.hpp", "map_body.inc"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -63,7 +63,8 @@ def build_hip(jobs: int = 8) -> Path:
         for fut in [ex.submit(_run, c) for c in cmds]:
             fut.result()
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz",
+          "-pthread"])
     os.replace(tmp, LIB)
     return LIB
 

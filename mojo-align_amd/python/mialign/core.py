@@ -282,6 +282,19 @@ class DeviceRecs:
     def field_ptr(self, f: str) -> int:
         return getattr(self.struct, f)
 
+    @classmethod
+    def from_host(cls, recs: dict, n: int, device: int = 0, stream=None) -> "DeviceRecs":
+        """Upload host record arrays (missing fields are zero)."""
+        d = cls(n, device)
+        raw = np.zeros(d.buf.nbytes, dtype=np.uint8)
+        n1 = max(n, 1)
+        for i, f in enumerate(REC_FIELDS):
+            if f in recs:
+                a = np.ascontiguousarray(recs[f][:n]).astype(REC_DTYPES[f], copy=False)
+                raw[i * n1 * 4 : i * n1 * 4 + n * 4] = a.view(np.uint8)
+        d.buf.upload(raw, stream)
+        return d
+
     def download(self) -> dict[str, np.ndarray]:
         raw = self.buf.download()
         out = {}

@@ -35,6 +35,8 @@ def _synth():
         L.ms_pack_2bit.argtypes = [P, P, I64, I64, I64, P]
         L.ms_gen_sort_records.restype = None
         L.ms_gen_sort_records.argtypes = [U64, I64, I64, I64, D, P, P, P]
+        L.ms_gen_names_quals.restype = I64
+        L.ms_gen_names_quals.argtypes = [U64, I64, P, I64, P, P, I64, P]
         _lib = L
     return _lib
 
@@ -147,3 +149,18 @@ def tid_of_cid(names: list) -> np.ndarray:
             seen[c] = len(seen)
         out[i] = seen[c]
     return out
+
+
+def gen_names_quals(seed: int, lens: np.ndarray, max_len: int):
+    """Illumina-style read names (blob + offsets [n+1]) and phred+33 quality
+    rows [n, max_len] for n reads (seeded)."""
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n = lens.size
+    cap = 48 * max(n, 1)
+    names = np.empty(cap, dtype=np.uint8)
+    off = np.empty(n + 1, dtype=np.int64)
+    qual = np.empty((n, max_len), dtype=np.uint8)
+    got = _synth().ms_gen_names_quals(seed, n, _p(lens), max_len, _p(qual), _p(names), cap, _p(off))
+    if got < 0:
+        raise ValueError("ms_gen_names_quals: name buffer too small")
+    return names[:max(int(got), 1)], off, qual

@@ -25,6 +25,12 @@
  *   or_sort_keys_se .................................. closed-form key formulas
  *   or_sort_markdup .................................. reference known answers
  *                                                    (test_bam_emit.py) + brute force
+ *   or_reg2bin / or_nt16 / or_nt16_comp ............. reference known answers
+ *        (test_bam_emit.py) + the SAM specification's bin definition by brute force
+ *   or_pack_bam_aln / or_bam_pack_fields / or_bam_pack_se ... reference known
+ *        answers for the 4-bit SEQ and the bin + an independent BAM decoder
+ *        (tests/bam_decode.py) checked field by field against the SAM text of
+ *        the same records + planted mistakes (swapped clips, unreversed QUAL)
  *   or_recount_nm (a checker, no reference counterpart) ... an independent Python
  *                                                    recount + planted one-base shifts
  */
@@ -1114,6 +1120,218 @@ void or_recount_nm(const uint8_t *seq, const uint64_t *contig_off, int64_t n_con
         }
         nm_out[i] = nm;
     }
+}
+
+/* ------------------------------------------------ BAM record pack (next row) */
+
+/* Follows _reg2bin, fq2bam-meth/src/linear_gpu_fm.mojo:481-495 (the SAM
+ * specification's binning; bam_emit.py:33-48 is the same function). */
+int64_t or_reg2bin(int64_t beg, int64_t end) {
+    if (end <= beg) end = beg + 1;
+    end -= 1;
+    if ((beg >> 14) == (end >> 14)) return 4681 + (beg >> 14);
+    if ((beg >> 17) == (end >> 17)) return 585 + (beg >> 17);
+    if ((beg >> 20) == (end >> 20)) return 73 + (beg >> 20);
+    if ((beg >> 23) == (end >> 23)) return 9 + (beg >> 23);
+    if ((beg >> 26) == (end >> 26)) return 1 + (beg >> 26);
+    return 0;
+}
+
+/* Follows _nt16 / _nt16_comp, linear_gpu_fm.mojo:457-478: A C G T (either
+ * case) -> 1 2 4 8, anything else 15; the complement swaps 1<->8 and 2<->4. */
+uint8_t or_nt16(uint8_t b) {
+    if (b == 'A' || b == 'a') return 1;
+    if (b == 'C' || b == 'c') return 2;
+    if (b == 'G' || b == 'g') return 4;
+    if (b == 'T' || b == 't') return 8;
+    return 15;
+}
+
+uint8_t or_nt16_comp(uint8_t c) {
+    if (c == 1) return 8;
+    if (c == 8) return 1;
+    if (c == 2) return 4;
+    if (c == 4) return 2;
+    return 15;
+}
+
+static void put32(uint8_t *p, int64_t off, int64_t v) {
+    uint64_t u = (uint64_t)v;
+    p[off] = (uint8_t)(u & 255);
+    p[off + 1] = (uint8_t)((u >> 8) & 255);
+    p[off + 2] = (uint8_t)((u >> 16) & 255);
+    p[off + 3] = (uint8_t)((u >> 24) & 255);
+}
+
+/* One BAM alignment record at p + off0 (block_size first); returns the offset
+ * after it, or -1 when it would pass cap ("BAM batch buffer overflow").
+ * Follows _pack_bam_aln, linear_gpu_fm.mojo:497-665 step by step: clips
+ * clamped and dropped when they cover the read, swapped for flag 16; CIGAR
+ * slS mM srS only for a mapped record; bin 4680 and tid/pos -1 otherwise;
+ * MAPQ and NM clamped to 0..255; SEQ 4-bit, reverse-complemented for flag 16;
+ * QUAL - 33 (reversed for flag 16; 255 where < 33) when qual_len == qlen,
+ * else 255 throughout; tags RG:Z:<rg> and NM:C. */
+int64_t or_pack_bam_aln(uint8_t *p, int64_t off0, int64_t cap, const uint8_t *name,
+                        int64_t name_len, int64_t flag, int64_t tid, int64_t pos0, int64_t mapq,
+                        int64_t sl, int64_t sr, const uint8_t *seq, int64_t qlen,
+                        const uint8_t *qual, int64_t qual_len, int64_t ntid, int64_t npos,
+                        int64_t tlen, int64_t nm, const uint8_t *rg, int64_t rg_n) {
+    int rev = (flag & 16) != 0;
+    int64_t slv = sl < 0 ? 0 : sl, srv = sr < 0 ? 0 : sr;
+    if (slv + srv >= qlen) { slv = 0; srv = 0; }
+    int64_t mid = qlen - slv - srv;
+    if (rev) { int64_t t = slv; slv = srv; srv = t; }
+    int mapped = tid >= 0 && (flag & 4) == 0;
+    int64_t n_cigar = 0;
+    if (mapped) n_cigar = (slv > 0) + (mid > 0) + (srv > 0);
+    int64_t l_qname = name_len + 1;
+    int64_t block = 32 + l_qname + 4 * n_cigar + (qlen + 1) / 2 + qlen + (3 + rg_n + 1 + 4);
+    if (off0 + 4 + block > cap) return -1;
+    int64_t bin = 4680, tid_w = -1, pos_w = -1;
+    if (mapped) {
+        tid_w = tid;
+        pos_w = pos0;
+        int64_t end = pos0 + mid;
+        if (end <= pos0) end = pos0 + 1;
+        bin = or_reg2bin(pos0, end);
+    }
+    if (ntid < 0) { ntid = -1; npos = -1; }
+    int64_t mq = mapq < 0 ? 0 : (mapq > 255 ? 255 : mapq);
+    put32(p, off0, block);
+    put32(p, off0 + 4, tid_w);
+    put32(p, off0 + 8, pos_w);
+    put32(p, off0 + 12, (int64_t)(uint32_t)((bin << 16) | (mq << 8) | l_qname));
+    put32(p, off0 + 16, (int64_t)(uint32_t)((flag << 16) | n_cigar));
+    put32(p, off0 + 20, qlen);
+    put32(p, off0 + 24, ntid);
+    put32(p, off0 + 28, npos);
+    put32(p, off0 + 32, tlen);
+    int64_t off = off0 + 36;
+    for (int64_t i = 0; i < name_len; i++) p[off + i] = name[i];
+    p[off + name_len] = 0;
+    off += l_qname;
+    if (n_cigar > 0) {
+        if (slv > 0) { put32(p, off, (slv << 4) | 4); off += 4; }
+        if (mid > 0) { put32(p, off, mid << 4); off += 4; }
+        if (srv > 0) { put32(p, off, (srv << 4) | 4); off += 4; }
+    }
+    int64_t si = 0;
+    while (si + 1 < qlen) {
+        int64_t ia = rev ? qlen - 1 - si : si, ib = rev ? qlen - 2 - si : si + 1;
+        uint8_t a = or_nt16(seq[ia]), b = or_nt16(seq[ib]);
+        if (rev) { a = or_nt16_comp(a); b = or_nt16_comp(b); }
+        p[off++] = (uint8_t)((a << 4) | b);
+        si += 2;
+    }
+    if (si < qlen) {
+        uint8_t c = or_nt16(seq[rev ? 0 : si]);
+        if (rev) c = or_nt16_comp(c);
+        p[off++] = (uint8_t)(c << 4);
+    }
+    if (qual_len == qlen && qual) {
+        for (int64_t i = 0; i < qlen; i++) {
+            uint8_t q = qual[rev ? qlen - 1 - i : i];
+            p[off + i] = q >= 33 ? (uint8_t)(q - 33) : 255;
+        }
+    } else {
+        for (int64_t i = 0; i < qlen; i++) p[off + i] = 255;
+    }
+    off += qlen;
+    p[off] = 'R'; p[off + 1] = 'G'; p[off + 2] = 'Z';
+    off += 3;
+    for (int64_t i = 0; i < rg_n; i++) p[off + i] = rg[i];
+    p[off + rg_n] = 0;
+    off += rg_n + 1;
+    p[off] = 'N'; p[off + 1] = 'M'; p[off + 2] = 'C';
+    p[off + 3] = (uint8_t)(nm < 0 ? 0 : (nm > 255 ? 255 : nm));
+    return off + 4;
+}
+
+/* Sets flag 0x400 in the record at p + rec_off; follows _or_dup_flag,
+ * linear_gpu_fm.mojo:408-420. */
+static void or_dup_flag(uint8_t *p, int64_t rec_off) {
+    uint32_t fnc = (uint32_t)p[rec_off + 16] | ((uint32_t)p[rec_off + 17] << 8) |
+                   ((uint32_t)p[rec_off + 18] << 16) | ((uint32_t)p[rec_off + 19] << 24);
+    uint32_t v = (((fnc >> 16) | 1024u) << 16) | (fnc & 65535u);
+    put32(p, rec_off + 16, v);
+}
+
+/* Packs n records from per-record fields (int32 arrays; ntid/npos/tlen may be
+ * NULL for -1/-1/0) in input order, then gathers them in perm order (NULL =
+ * identity) setting 0x400 where apply_dup and is_dup[orig] != 0. Read i: name
+ * names[name_off[i] .. name_off[i+1]), bases seq + i*stride (lens[i]), quality
+ * qual + i*stride of qual_lens[i] bytes (qual NULL = absent; qual_lens NULL =
+ * lens). Returns the stream length or -1 when it would pass cap. Follows the
+ * per-batch pack of _emit_bam_from_batch (linear_gpu_fm.mojo:766-800) and
+ * _gather_perm_to_writer (:177-225). */
+int64_t or_bam_pack_fields(int64_t n, const int32_t *flag, const int32_t *tid, const int32_t *pos,
+                           const int32_t *mapq, const int32_t *sl, const int32_t *sr,
+                           const int32_t *nm, const int32_t *ntid, const int32_t *npos,
+                           const int32_t *tlen, const uint8_t *names, const int64_t *name_off,
+                           const uint8_t *seq, int64_t stride, const uint32_t *lens,
+                           const uint8_t *qual, const uint32_t *qual_lens, const uint8_t *rg,
+                           int64_t rg_n, const uint32_t *perm, const uint32_t *is_dup,
+                           int apply_dup, uint8_t *out, int64_t cap) {
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t qlen = lens[i], l_qname = name_off[i + 1] - name_off[i] + 1;
+        total += 36 + l_qname + 12 + (qlen + 1) / 2 + qlen + 3 + rg_n + 1 + 4;
+    }
+    uint8_t *batch = (uint8_t *)malloc((size_t)(total > 0 ? total : 1));
+    int64_t *rec = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+    if (!batch || !rec) { free(batch); free(rec); return -1; }
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++) {
+        rec[i] = off;
+        off = or_pack_bam_aln(batch, off, total, names + name_off[i], name_off[i + 1] - name_off[i],
+                              flag[i], tid[i], pos[i], mapq[i], sl[i], sr[i], seq + i * stride,
+                              lens[i], qual ? qual + i * stride : NULL,
+                              qual ? (qual_lens ? qual_lens[i] : lens[i]) : 0,
+                              ntid ? ntid[i] : -1, npos ? npos[i] : -1, tlen ? tlen[i] : 0, nm[i],
+                              rg, rg_n);
+    }
+    rec[n] = off;
+    int64_t g = 0;
+    for (int64_t j = 0; j < n; j++) {
+        int64_t o = perm ? (int64_t)perm[j] : j, len = rec[o + 1] - rec[o];
+        if (g + len > cap) { free(batch); free(rec); return -1; }
+        memcpy(out + g, batch + rec[o], (size_t)len);
+        if (apply_dup && is_dup[o] != 0) or_dup_flag(out, g);
+        g += len;
+    }
+    free(batch);
+    free(rec);
+    return g;
+}
+
+/* Single-end records straight from merged map records: flag | 0x41 (paired +
+ * read 1, as the reference's SE branch sets them), tid from tid_of_cid (cid
+ * outside [0, n_tid) -> tid -1 and flag | 4), no mate. Follows
+ * _emit_bam_from_batch's SE branch, linear_gpu_fm.mojo:766-800. */
+int64_t or_bam_pack_se(int64_t n, const int32_t *cid, const uint32_t *pos, const uint32_t *mapq,
+                       const int32_t *rflag, const uint32_t *sl, const uint32_t *sr,
+                       const uint32_t *nm, const int32_t *tid_of_cid, int64_t n_tid,
+                       const uint8_t *names, const int64_t *name_off, const uint8_t *seq,
+                       int64_t stride, const uint32_t *lens, const uint8_t *qual,
+                       const uint32_t *qual_lens, const uint8_t *rg, int64_t rg_n,
+                       const uint32_t *perm, const uint32_t *is_dup, int apply_dup, uint8_t *out,
+                       int64_t cap) {
+    int32_t *f = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * 7 * sizeof(int32_t));
+    if (!f) return -1;
+    int32_t *fl = f, *tid = f + n, *ps = f + 2 * n, *mq = f + 3 * n, *l = f + 4 * n, *r = f + 5 * n,
+            *m = f + 6 * n;
+    for (int64_t i = 0; i < n; i++) {
+        int64_t fv = (int64_t)rflag[i] | 1 | 64, t = -1;
+        if (cid[i] >= 0 && cid[i] < n_tid) t = tid_of_cid[cid[i]];
+        else fv |= 4;
+        fl[i] = (int32_t)fv; tid[i] = (int32_t)t; ps[i] = (int32_t)pos[i];
+        mq[i] = (int32_t)mapq[i]; l[i] = (int32_t)sl[i]; r[i] = (int32_t)sr[i]; m[i] = (int32_t)nm[i];
+    }
+    int64_t g = or_bam_pack_fields(n, fl, tid, ps, mq, l, r, m, NULL, NULL, NULL, names, name_off,
+                                   seq, stride, lens, qual, qual_lens, rg, rg_n, perm, is_dup,
+                                   apply_dup, out, cap);
+    free(f);
+    return g;
 }
 
 int or_num_threads(void) {

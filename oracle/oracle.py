@@ -99,6 +99,20 @@ def lib():
         L.or_sort_markdup.restype = C.c_int
         L.or_sort_markdup.argtypes = [I64, P, P, P, P, P, P, P, C.c_int, P]
         L.or_num_threads.restype = C.c_int
+        L.or_reg2bin.restype = I64
+        L.or_reg2bin.argtypes = [I64, I64]
+        L.or_nt16.restype = C.c_uint8
+        L.or_nt16.argtypes = [C.c_uint8]
+        L.or_nt16_comp.restype = C.c_uint8
+        L.or_nt16_comp.argtypes = [C.c_uint8]
+        L.or_pack_bam_aln.restype = I64
+        L.or_pack_bam_aln.argtypes = [P, I64, I64, P, I64, I64, I64, I64, I64, I64, I64, P, I64, P,
+                                      I64, I64, I64, I64, I64, P, I64]
+        L.or_bam_pack_fields.restype = I64
+        L.or_bam_pack_fields.argtypes = [I64] + [P] * 13 + [I64, P, P, P, P, I64, P, P, C.c_int, P, I64]
+        L.or_bam_pack_se.restype = I64
+        L.or_bam_pack_se.argtypes = [I64] + [P] * 8 + [I64, P, P, P, I64, P, P, P, P, I64, P, P,
+                                                       C.c_int, P, I64]
         L.or_recount_nm.argtypes = [P, P, I64, P, I64, P, I64, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -229,3 +243,63 @@ def sort_markdup(coord, dhi, dlo, score, pair, do_markdup=True):
     if r != 0:
         raise MemoryError("or_sort_markdup failed")
     return is_dup, perm, dup_perm
+
+
+# ------------------------------------------------------------ BAM pack / BGZF
+
+def bam_record_bound(lens: np.ndarray, name_off: np.ndarray, rg_n: int) -> int:
+    lens = np.asarray(lens, dtype=np.int64)
+    nl = np.diff(np.asarray(name_off, dtype=np.int64))
+    return int((36 + nl + 1 + 12 + (lens + 1) // 2 + lens + 8 + rg_n).sum())
+
+
+def bam_pack_se(rec: dict, tid_of_cid: np.ndarray, names: np.ndarray, name_off: np.ndarray,
+                seq: np.ndarray, lens: np.ndarray, qual: np.ndarray | None, rg: bytes,
+                perm: np.ndarray | None = None, is_dup: np.ndarray | None = None,
+                apply_dup: bool = False, qual_lens: np.ndarray | None = None) -> bytes:
+    """Uncompressed BAM records of single-end map records (or_bam_pack_se)."""
+    n = int(lens.size)
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    stride = seq.shape[1] if seq.ndim == 2 else 0
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    names = np.ascontiguousarray(names, dtype=np.uint8)
+    name_off = np.ascontiguousarray(name_off, dtype=np.int64)
+    tid = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
+    q = None if qual is None else np.ascontiguousarray(qual, dtype=np.uint8)
+    ql = None if qual_lens is None else np.ascontiguousarray(qual_lens, dtype=np.uint32)
+    pm = None if perm is None else np.ascontiguousarray(perm, dtype=np.uint32)
+    dp = None if is_dup is None else np.ascontiguousarray(is_dup, dtype=np.uint32)
+    rgb = np.frombuffer(rg, dtype=np.uint8) if rg else np.zeros(1, np.uint8)
+    cols = {f: np.ascontiguousarray(rec[f]) for f in ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")}
+    cap = bam_record_bound(lens, name_off, len(rg))
+    out = np.empty(max(cap, 1), np.uint8)
+    g = lib().or_bam_pack_se(n, _p(cols["cid"]), _p(cols["pos"]), _p(cols["mapq"]), _p(cols["flag"]),
+                             _p(cols["sl"]), _p(cols["sr"]), _p(cols["nm"]), _p(tid), tid.size,
+                             _p(names), _p(name_off), _p(seq), stride, _p(lens), _p(q), _p(ql),
+                             _p(rgb), len(rg), _p(pm), _p(dp), int(bool(apply_dup)), _p(out), cap)
+    if g < 0:
+        raise RuntimeError("or_bam_pack_se: BAM batch buffer overflow")
+    return out[:g].tobytes()
+
+
+BGZF_MAX_BLOCK = 65280
+BGZF_EOF = bytes.fromhex("1f8b08040000000000ff0600424302001b0003000000000000000000")
+
+
+def bgzf_block(chunk: bytes, level: int) -> bytes:
+    """One BGZF member: gzip header with the BC extra field (BSIZE = member
+    length - 1), raw deflate at `level`, CRC32 and ISIZE; follows _bgzf_block,
+    fq2bam-meth/python/bam_emit.py:77-100 (zlib's raw deflate, wbits -15)."""
+    import struct
+    import zlib
+    cobj = zlib.compressobj(level, zlib.DEFLATED, -15)
+    payload = cobj.compress(chunk) + cobj.flush()
+    head = struct.pack("<BBBBLBBHBBHH", 31, 139, 8, 4, 0, 0, 255, 6, 66, 67, 2, 25 + len(payload))
+    return head + payload + struct.pack("<II", zlib.crc32(chunk) & 0xFFFFFFFF, len(chunk))
+
+
+def bgzf_stream(data: bytes, level: int = 1, eof: bool = True) -> bytes:
+    """A whole stream cut into consecutive 65280-byte members, as _BgzfWriter
+    writes it (bam_emit.py:103-185), then the 28-byte EOF member."""
+    out = [bgzf_block(data[i:i + BGZF_MAX_BLOCK], level) for i in range(0, len(data), BGZF_MAX_BLOCK)]
+    return b"".join(out) + (BGZF_EOF if eof else b"")
