@@ -292,6 +292,135 @@ def cpu_map_baseline(ix, dual, bases, lens, tid, ns, what):
 FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
 
 
+def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
+    """Names of reads idx (blob + offsets) in that order."""
+    ln = (off[idx + 1] - off[idx]).astype(np.int64)
+    new_off = np.zeros(idx.size + 1, np.int64)
+    np.cumsum(ln, out=new_off[1:])
+    src = np.repeat(off[idx] - new_off[:-1], ln) + np.arange(int(new_off[-1]), dtype=np.int64)
+    return blob[src], new_off
+
+
+def bam_section(sr, reads, tid, n_tid: int, with_cpu: bool, cpu_records: int, parity_records: int) -> dict:
+    """BAM emit of the step's records, resident in HBM: the device pack in
+    coordinate order with the duplicate flag (ma_bam_pack_se_dev: sizes + scan +
+    one wave per record), the stream's copy to pinned host memory and BGZF at
+    level 1 on the host's threads (to /dev/null). The pack's time comes from
+    HIP events on its stream; the pack kernel's algorithmic bytes are the
+    record stream written plus the names, bases, qualities and the 56 bytes of
+    per-record fields, permutation, flags and offsets read."""
+    import ctypes as C
+
+    import mialign
+    from mialign import bam_emit, synth
+    from mialign._lib import check, lib
+    from mialign.serve import pinned_empty
+
+    n, L = reads.lens.size, reads.bases.shape[1]
+    t0 = time.time()
+    blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
+    dreads = bam_emit.DeviceReads(blob, name_off, reads.bases, reads.lens, qual)
+    rg = b"mojo1"
+    cap = dreads.out_bound(len(rg))
+    pk = bam_emit.BamPacker(n, cap)
+    st = sr.stream
+    log(f"bam: names + qualities for {n} reads generated and uploaded in {time.time() - t0:.1f}s")
+    ev0, ev1 = mialign.Event(), mialign.Event()
+
+    def pack():
+        pk.pack_se(st, n, sr.recs, sr.d_tid, n_tid, dreads, rg, sr.outs["perm"], sr.outs["is_dup"], True)
+
+    pack()
+    total = pk.total(n, st)
+    pack_ms = []
+    for _ in range(5):
+        ev0.record(st)
+        pack()
+        ev1.record(st)
+        ev1.synchronize()
+        pack_ms.append(ev0.elapsed_ms(ev1))
+    dev_ms = float(np.median(pack_ms))
+    name_bytes = int(name_off[-1])
+    in_bytes = name_bytes + 2 * int(reads.lens.astype(np.int64).sum()) + 56 * n
+    achieved = (total + in_bytes) / (dev_ms / 1000.0) / 1e9
+    log(f"bam: pack {dev_ms:.2f} ms for {n} records, {total / 1e9:.2f} GB stream, {achieved:.0f} GB/s algorithmic")
+
+    # end to end: pack + copy to pinned host memory + BGZF level 1 (host threads)
+    host = pinned_empty((total,), np.uint8)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t_start = time.perf_counter()
+    pack()
+    check(lib().ma_memcpy_d2h(host.ctypes.data, pk.d_out.ptr, total, st.handle), "d2h")
+    st.sync()
+    t_copy = time.perf_counter()
+    w = bam_emit.BamWriter("/dev/null", ["c"], [1], threads=threads)
+    w.write_raw(host.ctypes.data, total)
+    z_bytes = w.close()
+    t_end = time.perf_counter()
+    emit_s = t_end - t_start
+    log(f"bam: emit {emit_s:.2f} s (pack + copy {t_copy - t_start:.2f} s, BGZF {t_end - t_copy:.2f} s on "
+        f"{threads} threads, {z_bytes / total:.3f} of the stream)")
+
+    # parity: the first records of the device stream == the oracle's pack of the same records
+    perm = sr.outs["perm"].download()
+    is_dup = sr.outs["is_dup"].download()
+    merged = sr.recs.download()
+    m = min(parity_records, n)
+    idx = perm[:m].astype(np.int64)
+    sub_blob, sub_off = _subset_names(blob, name_off, idx)
+    sub = {f: merged[f][idx] for f in FIELDS}
+    from oracle import oracle as orc
+
+    want = orc.bam_pack_se(sub, tid, sub_blob, sub_off, reads.bases[idx], reads.lens[idx], qual[idx], rg,
+                           is_dup=is_dup[idx], apply_dup=True)
+    ok = bytes(host[:len(want)]) == want
+    log(f"bam: first {m} records of the coordinate-ordered stream equal the oracle's: {ok}")
+
+    cpu = None
+    if with_cpu:
+        # the reference's host emit: one thread packs records (or_bam_pack_se on a
+        # sample), then zlib level 1 members on a pool of threads (bgzf_block)
+        from concurrent.futures import ThreadPoolExecutor
+
+        ns = min(cpu_records, n)
+        sub2 = {f: merged[f][:ns] for f in FIELDS}
+        b2, o2 = _subset_names(blob, name_off, np.arange(ns))
+        t0 = time.perf_counter()
+        s2 = orc.bam_pack_se(sub2, tid, b2, o2, reads.bases[:ns], reads.lens[:ns], qual[:ns], rg)
+        pack_s = time.perf_counter() - t0
+        chunk = bytes(host[: min(total, 512 << 20)])
+        blocks = [chunk[i:i + 65280] for i in range(0, len(chunk), 65280)]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            zs = list(ex.map(lambda b: orc.bgzf_block(b, 1), blocks))
+        z_s = time.perf_counter() - t0
+        per_read = total / n
+        rate = 1.0 / (pack_s / ns + per_read * z_s / len(chunk))
+        cpu = {"value": round(rate, 1), "unit": "reads/s", "cores": threads, "kind": "port",
+               "sample": f"oracle pack of {ns} records on 1 thread ({ns / pack_s:.0f} records/s) + zlib level 1 "
+                         f"BGZF of {len(chunk) >> 20} MiB of the stream on {threads} threads "
+                         f"({len(chunk) / z_s / 1e6:.0f} MB/s); {sum(map(len, zs)) / len(chunk):.3f} of the stream"}
+        del s2
+        log(f"bam: cpu baseline {rate:.0f} reads/s")
+    out = {
+        "workload": f"the step's {n} records (150 bp SE, coordinate order, duplicates flagged) as BAM",
+        "stream_bytes": total,
+        "pack_ms": round(dev_ms, 3),
+        "pack_records_per_s": round(n / (dev_ms / 1000.0), 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "bam_pack_kernel (+ size, scan)",
+                     "algorithmic_bytes_per_read": round((total + in_bytes) / n, 1)},
+        "emit_s": round(emit_s, 3),
+        "emit_reads_per_s": round(n / emit_s, 1),
+        "bgzf": {"level": 1, "threads": threads, "s": round(t_end - t_copy, 3),
+                 "ratio": round(z_bytes / total, 4)},
+        "parity_sample": {"records": m, "bytes_equal_oracle": bool(ok)},
+        "cpu_baseline": cpu,
+    }
+    del host, dreads, pk
+    return out
+
+
 def genome_scale_section(args) -> dict:
     """The 3.1 Gbp reference of the uneven-length and served configurations
     (24 contigs, 5% low-complexity repeat blocks, both conversions indexed:
@@ -402,6 +531,9 @@ def main() -> int:
     ap.add_argument("--gs-uneven-reads", type=int, default=50_000_000,
                     help="50-250 bp reads on the 3.1 Gbp index (one launch)")
     ap.add_argument("--gs-cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--no-bam", action="store_true", help="skip the BAM emit section")
+    ap.add_argument("--bam-cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--bam-parity-sample", type=int, default=1_000_000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -442,6 +574,9 @@ def main() -> int:
     merged = sr.recs.download()
     is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
+    bam = None
+    if not args.no_bam:
+        bam = bam_section(sr, reads, tid, tid.size, not args.no_cpu, args.bam_cpu_sample, args.bam_parity_sample)
     sr.close()
 
     # ---- served request loop vs plain one-batch-per-launch (same batches)
@@ -514,6 +649,7 @@ def main() -> int:
         "serving": serving,
         "sort_stress": sort_stress,
         "genome_scale": genome_scale,
+        "bam_emit": bam,
     }
     print(json.dumps(line), flush=True)
     return 0

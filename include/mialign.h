@@ -27,7 +27,8 @@
  *                             + _gather_perm_to_writer / _or_dup_flag,
  *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 766-800, 177-225, 408-420
  *   ma_bam_pack_dev ......... _pack_bam_aln from explicit per-record fields (:497-665)
- *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw,
+ *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw (host, zlib;
+ *                             ma_bgzf_compress_dev: the same members compressed on the GPU),
  *                             fq2bam-meth/python/bam_emit.py:77-185, 467-471
  *   ma_serve_* .............. no single reference counterpart: the request loop
  *                             generalises the batch loop of map_fastq_dense_gpu_locate
@@ -262,6 +263,18 @@ int ma_bam_pack_dev(void *stream, int64_t n, const ma_bam_fields *fields, const 
 int64_t ma_bgzf_bound(int64_t n);
 int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threads, int32_t add_eof,
                      uint8_t *dst, int64_t dst_cap, int64_t *out_n);
+/* BGZF on the device: the same 65280-byte members (header, raw deflate body,
+ * CRC32, ISIZE) of a device stream, compressed by the GPU: level 0 = stored
+ * blocks, otherwise one dynamic-Huffman block per member (a stored block where
+ * that is not smaller). The bodies are this library's own deflate, so the
+ * bytes differ from zlib's; the members decode to the same stream. Writes
+ * d_dst (ma_bgzf_bound(n) bytes suffice; 4-byte aligned, like d_src) and the
+ * total length to d_total (device); members past dst_cap are not written.
+ * Scratch: ma_bgzf_dev_scratch_bytes(n). Enqueued on stream. */
+int64_t ma_bgzf_dev_scratch_bytes(int64_t n);
+int ma_bgzf_compress_dev(void *stream, const uint8_t *d_src, int64_t n, int32_t level, int32_t add_eof,
+                         uint8_t *d_dst, int64_t dst_cap, uint64_t *d_total, void *d_scratch,
+                         int64_t scratch_bytes);
 /* A BGZF file writer: writes accumulate into 65280-byte members exactly as one
  * stream (a member never waits on a record boundary); close flushes the last
  * partial member and the EOF member. */

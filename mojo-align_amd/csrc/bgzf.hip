@@ -1,0 +1,567 @@
+// BGZF members compressed on the device.
+//
+// The stream is cut into consecutive 65280-byte blocks exactly as the
+// reference's writer cuts it (fq2bam-meth/python/bam_emit.py:103-185); each
+// block becomes one gzip member with the BC extra field, a raw deflate body,
+// CRC32 and ISIZE (_bgzf_block, :77-100). The deflate body is this build's own
+// compressor, so the compressed bytes differ from zlib's while the members
+// decode to the same stream:
+//
+//   * one 1024-thread workgroup per block (persistent grid), the block staged
+//     in LDS (64 bytes per thread);
+//   * a dynamic-Huffman block (RFC 1951 3.2.7) over the block's byte
+//     histogram: code lengths from a two-queue Huffman build limited to 15
+//     bits (7 for the code-length code) by Kraft-sum repair, canonical codes;
+//     the block falls back to a stored block when that is not smaller;
+//   * per-thread bit offsets from a block scan; each thread writes its
+//     segment's codes into an LDS bit buffer (plain stores for the words it
+//     owns, atomic OR for the two it may share);
+//   * CRC32: per-thread CRCs of the 64-byte segments joined pairwise by the
+//     GF(2) shift crc(A||B) = crc(A) * x^(8|B|) + crc(B) mod P;
+//   * a second kernel writes every member (18-byte header, body, trailer) at
+//     its offset from a scan of the member sizes, and the EOF member.
+#include "ma_sort.hpp"
+
+using namespace ma;
+
+namespace {
+
+constexpr int DF_T = 1024;            // threads per workgroup
+constexpr int DF_SEG = 64;            // input bytes per thread
+constexpr int DF_IN = 65280;          // bytes per BGZF block
+constexpr int DF_OUT_WORDS = 16384;   // 64 KiB deflate buffer
+constexpr int64_t DF_SLOT = 65536;    // staging bytes per block
+constexpr int NLIT = 286;             // literal/length alphabet (no matches: 0..256 used)
+constexpr int NDIST = 2;              // two 1-bit distance codes (a complete, unused code)
+constexpr int NCL = 19;               // code-length alphabet
+constexpr int NSEQ = NLIT + NDIST;    // code-length sequence entries at most
+constexpr uint32_t CRC_POLY = 0xEDB88320u;  // reflected CRC-32 polynomial
+constexpr uint32_t STORED = 0xFFFFFFFFu;
+
+struct DfShared {
+    uint32_t out[DF_OUT_WORDS];
+    uint8_t data[DF_IN];
+    uint32_t crc_tab[256];
+    uint32_t xpow[40];         // x^(2^k) mod P (reflected), k = 0..39
+    uint32_t part[DF_T];       // per-thread CRCs, then bit counts
+    uint32_t wsum[DF_T / 64];
+    uint32_t freq[NLIT];
+    uint32_t nf[NLIT];         // Huffman build: internal node weights
+    uint16_t par[2 * NLIT];    // Huffman build: parent of each node
+    uint8_t dep[2 * NLIT];     // Huffman build: node depth
+    uint16_t sorted[NLIT];     // used symbols by ascending (freq, symbol)
+    uint8_t lit_len[NLIT];
+    uint16_t lit_code[NLIT];   // bit-reversed canonical codes
+    uint8_t cl_sym[NSEQ];
+    uint8_t cl_extra[NSEQ];
+    uint32_t cl_freq[NCL];
+    uint8_t cl_len[NCL];
+    uint16_t cl_code[NCL];
+    uint16_t cl_sorted[NCL];
+    uint32_t blc[32];
+    uint32_t misc[8];          // 0 used symbols, 1 CL entries, 2 hlit, 3 hclen, 4 header bits, 5 crc
+};
+
+// CL alphabet transmission order (RFC 1951 3.2.7)
+__device__ __forceinline__ int cl_order(int i) {
+    constexpr uint32_t lo = 16 | 17 << 5 | 18 << 10 | 0 << 15 | 8 << 20 | 7 << 25;  // entries 0..5
+    constexpr uint32_t mid = 9 | 6 << 5 | 10 << 10 | 5 << 15 | 11 << 20 | 4 << 25;  // 6..11
+    constexpr uint32_t hi = 12 | 3 << 5 | 13 << 10 | 2 << 15 | 14 << 20 | 1 << 25;  // 12..17
+    if (i < 6) return (lo >> (5 * i)) & 31;
+    if (i < 12) return (mid >> (5 * (i - 6))) & 31;
+    if (i < 18) return (hi >> (5 * (i - 12))) & 31;
+    return 15;
+}
+
+// a * b mod P in the reflected representation (bit 31 = x^0).
+__device__ __forceinline__ uint32_t multmodp(uint32_t a, uint32_t b) {
+    uint32_t p = 0;
+#pragma unroll 4
+    for (int i = 0; i < 32; i++) {
+        if (a & (0x80000000u >> i)) p ^= b;
+        b = (b & 1) ? (b >> 1) ^ CRC_POLY : b >> 1;
+    }
+    return p;
+}
+
+// x^(8 len) mod P from the x^(2^k) table.
+__device__ __forceinline__ uint32_t x8n(const uint32_t *xpow, uint32_t len) {
+    uint32_t r = 0x80000000u;  // x^0
+    for (int k = 0; len; k++, len >>= 1)
+        if (len & 1) r = multmodp(xpow[k + 3], r);
+    return r;
+}
+
+__device__ __forceinline__ uint32_t bitrev(uint32_t c, int len) { return __brev(c) >> (32 - len); }
+
+// Code lengths (<= maxlen) for the m used symbols sorted[0..m) (ascending
+// (freq, symbol)); lengths of unused symbols stay 0. m >= 1. A single used
+// symbol gets length 1 and a second symbol `spare` length 1 too, so every
+// code is complete (inflate rejects incomplete code-length codes).
+__device__ void huff_lengths(const uint32_t *freq, const uint16_t *sorted, int m, int maxlen, int spare,
+                             uint8_t *len, uint32_t *nf, uint16_t *par, uint8_t *dep, uint32_t *blc) {
+    if (m == 1) {
+        len[sorted[0]] = 1;
+        len[sorted[0] == spare ? spare + 1 : spare] = 1;
+        return;
+    }
+    // two-queue Huffman: leaves 0..m-1 in sorted order, internal node k has id m + k
+    int i = 0, j = 0;
+    for (int k = 0; k < m - 1; k++) {
+        uint32_t f = 0;
+        for (int pick = 0; pick < 2; pick++) {
+            const bool leaf = i < m && (j >= k || freq[sorted[i]] <= nf[j]);
+            if (leaf) {
+                par[i] = (uint16_t)(m + k);
+                f += freq[sorted[i]];
+                i++;
+            } else {
+                par[m + j] = (uint16_t)(m + k);
+                f += nf[j];
+                j++;
+            }
+        }
+        nf[k] = f;
+    }
+    dep[2 * m - 2] = 0;
+    for (int node = 2 * m - 3; node >= 0; node--) {
+        const int d = dep[par[node]] + 1;
+        dep[node] = (uint8_t)(d > 255 ? 255 : d);
+    }
+    for (int l = 0; l < 32; l++) blc[l] = 0;
+    for (int s = 0; s < m; s++) blc[dep[s] > maxlen ? maxlen : dep[s]]++;
+    // Kraft repair after clamping: sum 2^(maxlen - l) over leaves must equal 2^maxlen
+    uint32_t kraft = 0;
+    for (int l = 1; l <= maxlen; l++) kraft += blc[l] << (maxlen - l);
+    const uint32_t target = 1u << maxlen;
+    // (the inner searches stop at length 1: every step below is bounded)
+    while (kraft > target) {  // lengthen the deepest code that can still grow
+        int l = maxlen - 1;
+        while (l > 1 && blc[l] == 0) l--;
+        if (blc[l] == 0) break;
+        blc[l]--;
+        blc[l + 1]++;
+        kraft -= 1u << (maxlen - l - 1);
+    }
+    while (kraft < target) {  // shorten the deepest code: its 2^(maxlen-l) divides the deficit
+        int l = maxlen;
+        while (l > 2 && blc[l] == 0) l--;
+        if (blc[l] == 0) break;
+        blc[l]--;
+        blc[l - 1]++;
+        kraft += 1u << (maxlen - l);
+    }
+    int s = 0;  // least frequent symbols take the longest codes
+    for (int l = maxlen; l >= 1; l--)
+        for (uint32_t c = 0; c < blc[l]; c++) len[sorted[s++]] = (uint8_t)l;
+}
+
+// Canonical codes (bit-reversed for LSB-first output) from lengths.
+__device__ void canonical(const uint8_t *len, int nsym, int maxlen, uint16_t *code, uint32_t *blc) {
+    for (int l = 0; l < 32; l++) blc[l] = 0;
+    for (int s = 0; s < nsym; s++) blc[len[s]]++;
+    blc[0] = 0;
+    uint32_t next[16];
+    uint32_t c = 0;
+    for (int l = 1; l <= maxlen; l++) {
+        c = (c + blc[l - 1]) << 1;
+        next[l] = c;
+    }
+    for (int s = 0; s < nsym; s++)
+        if (len[s]) code[s] = (uint16_t)bitrev(next[len[s]]++, len[s]);
+}
+
+// LSB-first bit writer into the shared buffer: words strictly inside the
+// writer's range are stored, the first and last (possibly shared) are ORed.
+struct BitWriter {
+    uint32_t *out;
+    uint32_t wi;
+    uint64_t acc;
+    int nb;
+    bool first;
+    __device__ BitWriter(uint32_t *o, uint32_t pos) : out(o), wi(pos >> 5), acc(0), nb((int)(pos & 31)), first(true) {}
+    __device__ __forceinline__ void put(uint32_t v, int n) {
+        acc |= (uint64_t)v << nb;
+        nb += n;
+        if (nb >= 32) {
+            if (first)
+                atomicOr(&out[wi], (uint32_t)acc);
+            else
+                out[wi] = (uint32_t)acc;
+            first = false;
+            wi++;
+            acc >>= 32;
+            nb -= 32;
+        }
+    }
+    __device__ __forceinline__ void finish() {
+        if (nb > 0) atomicOr(&out[wi], (uint32_t)acc);
+    }
+};
+
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *wsum, uint32_t &total) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (int k = 0; k < DF_T / 64; k++) {
+        const uint32_t s = wsum[k];
+        if (k < w) pre += s;
+        tot += s;
+    }
+    total = tot;
+    __syncthreads();
+    return pre + x - v;
+}
+
+__global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__restrict__ src, int64_t n,
+                                                            int64_t nb, int huffman,
+                                                            uint32_t *__restrict__ slots,
+                                                            uint32_t *__restrict__ body_len,
+                                                            uint32_t *__restrict__ crc_out,
+                                                            uint32_t *__restrict__ member_bytes) {
+    __shared__ DfShared S;
+    const int t = threadIdx.x;
+    if (t < 256) {
+        uint32_t c = (uint32_t)t;
+        for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
+        S.crc_tab[t] = c;
+    }
+    if (t == 0) {
+        uint32_t x = 0x40000000u;  // x^1
+        for (int k = 0; k < 40; k++) {
+            S.xpow[k] = x;
+            x = multmodp(x, x);
+        }
+    }
+    __syncthreads();
+    for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        const int64_t base = b * DF_IN;
+        const int bl = (int)(n - base < DF_IN ? n - base : DF_IN);
+        // 1. stage the block, clear the buffers
+        for (int i = t; i < DF_OUT_WORDS; i += DF_T) S.out[i] = 0;
+        for (int i = t; i < NLIT; i += DF_T) {
+            S.freq[i] = 0;
+            S.lit_len[i] = 0;
+        }
+        if (t < NCL) {
+            S.cl_freq[t] = 0;
+            S.cl_len[t] = 0;
+        }
+        if (t < 8) S.misc[t] = 0;
+        const uint8_t *blk = src + base;
+        for (int i = 4 * t; i < bl; i += 4 * DF_T) {
+            if (i + 4 <= bl) {
+                *reinterpret_cast<uint32_t *>(&S.data[i]) = *reinterpret_cast<const uint32_t *>(blk + i);
+            } else {
+                for (int k = i; k < bl; k++) S.data[k] = blk[k];
+            }
+        }
+        __syncthreads();
+        // 2. histogram and per-segment CRC
+        const int s0 = t * DF_SEG, s1 = min(bl, s0 + DF_SEG);
+        uint32_t c = 0xFFFFFFFFu;
+        for (int i = s0; i < s1; i++) {
+            const uint32_t v = S.data[i];
+            if (huffman) atomicAdd(&S.freq[v], 1u);
+            c = S.crc_tab[(c ^ v) & 255] ^ (c >> 8);
+        }
+        S.part[t] = ~c;
+        if (t == 0) S.freq[256] = 1;  // end of block
+        __syncthreads();
+        // 3. CRC of the block: join segment CRCs pairwise
+        for (int s = 1; s < DF_T; s <<= 1) {
+            if ((t & (2 * s - 1)) == 0) {
+                const int rstart = (t + s) * DF_SEG;
+                int rlen = bl - rstart;
+                rlen = rlen < 0 ? 0 : (rlen > s * DF_SEG ? s * DF_SEG : rlen);
+                if (rlen > 0) S.part[t] = multmodp(x8n(S.xpow, (uint32_t)rlen), S.part[t]) ^ S.part[t + s];
+            }
+            __syncthreads();
+        }
+        const uint32_t crc = S.part[0];
+        uint32_t nbytes = 0xFFFFFFFFu;
+        if (huffman) {
+            // 4. literal/length code: used symbols ranked by (freq, symbol)
+            for (int s = t; s < NLIT; s += DF_T) {
+                const uint32_t f = S.freq[s];
+                if (f) {
+                    int r = 0;
+                    for (int u = 0; u < NLIT; u++) {
+                        const uint32_t g = S.freq[u];
+                        r += g && (g < f || (g == f && u < s));
+                    }
+                    S.sorted[r] = (uint16_t)s;
+                    atomicAdd(&S.misc[0], 1u);
+                }
+            }
+            __syncthreads();
+            if (t == 0) {
+                huff_lengths(S.freq, S.sorted, (int)S.misc[0], 15, 0, S.lit_len, S.nf, S.par, S.dep, S.blc);
+                canonical(S.lit_len, NLIT, 15, S.lit_code, S.blc);
+                // code-length sequence (lit/len lengths then the two 1-bit distance codes), run-length coded
+                int hlit = NLIT;
+                while (hlit > 257 && S.lit_len[hlit - 1] == 0) hlit--;
+                const int total = hlit + NDIST;
+                int i = 0, ns = 0;
+                while (i < total) {
+                    const int v = i < hlit ? S.lit_len[i] : 1;
+                    int run = 1;
+                    while (i + run < total && (i + run < hlit ? S.lit_len[i + run] : 1) == v) run++;
+                    i += run;
+                    if (v == 0) {
+                        while (run >= 11) {
+                            const int r = run < 138 ? run : 138;
+                            S.cl_sym[ns] = 18;
+                            S.cl_extra[ns++] = (uint8_t)(r - 11);
+                            run -= r;
+                        }
+                        if (run >= 3) {
+                            S.cl_sym[ns] = 17;
+                            S.cl_extra[ns++] = (uint8_t)(run - 3);
+                            run = 0;
+                        }
+                    } else {
+                        S.cl_sym[ns] = (uint8_t)v;
+                        S.cl_extra[ns++] = 0;
+                        run--;
+                        while (run >= 3) {
+                            const int r = run < 6 ? run : 6;
+                            S.cl_sym[ns] = 16;
+                            S.cl_extra[ns++] = (uint8_t)(r - 3);
+                            run -= r;
+                        }
+                    }
+                    for (; run > 0; run--) {
+                        S.cl_sym[ns] = (uint8_t)v;
+                        S.cl_extra[ns++] = 0;
+                    }
+                }
+                for (int k = 0; k < ns; k++) S.cl_freq[S.cl_sym[k]]++;
+                int m = 0;
+                for (int s = 0; s < NCL; s++)
+                    if (S.cl_freq[s]) {
+                        int r = 0;
+                        for (int u = 0; u < NCL; u++) {
+                            const uint32_t g = S.cl_freq[u], f = S.cl_freq[s];
+                            r += g && (g < f || (g == f && u < s));
+                        }
+                        S.cl_sorted[r] = (uint16_t)s;
+                        m++;
+                    }
+                huff_lengths(S.cl_freq, S.cl_sorted, m, 7, 0, S.cl_len, S.nf, S.par, S.dep, S.blc);
+                canonical(S.cl_len, NCL, 7, S.cl_code, S.blc);
+                int hclen = NCL;
+                while (hclen > 4 && S.cl_len[cl_order(hclen - 1)] == 0) hclen--;
+                uint32_t hb = 3 + 5 + 5 + 4 + 3 * hclen;
+                for (int k = 0; k < ns; k++) {
+                    const int sym = S.cl_sym[k];
+                    hb += S.cl_len[sym] + (sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0);
+                }
+                S.misc[1] = (uint32_t)ns;
+                S.misc[2] = (uint32_t)hlit;
+                S.misc[3] = (uint32_t)hclen;
+                S.misc[4] = hb;
+            }
+            __syncthreads();
+            // 5. bit offsets of the segments
+            uint32_t bits = 0;
+            for (int i = s0; i < s1; i++) bits += S.lit_len[S.data[i]];
+            uint32_t dbits;
+            const uint32_t off = block_excl_scan_1024(bits, S.wsum, dbits);
+            const uint32_t hb = S.misc[4];
+            const uint32_t total_bits = hb + dbits + S.lit_len[256];
+            nbytes = (total_bits + 7) >> 3;
+            if (nbytes < (uint32_t)bl + 5) {
+                // 6. header (thread 0), then every segment's codes, EOB after the last byte
+                if (t == 0) {
+                    BitWriter w(S.out, 0);
+                    w.put(1, 1);  // BFINAL
+                    w.put(2, 2);  // BTYPE = dynamic Huffman
+                    w.put(S.misc[2] - 257, 5);
+                    w.put(NDIST - 1, 5);
+                    w.put(S.misc[3] - 4, 4);
+                    for (uint32_t k = 0; k < S.misc[3]; k++) w.put(S.cl_len[cl_order((int)k)], 3);
+                    for (uint32_t k = 0; k < S.misc[1]; k++) {
+                        const int sym = S.cl_sym[k];
+                        w.put(S.cl_code[sym], S.cl_len[sym]);
+                        if (sym == 16) w.put(S.cl_extra[k], 2);
+                        if (sym == 17) w.put(S.cl_extra[k], 3);
+                        if (sym == 18) w.put(S.cl_extra[k], 7);
+                    }
+                    w.finish();
+                }
+                if (s0 < bl) {
+                    BitWriter w(S.out, hb + off);
+                    for (int i = s0; i < s1; i++) {
+                        const uint32_t v = S.data[i];
+                        w.put(S.lit_code[v], S.lit_len[v]);
+                    }
+                    if (s1 == bl) w.put(S.lit_code[256], S.lit_len[256]);
+                    w.finish();
+                }
+            } else {
+                nbytes = 0xFFFFFFFFu;
+            }
+        }
+        __syncthreads();
+        // 7. body to the block's staging slot, sizes and CRC
+        uint32_t *slot = slots + b * (DF_SLOT / 4);
+        if (nbytes != 0xFFFFFFFFu)
+            for (uint32_t i = (uint32_t)t; i < (nbytes + 3) / 4; i += DF_T) slot[i] = S.out[i];
+        if (t == 0) {
+            body_len[b] = nbytes == 0xFFFFFFFFu ? STORED : nbytes;
+            crc_out[b] = crc;
+            member_bytes[b] = 18 + (nbytes == 0xFFFFFFFFu ? (uint32_t)bl + 5 : nbytes) + 8;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint32_t member_byte(int64_t m, int64_t nb, int64_t n, uint32_t j, uint32_t size,
+                                                const uint8_t *__restrict__ src,
+                                                const uint8_t *__restrict__ slot, uint32_t body,
+                                                uint32_t crc) {
+    if (m == nb) {  // EOF member: empty block, CRC 0, ISIZE 0
+        constexpr uint8_t eof[28] = {0x1f, 0x8b, 0x08, 0x04, 0, 0, 0, 0, 0, 0xff, 0x06, 0, 0x42, 0x43,
+                                     0x02, 0, 0x1b, 0, 0x03, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 28; k++) v = (uint32_t)k == j ? eof[k] : v;
+        return v;
+    }
+    const int64_t base = m * DF_IN;
+    const uint32_t bl = (uint32_t)(n - base < DF_IN ? n - base : DF_IN);
+    if (j < 18) {
+        if (j == 16) return (size - 1) & 255;
+        if (j == 17) return (size - 1) >> 8;
+        constexpr uint32_t h0 = 0x04088b1fu, h2 = 0x0006ff00u, h3 = 0x00024342u;  // bytes 0-3, 8-11, 12-15
+        return j < 4 ? (h0 >> (8 * j)) & 255 : j < 8 ? 0u : j < 12 ? (h2 >> (8 * (j - 8))) & 255 : (h3 >> (8 * (j - 12))) & 255;
+    }
+    if (j < size - 8) {
+        const uint32_t p = j - 18;
+        if (body != STORED) return slot[p];
+        if (p >= 5) return src[base + p - 5];
+        const uint32_t nl = ~bl & 0xFFFF;
+        return p == 0 ? 1u : p == 1 ? bl & 255 : p == 2 ? bl >> 8 : p == 3 ? nl & 255 : nl >> 8;
+    }
+    const uint32_t q = j - (size - 8);
+    return q < 4 ? (crc >> (8 * q)) & 255 : (bl >> (8 * (q - 4))) & 255;
+}
+
+// One wave per member; lanes build 4-byte-aligned words of the output.
+__global__ __launch_bounds__(256) void bgzf_assemble_kernel(const uint8_t *__restrict__ src, int64_t n, int64_t nb,
+                                                            int64_t n_members,
+                                                            const uint8_t *__restrict__ slots,
+                                                            const uint32_t *__restrict__ body_len,
+                                                            const uint32_t *__restrict__ crc_in,
+                                                            const uint32_t *__restrict__ member_bytes,
+                                                            const uint64_t *__restrict__ member_off,
+                                                            uint8_t *__restrict__ dst, int64_t dst_cap) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    uint32_t *const dst32 = reinterpret_cast<uint32_t *>(dst);
+    for (int64_t m = wave; m < n_members; m += (int64_t)gridDim.x * 4) {
+        const uint32_t size = member_bytes[m];
+        const int64_t off = (int64_t)member_off[m];
+        if (off + size > dst_cap) continue;
+        const uint32_t body = m < nb ? body_len[m] : 0;
+        const uint32_t crc = m < nb ? crc_in[m] : 0;
+        const uint8_t *slot = slots + m * DF_SLOT;
+        const int64_t d0 = off >> 2, d1 = (off + size - 1) >> 2;
+        for (int64_t d = d0 + lane; d <= d1; d += 64) {
+            const int64_t b0 = 4 * d - off;
+            if (b0 >= 0 && b0 + 4 <= size) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    w |= member_byte(m, nb, n, (uint32_t)b0 + k, size, src, slot, body, crc) << (8 * k);
+                dst32[d] = w;
+            } else {
+                for (int k = 0; k < 4; k++) {
+                    const int64_t j = b0 + k;
+                    if (j >= 0 && j < size)
+                        dst[off + j] = (uint8_t)member_byte(m, nb, n, (uint32_t)j, size, src, slot, body, crc);
+                }
+            }
+        }
+    }
+}
+
+__global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+struct DfScratch {
+    size_t slots, body, crc, bytes, off, partial, total;
+};
+
+DfScratch df_layout(int64_t n) {
+    const int64_t nb = (n + DF_IN - 1) / DF_IN, nm = nb + 1;
+    DfScratch s;
+    size_t o = 0;
+    auto take = [&](size_t b) {
+        const size_t r = o;
+        o += (b + 255) & ~(size_t)255;
+        return r;
+    };
+    s.slots = take((size_t)(nb > 0 ? nb : 1) * DF_SLOT);
+    s.body = take((size_t)nm * 4);
+    s.crc = take((size_t)nm * 4);
+    s.bytes = take((size_t)nm * 4);
+    s.off = take((size_t)(nm + 1) * 8);
+    s.partial = take(scan_scratch_bytes(nm));
+    s.total = o;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ma_bgzf_dev_scratch_bytes(int64_t n) { return (int64_t)df_layout(n < 0 ? 0 : n).total; }
+
+int ma_bgzf_compress_dev(void *stream, const uint8_t *d_src, int64_t n, int32_t level, int32_t add_eof,
+                         uint8_t *d_dst, int64_t dst_cap, uint64_t *d_total, void *d_scratch,
+                         int64_t scratch_bytes) {
+    MA_REQUIRE(n >= 0 && (n == 0 || d_src) && d_dst && d_total, "ma_bgzf_compress_dev: bad arguments");
+    MA_REQUIRE(level >= 0 && level <= 9, "ma_bgzf_compress_dev: level outside 0..9");
+    MA_REQUIRE(((uintptr_t)d_src & 3) == 0 && ((uintptr_t)d_dst & 3) == 0,
+               "ma_bgzf_compress_dev: buffers not 4-byte aligned");
+    const DfScratch L = df_layout(n);
+    MA_REQUIRE(scratch_bytes >= (int64_t)L.total, "ma_bgzf_compress_dev: scratch too small");
+    hipStream_t st = (hipStream_t)stream;
+    char *s = (char *)d_scratch;
+    const int64_t nb = (n + DF_IN - 1) / DF_IN;
+    const int64_t nm = nb + (add_eof ? 1 : 0);
+    uint32_t *body = (uint32_t *)(s + L.body), *crc = (uint32_t *)(s + L.crc), *bytes = (uint32_t *)(s + L.bytes);
+    uint64_t *off = (uint64_t *)(s + L.off);
+    if (nb > 0) {
+        int dev = 0, n_cu = 256;
+        MA_HIP(hipGetDevice(&dev));
+        MA_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int64_t grid = nb < n_cu ? nb : n_cu;
+        bgzf_deflate_kernel<<<(unsigned)grid, DF_T, 0, st>>>(d_src, n, nb, level > 0, (uint32_t *)(s + L.slots),
+                                                              body, crc, bytes);
+        MA_LAUNCHED();
+    }
+    if (add_eof) {
+        set_u32_kernel<<<1, 64, 0, st>>>(bytes + nb, 28u);
+        MA_LAUNCHED();
+    }
+    if (nm == 0) return fill_u64(st, d_total, 1, 0);
+    MA_TRY(exclusive_scan_u32(st, nm, bytes, off, (uint64_t *)(s + L.partial), d_total));
+    const int64_t blocks = (nm + 3) / 4 < 65536 ? (nm + 3) / 4 : 65536;
+    bgzf_assemble_kernel<<<(unsigned)blocks, 256, 0, st>>>(d_src, n, nb, nm, (const uint8_t *)(s + L.slots), body,
+                                                           crc, bytes, off, d_dst, dst_cap);
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
+}  // extern "C"

@@ -142,6 +142,47 @@ def bgzf_compress(data, level: int = 1, threads: Optional[int] = None, eof: bool
     return out[: n.value].tobytes()
 
 
+class DeviceBgzf:
+    """BGZF of device streams of up to `max_n` bytes, compressed on the GPU
+    (ma_bgzf_compress_dev); the members land in a device buffer."""
+
+    def __init__(self, max_n: int, device: int = 0):
+        self.max_n = int(max_n)
+        self.cap = int(lib().ma_bgzf_bound(self.max_n))
+        self.d_dst = DeviceArray((max(self.cap, 4),), np.uint8, device)
+        sb = int(lib().ma_bgzf_dev_scratch_bytes(self.max_n))
+        self.d_scratch = DeviceArray((sb,), np.uint8, device)
+        self.scratch_bytes = sb
+        self.d_total = DeviceArray((1,), np.uint64, device)
+
+    def compress(self, stream, d_src_ptr: int, n: int, level: int = 1, eof: bool = True) -> None:
+        """Enqueue the compression of n bytes at device address d_src_ptr."""
+        if n > self.max_n:
+            raise MialignError("DeviceBgzf: stream longer than its capacity")
+        h = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().ma_bgzf_compress_dev(h, d_src_ptr, int(n), int(level), int(bool(eof)), self.d_dst.ptr,
+                                         self.cap, self.d_total.ptr, self.d_scratch.ptr, self.scratch_bytes),
+              "ma_bgzf_compress_dev")
+
+    def total(self, stream=None) -> int:
+        t = np.zeros(1, np.uint64)
+        h = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().ma_memcpy_d2h(ptr(t), self.d_total.ptr, 8, h), "ma_memcpy_d2h")
+        if h is not None:
+            check(lib().ma_stream_sync(h), "ma_stream_sync")
+        return int(t[0])
+
+    def download(self, stream=None) -> bytes:
+        total = self.total(stream)
+        out = np.empty(max(total, 1), np.uint8)
+        h = stream.handle if isinstance(stream, Stream) else stream
+        if total:
+            check(lib().ma_memcpy_d2h(ptr(out), self.d_dst.ptr, total, h), "ma_memcpy_d2h")
+            if h is not None:
+                check(lib().ma_stream_sync(h), "ma_stream_sync")
+        return out[:total].tobytes()
+
+
 class DeviceReads:
     """A batch's names, original bases and qualities resident on the device,
     in the layout ma_bam_reads describes (rows of `stride` bytes)."""
@@ -288,6 +329,7 @@ def tid_table(contig_names: Sequence[str]):
 
 __all__ = [
     "BamPacker",
+    "DeviceBgzf",
     "BamWriter",
     "DeviceReads",
     "bam_header",

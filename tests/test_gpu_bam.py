@@ -5,6 +5,7 @@ the duplicate flag; BGZF of the result decodes back to it."""
 
 from __future__ import annotations
 
+import gzip
 import sys
 
 import numpy as np
@@ -162,3 +163,49 @@ def test_bam_after_map_and_sort_matches_oracle(tmp_path):
     head = bam_emit.bam_header(names, lens_sq, sort_order="coordinate")
     assert b"".join(pl for _, pl in members) == head + stream
     assert raw == orc.bgzf_stream(head + stream, 1)
+
+
+@pytest.mark.parametrize("level", [1, 0])
+def test_device_bgzf_members_decode_to_the_stream(level):
+    """GPU BGZF: every member's framing, BSIZE, CRC32 and ISIZE check out
+    (zlib inflates each body), the members are the stream's consecutive
+    65280-byte blocks, the EOF member closes the file; incompressible blocks
+    fall back to stored ones. Sizes cover empty, one byte, block edges and
+    the BAM stream of mapped records."""
+    say(f"test_device_bgzf_members_decode_to_the_stream level={level}")
+    rng = np.random.default_rng(11)
+    rec, tid, names, blob, name_off, seqs, lens, quals, qual_lens = random_se_batch(12, 3000)
+    bam = orc.bam_pack_se(rec, tid, blob, name_off, seqs, lens, quals, b"mojo1", qual_lens=qual_lens)
+    cases = {
+        "empty": b"", "one": b"A", "block": bytes(rng.integers(0, 40, 65280).astype(np.uint8)),
+        "block+1": bytes(rng.integers(0, 40, 65281).astype(np.uint8)), "zeros": bytes(200_000),
+        "random": bytes(rng.integers(0, 256, 150_000).astype(np.uint8)),
+        "two symbols": bytes(rng.integers(0, 2, 70_000).astype(np.uint8)),
+        "skewed": bytes(np.minimum(rng.geometric(0.5, 300_000), 255).astype(np.uint8)),
+        "bam": bam,
+    }
+    st = Stream(0)
+    dz = bam_emit.DeviceBgzf(max(len(v) for v in cases.values()))
+    for name, data in cases.items():
+        src = DeviceArray.from_host(np.frombuffer(data or b"\0", np.uint8))
+        for eof in (True, False):
+            dz.compress(st, src.ptr, len(data), level, eof)
+            z = dz.download(st)
+            members = bd.bgzf_members(z)
+            want_n = (len(data) + 65279) // 65280
+            assert len(members) == want_n + int(eof), name
+            for k in range(want_n):
+                assert members[k][1] == data[65280 * k:65280 * (k + 1)], (name, k)
+            if eof:
+                assert members[-1][0] == orc.BGZF_EOF
+            assert gzip.decompress(z) == data if z else data == b""
+            if level == 0:
+                assert len(z) == sum(18 + len(p) + 5 + 8 for p in (data[i:i + 65280] for i in
+                                     range(0, len(data), 65280))) + 28 * int(eof)
+        say(f"  {name}: {len(data)} -> {len(z)} bytes")
+    if level:
+        # incompressible blocks are stored; a BAM stream compresses
+        dz.compress(st, DeviceArray.from_host(np.frombuffer(cases["random"], np.uint8)).ptr, 150_000, level, False)
+        assert dz.total(st) <= 150_000 + 3 * (18 + 5 + 8)
+        dz.compress(st, DeviceArray.from_host(np.frombuffer(bam, np.uint8)).ptr, len(bam), level, False)
+        assert dz.total(st) < 0.8 * len(bam)
