@@ -265,8 +265,8 @@ int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threa
                      uint8_t *dst, int64_t dst_cap, int64_t *out_n);
 /* BGZF on the device: the same 65280-byte members (header, raw deflate body,
  * CRC32, ISIZE) of a device stream, compressed by the GPU: level 0 = stored
- * blocks, otherwise one dynamic-Huffman block per member (a stored block where
- * that is not smaller). The bodies are this library's own deflate, so the
+ * blocks, otherwise LZ77 (greedy, matches of 4..64 bytes) and one
+ * dynamic-Huffman block per member (a stored block where that is not smaller). The bodies are this library's own deflate, so the
  * bytes differ from zlib's; the members decode to the same stream. Writes
  * d_dst (ma_bgzf_bound(n) bytes suffice; 4-byte aligned, like d_src) and the
  * total length to d_total (device); members past dst_cap are not written.

@@ -9,13 +9,18 @@
 //
 //   * one 1024-thread workgroup per block (persistent grid), the block staged
 //     in LDS (64 bytes per thread);
-//   * a dynamic-Huffman block (RFC 1951 3.2.7) over the block's byte
-//     histogram: code lengths from a two-queue Huffman build limited to 15
-//     bits (7 for the code-length code) by Kraft-sum repair, canonical codes;
-//     the block falls back to a stored block when that is not smaller;
+//   * LZ77: a 4-byte hash table in LDS filled chunk by chunk (1024 positions
+//     per step, in order), so each position gets the latest earlier-chunk
+//     position with its hash as a match candidate, plus distance 1 for runs;
+//     each thread parses its 64-byte segment greedily (matches of 4..64 bytes
+//     stay inside the segment, so the parses are independent);
+//   * one dynamic-Huffman block (RFC 1951 3.2.7) per member: literal/length
+//     and distance code lengths from two-queue Huffman builds (two threads at
+//     once) limited to 15 bits (7 for the code-length code) by Kraft-sum
+//     repair, canonical codes; a stored block when that is not smaller;
 //   * per-thread bit offsets from a block scan; each thread writes its
-//     segment's codes into an LDS bit buffer (plain stores for the words it
-//     owns, atomic OR for the two it may share);
+//     segment's codes straight into the block's output slot (plain stores for
+//     the words it owns, atomic OR for the two it may share);
 //   * CRC32: per-thread CRCs of the 64-byte segments joined pairwise by the
 //     GF(2) shift crc(A||B) = crc(A) * x^(8|B|) + crc(B) mod P;
 //   * a second kernel writes every member (18-byte header, body, trailer) at
@@ -29,38 +34,15 @@ namespace {
 constexpr int DF_T = 1024;            // threads per workgroup
 constexpr int DF_SEG = 64;            // input bytes per thread
 constexpr int DF_IN = 65280;          // bytes per BGZF block
-constexpr int DF_OUT_WORDS = 16384;   // 64 KiB deflate buffer
+constexpr int DF_OUT_WORDS = 16384;   // 64 KiB output slot per block
 constexpr int64_t DF_SLOT = 65536;    // staging bytes per block
-constexpr int NLIT = 286;             // literal/length alphabet (no matches: 0..256 used)
-constexpr int NDIST = 2;              // two 1-bit distance codes (a complete, unused code)
+constexpr int NLIT = 286;             // literal/length alphabet
+constexpr int NDC = 30;               // distance alphabet
 constexpr int NCL = 19;               // code-length alphabet
-constexpr int NSEQ = NLIT + NDIST;    // code-length sequence entries at most
+constexpr int NSEQ = NLIT + NDC;      // code-length sequence entries at most
+constexpr int HASH_BITS = 13;         // match-candidate table: 8192 entries
 constexpr uint32_t CRC_POLY = 0xEDB88320u;  // reflected CRC-32 polynomial
 constexpr uint32_t STORED = 0xFFFFFFFFu;
-
-struct DfShared {
-    uint32_t out[DF_OUT_WORDS];
-    uint8_t data[DF_IN];
-    uint32_t crc_tab[256];
-    uint32_t xpow[40];         // x^(2^k) mod P (reflected), k = 0..39
-    uint32_t part[DF_T];       // per-thread CRCs, then bit counts
-    uint32_t wsum[DF_T / 64];
-    uint32_t freq[NLIT];
-    uint32_t nf[NLIT];         // Huffman build: internal node weights
-    uint16_t par[2 * NLIT];    // Huffman build: parent of each node
-    uint8_t dep[2 * NLIT];     // Huffman build: node depth
-    uint16_t sorted[NLIT];     // used symbols by ascending (freq, symbol)
-    uint8_t lit_len[NLIT];
-    uint16_t lit_code[NLIT];   // bit-reversed canonical codes
-    uint8_t cl_sym[NSEQ];
-    uint8_t cl_extra[NSEQ];
-    uint32_t cl_freq[NCL];
-    uint8_t cl_len[NCL];
-    uint16_t cl_code[NCL];
-    uint16_t cl_sorted[NCL];
-    uint32_t blc[32];
-    uint32_t misc[8];          // 0 used symbols, 1 CL entries, 2 hlit, 3 hclen, 4 header bits, 5 crc
-};
 
 // CL alphabet transmission order (RFC 1951 3.2.7)
 __device__ __forceinline__ int cl_order(int i) {
@@ -156,19 +138,19 @@ __device__ void huff_lengths(const uint32_t *freq, const uint16_t *sorted, int m
         for (uint32_t c = 0; c < blc[l]; c++) len[sorted[s++]] = (uint8_t)l;
 }
 
-// Canonical codes (bit-reversed for LSB-first output) from lengths.
+// Canonical codes (bit-reversed for LSB-first output) from lengths; blc has
+// 32 entries: counts in 0..15, next codes in 16..31 (shared memory).
 __device__ void canonical(const uint8_t *len, int nsym, int maxlen, uint16_t *code, uint32_t *blc) {
     for (int l = 0; l < 32; l++) blc[l] = 0;
     for (int s = 0; s < nsym; s++) blc[len[s]]++;
     blc[0] = 0;
-    uint32_t next[16];
     uint32_t c = 0;
     for (int l = 1; l <= maxlen; l++) {
         c = (c + blc[l - 1]) << 1;
-        next[l] = c;
+        blc[16 + l] = c;
     }
     for (int s = 0; s < nsym; s++)
-        if (len[s]) code[s] = (uint16_t)bitrev(next[len[s]]++, len[s]);
+        if (len[s]) code[s] = (uint16_t)bitrev(blc[16 + len[s]]++, len[s]);
 }
 
 // LSB-first bit writer into the shared buffer: words strictly inside the
@@ -220,14 +202,119 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *w
     return pre + x - v;
 }
 
+struct DfShared {
+    uint8_t data[DF_IN];
+    uint32_t head[1 << HASH_BITS];  // per 4-byte hash: 1 + the latest position of earlier chunks
+    uint32_t crc_tab[256];
+    uint32_t xpow[40];         // x^(2^k) mod P (reflected), k = 0..39
+    uint32_t part[DF_T];       // per-thread CRCs, then bit counts
+    uint32_t wsum[DF_T / 64];
+    uint8_t ntok[DF_T];        // tokens of each thread's segment
+    uint32_t freq[NLIT];
+    uint32_t dfreq[NDC];
+    uint8_t lit_len[NLIT];
+    uint16_t lit_code[NLIT];   // bit-reversed canonical codes
+    uint8_t d_len[NDC];
+    uint16_t d_code[NDC];
+    uint16_t sorted[NLIT];     // used symbols by ascending (freq, symbol)
+    uint16_t dsorted[NDC];
+    uint32_t nf[NLIT];         // Huffman build scratch (literal/length and code-length trees)
+    uint16_t par[2 * NLIT];
+    uint8_t dep[2 * NLIT];
+    uint32_t dnf[NDC];         // Huffman build scratch (distance tree, built concurrently)
+    uint16_t dpar[2 * NDC];
+    uint8_t ddep[2 * NDC];
+    uint32_t dblc[32];
+    uint8_t cl_sym[NSEQ];
+    uint8_t cl_extra[NSEQ];
+    uint32_t cl_freq[NCL];
+    uint8_t cl_len[NCL];
+    uint16_t cl_code[NCL];
+    uint16_t cl_sorted[NCL];
+    uint32_t blc[32];
+    uint32_t misc[8];          // 0 used lit/len symbols, 1 CL entries, 2 hlit, 3 hclen, 4 header bits,
+                               // 5 used distance symbols, 6 hdist
+};
+
+// Length symbol (257..285) and extra bits of a match length 3..258 (RFC 1951 3.2.5).
+__device__ __forceinline__ void len_code(uint32_t L, uint32_t &sym, uint32_t &ebits, uint32_t &extra) {
+    if (L == 258) {
+        sym = 285;
+        ebits = extra = 0;
+        return;
+    }
+    const uint32_t v = L - 3;
+    if (v < 8) {
+        sym = 257 + v;
+        ebits = extra = 0;
+        return;
+    }
+    const uint32_t e = 31 - __clz(v) - 2;
+    sym = 257 + 4 * (e + 1) + ((v >> e) & 3);
+    ebits = e;
+    extra = v & ((1u << e) - 1);
+}
+
+// Distance symbol (0..29) and extra bits of a distance 1..32768.
+__device__ __forceinline__ void dist_code(uint32_t D, uint32_t &sym, uint32_t &ebits, uint32_t &extra) {
+    const uint32_t d = D - 1;
+    if (d < 4) {
+        sym = d;
+        ebits = extra = 0;
+        return;
+    }
+    const uint32_t e = 31 - __clz(d) - 1;
+    sym = 2 * (e + 1) + ((d >> e) & 1);
+    ebits = e;
+    extra = d & ((1u << e) - 1);
+}
+
+__device__ __forceinline__ uint32_t hash4(const uint8_t *d, int p) {
+    const uint32_t x = (uint32_t)d[p] | (uint32_t)d[p + 1] << 8 | (uint32_t)d[p + 2] << 16 | (uint32_t)d[p + 3] << 24;
+    return (x * 2654435761u) >> (32 - HASH_BITS);
+}
+
+__device__ __forceinline__ int match_len(const uint8_t *d, int q, int p, int maxl) {
+    int i = 0;
+    while (i < maxl && d[q + i] == d[p + i]) i++;
+    return i;
+}
+
+// Ranks the used symbols of freq[0..nsym) by (freq, symbol) into sorted[];
+// threads [t0, t0 + nsym) take one symbol each; counts them into *m.
+__device__ __forceinline__ void rank_symbols(const uint32_t *freq, int nsym, uint16_t *sorted, uint32_t *m, int t0) {
+    const int s = (int)threadIdx.x - t0;
+    if (s >= 0 && s < nsym) {
+        const uint32_t f = freq[s];
+        if (f) {
+            int r = 0;
+            for (int u = 0; u < nsym; u++) {
+                const uint32_t g = freq[u];
+                r += g && (g < f || (g == f && u < s));
+            }
+            sorted[r] = (uint16_t)s;
+            atomicAdd(m, 1u);
+        }
+    }
+}
+
+// Tokens: a literal byte b is b; a match is 1 << 31 | length << 16 | distance.
+constexpr uint32_t TOK_MATCH = 1u << 31;
+constexpr int MIN_MATCH = 4;
+
 __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__restrict__ src, int64_t n,
-                                                            int64_t nb, int huffman,
+                                                            int64_t nb, int level,
                                                             uint32_t *__restrict__ slots,
                                                             uint32_t *__restrict__ body_len,
                                                             uint32_t *__restrict__ crc_out,
-                                                            uint32_t *__restrict__ member_bytes) {
+                                                            uint32_t *__restrict__ member_bytes,
+                                                            uint16_t *__restrict__ cand_all,
+                                                            uint32_t *__restrict__ tok_all) {
     __shared__ DfShared S;
     const int t = threadIdx.x;
+    const bool huffman = level > 0, lz = level > 0;
+    uint16_t *const cand = cand_all + (int64_t)blockIdx.x * DF_IN;        // this workgroup's scratch
+    uint32_t *const tok = tok_all + ((int64_t)blockIdx.x * DF_T + t) * DF_SEG;
     if (t < 256) {
         uint32_t c = (uint32_t)t;
         for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
@@ -244,11 +331,17 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
     for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const int64_t base = b * DF_IN;
         const int bl = (int)(n - base < DF_IN ? n - base : DF_IN);
-        // 1. stage the block, clear the buffers
-        for (int i = t; i < DF_OUT_WORDS; i += DF_T) S.out[i] = 0;
+        uint32_t *const slot = slots + b * (DF_SLOT / 4);
+        // 1. stage the block, clear the tables and the block's output slot
+        for (int i = t; i < DF_OUT_WORDS; i += DF_T) slot[i] = 0;
+        for (int i = t; i < (1 << HASH_BITS); i += DF_T) S.head[i] = 0;
         for (int i = t; i < NLIT; i += DF_T) {
             S.freq[i] = 0;
             S.lit_len[i] = 0;
+        }
+        if (t < NDC) {
+            S.dfreq[t] = 0;
+            S.d_len[t] = 0;
         }
         if (t < NCL) {
             S.cl_freq[t] = 0;
@@ -264,18 +357,28 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
         }
         __syncthreads();
-        // 2. histogram and per-segment CRC
+        // 2. per-segment CRC
         const int s0 = t * DF_SEG, s1 = min(bl, s0 + DF_SEG);
         uint32_t c = 0xFFFFFFFFu;
-        for (int i = s0; i < s1; i++) {
-            const uint32_t v = S.data[i];
-            if (huffman) atomicAdd(&S.freq[v], 1u);
-            c = S.crc_tab[(c ^ v) & 255] ^ (c >> 8);
-        }
+        for (int i = s0; i < s1; i++) c = S.crc_tab[(c ^ S.data[i]) & 255] ^ (c >> 8);
         S.part[t] = ~c;
         if (t == 0) S.freq[256] = 1;  // end of block
-        __syncthreads();
-        // 3. CRC of the block: join segment CRCs pairwise
+        // 3. match candidates: chunks of DF_T consecutive positions in order;
+        //    a position sees the latest earlier-chunk position with its hash
+        if (lz) {
+            for (int c0 = 0; c0 < bl; c0 += DF_T) {
+                const int p = c0 + t;
+                const bool has = p + MIN_MATCH <= bl;
+                const uint32_t h = has ? hash4(S.data, p) : 0;
+                if (p < bl) cand[p] = has ? (uint16_t)S.head[h] : (uint16_t)0;
+                __syncthreads();
+                if (has) atomicMax(&S.head[h], (uint32_t)(p + 1));
+                __syncthreads();
+            }
+        } else {
+            __syncthreads();
+        }
+        // 4. CRC of the block: join segment CRCs pairwise
         for (int s = 1; s < DF_T; s <<= 1) {
             if ((t & (2 * s - 1)) == 0) {
                 const int rstart = (t + s) * DF_SEG;
@@ -288,32 +391,74 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
         const uint32_t crc = S.part[0];
         uint32_t nbytes = 0xFFFFFFFFu;
         if (huffman) {
-            // 4. literal/length code: used symbols ranked by (freq, symbol)
-            for (int s = t; s < NLIT; s += DF_T) {
-                const uint32_t f = S.freq[s];
-                if (f) {
-                    int r = 0;
-                    for (int u = 0; u < NLIT; u++) {
-                        const uint32_t g = S.freq[u];
-                        r += g && (g < f || (g == f && u < s));
+            // 5. greedy parse of the segment (matches stay inside it), symbol counts
+            int k = 0;
+            for (int p = s0; p < s1;) {
+                int best = 0, bd = 0;
+                const int maxl = min(258, s1 - p);
+                if (lz && maxl >= MIN_MATCH) {
+                    const int cq = cand[p];
+                    if (cq && p - (cq - 1) <= 32768) {
+                        const int l = match_len(S.data, cq - 1, p, maxl);
+                        if (l > best) {
+                            best = l;
+                            bd = p - (cq - 1);
+                        }
                     }
-                    S.sorted[r] = (uint16_t)s;
-                    atomicAdd(&S.misc[0], 1u);
+                    if (p >= 1 && best < maxl) {  // runs: distance 1
+                        const int l = match_len(S.data, p - 1, p, maxl);
+                        if (l > best) {
+                            best = l;
+                            bd = 1;
+                        }
+                    }
                 }
+                uint32_t tk;
+                if (best >= MIN_MATCH) {
+                    uint32_t ls, le, lx, ds, de, dx;
+                    len_code((uint32_t)best, ls, le, lx);
+                    dist_code((uint32_t)bd, ds, de, dx);
+                    atomicAdd(&S.freq[ls], 1u);
+                    atomicAdd(&S.dfreq[ds], 1u);
+                    tk = TOK_MATCH | (uint32_t)best << 16 | (uint32_t)bd;
+                    p += best;
+                } else {
+                    tk = S.data[p];
+                    atomicAdd(&S.freq[tk], 1u);
+                    p += 1;
+                }
+                tok[k++] = tk;
             }
+            S.ntok[t] = (uint8_t)k;
+            __syncthreads();
+            // 6. the two codes: symbols ranked in parallel, trees built by threads 0 and 64 at once
+            rank_symbols(S.freq, NLIT, S.sorted, &S.misc[0], 0);
+            rank_symbols(S.dfreq, NDC, S.dsorted, &S.misc[5], 512);
             __syncthreads();
             if (t == 0) {
                 huff_lengths(S.freq, S.sorted, (int)S.misc[0], 15, 0, S.lit_len, S.nf, S.par, S.dep, S.blc);
                 canonical(S.lit_len, NLIT, 15, S.lit_code, S.blc);
-                // code-length sequence (lit/len lengths then the two 1-bit distance codes), run-length coded
+            } else if (t == 64) {
+                if (S.misc[5] == 0) {  // no match: two 1-bit codes, a complete and unused distance code
+                    S.d_len[0] = S.d_len[1] = 1;
+                } else {
+                    huff_lengths(S.dfreq, S.dsorted, (int)S.misc[5], 15, 0, S.d_len, S.dnf, S.dpar, S.ddep, S.dblc);
+                }
+                canonical(S.d_len, NDC, 15, S.d_code, S.dblc);
+            }
+            __syncthreads();
+            if (t == 0) {
+                // code-length sequence (literal/length then distance lengths), run-length coded
                 int hlit = NLIT;
                 while (hlit > 257 && S.lit_len[hlit - 1] == 0) hlit--;
-                const int total = hlit + NDIST;
+                int hdist = NDC;
+                while (hdist > 1 && S.d_len[hdist - 1] == 0) hdist--;
+                const int total = hlit + hdist;
                 int i = 0, ns = 0;
                 while (i < total) {
-                    const int v = i < hlit ? S.lit_len[i] : 1;
+                    const int v = i < hlit ? S.lit_len[i] : S.d_len[i - hlit];
                     int run = 1;
-                    while (i + run < total && (i + run < hlit ? S.lit_len[i + run] : 1) == v) run++;
+                    while (i + run < total && (i + run < hlit ? S.lit_len[i + run] : S.d_len[i + run - hlit]) == v) run++;
                     i += run;
                     if (v == 0) {
                         while (run >= 11) {
@@ -343,7 +488,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                         S.cl_extra[ns++] = 0;
                     }
                 }
-                for (int k = 0; k < ns; k++) S.cl_freq[S.cl_sym[k]]++;
+                for (int q = 0; q < ns; q++) S.cl_freq[S.cl_sym[q]]++;
                 int m = 0;
                 for (int s = 0; s < NCL; s++)
                     if (S.cl_freq[s]) {
@@ -360,48 +505,70 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 int hclen = NCL;
                 while (hclen > 4 && S.cl_len[cl_order(hclen - 1)] == 0) hclen--;
                 uint32_t hb = 3 + 5 + 5 + 4 + 3 * hclen;
-                for (int k = 0; k < ns; k++) {
-                    const int sym = S.cl_sym[k];
+                for (int q = 0; q < ns; q++) {
+                    const int sym = S.cl_sym[q];
                     hb += S.cl_len[sym] + (sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0);
                 }
                 S.misc[1] = (uint32_t)ns;
                 S.misc[2] = (uint32_t)hlit;
                 S.misc[3] = (uint32_t)hclen;
                 S.misc[4] = hb;
+                S.misc[6] = (uint32_t)hdist;
             }
             __syncthreads();
-            // 5. bit offsets of the segments
+            // 7. bit offsets of the segments
+            const int ntok = S.ntok[t];
             uint32_t bits = 0;
-            for (int i = s0; i < s1; i++) bits += S.lit_len[S.data[i]];
+            for (int q = 0; q < ntok; q++) {
+                const uint32_t tk = tok[q];
+                if (tk & TOK_MATCH) {
+                    uint32_t ls, le, lx, ds, de, dx;
+                    len_code((tk >> 16) & 0x1FF, ls, le, lx);
+                    dist_code(tk & 0xFFFF ? tk & 0xFFFF : 65536, ds, de, dx);
+                    bits += S.lit_len[ls] + le + S.d_len[ds] + de;
+                } else {
+                    bits += S.lit_len[tk];
+                }
+            }
             uint32_t dbits;
             const uint32_t off = block_excl_scan_1024(bits, S.wsum, dbits);
             const uint32_t hb = S.misc[4];
             const uint32_t total_bits = hb + dbits + S.lit_len[256];
             nbytes = (total_bits + 7) >> 3;
             if (nbytes < (uint32_t)bl + 5) {
-                // 6. header (thread 0), then every segment's codes, EOB after the last byte
+                // 8. header (thread 0), then every segment's codes, EOB after the last one
                 if (t == 0) {
-                    BitWriter w(S.out, 0);
+                    BitWriter w(slot, 0);
                     w.put(1, 1);  // BFINAL
                     w.put(2, 2);  // BTYPE = dynamic Huffman
                     w.put(S.misc[2] - 257, 5);
-                    w.put(NDIST - 1, 5);
+                    w.put(S.misc[6] - 1, 5);
                     w.put(S.misc[3] - 4, 4);
-                    for (uint32_t k = 0; k < S.misc[3]; k++) w.put(S.cl_len[cl_order((int)k)], 3);
-                    for (uint32_t k = 0; k < S.misc[1]; k++) {
-                        const int sym = S.cl_sym[k];
+                    for (uint32_t q = 0; q < S.misc[3]; q++) w.put(S.cl_len[cl_order((int)q)], 3);
+                    for (uint32_t q = 0; q < S.misc[1]; q++) {
+                        const int sym = S.cl_sym[q];
                         w.put(S.cl_code[sym], S.cl_len[sym]);
-                        if (sym == 16) w.put(S.cl_extra[k], 2);
-                        if (sym == 17) w.put(S.cl_extra[k], 3);
-                        if (sym == 18) w.put(S.cl_extra[k], 7);
+                        if (sym == 16) w.put(S.cl_extra[q], 2);
+                        if (sym == 17) w.put(S.cl_extra[q], 3);
+                        if (sym == 18) w.put(S.cl_extra[q], 7);
                     }
                     w.finish();
                 }
                 if (s0 < bl) {
-                    BitWriter w(S.out, hb + off);
-                    for (int i = s0; i < s1; i++) {
-                        const uint32_t v = S.data[i];
-                        w.put(S.lit_code[v], S.lit_len[v]);
+                    BitWriter w(slot, hb + off);
+                    for (int q = 0; q < ntok; q++) {
+                        const uint32_t tk = tok[q];
+                        if (tk & TOK_MATCH) {
+                            uint32_t ls, le, lx, ds, de, dx;
+                            len_code((tk >> 16) & 0x1FF, ls, le, lx);
+                            dist_code(tk & 0xFFFF ? tk & 0xFFFF : 65536, ds, de, dx);
+                            w.put(S.lit_code[ls], S.lit_len[ls]);
+                            if (le) w.put(lx, (int)le);
+                            w.put(S.d_code[ds], S.d_len[ds]);
+                            if (de) w.put(dx, (int)de);
+                        } else {
+                            w.put(S.lit_code[tk], S.lit_len[tk]);
+                        }
                     }
                     if (s1 == bl) w.put(S.lit_code[256], S.lit_len[256]);
                     w.finish();
@@ -411,10 +578,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
         }
         __syncthreads();
-        // 7. body to the block's staging slot, sizes and CRC
-        uint32_t *slot = slots + b * (DF_SLOT / 4);
-        if (nbytes != 0xFFFFFFFFu)
-            for (uint32_t i = (uint32_t)t; i < (nbytes + 3) / 4; i += DF_T) slot[i] = S.out[i];
+        // 9. sizes and CRC (the body is already in the block's slot)
         if (t == 0) {
             body_len[b] = nbytes == 0xFFFFFFFFu ? STORED : nbytes;
             crc_out[b] = crc;
@@ -498,12 +662,15 @@ __global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
     if (threadIdx.x == 0) *p = v;
 }
 
+constexpr int DF_MAX_GRID = 256;  // resident workgroups at most (one per CU)
+
 struct DfScratch {
-    size_t slots, body, crc, bytes, off, partial, total;
+    size_t slots, body, crc, bytes, off, partial, cand, tok, total;
 };
 
 DfScratch df_layout(int64_t n) {
     const int64_t nb = (n + DF_IN - 1) / DF_IN, nm = nb + 1;
+    const int64_t g = nb < DF_MAX_GRID ? (nb > 0 ? nb : 1) : DF_MAX_GRID;
     DfScratch s;
     size_t o = 0;
     auto take = [&](size_t b) {
@@ -517,6 +684,8 @@ DfScratch df_layout(int64_t n) {
     s.bytes = take((size_t)nm * 4);
     s.off = take((size_t)(nm + 1) * 8);
     s.partial = take(scan_scratch_bytes(nm));
+    s.cand = take((size_t)g * DF_IN * 2);
+    s.tok = take((size_t)g * DF_T * DF_SEG * 4);
     s.total = o;
     return s;
 }
@@ -546,9 +715,11 @@ int ma_bgzf_compress_dev(void *stream, const uint8_t *d_src, int64_t n, int32_t 
         int dev = 0, n_cu = 256;
         MA_HIP(hipGetDevice(&dev));
         MA_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev));
-        const int64_t grid = nb < n_cu ? nb : n_cu;
-        bgzf_deflate_kernel<<<(unsigned)grid, DF_T, 0, st>>>(d_src, n, nb, level > 0, (uint32_t *)(s + L.slots),
-                                                              body, crc, bytes);
+        int64_t grid = nb < n_cu ? nb : n_cu;
+        if (grid > DF_MAX_GRID) grid = DF_MAX_GRID;
+        bgzf_deflate_kernel<<<(unsigned)grid, DF_T, 0, st>>>(d_src, n, nb, level, (uint32_t *)(s + L.slots),
+                                                              body, crc, bytes, (uint16_t *)(s + L.cand),
+                                                              (uint32_t *)(s + L.tok));
         MA_LAUNCHED();
     }
     if (add_eof) {
