@@ -207,6 +207,11 @@ struct DfShared {
     uint32_t head[1 << HASH_BITS];  // per 4-byte hash: 1 + the latest position of earlier chunks
     uint32_t crc_tab[256];
     uint32_t xpow[40];         // x^(2^k) mod P (reflected), k = 0..39
+    uint32_t shift64[DF_T];    // x^(8 * 64 k) mod P: the CRC shift over k whole segments
+    uint32_t crcw[DF_T / 64];  // per-wave XOR of the shifted segment CRCs
+    uint32_t sf[NLIT];         // frequencies in ranked order
+    uint32_t dsf[NDC];
+    uint32_t clblc[32];
     uint32_t part[DF_T];       // per-thread CRCs, then bit counts
     uint32_t wsum[DF_T / 64];
     uint8_t ntok[DF_T];        // tokens of each thread's segment
@@ -232,8 +237,9 @@ struct DfShared {
     uint16_t cl_code[NCL];
     uint16_t cl_sorted[NCL];
     uint32_t blc[32];
-    uint32_t misc[8];          // 0 used lit/len symbols, 1 CL entries, 2 hlit, 3 hclen, 4 header bits,
-                               // 5 used distance symbols, 6 hdist
+    uint32_t misc[8];          // 0 used lit/len symbols, 1 CL entries, 2 last used lit/len symbol + 1,
+                               // 3 hclen, 5 used distance symbols, 6 last used distance symbol + 1,
+                               // 7 used CL symbols
 };
 
 // Length symbol (257..285) and extra bits of a match length 3..258 (RFC 1951 3.2.5).
@@ -282,7 +288,8 @@ __device__ __forceinline__ int match_len(const uint8_t *d, int q, int p, int max
 
 // Ranks the used symbols of freq[0..nsym) by (freq, symbol) into sorted[];
 // threads [t0, t0 + nsym) take one symbol each; counts them into *m.
-__device__ __forceinline__ void rank_symbols(const uint32_t *freq, int nsym, uint16_t *sorted, uint32_t *m, int t0) {
+__device__ __forceinline__ void rank_symbols(const uint32_t *freq, int nsym, uint16_t *sorted, uint32_t *sf,
+                                             uint32_t *m, int t0) {
     const int s = (int)threadIdx.x - t0;
     if (s >= 0 && s < nsym) {
         const uint32_t f = freq[s];
@@ -293,9 +300,124 @@ __device__ __forceinline__ void rank_symbols(const uint32_t *freq, int nsym, uin
                 r += g && (g < f || (g == f && u < s));
             }
             sorted[r] = (uint16_t)s;
+            sf[r] = f;
             atomicAdd(m, 1u);
         }
     }
+}
+
+// Two-queue Huffman over m >= 2 leaves with ascending weights sf[0..m):
+// internal node k (id m + k) gets weight nf[k]; par[] = parent id of every
+// node (leaves first). One thread.
+__device__ void huff_tree(const uint32_t *sf, int m, uint32_t *nf, uint16_t *par) {
+    int i = 0, j = 0;
+    uint32_t li = sf[0];
+    for (int k = 0; k < m - 1; k++) {
+        uint32_t f = 0;
+        for (int pick = 0; pick < 2; pick++) {
+            if (i < m && (j >= k || li <= nf[j])) {
+                par[i] = (uint16_t)(m + k);
+                f += li;
+                i++;
+                li = i < m ? sf[i] : 0;
+            } else {
+                par[m + j] = (uint16_t)(m + k);
+                f += nf[j];
+                j++;
+            }
+        }
+        nf[k] = f;
+    }
+}
+
+// Depth of leaf x: steps up to the root (id root).
+__device__ __forceinline__ int leaf_depth(const uint16_t *par, int x, int root) {
+    int d = 0;
+    while (x != root && d < 2 * NLIT) {
+        x = par[x];
+        d++;
+    }
+    return d;
+}
+
+// Per-length counts blc[1..maxlen] (deeper leaves already counted at maxlen)
+// repaired so that the Kraft sum is exactly 1. One thread.
+__device__ void kraft_repair(uint32_t *blc, int maxlen) {
+    uint32_t kraft = 0;
+    for (int l = 1; l <= maxlen; l++) kraft += blc[l] << (maxlen - l);
+    const uint32_t target = 1u << maxlen;
+    while (kraft > target) {  // lengthen the deepest code that can still grow
+        int l = maxlen - 1;
+        while (l > 1 && blc[l] == 0) l--;
+        if (blc[l] == 0) break;
+        blc[l]--;
+        blc[l + 1]++;
+        kraft -= 1u << (maxlen - l - 1);
+    }
+    while (kraft < target) {  // shorten the deepest code: its 2^(maxlen-l) divides the deficit
+        int l = maxlen;
+        while (l > 2 && blc[l] == 0) l--;
+        if (blc[l] == 0) break;
+        blc[l]--;
+        blc[l - 1]++;
+        kraft += 1u << (maxlen - l);
+    }
+}
+
+// Length of the r-th least frequent symbol: lengths handed out longest first.
+__device__ __forceinline__ uint8_t rank_length(const uint32_t *blc, int maxlen, int r) {
+    uint32_t acc = 0;
+    for (int l = maxlen; l >= 1; l--) {
+        acc += blc[l];
+        if ((uint32_t)r < acc) return (uint8_t)l;
+    }
+    return 1;
+}
+
+// Canonical code of symbol s (bit-reversed): the first code of its length
+// plus the number of smaller symbols of that length.
+__device__ __forceinline__ uint16_t canon_code(const uint8_t *len, const uint32_t *blc, int s) {
+    const int l = len[s];
+    if (l == 0) return 0;
+    uint32_t code = 0;
+    for (int k = 1; k <= l; k++) code = (code + (k > 1 ? blc[k - 1] : 0)) << 1;
+    for (int u = 0; u < s; u++) code += len[u] == l;
+    return (uint16_t)bitrev(code, l);
+}
+
+// Code-length symbols of a run of `run` equal lengths v (RFC 1951 3.2.7:
+// 16 repeats the previous length 3..6 times, 17 / 18 give 3..10 / 11..138
+// zeros); writes them when sym is not null; returns their number.
+__device__ int rle_emit(int v, int run, uint8_t *sym, uint8_t *extra) {
+    int ns = 0;
+    auto put = [&](int sv, int ev) {
+        if (sym) {
+            sym[ns] = (uint8_t)sv;
+            extra[ns] = (uint8_t)ev;
+        }
+        ns++;
+    };
+    if (v == 0) {
+        while (run >= 11) {
+            const int r = run < 138 ? run : 138;
+            put(18, r - 11);
+            run -= r;
+        }
+        if (run >= 3) {
+            put(17, run - 3);
+            run = 0;
+        }
+    } else {
+        put(v, 0);
+        run--;
+        while (run >= 3) {
+            const int r = run < 6 ? run : 6;
+            put(16, r - 3);
+            run -= r;
+        }
+    }
+    for (; run > 0; run--) put(v, 0);
+    return ns;
 }
 
 // Tokens: a literal byte b is b; a match is 1 << 31 | length << 16 | distance.
@@ -327,6 +449,8 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             x = multmodp(x, x);
         }
     }
+    __syncthreads();
+    S.shift64[t] = x8n(S.xpow, (uint32_t)(DF_SEG * t));
     __syncthreads();
     for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const int64_t base = b * DF_IN;
@@ -361,7 +485,17 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
         const int s0 = t * DF_SEG, s1 = min(bl, s0 + DF_SEG);
         uint32_t c = 0xFFFFFFFFu;
         for (int i = s0; i < s1; i++) c = S.crc_tab[(c ^ S.data[i]) & 255] ^ (c >> 8);
-        S.part[t] = ~c;
+        {
+            // crc(S_0 .. S_k) = XOR over segments of crc(S_t) * x^(8 * bytes after S_t) mod P
+            uint32_t v = 0;
+            if (s0 < bl) {
+                const int after = bl - s1;
+                v = multmodp(after % DF_SEG == 0 ? S.shift64[after / DF_SEG] : x8n(S.xpow, (uint32_t)after), ~c);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
+            if ((t & 63) == 0) S.crcw[t >> 6] = v;
+        }
         if (t == 0) S.freq[256] = 1;  // end of block
         // 3. match candidates: chunks of DF_T consecutive positions in order;
         //    a position sees the latest earlier-chunk position with its hash
@@ -378,17 +512,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
         } else {
             __syncthreads();
         }
-        // 4. CRC of the block: join segment CRCs pairwise
-        for (int s = 1; s < DF_T; s <<= 1) {
-            if ((t & (2 * s - 1)) == 0) {
-                const int rstart = (t + s) * DF_SEG;
-                int rlen = bl - rstart;
-                rlen = rlen < 0 ? 0 : (rlen > s * DF_SEG ? s * DF_SEG : rlen);
-                if (rlen > 0) S.part[t] = multmodp(x8n(S.xpow, (uint32_t)rlen), S.part[t]) ^ S.part[t + s];
-            }
-            __syncthreads();
-        }
-        const uint32_t crc = S.part[0];
+        // 4. CRC of the block (the barriers above ordered the per-wave parts)
+        uint32_t crc = 0;
+        for (int w = 0; w < DF_T / 64; w++) crc ^= S.crcw[w];
         uint32_t nbytes = 0xFFFFFFFFu;
         if (huffman) {
             // 5. greedy parse of the segment (matches stay inside it), symbol counts
@@ -431,91 +557,92 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
             S.ntok[t] = (uint8_t)k;
             __syncthreads();
-            // 6. the two codes: symbols ranked in parallel, trees built by threads 0 and 64 at once
-            rank_symbols(S.freq, NLIT, S.sorted, &S.misc[0], 0);
-            rank_symbols(S.dfreq, NDC, S.dsorted, &S.misc[5], 512);
-            __syncthreads();
-            if (t == 0) {
-                huff_lengths(S.freq, S.sorted, (int)S.misc[0], 15, 0, S.lit_len, S.nf, S.par, S.dep, S.blc);
-                canonical(S.lit_len, NLIT, 15, S.lit_code, S.blc);
-            } else if (t == 64) {
-                if (S.misc[5] == 0) {  // no match: two 1-bit codes, a complete and unused distance code
-                    S.d_len[0] = S.d_len[1] = 1;
-                } else {
-                    huff_lengths(S.dfreq, S.dsorted, (int)S.misc[5], 15, 0, S.d_len, S.dnf, S.dpar, S.ddep, S.dblc);
-                }
-                canonical(S.d_len, NDC, 15, S.d_code, S.dblc);
+            // 6. the two codes: symbols ranked in parallel, the two-queue merges by
+            //    threads 0 and 64 at once, then depths, lengths and codes in parallel
+            rank_symbols(S.freq, NLIT, S.sorted, S.sf, &S.misc[0], 0);
+            rank_symbols(S.dfreq, NDC, S.dsorted, S.dsf, &S.misc[5], 512);
+            if (t < 32) {
+                S.blc[t] = 0;
+                S.dblc[t] = 0;
+                S.clblc[t] = 0;
             }
             __syncthreads();
-            if (t == 0) {
-                // code-length sequence (literal/length then distance lengths), run-length coded
-                int hlit = NLIT;
-                while (hlit > 257 && S.lit_len[hlit - 1] == 0) hlit--;
-                int hdist = NDC;
-                while (hdist > 1 && S.d_len[hdist - 1] == 0) hdist--;
-                const int total = hlit + hdist;
-                int i = 0, ns = 0;
-                while (i < total) {
-                    const int v = i < hlit ? S.lit_len[i] : S.d_len[i - hlit];
-                    int run = 1;
-                    while (i + run < total && (i + run < hlit ? S.lit_len[i + run] : S.d_len[i + run - hlit]) == v) run++;
-                    i += run;
-                    if (v == 0) {
-                        while (run >= 11) {
-                            const int r = run < 138 ? run : 138;
-                            S.cl_sym[ns] = 18;
-                            S.cl_extra[ns++] = (uint8_t)(r - 11);
-                            run -= r;
-                        }
-                        if (run >= 3) {
-                            S.cl_sym[ns] = 17;
-                            S.cl_extra[ns++] = (uint8_t)(run - 3);
-                            run = 0;
-                        }
+            const int m = (int)S.misc[0], md = (int)S.misc[5];  // m >= 2: a byte and end of block
+            if (t == 0) huff_tree(S.sf, m, S.nf, S.par);
+            if (t == 64 && md >= 2) huff_tree(S.dsf, md, S.dnf, S.dpar);
+            __syncthreads();
+            if (t < m) atomicAdd(&S.blc[min(leaf_depth(S.par, t, 2 * m - 2), 15)], 1u);
+            if (t >= 512 && t - 512 < md && md >= 2)
+                atomicAdd(&S.dblc[min(leaf_depth(S.dpar, t - 512, 2 * md - 2), 15)], 1u);
+            __syncthreads();
+            if (t == 0) kraft_repair(S.blc, 15);
+            if (t == 64) {
+                if (md >= 2) {
+                    kraft_repair(S.dblc, 15);
+                } else {  // no match: two 1-bit codes; one distance symbol: it and a spare
+                    if (md == 0) {
+                        S.d_len[0] = S.d_len[1] = 1;
                     } else {
-                        S.cl_sym[ns] = (uint8_t)v;
-                        S.cl_extra[ns++] = 0;
-                        run--;
-                        while (run >= 3) {
-                            const int r = run < 6 ? run : 6;
-                            S.cl_sym[ns] = 16;
-                            S.cl_extra[ns++] = (uint8_t)(r - 3);
-                            run -= r;
-                        }
+                        const int s = S.dsorted[0];
+                        S.d_len[s] = 1;
+                        S.d_len[s == 0 ? 1 : 0] = 1;
                     }
-                    for (; run > 0; run--) {
-                        S.cl_sym[ns] = (uint8_t)v;
-                        S.cl_extra[ns++] = 0;
-                    }
+                    S.dblc[1] = 2;
                 }
-                for (int q = 0; q < ns; q++) S.cl_freq[S.cl_sym[q]]++;
-                int m = 0;
-                for (int s = 0; s < NCL; s++)
-                    if (S.cl_freq[s]) {
-                        int r = 0;
-                        for (int u = 0; u < NCL; u++) {
-                            const uint32_t g = S.cl_freq[u], f = S.cl_freq[s];
-                            r += g && (g < f || (g == f && u < s));
-                        }
-                        S.cl_sorted[r] = (uint16_t)s;
-                        m++;
-                    }
-                huff_lengths(S.cl_freq, S.cl_sorted, m, 7, 0, S.cl_len, S.nf, S.par, S.dep, S.blc);
-                canonical(S.cl_len, NCL, 7, S.cl_code, S.blc);
+            }
+            __syncthreads();
+            if (t < m) S.lit_len[S.sorted[t]] = rank_length(S.blc, 15, t);
+            if (t >= 512 && t - 512 < md && md >= 2) S.d_len[S.dsorted[t - 512]] = rank_length(S.dblc, 15, t - 512);
+            __syncthreads();
+            if (t < NLIT) {
+                S.lit_code[t] = canon_code(S.lit_len, S.blc, t);
+                if (S.lit_len[t]) atomicMax(&S.misc[2], (uint32_t)(t + 1));
+            }
+            if (t >= 512 && t - 512 < NDC) {
+                S.d_code[t - 512] = canon_code(S.d_len, S.dblc, t - 512);
+                if (S.d_len[t - 512]) atomicMax(&S.misc[6], (uint32_t)(t - 511));
+            }
+            __syncthreads();
+            // code-length sequence (literal/length then distance lengths): maximal runs
+            // found in parallel, each coded at its scanned offset
+            const int hlit = max(257, (int)S.misc[2]), hdist = max(1, (int)S.misc[6]);
+            const int total = hlit + hdist;
+            int rv = 0, run = 0;
+            uint32_t cnt = 0;
+            if (t < total) {
+                rv = t < hlit ? S.lit_len[t] : S.d_len[t - hlit];
+                const int pv = t == 0 ? -1 : (t - 1 < hlit ? S.lit_len[t - 1] : S.d_len[t - 1 - hlit]);
+                if (rv != pv) {
+                    run = 1;
+                    while (t + run < total && (t + run < hlit ? S.lit_len[t + run] : S.d_len[t + run - hlit]) == rv) run++;
+                    cnt = (uint32_t)rle_emit(rv, run, nullptr, nullptr);
+                }
+            }
+            uint32_t ns;
+            const uint32_t co = block_excl_scan_1024(cnt, S.wsum, ns);
+            if (cnt) rle_emit(rv, run, S.cl_sym + co, S.cl_extra + co);
+            __syncthreads();
+            for (uint32_t q = (uint32_t)t; q < ns; q += DF_T) atomicAdd(&S.cl_freq[S.cl_sym[q]], 1u);
+            __syncthreads();
+            rank_symbols(S.cl_freq, NCL, S.cl_sorted, S.sf, &S.misc[7], 0);
+            __syncthreads();
+            if (t == 0) {
+                huff_lengths(S.cl_freq, S.cl_sorted, (int)S.misc[7], 7, 0, S.cl_len, S.nf, S.par, S.dep, S.clblc);
+                canonical(S.cl_len, NCL, 7, S.cl_code, S.clblc);
                 int hclen = NCL;
                 while (hclen > 4 && S.cl_len[cl_order(hclen - 1)] == 0) hclen--;
-                uint32_t hb = 3 + 5 + 5 + 4 + 3 * hclen;
-                for (int q = 0; q < ns; q++) {
-                    const int sym = S.cl_sym[q];
-                    hb += S.cl_len[sym] + (sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0);
-                }
-                S.misc[1] = (uint32_t)ns;
-                S.misc[2] = (uint32_t)hlit;
+                S.misc[1] = ns;
                 S.misc[3] = (uint32_t)hclen;
-                S.misc[4] = hb;
-                S.misc[6] = (uint32_t)hdist;
             }
             __syncthreads();
+            uint32_t hpart = 0;
+            for (uint32_t q = (uint32_t)t; q < ns; q += DF_T) {
+                const int sym = S.cl_sym[q];
+                hpart += S.cl_len[sym] + (sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0);
+            }
+            uint32_t hcl;
+            block_excl_scan_1024(hpart, S.wsum, hcl);
+            const uint32_t hb_total = 3 + 5 + 5 + 4 + 3 * S.misc[3] + hcl;
             // 7. bit offsets of the segments
             const int ntok = S.ntok[t];
             uint32_t bits = 0;
@@ -532,7 +659,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
             uint32_t dbits;
             const uint32_t off = block_excl_scan_1024(bits, S.wsum, dbits);
-            const uint32_t hb = S.misc[4];
+            const uint32_t hb = hb_total;
             const uint32_t total_bits = hb + dbits + S.lit_len[256];
             nbytes = (total_bits + 7) >> 3;
             if (nbytes < (uint32_t)bl + 5) {
@@ -541,8 +668,8 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                     BitWriter w(slot, 0);
                     w.put(1, 1);  // BFINAL
                     w.put(2, 2);  // BTYPE = dynamic Huffman
-                    w.put(S.misc[2] - 257, 5);
-                    w.put(S.misc[6] - 1, 5);
+                    w.put((uint32_t)hlit - 257, 5);
+                    w.put((uint32_t)hdist - 1, 5);
                     w.put(S.misc[3] - 4, 4);
                     for (uint32_t q = 0; q < S.misc[3]; q++) w.put(S.cl_len[cl_order((int)q)], 3);
                     for (uint32_t q = 0; q < S.misc[1]; q++) {
