@@ -8,9 +8,10 @@
 //
 // Design: two kernels and a scan. bam_size_kernel writes every output record's
 // length (output order j, source record perm[j]); an exclusive scan turns them
-// into byte offsets (the total lands in rec_off[n]); bam_pack_kernel then
-// gives each record one wave, whose lanes each produce a 4-byte-aligned
-// word of the output stream (byte stores only where a word is shared with a
+// into byte offsets (the total lands in rec_off[n]); in bam_pack_kernel each
+// wave lays out 64 records at once (a lane per record, the layouts in LDS) and
+// then writes them in turn, its lanes each producing a 4-byte-aligned word of
+// the output stream (byte stores only where a word is shared with a
 // neighbouring record), so the stream is written once, in order, with no host
 // gather and no staging copy.
 #include <zlib.h>
@@ -26,7 +27,8 @@ using namespace ma;
 namespace {
 
 constexpr int BAM_RG_MAX = 240;   // read-group id bytes carried in the kernel arguments
-constexpr int BAM_WPB = 4;        // waves (records in flight) per 256-thread block
+constexpr int BAM_WPB = 4;        // waves per 256-thread block
+constexpr int STAGE = 256;        // name / bases / qualities bytes staged in LDS per record
 constexpr int64_t BGZF_BLOCK = 65280;  // uncompressed bytes per BGZF member (bam_emit.py:25)
 
 struct RgArg {
@@ -228,6 +230,13 @@ __device__ __forceinline__ uint32_t rec_byte(const RecLay &L, const uint32_t *wo
     return u == 0 ? 'N' : (u == 1 ? 'M' : (u == 2 ? 'C' : (uint32_t)L.nm8));
 }
 
+// A wave-uniform 64-bit value into scalar registers (both halves through uint32_t).
+__device__ __forceinline__ int64_t uni64(uint64_t v) {
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)v);
+    return (int64_t)(((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
 __device__ __forceinline__ int64_t src_of(const uint32_t *perm, int64_t j) {
     return perm ? (int64_t)perm[j] : j;
 }
@@ -251,53 +260,140 @@ __global__ void bam_rg_kernel(RgArg rg, uint8_t *__restrict__ dst) {
     for (int t = threadIdx.x; t < BAM_RG_MAX; t += blockDim.x) dst[t] = rg.b[t];
 }
 
+// One record's bytes at out + base: each lane builds 4-byte-aligned words
+// (byte stores only for the two words shared with the neighbouring records).
+__device__ __forceinline__ void write_record(const RecLay &L, const uint32_t *wl, int64_t base,
+                                             const uint8_t *name, const uint8_t *seq, const uint8_t *qual,
+                                             const uint8_t *rg, int rg_n, int lane, uint8_t *out,
+                                             uint32_t *out32) {
+    const int64_t d0 = base >> 2, d1 = (base + L.total - 1) >> 2;
+    for (int64_t d = d0 + lane; d <= d1; d += 64) {
+        const int b0 = (int)(4 * d - base);
+        if (b0 >= 0 && b0 + 4 <= L.total) {
+            uint32_t w = rec_byte(L, wl, b0, name, seq, qual, rg, rg_n);
+            w |= rec_byte(L, wl, b0 + 1, name, seq, qual, rg, rg_n) << 8;
+            w |= rec_byte(L, wl, b0 + 2, name, seq, qual, rg, rg_n) << 16;
+            w |= rec_byte(L, wl, b0 + 3, name, seq, qual, rg, rg_n) << 24;
+            out32[d] = w;
+        } else {
+            for (int k = 0; k < 4; k++) {
+                const int b = b0 + k;
+                if (b >= 0 && b < L.total) out[base + b] = (uint8_t)rec_byte(L, wl, b, name, seq, qual, rg, rg_n);
+            }
+        }
+    }
+}
+
+// A record's layout as one lane computed it, kept in LDS for the wave.
+struct RecSlot {
+    uint32_t w[12];       // header words 0..8, CIGAR words 9..11
+    int64_t name;         // first name byte in the names array
+    int64_t row;          // read index (bases and qualities at row * stride)
+    int64_t base;         // output offset
+    uint32_t qlen_flags;  // qlen | rev << 16 | has_qual << 17 | n_cigar << 18
+    uint32_t name_nm;     // name_len | nm8 << 16
+};
+
 __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, ReadSrc rs,
                                                        const uint8_t *__restrict__ rg, int rg_n,
                                                        const uint32_t *__restrict__ perm,
                                                        const uint32_t *__restrict__ is_dup, int apply_dup,
                                                        const uint64_t *__restrict__ rec_off,
                                                        uint8_t *__restrict__ out, int64_t out_cap) {
+    // Each wave takes 64 consecutive output records: every lane lays out one
+    // (its loads of the permutation, fields and offsets all in flight at
+    // once), then the wave writes the 64 records one after another.
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * BAM_WPB + (threadIdx.x >> 6));
     const int64_t n_waves = (int64_t)gridDim.x * BAM_WPB;
     uint32_t *const out32 = reinterpret_cast<uint32_t *>(out);
-    __shared__ uint32_t s_words[BAM_WPB][12];
-    uint32_t *const wl = s_words[threadIdx.x >> 6];
-    for (int64_t j = wave; j < n; j += n_waves) {
-        const int64_t i = src_of(perm, j);
-        RecF f = load_fields(fs, i);
-        if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
-        const int64_t qlen = rs.lens[i];
-        const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
-        const int64_t nb = rs.name_off[i];
-        const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
-        const int64_t base = (int64_t)rec_off[j];
-        if (base + L.total > out_cap) continue;  // the caller sees rec_off[n] > out_cap
-        __builtin_amdgcn_wave_barrier();  // the previous record's reads of wl are done
+    __shared__ RecSlot s_rec[BAM_WPB][64];
+    __shared__ uint8_t s_stage[BAM_WPB][3 * STAGE];
+    RecSlot *const slots = s_rec[threadIdx.x >> 6];
+    for (int64_t j0 = wave * 64; j0 < n; j0 += n_waves * 64) {
+        const int64_t j = j0 + lane;
+        if (j < n) {
+            const int64_t i = src_of(perm, j);
+            RecF f = load_fields(fs, i);
+            if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
+            const int64_t qlen = rs.lens[i];
+            const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
+            const int64_t nb = rs.name_off[i];
+            const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
+            RecSlot &S = slots[lane];
 #pragma unroll
-        for (int k = 0; k < 12; k++)
-            if (lane == k) wl[k] = k < 9 ? L.hdr[k] : L.cig[k - 9];
-        __builtin_amdgcn_wave_barrier();
-        const uint8_t *name = rs.names + nb;
-        const uint8_t *seq = rs.seq + i * rs.stride;
-        const uint8_t *qual = rs.qual ? rs.qual + i * rs.stride : nullptr;
-        const int64_t d0 = base >> 2, d1 = (base + L.total - 1) >> 2;
-        for (int64_t d = d0 + lane; d <= d1; d += 64) {
-            const int b0 = (int)(4 * d - base);
-            if (b0 >= 0 && b0 + 4 <= L.total) {
-                uint32_t w = rec_byte(L, wl, b0, name, seq, qual, rg, rg_n);
-                w |= rec_byte(L, wl, b0 + 1, name, seq, qual, rg, rg_n) << 8;
-                w |= rec_byte(L, wl, b0 + 2, name, seq, qual, rg, rg_n) << 16;
-                w |= rec_byte(L, wl, b0 + 3, name, seq, qual, rg, rg_n) << 24;
-                out32[d] = w;
-            } else {
-                // a word shared with the neighbouring record: this record's bytes only
-                for (int k = 0; k < 4; k++) {
-                    const int b = b0 + k;
-                    if (b >= 0 && b < L.total) out[base + b] = (uint8_t)rec_byte(L, wl, b, name, seq, qual, rg, rg_n);
-                }
-            }
+            for (int k = 0; k < 9; k++) S.w[k] = L.hdr[k];
+#pragma unroll
+            for (int k = 0; k < 3; k++) S.w[9 + k] = L.cig[k];
+            S.name = nb;
+            S.row = i;
+            S.base = (int64_t)rec_off[j];
+            S.qlen_flags = (uint32_t)L.qlen | (uint32_t)L.rev << 16 | (uint32_t)L.has_qual << 17 |
+                           (uint32_t)((L.o_seq - L.o_cig) / 4) << 18;
+            S.name_nm = (uint32_t)L.name_len | (uint32_t)L.nm8 << 16;
         }
+        __builtin_amdgcn_wave_barrier();
+        const int cnt = (int)(n - j0 < 64 ? n - j0 : 64);
+        // record r's name, bases and qualities: 4 bytes of each per lane, loaded
+        // one record ahead so the loads of r + 1 overlap the writing of r
+        uint32_t pn[STAGE / 64], ps[STAGE / 64], pq[STAGE / 64];
+        auto fetch = [&](int r) {
+            const RecSlot &S = slots[r];
+            const uint32_t qf = __builtin_amdgcn_readfirstlane(S.qlen_flags);
+            const int nl = (int)(__builtin_amdgcn_readfirstlane(S.name_nm) & 0xFFFF), ql = (int)(qf & 0xFFFF);
+            const int64_t row = uni64((uint64_t)S.row);
+            const uint8_t *nm = rs.names + uni64((uint64_t)S.name);
+            const uint8_t *sq = rs.seq + row * rs.stride;
+            const uint8_t *qu = ((qf >> 17) & 1) ? rs.qual + row * rs.stride : nullptr;
+#pragma unroll
+            for (int k = 0; k < STAGE / 64; k++) {
+                const int x = lane + 64 * k;
+                pn[k] = x < nl ? nm[x] : 0u;
+                ps[k] = x < ql ? sq[x] : 0u;
+                pq[k] = (qu && x < ql) ? qu[x] : 0u;
+            }
+        };
+        uint8_t *const st_name = s_stage[threadIdx.x >> 6];
+        uint8_t *const st_seq = st_name + STAGE;
+        uint8_t *const st_qual = st_seq + STAGE;
+        if (cnt > 0) fetch(0);
+        for (int r = 0; r < cnt; r++) {
+#pragma unroll
+            for (int k = 0; k < STAGE / 64; k++) {
+                st_name[lane + 64 * k] = (uint8_t)pn[k];
+                st_seq[lane + 64 * k] = (uint8_t)ps[k];
+                st_qual[lane + 64 * k] = (uint8_t)pq[k];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (r + 1 < cnt) fetch(r + 1);
+            const RecSlot &S = slots[r];
+            const uint32_t qf = __builtin_amdgcn_readfirstlane(S.qlen_flags);
+            const uint32_t nn = __builtin_amdgcn_readfirstlane(S.name_nm);
+            RecLay L;
+            L.qlen = (int32_t)(qf & 0xFFFF);
+            L.rev = (qf >> 16) & 1;
+            L.has_qual = (qf >> 17) & 1;
+            L.name_len = (int32_t)(nn & 0xFFFF);
+            L.nm8 = (int32_t)(nn >> 16);
+            L.o_cig = 36 + L.name_len + 1;
+            L.o_seq = L.o_cig + 4 * (int32_t)(qf >> 18);
+            L.o_qual = L.o_seq + (L.qlen + 1) / 2;
+            L.o_tags = L.o_qual + L.qlen;
+            L.total = (int32_t)__builtin_amdgcn_readfirstlane(S.w[0]) + 4;
+            const int64_t base = uni64((uint64_t)S.base);
+            const int64_t row = uni64((uint64_t)S.row);
+            // staged copies unless the record is longer than the stage (then from HBM);
+            // two calls so that the staged one reads LDS with LDS instructions
+            if (base + L.total <= out_cap) {  // else the caller sees rec_off[n] > out_cap
+                if (L.name_len <= STAGE && L.qlen <= STAGE)
+                    write_record(L, S.w, base, st_name, st_seq, st_qual, rg, rg_n, lane, out, out32);
+                else
+                    write_record(L, S.w, base, rs.names + uni64((uint64_t)S.name), rs.seq + row * rs.stride,
+                                 rs.qual ? rs.qual + row * rs.stride : nullptr, rg, rg_n, lane, out, out32);
+            }
+            __builtin_amdgcn_wave_barrier();  // the stage is rewritten for the next record
+        }
+        __builtin_amdgcn_wave_barrier();  // the slots are reused by the next batch
     }
 }
 
@@ -347,7 +443,7 @@ int bam_pack(void *stream, int64_t n, const FieldSrc &fs, const ma_bam_reads *r,
     uint8_t *d_rg = (uint8_t *)(s + S.rg);
     bam_rg_kernel<<<1, 64, 0, st>>>(ra, d_rg);
     MA_LAUNCHED();
-    const int64_t blocks = std::min<int64_t>((n + BAM_WPB - 1) / BAM_WPB, 256 * 64);
+    const int64_t blocks = std::min<int64_t>((n + 64 * BAM_WPB - 1) / (64 * BAM_WPB), 256 * 16);
     bam_pack_kernel<<<(unsigned)blocks, 256, 0, st>>>(n, fs, rs, d_rg, rg_n, d_perm, d_is_dup, apply_dup,
                                                       d_rec_off, d_out, out_cap);
     MA_LAUNCHED();
