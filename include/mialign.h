@@ -25,11 +25,11 @@
  *                             gpu-common/python/gpu_mem.py:88-121
  *   ma_bam_pack_se_dev ...... _pack_bam_aln over the SE branch of _emit_bam_from_batch
  *                             + _gather_perm_to_writer / _or_dup_flag,
- *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 766-800, 177-225, 408-420
+ *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 755-800, 177-225, 408-420
  *   ma_bam_pack_dev ......... _pack_bam_aln from explicit per-record fields (:497-665)
  *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw (host, zlib;
  *                             ma_bgzf_compress_dev: the same members compressed on the GPU),
- *                             fq2bam-meth/python/bam_emit.py:77-185, 467-471
+ *                             fq2bam-meth/python/bam_emit.py:77-185, 477-481
  *   ma_serve_* .............. no single reference counterpart: the request loop
  *                             generalises the batch loop of map_fastq_dense_gpu_locate
  *                             with admission by free HBM (gpu_mem.py:203-377)

@@ -3,7 +3,7 @@
 //
 // Record format: _pack_bam_aln, fq2bam-meth/src/linear_gpu_fm.mojo:497-665;
 // gather + 0x400: _gather_perm_to_writer (:177-225) and _or_dup_flag (:408-420);
-// SE field derivation: _emit_bam_from_batch (:766-800); BGZF members:
+// SE field derivation: _emit_bam_from_batch (:755-800); BGZF members:
 // _bgzf_block / _BgzfWriter, fq2bam-meth/python/bam_emit.py:77-185.
 //
 // Design: two kernels and a scan. bam_size_kernel writes every output record's

@@ -2,10 +2,10 @@
 duplicate flag, BGZF members compressed by native worker threads.
 
 Mirrors the reference's BAM writer module (fq2bam-meth/python/bam_emit.py:
-BamWriter header :318-356, write_raw :467-471, write_bytes :473-476, close;
-_reg2bin :33-48 and _pack_seq :51-61 as small host helpers) and the fm
+BamWriter header :318-356, write_raw :477-481, write_bytes :483-487, close;
+_reg2bin :34-48 and _pack_seq :52-61 as small host helpers) and the fm
 driver's native-BAM path (fq2bam-meth/src/linear_gpu_fm.mojo: _open_bam_writer
-:91-125, the SE branch of _emit_bam_from_batch :766-800, _pack_bam_aln
+:91-125, the SE branch of _emit_bam_from_batch :755-823, _pack_bam_aln
 :497-665, _gather_perm_to_writer :177-225). The records themselves are built by
 ma_bam_pack_se_dev / ma_bam_pack_dev (one wave per record, output order = the
 sort's permutation, 0x400 applied in the same pass); the host only frames and
@@ -34,7 +34,7 @@ for _i, _c in enumerate(b"=ACMGRSVTWYHKDBN"):
 
 
 def _reg2bin(beg: int, end: int) -> int:
-    """SAM-specification bin of [beg, end) (bam_emit.py:33-48)."""
+    """SAM-specification bin of [beg, end) (bam_emit.py:34-48)."""
     if end <= beg:
         end = beg + 1
     end -= 1
@@ -45,7 +45,7 @@ def _reg2bin(beg: int, end: int) -> int:
 
 
 def _pack_seq(seq: bytes) -> bytes:
-    """4-bit BAM SEQ of an ASCII sequence (bam_emit.py:51-61)."""
+    """4-bit BAM SEQ of an ASCII sequence (bam_emit.py:52-61)."""
     out = bytearray((len(seq) + 1) // 2)
     for i, b in enumerate(seq):
         out[i >> 1] |= _SEQ_NT16[b] << (4 * (1 - (i & 1)))

@@ -1124,8 +1124,8 @@ void or_recount_nm(const uint8_t *seq, const uint64_t *contig_off, int64_t n_con
 
 /* ------------------------------------------------ BAM record pack (next row) */
 
-/* Follows _reg2bin, fq2bam-meth/src/linear_gpu_fm.mojo:481-495 (the SAM
- * specification's binning; bam_emit.py:33-48 is the same function). */
+/* Follows _reg2bin, fq2bam-meth/src/linear_gpu_fm.mojo:479-495 (the SAM
+ * specification's binning; bam_emit.py:34-48 is the same function). */
 int64_t or_reg2bin(int64_t beg, int64_t end) {
     if (end <= beg) end = beg + 1;
     end -= 1;
@@ -1137,7 +1137,7 @@ int64_t or_reg2bin(int64_t beg, int64_t end) {
     return 0;
 }
 
-/* Follows _nt16 / _nt16_comp, linear_gpu_fm.mojo:457-478: A C G T (either
+/* Follows _nt16 / _nt16_comp, linear_gpu_fm.mojo:455-477: A C G T (either
  * case) -> 1 2 4 8, anything else 15; the complement swaps 1<->8 and 2<->4. */
 uint8_t or_nt16(uint8_t b) {
     if (b == 'A' || b == 'a') return 1;
@@ -1262,7 +1262,7 @@ static void or_dup_flag(uint8_t *p, int64_t rec_off) {
  * names[name_off[i] .. name_off[i+1]), bases seq + i*stride (lens[i]), quality
  * qual + i*stride of qual_lens[i] bytes (qual NULL = absent; qual_lens NULL =
  * lens). Returns the stream length or -1 when it would pass cap. Follows the
- * per-batch pack of _emit_bam_from_batch (linear_gpu_fm.mojo:766-800) and
+ * per-batch pack of _emit_bam_from_batch (linear_gpu_fm.mojo:755-800) and
  * _gather_perm_to_writer (:177-225). */
 int64_t or_bam_pack_fields(int64_t n, const int32_t *flag, const int32_t *tid, const int32_t *pos,
                            const int32_t *mapq, const int32_t *sl, const int32_t *sr,
@@ -1307,7 +1307,7 @@ int64_t or_bam_pack_fields(int64_t n, const int32_t *flag, const int32_t *tid, c
 /* Single-end records straight from merged map records: flag | 0x41 (paired +
  * read 1, as the reference's SE branch sets them), tid from tid_of_cid (cid
  * outside [0, n_tid) -> tid -1 and flag | 4), no mate. Follows
- * _emit_bam_from_batch's SE branch, linear_gpu_fm.mojo:766-800. */
+ * _emit_bam_from_batch's SE branch, linear_gpu_fm.mojo:755-800. */
 int64_t or_bam_pack_se(int64_t n, const int32_t *cid, const uint32_t *pos, const uint32_t *mapq,
                        const int32_t *rflag, const uint32_t *sl, const uint32_t *sr,
                        const uint32_t *nm, const int32_t *tid_of_cid, int64_t n_tid,
