@@ -12,7 +12,11 @@ duplicate marking, all on the device.
 Prints one JSON line. `roofline` is for the dominant kernel (the map kernel);
 `cpu_baseline` times the oracle (the C restatement of the reference's CPU
 science, OpenMP over reads + the stable-sort markdup) on a bounded sample of
-the same reads on this host.
+the same reads on this host. Extra sections: `serving` (the served request
+loop against plain launches), `sort_stress` (200M sort + markdup records),
+`genome_scale` (the 3.1 Gbp reference) and `bam_emit` (the step's records as
+coordinate-ordered BAM with duplicate flags: device pack, BGZF on the device
+or on host threads, with its own roofline, parity sample and CPU baseline).
 """
 
 from __future__ import annotations
