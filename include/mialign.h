@@ -238,7 +238,8 @@ int64_t ma_bam_scratch_bytes(int64_t n);
 int64_t ma_bam_record_bound(int32_t name_len, int32_t len, int32_t rg_n);
 /* Uncompressed BAM alignment records (block_size first, little endian) of n
  * single-end merged map records, written back to back into d_out in output
- * order: output record j is read perm[j] (d_perm NULL = identity), with flag
+ * order: output record j is read perm[j] (d_perm NULL = identity; otherwise a
+ * permutation of [0, n) — other values give empty records, never a fault), with flag
  * 0x400 set when apply_dup and d_is_dup[perm[j]] != 0. Fields as the
  * reference's SE batch emit: flag = record flag | 0x41, tid = d_tid_of_cid[cid]
  * (cid outside [0, n_tid) -> unmapped, flag | 4), CIGAR slS mM srS (clips

@@ -6,14 +6,16 @@
 // SE field derivation: _emit_bam_from_batch (:755-800); BGZF members:
 // _bgzf_block / _BgzfWriter, fq2bam-meth/python/bam_emit.py:77-185.
 //
-// Design: two kernels and a scan. bam_size_kernel writes every output record's
-// length (output order j, source record perm[j]); an exclusive scan turns them
-// into byte offsets (the total lands in rec_off[n]); in bam_pack_kernel each
-// wave lays out 64 records at once (a lane per record, the layouts in LDS) and
-// then writes them in turn, its lanes each producing a 4-byte-aligned word of
-// the output stream (byte stores only where a word is shared with a
-// neighbouring record), so the stream is written once, in order, with no host
-// gather and no staging copy.
+// Design: bam_layout_kernel reads the records in source order (the field
+// arrays in sequence) and writes each one's 64-byte layout (header and CIGAR
+// words, name / row / flags) to its output position inv[i] (inv = the
+// permutation's inverse); the lengths in output order are scanned into byte
+// offsets (the total lands in rec_off[n]); in bam_pack_kernel each wave takes
+// 64 consecutive layouts into LDS and writes the records in turn, its lanes
+// each producing a 4-byte-aligned word of the output stream (byte stores only
+// where a word is shared with a neighbouring record) from the record's name,
+// bases and qualities staged in LDS (the next record's loads in flight), so the
+// stream is written once, in order, with no host gather and no staging copy.
 #include <zlib.h>
 
 #include <algorithm>
@@ -237,21 +239,54 @@ __device__ __forceinline__ int64_t uni64(uint64_t v) {
     return (int64_t)(((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
-__device__ __forceinline__ int64_t src_of(const uint32_t *perm, int64_t j) {
-    return perm ? (int64_t)perm[j] : j;
+// One record's layout, 64 bytes, in output order.
+struct RecLayout {
+    uint32_t w[12];   // header words 0..8 (word 5 = l_seq), CIGAR words 9..11
+    uint64_t name;    // first name byte in the names array
+    uint32_t row;     // read index (bases and qualities at row * stride)
+    uint32_t flags;   // name_len | rev << 16 | has_qual << 17 | n_cigar << 18 | nm8 << 20
+};
+static_assert(sizeof(RecLayout) == 64, "one 64-byte line per record");
+
+// inv[perm[j]] = j: where each source record goes (entries outside [0, n) are
+// skipped; inv starts all ones, so a source that perm never names is skipped too).
+__global__ void bam_inverse_kernel(int64_t n, const uint32_t *__restrict__ perm, uint32_t *__restrict__ inv) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = perm[j];
+        if (i < n) inv[i] = (uint32_t)j;
+    }
 }
 
-__global__ void bam_size_kernel(int64_t n, FieldSrc fs, ReadSrc rs, int rg_n,
-                                const uint32_t *__restrict__ perm, uint32_t *__restrict__ sizes) {
-    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n;
-         j += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = src_of(perm, j);
-        const RecF f = load_fields(fs, i);
+// In source order (the field arrays read in sequence): each record's layout,
+// 0x400 applied, written to its output slot.
+__global__ void bam_layout_kernel(int64_t n, FieldSrc fs, ReadSrc rs, int rg_n, const uint32_t *__restrict__ inv,
+                                  const uint32_t *__restrict__ is_dup, int apply_dup,
+                                  RecLayout *__restrict__ lay) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        RecF f = load_fields(fs, i);
+        if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
         const int64_t qlen = rs.lens[i];
         const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
-        const RecLay L = lay_out(f, rs.name_off[i + 1] - rs.name_off[i], qlen, ql, rs.qual != nullptr, rg_n);
-        sizes[j] = (uint32_t)L.total;
+        const int64_t nb = rs.name_off[i];
+        const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
+        RecLayout o;
+#pragma unroll
+        for (int k = 0; k < 9; k++) o.w[k] = L.hdr[k];
+#pragma unroll
+        for (int k = 0; k < 3; k++) o.w[9 + k] = L.cig[k];
+        o.name = (uint64_t)nb;
+        o.row = (uint32_t)i;
+        o.flags = (uint32_t)L.name_len | (uint32_t)L.rev << 16 | (uint32_t)L.has_qual << 17 |
+                  (uint32_t)((L.o_seq - L.o_cig) / 4) << 18 | (uint32_t)L.nm8 << 20;
+        const uint32_t j = inv ? inv[i] : (uint32_t)i;
+        if (j < n) lay[j] = o;
     }
+}
+
+// Record lengths in output order (for the offset scan).
+__global__ void bam_sizes_kernel(int64_t n, const RecLayout *__restrict__ lay, uint32_t *__restrict__ sizes) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        sizes[j] = lay[j].w[0] + 4;
 }
 
 // The read-group id bytes, from the kernel arguments into device memory (one
@@ -284,53 +319,27 @@ __device__ __forceinline__ void write_record(const RecLay &L, const uint32_t *wl
     }
 }
 
-// A record's layout as one lane computed it, kept in LDS for the wave.
-struct RecSlot {
-    uint32_t w[12];       // header words 0..8, CIGAR words 9..11
-    int64_t name;         // first name byte in the names array
-    int64_t row;          // read index (bases and qualities at row * stride)
-    int64_t base;         // output offset
-    uint32_t qlen_flags;  // qlen | rev << 16 | has_qual << 17 | n_cigar << 18
-    uint32_t name_nm;     // name_len | nm8 << 16
-};
-
-__global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, ReadSrc rs,
+__global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, ReadSrc rs, const RecLayout *__restrict__ lay,
                                                        const uint8_t *__restrict__ rg, int rg_n,
-                                                       const uint32_t *__restrict__ perm,
-                                                       const uint32_t *__restrict__ is_dup, int apply_dup,
                                                        const uint64_t *__restrict__ rec_off,
                                                        uint8_t *__restrict__ out, int64_t out_cap) {
-    // Each wave takes 64 consecutive output records: every lane lays out one
-    // (its loads of the permutation, fields and offsets all in flight at
-    // once), then the wave writes the 64 records one after another.
+    // Each wave takes 64 consecutive output records: their layouts (one 64-byte
+    // line each, read in sequence) go to LDS, then the wave writes the 64
+    // records one after another.
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * BAM_WPB + (threadIdx.x >> 6));
     const int64_t n_waves = (int64_t)gridDim.x * BAM_WPB;
     uint32_t *const out32 = reinterpret_cast<uint32_t *>(out);
-    __shared__ RecSlot s_rec[BAM_WPB][64];
+    __shared__ RecLayout s_rec[BAM_WPB][64];
+    __shared__ uint64_t s_base[BAM_WPB][64];
     __shared__ uint8_t s_stage[BAM_WPB][3 * STAGE];
-    RecSlot *const slots = s_rec[threadIdx.x >> 6];
+    RecLayout *const slots = s_rec[threadIdx.x >> 6];
+    uint64_t *const bases = s_base[threadIdx.x >> 6];
     for (int64_t j0 = wave * 64; j0 < n; j0 += n_waves * 64) {
         const int64_t j = j0 + lane;
         if (j < n) {
-            const int64_t i = src_of(perm, j);
-            RecF f = load_fields(fs, i);
-            if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
-            const int64_t qlen = rs.lens[i];
-            const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
-            const int64_t nb = rs.name_off[i];
-            const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
-            RecSlot &S = slots[lane];
-#pragma unroll
-            for (int k = 0; k < 9; k++) S.w[k] = L.hdr[k];
-#pragma unroll
-            for (int k = 0; k < 3; k++) S.w[9 + k] = L.cig[k];
-            S.name = nb;
-            S.row = i;
-            S.base = (int64_t)rec_off[j];
-            S.qlen_flags = (uint32_t)L.qlen | (uint32_t)L.rev << 16 | (uint32_t)L.has_qual << 17 |
-                           (uint32_t)((L.o_seq - L.o_cig) / 4) << 18;
-            S.name_nm = (uint32_t)L.name_len | (uint32_t)L.nm8 << 16;
+            slots[lane] = lay[j];
+            bases[lane] = rec_off[j];
         }
         __builtin_amdgcn_wave_barrier();
         const int cnt = (int)(n - j0 < 64 ? n - j0 : 64);
@@ -338,13 +347,13 @@ __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, R
         // one record ahead so the loads of r + 1 overlap the writing of r
         uint32_t pn[STAGE / 64], ps[STAGE / 64], pq[STAGE / 64];
         auto fetch = [&](int r) {
-            const RecSlot &S = slots[r];
-            const uint32_t qf = __builtin_amdgcn_readfirstlane(S.qlen_flags);
-            const int nl = (int)(__builtin_amdgcn_readfirstlane(S.name_nm) & 0xFFFF), ql = (int)(qf & 0xFFFF);
-            const int64_t row = uni64((uint64_t)S.row);
-            const uint8_t *nm = rs.names + uni64((uint64_t)S.name);
+            const RecLayout &S = slots[r];
+            const uint32_t fl = __builtin_amdgcn_readfirstlane(S.flags);
+            const int nl = (int)(fl & 0xFFFF), ql = (int)__builtin_amdgcn_readfirstlane(S.w[5]);
+            const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane(S.row);
+            const uint8_t *nm = rs.names + uni64(S.name);
             const uint8_t *sq = rs.seq + row * rs.stride;
-            const uint8_t *qu = ((qf >> 17) & 1) ? rs.qual + row * rs.stride : nullptr;
+            const uint8_t *qu = ((fl >> 17) & 1) ? rs.qual + row * rs.stride : nullptr;
 #pragma unroll
             for (int k = 0; k < STAGE / 64; k++) {
                 const int x = lane + 64 * k;
@@ -366,29 +375,28 @@ __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, R
             }
             __builtin_amdgcn_wave_barrier();
             if (r + 1 < cnt) fetch(r + 1);
-            const RecSlot &S = slots[r];
-            const uint32_t qf = __builtin_amdgcn_readfirstlane(S.qlen_flags);
-            const uint32_t nn = __builtin_amdgcn_readfirstlane(S.name_nm);
+            const RecLayout &S = slots[r];
+            const uint32_t fl = __builtin_amdgcn_readfirstlane(S.flags);
             RecLay L;
-            L.qlen = (int32_t)(qf & 0xFFFF);
-            L.rev = (qf >> 16) & 1;
-            L.has_qual = (qf >> 17) & 1;
-            L.name_len = (int32_t)(nn & 0xFFFF);
-            L.nm8 = (int32_t)(nn >> 16);
+            L.qlen = (int32_t)__builtin_amdgcn_readfirstlane(S.w[5]);
+            L.rev = (fl >> 16) & 1;
+            L.has_qual = (fl >> 17) & 1;
+            L.name_len = (int32_t)(fl & 0xFFFF);
+            L.nm8 = (int32_t)((fl >> 20) & 0xFF);
             L.o_cig = 36 + L.name_len + 1;
-            L.o_seq = L.o_cig + 4 * (int32_t)(qf >> 18);
+            L.o_seq = L.o_cig + 4 * (int32_t)((fl >> 18) & 3);
             L.o_qual = L.o_seq + (L.qlen + 1) / 2;
             L.o_tags = L.o_qual + L.qlen;
             L.total = (int32_t)__builtin_amdgcn_readfirstlane(S.w[0]) + 4;
-            const int64_t base = uni64((uint64_t)S.base);
-            const int64_t row = uni64((uint64_t)S.row);
+            const int64_t base = uni64(bases[r]);
+            const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane(S.row);
             // staged copies unless the record is longer than the stage (then from HBM);
             // two calls so that the staged one reads LDS with LDS instructions
             if (base + L.total <= out_cap) {  // else the caller sees rec_off[n] > out_cap
                 if (L.name_len <= STAGE && L.qlen <= STAGE)
                     write_record(L, S.w, base, st_name, st_seq, st_qual, rg, rg_n, lane, out, out32);
                 else
-                    write_record(L, S.w, base, rs.names + uni64((uint64_t)S.name), rs.seq + row * rs.stride,
+                    write_record(L, S.w, base, rs.names + uni64(S.name), rs.seq + row * rs.stride,
                                  rs.qual ? rs.qual + row * rs.stride : nullptr, rg, rg_n, lane, out, out32);
             }
             __builtin_amdgcn_wave_barrier();  // the stage is rewritten for the next record
@@ -398,7 +406,7 @@ __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, FieldSrc fs, R
 }
 
 struct BamScratch {
-    size_t sizes, partial, rg, total;
+    size_t sizes, partial, rg, inv, lay, total;
 };
 
 BamScratch bam_layout(int64_t n) {
@@ -412,6 +420,8 @@ BamScratch bam_layout(int64_t n) {
     s.sizes = take((size_t)(n < 1 ? 1 : n) * 4);
     s.partial = take(scan_scratch_bytes(n < 1 ? 1 : n));
     s.rg = take(BAM_RG_MAX);
+    s.inv = take((size_t)(n < 1 ? 1 : n) * 4);
+    s.lay = take((size_t)(n < 1 ? 1 : n) * sizeof(RecLayout));
     s.total = off;
     return s;
 }
@@ -437,15 +447,25 @@ int bam_pack(void *stream, int64_t n, const FieldSrc &fs, const ma_bam_reads *r,
     for (int k = 0; k < BAM_RG_MAX; k++) ra.b[k] = k < rg_n ? (uint8_t)rg[k] : 0;
     char *s = (char *)d_scratch;
     uint32_t *sizes = (uint32_t *)(s + S.sizes);
-    bam_size_kernel<<<grid_for(n), 256, 0, st>>>(n, fs, rs, rg_n, d_perm, sizes);
+    uint32_t *inv = d_perm ? (uint32_t *)(s + S.inv) : nullptr;
+    RecLayout *lay = (RecLayout *)(s + S.lay);
+    if (inv) {
+        // a permutation fills every slot; anything else leaves empty layouts (4-byte records), never a wild read
+        MA_TRY(fill_u64(st, (uint64_t *)inv, (n + 1) / 2, ~0ull));
+        MA_TRY(fill_u64(st, (uint64_t *)lay, n * (int64_t)(sizeof(RecLayout) / 8), 0));
+        bam_inverse_kernel<<<grid_for(n), 256, 0, st>>>(n, d_perm, inv);
+        MA_LAUNCHED();
+    }
+    bam_layout_kernel<<<grid_for(n), 256, 0, st>>>(n, fs, rs, rg_n, inv, d_is_dup, apply_dup, lay);
+    MA_LAUNCHED();
+    bam_sizes_kernel<<<grid_for(n), 256, 0, st>>>(n, lay, sizes);
     MA_LAUNCHED();
     MA_TRY(exclusive_scan_u32(st, n, sizes, d_rec_off, (uint64_t *)(s + S.partial), d_rec_off + n));
     uint8_t *d_rg = (uint8_t *)(s + S.rg);
     bam_rg_kernel<<<1, 64, 0, st>>>(ra, d_rg);
     MA_LAUNCHED();
     const int64_t blocks = std::min<int64_t>((n + 64 * BAM_WPB - 1) / (64 * BAM_WPB), 256 * 16);
-    bam_pack_kernel<<<(unsigned)blocks, 256, 0, st>>>(n, fs, rs, d_rg, rg_n, d_perm, d_is_dup, apply_dup,
-                                                      d_rec_off, d_out, out_cap);
+    bam_pack_kernel<<<(unsigned)blocks, 256, 0, st>>>(n, rs, lay, d_rg, rg_n, d_rec_off, d_out, out_cap);
     MA_LAUNCHED();
     return MA_OK;
 }

@@ -20,7 +20,9 @@
 //     repair, canonical codes; a stored block when that is not smaller;
 //   * per-thread bit offsets from a block scan; each thread writes its
 //     segment's codes straight into the block's output slot (plain stores for
-//     the words it owns, atomic OR for the two it may share);
+//     the words it owns, atomic OR for the two it may share, which alone are
+//     cleared first); tokens live in a scratch laid out so a wave's accesses
+//     are 64 consecutive words;
 //   * CRC32: per-thread CRCs of the 64-byte segments joined pairwise by the
 //     GF(2) shift crc(A||B) = crc(A) * x^(8|B|) + crc(B) mod P;
 //   * a second kernel writes every member (18-byte header, body, trailer) at
@@ -436,7 +438,8 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
     const int t = threadIdx.x;
     const bool huffman = level > 0, lz = level > 0;
     uint16_t *const cand = cand_all + (int64_t)blockIdx.x * DF_IN;        // this workgroup's scratch
-    uint32_t *const tok = tok_all + ((int64_t)blockIdx.x * DF_T + t) * DF_SEG;
+    // token k of thread t at tok[k * DF_T]: a wave's k-th tokens are 64 consecutive words
+    uint32_t *const tok = tok_all + (int64_t)blockIdx.x * DF_T * DF_SEG + t;
     if (t < 256) {
         uint32_t c = (uint32_t)t;
         for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
@@ -457,7 +460,6 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
         const int bl = (int)(n - base < DF_IN ? n - base : DF_IN);
         uint32_t *const slot = slots + b * (DF_SLOT / 4);
         // 1. stage the block, clear the tables and the block's output slot
-        for (int i = t; i < DF_OUT_WORDS; i += DF_T) slot[i] = 0;
         for (int i = t; i < (1 << HASH_BITS); i += DF_T) S.head[i] = 0;
         for (int i = t; i < NLIT; i += DF_T) {
             S.freq[i] = 0;
@@ -553,7 +555,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                     atomicAdd(&S.freq[tk], 1u);
                     p += 1;
                 }
-                tok[k++] = tk;
+                tok[(k++) * DF_T] = tk;
             }
             S.ntok[t] = (uint8_t)k;
             __syncthreads();
@@ -647,7 +649,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             const int ntok = S.ntok[t];
             uint32_t bits = 0;
             for (int q = 0; q < ntok; q++) {
-                const uint32_t tk = tok[q];
+                const uint32_t tk = tok[q * DF_T];
                 if (tk & TOK_MATCH) {
                     uint32_t ls, le, lx, ds, de, dx;
                     len_code((tk >> 16) & 0x1FF, ls, le, lx);
@@ -663,7 +665,19 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             const uint32_t total_bits = hb + dbits + S.lit_len[256];
             nbytes = (total_bits + 7) >> 3;
             if (nbytes < (uint32_t)bl + 5) {
-                // 8. header (thread 0), then every segment's codes, EOB after the last one
+                // 8. header (thread 0), then every segment's codes, EOB after the last one.
+                //    Only the first and last word of each writer's bit range can be shared
+                //    (ORed by two writers): those are cleared first; the others are stored whole.
+                const uint32_t a = hb + off, e = a + bits + (s1 == bl ? S.lit_len[256] : 0u);
+                if (s0 < bl) {
+                    slot[a >> 5] = 0;
+                    slot[(e - 1) >> 5] = 0;
+                }
+                if (t == 0) {
+                    slot[0] = 0;
+                    slot[(hb - 1) >> 5] = 0;
+                }
+                __syncthreads();
                 if (t == 0) {
                     BitWriter w(slot, 0);
                     w.put(1, 1);  // BFINAL
@@ -684,7 +698,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 if (s0 < bl) {
                     BitWriter w(slot, hb + off);
                     for (int q = 0; q < ntok; q++) {
-                        const uint32_t tk = tok[q];
+                        const uint32_t tk = tok[q * DF_T];
                         if (tk & TOK_MATCH) {
                             uint32_t ls, le, lx, ds, de, dx;
                             len_code((tk >> 16) & 0x1FF, ls, le, lx);
