@@ -30,6 +30,9 @@
  *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw (host, zlib;
  *                             ma_bgzf_compress_dev: the same members compressed on the GPU),
  *                             fq2bam-meth/python/bam_emit.py:77-185, 477-481
+ *   ma_fastq_parse_dev ...... the bulk FASTQ reader: _pipe_line / _store_rec / _qname_span /
+ *                             _cvt / fq_pack_bases, fq2bam-meth/src/linear_fastq.mojo:37-46,
+ *                             229-470 (FastqPairReader, fq2bam-meth/python/fastq_batch.py:230-317)
  *   ma_serve_* .............. no single reference counterpart: the request loop
  *                             generalises the batch loop of map_fastq_dense_gpu_locate
  *                             with admission by free HBM (gpu_mem.py:203-377)
@@ -283,6 +286,27 @@ typedef struct ma_bgzf ma_bgzf;
 int ma_bgzf_open(const char *path, int32_t level, int32_t threads, ma_bgzf **out);
 int ma_bgzf_write(ma_bgzf *w, const void *data, int64_t n);
 int ma_bgzf_close(ma_bgzf *w, int64_t *bytes_written);
+
+/* ---- FASTQ ingest ---- */
+/* Parses raw FASTQ bytes resident in HBM into a read batch, as the reference's
+ * bulk reader does (lines end at '\n', one trailing '\r' dropped; a record is
+ * four lines; the name skips '@' and stops at ' ' or '/'; bs_mode 1 = C2T,
+ * C/c -> T, 2 = G2A, G/g -> A, 0 = none). Up to max_records records: d_rows
+ * [max_records x stride] original bases padded with 'N'; d_conv_rows the
+ * converted bases (may be NULL); d_qual_rows the first `stride` quality bytes,
+ * 0 padded (may be NULL); d_lens / d_qual_lens; names back to back in d_names
+ * at d_name_off[max_records + 1]. d_info[8] (device u64): records, bytes
+ * consumed (through the last record's quality line), longest read, status
+ * (0 ok; 1 / 2 / 3 = the input ends inside a record, missing SEQ / '+' / QUAL,
+ * only when `final`; 4 = a read longer than stride; 5 = names past names_cap —
+ * the batch stops before that record), name bytes, newlines, lines. Without
+ * `final` an unterminated or incomplete tail stays unconsumed for the next
+ * buffer. Scratch: ma_fastq_scratch_bytes. Enqueued on stream. */
+int64_t ma_fastq_scratch_bytes(int64_t nbytes, int64_t max_records);
+int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32_t final, int64_t max_records,
+                       int32_t stride, int32_t bs_mode, uint8_t *d_rows, uint8_t *d_conv_rows, uint8_t *d_qual_rows,
+                       uint32_t *d_lens, uint32_t *d_qual_lens, uint8_t *d_names, int64_t names_cap,
+                       int64_t *d_name_off, uint64_t *d_info, void *d_scratch, int64_t scratch_bytes);
 
 /* ---- gpu-common seed kernels ---- */
 /* ASCII reads [n x max_len] -> per-position Wang hash of the MSB-first k-mer

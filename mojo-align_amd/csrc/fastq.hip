@@ -1,0 +1,338 @@
+// FASTQ ingest on the device: raw FASTQ bytes in HBM -> the read batch the
+// map and the BAM pack take (rows of original bases padded with 'N', rows of
+// converted bases, quality rows, lengths, names back to back).
+//
+// Semantics follow the reference's bulk reader (fq2bam-meth/src/linear_fastq.mojo):
+// lines end at '\n' with one trailing '\r' dropped (_pipe_line, :229-266), an
+// unterminated last line counts at the end of the input when non-empty; a
+// record is four lines (_store_rec, :342-394; truncation is an error); the
+// read name skips a leading '@' and stops at ' ' or '/' (_qname_span, :322-340);
+// conversion C/c -> T or G/g -> A (_cvt, :37-46); rows padded with 'N'
+// (fq_pack_bases, :432-470).
+//
+// Design: the buffer is cut into 16 KiB tiles (a 256-thread block, 64 bytes a
+// thread): one sweep counts the tile's newlines, a scan gives every tile its
+// first line number, a second sweep writes each newline's position at its line
+// number; one thread then settles the record count, the bytes consumed and the
+// status; a thread per record measures its four lines and name; a scan of the
+// name lengths places the names; a wave per record copies bases, converted
+// bases, qualities and the name.
+#include "ma_sort.hpp"
+
+using namespace ma;
+
+namespace {
+
+constexpr int FQ_TB = 256;                   // threads per block
+constexpr int FQ_BYTES = 64;                 // bytes per thread per sweep
+constexpr int FQ_TILE = FQ_TB * FQ_BYTES;    // 16 KiB
+
+enum : uint32_t { FQ_OK = 0, FQ_NO_SEQ = 1, FQ_NO_PLUS = 2, FQ_NO_QUAL = 3, FQ_TOO_LONG = 4, FQ_NAMES = 5 };
+
+// info words (u64): 0 records, 1 bytes consumed, 2 longest read, 3 status,
+// 4 name bytes, 5 newlines, 6 lines, 7 first read longer than stride
+
+__device__ __forceinline__ uint32_t count_nl(const uint8_t *buf, int64_t n, int64_t p0) {
+    uint32_t c = 0;
+    if (p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0) {
+        const uint4 *v = reinterpret_cast<const uint4 *>(buf + p0);
+#pragma unroll
+        for (int k = 0; k < FQ_BYTES / 16; k++) {
+            const uint4 w = v[k];
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t x = ws[q] ^ 0x0A0A0A0Au;  // zero bytes where '\n'
+                // exact zero-byte flags (no borrow between bytes): high bit of each zero byte
+                c += __popc(~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu));
+            }
+        }
+    } else {
+        for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++) c += buf[p] == '\n';
+    }
+    return c;
+}
+
+__device__ __forceinline__ uint32_t block_sum_256(uint32_t v, uint32_t *sh) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    const uint32_t s = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return s;
+}
+
+__global__ __launch_bounds__(FQ_TB) void fq_count_kernel(const uint8_t *__restrict__ buf, int64_t n, int64_t tiles,
+                                                         uint32_t *__restrict__ counts) {
+    __shared__ uint32_t sh[4];
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t p0 = tile * FQ_TILE + (int64_t)threadIdx.x * FQ_BYTES;
+        const uint32_t s = block_sum_256(count_nl(buf, n, p0), sh);
+        if (threadIdx.x == 0) counts[tile] = s;
+    }
+}
+
+// nl[L] = position of the newline that ends line L (for L < cap).
+__global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restrict__ buf, int64_t n, int64_t tiles,
+                                                         const uint64_t *__restrict__ tile_base,
+                                                         uint64_t *__restrict__ nl, int64_t cap) {
+    __shared__ uint64_t sh[4];
+    for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const int64_t p0 = tile * FQ_TILE + (int64_t)threadIdx.x * FQ_BYTES;
+        const uint32_t c = count_nl(buf, n, p0);
+        uint64_t tot;
+        uint64_t L = tile_base[tile] + block_excl_scan_256(c, sh, tot);
+        if (c && L < (uint64_t)cap)
+            for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
+                if (buf[p] == '\n') {
+                    if (L < (uint64_t)cap) nl[L] = (uint64_t)p;
+                    L++;
+                }
+    }
+}
+
+__device__ __forceinline__ int64_t line_start(const uint64_t *nl, int64_t L) { return L == 0 ? 0 : (int64_t)nl[L - 1] + 1; }
+
+// Line L's bytes [s, e) without its '\r'.
+__device__ __forceinline__ void line_span(const uint8_t *buf, const uint64_t *nl, int64_t L, int64_t &s, int64_t &e) {
+    s = line_start(nl, L);
+    e = (int64_t)nl[L];
+    if (e > s && buf[e - 1] == '\r') e--;
+}
+
+// Record count, bytes consumed and status from the newline count.
+__global__ void fq_settle_kernel(const uint8_t *__restrict__ buf, int64_t n, int final, int64_t max_records,
+                                 uint64_t *__restrict__ nl, int64_t cap, const uint64_t *__restrict__ total_nl,
+                                 uint64_t *__restrict__ info) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t nnl = *total_nl;
+    uint64_t lines = nnl;
+    if (final && n > 0 && buf[n - 1] != '\n') {
+        // an unterminated rest counts as a line when it is not empty after the '\r' strip
+        const int64_t s = nnl == 0 ? 0 : (nnl - 1 < (uint64_t)cap ? (int64_t)nl[nnl - 1] + 1 : n);
+        int64_t e = n;
+        if (e > s && buf[e - 1] == '\r') e--;
+        if (e > s && nnl < (uint64_t)cap) {
+            nl[nnl] = (uint64_t)n;  // a virtual newline just past the end
+            lines = nnl + 1;
+        } else if (e > s) {
+            lines = nnl + 1;  // beyond the records asked for: never read
+        }
+    }
+    const uint64_t complete = lines / 4;
+    const uint64_t recs = complete < (uint64_t)max_records ? complete : (uint64_t)max_records;
+    uint32_t status = FQ_OK;
+    if (final && recs == complete && lines % 4) status = (uint32_t)(lines % 4);  // 1 no SEQ, 2 no '+', 3 no QUAL
+    info[0] = recs;
+    uint64_t consumed = recs ? nl[4 * recs - 1] + 1 : 0;
+    info[1] = consumed > (uint64_t)n ? (uint64_t)n : consumed;
+    info[2] = 0;
+    info[3] = status;
+    info[4] = 0;
+    info[5] = nnl;
+    info[6] = lines;
+    info[7] = ~0ull;
+}
+
+// Per record: lengths and name span; the first read longer than stride.
+__global__ void fq_measure_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
+                                  int64_t max_records, int64_t stride, uint64_t *__restrict__ info,
+                                  uint32_t *__restrict__ lens, uint32_t *__restrict__ qual_lens,
+                                  uint32_t *__restrict__ name_len, uint64_t *__restrict__ name_src) {
+    const int64_t recs = (int64_t)info[0];
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < max_records;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        if (r >= recs) {
+            name_len[r] = 0;
+            continue;
+        }
+        int64_t hs, he, ss, se, qs, qe;
+        line_span(buf, nl, 4 * r, hs, he);
+        line_span(buf, nl, 4 * r + 1, ss, se);
+        line_span(buf, nl, 4 * r + 3, qs, qe);
+        int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs, e0 = he;
+        for (int64_t i = s0; i < he; i++)
+            if (buf[i] == ' ' || buf[i] == '/') {
+                e0 = i;
+                break;
+            }
+        const int64_t slen = se - ss;
+        lens[r] = (uint32_t)slen;
+        qual_lens[r] = (uint32_t)(qe - qs);
+        name_len[r] = (uint32_t)(e0 - s0);
+        name_src[r] = (uint64_t)s0;
+        if (slen > stride) atomicMin((unsigned long long *)&info[7], (unsigned long long)r);
+    }
+}
+
+__device__ __forceinline__ uint8_t cvt(uint8_t c, int mode) {
+    if (mode == 1) return (c == 'C' || c == 'c') ? 'T' : c;
+    if (mode == 2) return (c == 'G' || c == 'g') ? 'A' : c;
+    return c;
+}
+
+// A wave per record: bases, converted bases and qualities into their rows,
+// the name at its offset.
+__global__ __launch_bounds__(256) void fq_copy_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
+                                                      int64_t stride, int mode, const uint64_t *__restrict__ info,
+                                                      const uint32_t *__restrict__ name_len,
+                                                      const uint64_t *__restrict__ name_src,
+                                                      const int64_t *__restrict__ name_off, int64_t names_cap,
+                                                      uint8_t *__restrict__ rows, uint8_t *__restrict__ conv_rows,
+                                                      uint8_t *__restrict__ qual_rows, uint8_t *__restrict__ names) {
+    const int64_t recs = (int64_t)info[0];
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    for (int64_t r = wave; r < recs; r += (int64_t)gridDim.x * 4) {
+        int64_t ss, se, qs, qe;
+        line_span(buf, nl, 4 * r + 1, ss, se);
+        line_span(buf, nl, 4 * r + 3, qs, qe);
+        const int64_t slen = se - ss, qlen = qe - qs;
+        uint8_t *row = rows + r * stride;
+        uint8_t *crow = conv_rows ? conv_rows + r * stride : nullptr;
+        uint8_t *qrow = qual_rows ? qual_rows + r * stride : nullptr;
+        for (int64_t x = lane; x < stride; x += 64) {
+            const uint8_t b = x < slen ? buf[ss + x] : (uint8_t)'N';
+            row[x] = b;
+            if (crow) crow[x] = x < slen ? cvt(b, mode) : (uint8_t)'N';
+            if (qrow) qrow[x] = x < qlen ? buf[qs + x] : (uint8_t)0;
+        }
+        const int64_t no = name_off[r], nlen = name_len[r], src = (int64_t)name_src[r];
+        if (no + nlen <= names_cap)
+            for (int64_t x = lane; x < nlen; x += 64) names[no + x] = buf[src + x];
+    }
+}
+
+// The batch stops at its first read longer than stride (status 4) or whose
+// name passes names_cap (status 5), as a sequential reader would: the records
+// before it stand, and the bytes consumed end with them.
+__global__ void fq_finish_kernel(uint64_t *__restrict__ info, const uint64_t *__restrict__ nl,
+                                 const int64_t *__restrict__ name_off, int64_t names_cap, int64_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint64_t recs = info[0];
+    uint64_t stop = recs;
+    uint32_t why = 0;
+    if (info[7] < stop) {
+        stop = info[7];
+        why = FQ_TOO_LONG;
+    }
+    if (recs && (uint64_t)name_off[recs] > (uint64_t)names_cap) {
+        uint64_t lo = 0, hi = recs - 1;  // first r with name_off[r + 1] > names_cap
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            if ((uint64_t)name_off[mid + 1] > (uint64_t)names_cap)
+                hi = mid;
+            else
+                lo = mid + 1;
+        }
+        if (lo < stop) {
+            stop = lo;
+            why = FQ_NAMES;
+        }
+    }
+    if (stop < recs) {
+        info[0] = stop;
+        info[3] = why;
+        const uint64_t consumed = stop ? nl[4 * stop - 1] + 1 : 0;
+        info[1] = consumed > (uint64_t)n ? (uint64_t)n : consumed;
+    }
+    info[4] = info[0] ? (uint64_t)name_off[info[0]] : 0;
+}
+
+__global__ void fq_maxlen_kernel(const uint32_t *__restrict__ lens, uint64_t *__restrict__ info) {
+    const int64_t recs = (int64_t)info[0];
+    uint32_t m = 0;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < recs; r += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, lens[r]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax((unsigned long long *)&info[2], (unsigned long long)m);
+}
+
+struct FqScratch {
+    size_t counts, tile_base, partial, total_nl, nl, name_len, name_src, name_part, total;
+};
+
+FqScratch fq_layout(int64_t n, int64_t max_records) {
+    const int64_t tiles = (n + FQ_TILE - 1) / FQ_TILE, t1 = tiles > 0 ? tiles : 1;
+    const int64_t r1 = max_records > 0 ? max_records : 1;
+    FqScratch s;
+    size_t o = 0;
+    auto take = [&](size_t b) {
+        const size_t r = o;
+        o += (b + 255) & ~(size_t)255;
+        return r;
+    };
+    s.counts = take((size_t)t1 * 4);
+    s.tile_base = take((size_t)(t1 + 1) * 8);
+    s.partial = take(scan_scratch_bytes(t1 > r1 ? t1 : r1));
+    s.total_nl = take(8);
+    s.nl = take((size_t)(4 * r1 + 4) * 8);
+    s.name_len = take((size_t)r1 * 4);
+    s.name_src = take((size_t)r1 * 8);
+    s.name_part = take(scan_scratch_bytes(r1));
+    s.total = o;
+    return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t ma_fastq_scratch_bytes(int64_t nbytes, int64_t max_records) {
+    return (int64_t)fq_layout(nbytes < 0 ? 0 : nbytes, max_records).total;
+}
+
+int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32_t final, int64_t max_records,
+                       int32_t stride, int32_t bs_mode, uint8_t *d_rows, uint8_t *d_conv_rows, uint8_t *d_qual_rows,
+                       uint32_t *d_lens, uint32_t *d_qual_lens, uint8_t *d_names, int64_t names_cap,
+                       int64_t *d_name_off, uint64_t *d_info, void *d_scratch, int64_t scratch_bytes) {
+    MA_REQUIRE(nbytes >= 0 && (nbytes == 0 || d_buf) && max_records >= 1 && stride >= 1,
+               "ma_fastq_parse_dev: bad arguments");
+    MA_REQUIRE(d_rows && d_lens && d_qual_lens && d_names && d_name_off && d_info, "ma_fastq_parse_dev: null output");
+    MA_REQUIRE(bs_mode >= 0 && bs_mode <= 2, "ma_fastq_parse_dev: conversion must be none, C2T (1) or G2A (2)");
+    MA_REQUIRE(max_records <= 0x7FFFFFFFll, "ma_fastq_parse_dev: max_records too large");
+    const FqScratch L = fq_layout(nbytes, max_records);
+    MA_REQUIRE(scratch_bytes >= (int64_t)L.total, "ma_fastq_parse_dev: scratch too small");
+    hipStream_t st = (hipStream_t)stream;
+    char *s = (char *)d_scratch;
+    const int64_t tiles = (nbytes + FQ_TILE - 1) / FQ_TILE;
+    uint32_t *counts = (uint32_t *)(s + L.counts);
+    uint64_t *tile_base = (uint64_t *)(s + L.tile_base), *total_nl = (uint64_t *)(s + L.total_nl);
+    uint64_t *nl = (uint64_t *)(s + L.nl);
+    const int64_t cap = 4 * max_records + 4;
+    if (tiles > 0) {
+        const unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
+        fq_count_kernel<<<grid, FQ_TB, 0, st>>>(d_buf, nbytes, tiles, counts);
+        MA_LAUNCHED();
+        MA_TRY(exclusive_scan_u32(st, tiles, counts, tile_base, (uint64_t *)(s + L.partial), total_nl));
+        fq_lines_kernel<<<grid, FQ_TB, 0, st>>>(d_buf, nbytes, tiles, tile_base, nl, cap);
+        MA_LAUNCHED();
+    } else {
+        MA_TRY(fill_u64(st, total_nl, 1, 0));
+    }
+    fq_settle_kernel<<<1, 64, 0, st>>>(d_buf, nbytes, final, max_records, nl, cap, total_nl, d_info);
+    MA_LAUNCHED();
+    uint32_t *name_len = (uint32_t *)(s + L.name_len);
+    uint64_t *name_src = (uint64_t *)(s + L.name_src);
+    const unsigned g = (unsigned)grid_for(max_records);
+    fq_measure_kernel<<<g, 256, 0, st>>>(d_buf, nl, max_records, stride, d_info, d_lens, d_qual_lens, name_len,
+                                         name_src);
+    MA_LAUNCHED();
+    MA_TRY(exclusive_scan_u32(st, max_records, name_len, (uint64_t *)d_name_off, (uint64_t *)(s + L.name_part),
+                              (uint64_t *)d_name_off + max_records));
+    fq_finish_kernel<<<1, 64, 0, st>>>(d_info, nl, d_name_off, names_cap, nbytes);
+    MA_LAUNCHED();
+    fq_maxlen_kernel<<<g, 256, 0, st>>>(d_lens, d_info);
+    MA_LAUNCHED();
+    const int64_t blocks = (max_records + 3) / 4 < 65536 ? (max_records + 3) / 4 : 65536;
+    fq_copy_kernel<<<(unsigned)blocks, 256, 0, st>>>(d_buf, nl, stride, bs_mode, d_info, name_len, name_src,
+                                                     d_name_off, names_cap, d_rows, d_conv_rows, d_qual_rows,
+                                                     d_names);
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
+}  // extern "C"

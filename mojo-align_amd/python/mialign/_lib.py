@@ -156,6 +156,8 @@ SIGNATURES = {
                                      I32, P, P, I32, P, I64, P, P, I64]),
     "ma_bam_pack_dev": (C.c_int, [P, I64, C.POINTER(BamFields), C.POINTER(BamReads), C.c_char_p, I32, P,
                                   P, I32, P, I64, P, P, I64]),
+    "ma_fastq_scratch_bytes": (I64, [I64, I64]),
+    "ma_fastq_parse_dev": (C.c_int, [P, P, I64, I32, I64, I32, I32, P, P, P, P, P, P, I64, P, P, P, I64]),
     "ma_bgzf_bound": (I64, [I64]),
     "ma_bgzf_dev_scratch_bytes": (I64, [I64]),
     "ma_bgzf_compress_dev": (C.c_int, [P, P, I64, I32, I32, P, I64, P, P, I64]),

@@ -1,0 +1,216 @@
+"""FASTQ ingest on the device.
+
+Mirrors the reference's bulk FASTQ path (fq2bam-meth/src/linear_fastq.mojo:
+fq_open / fq_read_batch / fq_pack_bases / fq_close, :411-470; its Python twin
+FastqPairReader, fq2bam-meth/python/fastq_batch.py:230-317): the host only
+moves bytes (file chunks, gzip inflated on the host like the reference's pigz
+pipe), and ma_fastq_parse_dev splits the lines, cuts the names, converts and
+packs the rows in HBM, so a batch is ready for map_batch_dev and the BAM pack
+without a host pass over the reads.
+
+Semantics are the Mojo reader's: C2T maps C and c to 'T' and G2A maps G and g
+to 'A' (upper case; the Python reader keeps case), rows are padded with 'N'.
+"""
+
+from __future__ import annotations
+
+import gzip
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from ._lib import MialignError, check, lib, ptr
+from .core import DeviceArray, Stream
+
+BS_CODES = {"": 0, "none": 0, "C2T": 1, "G2A": 2}
+FQ_ERRORS = {
+    1: "truncated FASTQ (missing SEQ)",
+    2: "truncated FASTQ (missing +)",
+    3: "truncated FASTQ (missing QUAL)",
+}
+
+
+@dataclass
+class ParseInfo:
+    records: int
+    consumed: int
+    max_len: int
+    status: int
+    name_bytes: int
+
+
+class DeviceFastqBatch:
+    """One parsed batch resident in HBM (layout of ma_fastq_parse_dev)."""
+
+    def __init__(self, max_records: int, stride: int, names_cap: int, device: int = 0, with_conv: bool = True,
+                 with_qual: bool = True):
+        self.max_records, self.stride, self.names_cap = int(max_records), int(stride), int(names_cap)
+        r = max(self.max_records, 1)
+        self.rows = DeviceArray((r, self.stride), np.uint8, device)
+        self.conv = DeviceArray((r, self.stride), np.uint8, device) if with_conv else None
+        self.qual = DeviceArray((r, self.stride), np.uint8, device) if with_qual else None
+        self.lens = DeviceArray((r,), np.uint32, device)
+        self.qual_lens = DeviceArray((r,), np.uint32, device)
+        self.names = DeviceArray((max(self.names_cap, 1),), np.uint8, device)
+        self.name_off = DeviceArray((r + 1,), np.int64, device)
+        self.info = DeviceArray((8,), np.uint64, device)
+        self.n = 0
+        self.max_len = 0
+
+    def download(self) -> dict:
+        """Host copies of the batch's first n records (for tests and tools)."""
+        n = self.n
+        out = {"rows": self.rows.download()[:n], "lens": self.lens.download()[:n],
+               "qual_lens": self.qual_lens.download()[:n], "name_off": self.name_off.download()[:n + 1]}
+        out["names"] = self.names.download()[: int(out["name_off"][-1]) if n else 0]
+        if self.conv is not None:
+            out["conv"] = self.conv.download()[:n]
+        if self.qual is not None:
+            out["qual"] = self.qual.download()[:n]
+        return out
+
+
+class FastqParser:
+    """ma_fastq_parse_dev over device buffers of up to `max_bytes` bytes."""
+
+    def __init__(self, max_bytes: int, max_records: int, device: int = 0):
+        self.max_bytes, self.max_records = int(max_bytes), int(max_records)
+        sb = int(lib().ma_fastq_scratch_bytes(self.max_bytes, self.max_records))
+        self.scratch = DeviceArray((sb,), np.uint8, device)
+        self.scratch_bytes = sb
+
+    def parse(self, stream, d_buf_ptr: int, nbytes: int, final: bool, batch: DeviceFastqBatch, bs_mode: int,
+              max_records: Optional[int] = None) -> ParseInfo:
+        mr = min(batch.max_records, self.max_records if max_records is None else int(max_records))
+        if nbytes > self.max_bytes:
+            raise MialignError("FastqParser: buffer larger than its capacity")
+        h = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().ma_fastq_parse_dev(h, d_buf_ptr if nbytes else None, int(nbytes), int(bool(final)), mr,
+                                       batch.stride, int(bs_mode), batch.rows.ptr,
+                                       batch.conv.ptr if batch.conv is not None else None,
+                                       batch.qual.ptr if batch.qual is not None else None, batch.lens.ptr,
+                                       batch.qual_lens.ptr, batch.names.ptr, batch.names_cap, batch.name_off.ptr,
+                                       batch.info.ptr, self.scratch.ptr, self.scratch_bytes), "ma_fastq_parse_dev")
+        info = np.zeros(8, np.uint64)
+        check(lib().ma_memcpy_d2h(ptr(info), batch.info.ptr, 64, h), "ma_memcpy_d2h")
+        if h is not None:
+            check(lib().ma_stream_sync(h), "ma_stream_sync")
+        pi = ParseInfo(int(info[0]), int(info[1]), int(info[2]), int(info[3]), int(info[4]))
+        batch.n, batch.max_len = pi.records, pi.max_len
+        return pi
+
+
+def _open_stream(path: str):
+    low = path.lower()
+    if low.endswith(".gz") or low.endswith(".gzip"):
+        return gzip.open(path, "rb")
+    return open(path, "rb", buffering=0)
+
+
+class _Side:
+    """One FASTQ file: host chunks, a carried tail, a device staging buffer."""
+
+    def __init__(self, path: str, bs: str, chunk: int, device: int):
+        if bs not in BS_CODES:
+            raise MialignError(f"unknown conversion mode {bs!r} (C2T, G2A or none)")
+        self.fh = _open_stream(path)
+        self.mode = BS_CODES[bs]
+        self.chunk = int(chunk)
+        self.carry = b""
+        self.eof = False
+        self.dbuf = DeviceArray((2 * self.chunk + 16,), np.uint8, device)
+
+    def fill(self) -> bytes:
+        """Carried tail + the next chunk (the buffer the parser sees)."""
+        while not self.eof and len(self.carry) < self.chunk:
+            data = self.fh.read(self.chunk)
+            if not data:
+                self.eof = True
+                break
+            self.carry += data
+        return self.carry
+
+    def close(self):
+        self.fh.close()
+
+
+class DeviceFastqReader:
+    """Streaming SE/PE FASTQ reader whose batches land in HBM, parsed on the
+    device. read_batch() returns (batch R1, batch R2 or None), each with n
+    records (R2 has as many as R1, else "paired FASTQ length mismatch (R2 ended
+    early)" as in the reference)."""
+
+    def __init__(self, path1: str, path2: str = "", bs_r1: str = "", bs_r2: str = "", batch_reads: int = 262144,
+                 stride: int = 256, chunk_bytes: int = 64 << 20, device: int = 0):
+        self.stride = int(stride)
+        self.batch_reads = int(batch_reads)
+        self.sides = [_Side(path1, bs_r1, chunk_bytes, device)]
+        if path2:
+            self.sides.append(_Side(path2, bs_r2, chunk_bytes, device))
+        self.stream = Stream(device)
+        self.parser = FastqParser(2 * chunk_bytes + 16, self.batch_reads, device)
+        names_cap = 2 * chunk_bytes
+        self.batches = [DeviceFastqBatch(self.batch_reads, self.stride, names_cap, device) for _ in self.sides]
+
+    def _parse_side(self, k: int, max_records: int) -> ParseInfo:
+        """Parse up to max_records records of side k, reading more input until
+        that many are complete or the file ends."""
+        side, batch = self.sides[k], self.batches[k]
+        while True:
+            buf = side.fill()
+            if len(buf) + 16 > side.dbuf.nbytes:
+                side.dbuf = DeviceArray((2 * len(buf) + 16,), np.uint8, side.dbuf.device)
+            if len(buf) > self.parser.max_bytes:
+                self.parser = FastqParser(2 * len(buf), self.batch_reads, side.dbuf.device)
+            if len(buf) > batch.names_cap:  # names never take more bytes than the buffer
+                batch.names_cap = 2 * len(buf)
+                batch.names = DeviceArray((batch.names_cap,), np.uint8, side.dbuf.device)
+            if buf:
+                check(lib().ma_memcpy_h2d(side.dbuf.ptr, buf, len(buf), None), "ma_memcpy_h2d")
+            info = self.parser.parse(self.stream, side.dbuf.ptr, len(buf), side.eof, batch, side.mode, max_records)
+            if info.status in FQ_ERRORS:
+                raise MialignError(FQ_ERRORS[info.status])
+            if info.status == 4:
+                raise MialignError(f"FASTQ read longer than {self.stride} bases")
+            if info.status == 5:
+                raise MialignError("FASTQ names larger than the batch's name buffer")
+            if info.records >= max_records or side.eof:
+                return info
+            side.chunk *= 2  # fewer complete records than asked: read on
+
+    def read_batch(self):
+        """(R1 batch, R2 batch or None) of n records each, n = 0 at the end."""
+        info1 = self._parse_side(0, self.batch_reads)
+        n = info1.records
+        if len(self.sides) == 2:
+            if n > 0:
+                info2 = self._parse_side(1, n)
+                if info2.records < n:
+                    raise MialignError("paired FASTQ length mismatch (R2 ended early)")
+                self.sides[1].carry = self.sides[1].carry[info2.consumed:]
+            else:
+                self.batches[1].n = 0
+        self.sides[0].carry = self.sides[0].carry[info1.consumed:]
+        return self.batches[0], (self.batches[1] if len(self.sides) == 2 else None)
+
+    def close(self):
+        for s in self.sides:
+            s.close()
+
+
+def parse_fastq_bytes(data: bytes, stride: int, bs: str = "", final: bool = True, max_records: Optional[int] = None,
+                      names_cap: Optional[int] = None, device: int = 0) -> tuple[ParseInfo, dict]:
+    """Parse one in-memory FASTQ buffer on the device; returns (info, host copies)."""
+    mr = max_records if max_records is not None else max(1, data.count(b"\n") // 4 + 1)
+    nc = names_cap if names_cap is not None else max(1, len(data))
+    st = Stream(device)
+    d = DeviceArray((max(len(data), 1) + 16,), np.uint8, device)
+    if data:
+        check(lib().ma_memcpy_h2d(d.ptr, data, len(data), None), "ma_memcpy_h2d")
+    batch = DeviceFastqBatch(mr, stride, nc, device)
+    info = FastqParser(len(data), mr, device).parse(st, d.ptr, len(data), final, batch, BS_CODES[bs])
+    return info, batch.download()
+
+
+__all__ = ["DeviceFastqBatch", "DeviceFastqReader", "FastqParser", "ParseInfo", "parse_fastq_bytes"]

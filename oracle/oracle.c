@@ -31,6 +31,11 @@
  *        answers for the 4-bit SEQ and the bin + an independent BAM decoder
  *        (tests/bam_decode.py) checked field by field against the SAM text of
  *        the same records + planted mistakes (swapped clips, unreversed QUAL)
+ *   or_fq_cvt / or_fastq_parse ........................ reference known answers
+ *        (test_linear_fastq.mojo, test_fastq_batch.py, the reference's FASTQ
+ *        fixture) + the host reader of the parity path (read_one) on random
+ *        FASTQ text + hand-cut edge cases (CRLF, missing final newline, truncated
+ *        records, empty lines)
  *   or_recount_nm (a checker, no reference counterpart) ... an independent Python
  *                                                    recount + planted one-base shifts
  */
@@ -1332,6 +1337,90 @@ int64_t or_bam_pack_se(int64_t n, const int32_t *cid, const uint32_t *pos, const
                                    apply_dup, out, cap);
     free(f);
     return g;
+}
+
+/* ---------------------------------------------------- FASTQ ingest (next row) */
+
+/* Follows _cvt, fq2bam-meth/src/linear_fastq.mojo:37-46: mode 1 (C2T) maps C and
+ * c to 'T', mode 2 (G2A) maps G and g to 'A' (upper case out), else as is. */
+uint8_t or_fq_cvt(uint8_t c, int mode) {
+    if (mode == 1) return (c == 'C' || c == 'c') ? 'T' : c;
+    if (mode == 2) return (c == 'G' || c == 'g') ? 'A' : c;
+    return c;
+}
+
+/* Next line of buf[*pos..n): follows _pipe_line (linear_fastq.mojo:229-266)
+ * over a whole in-memory buffer. A line ends at '\n' (one trailing '\r'
+ * dropped); at the end, an unterminated rest is a line only when `final` and
+ * non-empty after the '\r' strip. Returns 1 with [*s, *e) or 0. */
+static int fq_line(const uint8_t *buf, int64_t n, int64_t *pos, int final, int64_t *s, int64_t *e) {
+    int64_t i = *pos;
+    while (i < n && buf[i] != '\n') i++;
+    if (i < n) {
+        *s = *pos;
+        *e = (i > *pos && buf[i - 1] == '\r') ? i - 1 : i;
+        *pos = i + 1;
+        return 1;
+    }
+    if (!final || *pos >= n) return 0;
+    *s = *pos;
+    *e = (n > *pos && buf[n - 1] == '\r') ? n - 1 : n;
+    *pos = n;
+    return *e > *s;
+}
+
+/* Follows _qname_span (linear_fastq.mojo:322-340): skip a leading '@', stop at
+ * the first ' ' or '/'. */
+static void fq_qname(const uint8_t *h, int64_t n, int64_t *start, int64_t *len) {
+    int64_t s = (n > 0 && h[0] == '@') ? 1 : 0, e = n;
+    for (int64_t i = s; i < e; i++)
+        if (h[i] == ' ' || h[i] == '/') { e = i; break; }
+    *start = s;
+    *len = e - s < 0 ? 0 : e - s;
+}
+
+/* Parses up to max_records FASTQ records of buf[0..n) (follows _store_rec +
+ * fq_read_batch, :342-414, and fq_pack_bases, :432-470): rows[r * stride] =
+ * the original bases padded with 'N', conv_rows the same converted with mode
+ * (NULL: not written), qual_rows the qualities (first `stride` bytes, 0
+ * padded; NULL: not written), lens / qual_lens, names back to back at name_off.
+ * info: [0] records, [1] bytes consumed (through the last record's quality
+ * line), [2] longest read, [3] status (0 ok; 1, 2, 3 = truncated record
+ * missing SEQ, '+', QUAL — only when `final`; 4 = a read longer than stride;
+ * 5 = names past names_cap), [4] name bytes. Without `final` an unterminated
+ * or incomplete tail is left unconsumed for the next buffer. */
+void or_fastq_parse(const uint8_t *buf, int64_t n, int final, int64_t max_records, int64_t stride, int mode,
+                    uint8_t *rows, uint8_t *conv_rows, uint8_t *qual_rows, uint32_t *lens,
+                    uint32_t *qual_lens, uint8_t *names, int64_t names_cap, int64_t *name_off,
+                    int64_t *info) {
+    int64_t pos = 0, r = 0, nb = 0, mx = 0, consumed = 0, status = 0;
+    name_off[0] = 0;
+    while (r < max_records) {
+        int64_t hs, he, ss, se, ps, pe, qs, qe;
+        if (!fq_line(buf, n, &pos, final, &hs, &he)) break;
+        if (!fq_line(buf, n, &pos, final, &ss, &se)) { if (final) status = 1; break; }
+        if (!fq_line(buf, n, &pos, final, &ps, &pe)) { if (final) status = 2; break; }
+        if (!fq_line(buf, n, &pos, final, &qs, &qe)) { if (final) status = 3; break; }
+        (void)ps; (void)pe;
+        int64_t slen = se - ss, qlen = qe - qs, q0, ql;
+        if (slen > stride) { status = 4; break; }
+        fq_qname(buf + hs, he - hs, &q0, &ql);
+        if (nb + ql > names_cap) { status = 5; break; }
+        memcpy(names + nb, buf + hs + q0, (size_t)ql);
+        nb += ql;
+        name_off[r + 1] = nb;
+        for (int64_t x = 0; x < stride; x++) {
+            rows[r * stride + x] = x < slen ? buf[ss + x] : 'N';
+            if (conv_rows) conv_rows[r * stride + x] = x < slen ? or_fq_cvt(buf[ss + x], mode) : 'N';
+            if (qual_rows) qual_rows[r * stride + x] = x < qlen ? buf[qs + x] : 0;
+        }
+        lens[r] = (uint32_t)slen;
+        qual_lens[r] = (uint32_t)qlen;
+        if (slen > mx) mx = slen;
+        r++;
+        consumed = pos;
+    }
+    info[0] = r; info[1] = consumed; info[2] = mx; info[3] = status; info[4] = nb;
 }
 
 int or_num_threads(void) {

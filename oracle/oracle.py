@@ -108,6 +108,9 @@ def lib():
         L.or_pack_bam_aln.restype = I64
         L.or_pack_bam_aln.argtypes = [P, I64, I64, P, I64, I64, I64, I64, I64, I64, I64, P, I64, P,
                                       I64, I64, I64, I64, I64, P, I64]
+        L.or_fq_cvt.restype = C.c_uint8
+        L.or_fq_cvt.argtypes = [C.c_uint8, C.c_int]
+        L.or_fastq_parse.argtypes = [P, I64, C.c_int, I64, I64, C.c_int, P, P, P, P, P, P, I64, P, P]
         L.or_bam_pack_fields.restype = I64
         L.or_bam_pack_fields.argtypes = [I64] + [P] * 13 + [I64, P, P, P, P, I64, P, P, C.c_int, P, I64]
         L.or_bam_pack_se.restype = I64
@@ -303,3 +306,28 @@ def bgzf_stream(data: bytes, level: int = 1, eof: bool = True) -> bytes:
     writes it (bam_emit.py:103-185), then the 28-byte EOF member."""
     out = [bgzf_block(data[i:i + BGZF_MAX_BLOCK], level) for i in range(0, len(data), BGZF_MAX_BLOCK)]
     return b"".join(out) + (BGZF_EOF if eof else b"")
+
+
+# ------------------------------------------------------------------ FASTQ ingest
+
+def fastq_parse(buf: bytes, stride: int, mode: int = 0, final: bool = True, max_records: int | None = None,
+                names_cap: int | None = None) -> dict:
+    """or_fastq_parse over a FASTQ byte buffer; returns the rows and the info fields."""
+    a = np.frombuffer(buf, dtype=np.uint8) if buf else np.zeros(1, np.uint8)
+    n = len(buf)
+    cap = max_records if max_records is not None else max(1, buf.count(b"\n") // 4 + 1)
+    names_cap = names_cap if names_cap is not None else max(1, n)
+    rows = np.zeros((cap, stride), np.uint8)
+    conv = np.zeros((cap, stride), np.uint8)
+    qual = np.zeros((cap, stride), np.uint8)
+    lens = np.zeros(cap, np.uint32)
+    qlens = np.zeros(cap, np.uint32)
+    names = np.zeros(names_cap, np.uint8)
+    off = np.zeros(cap + 1, np.int64)
+    info = np.zeros(5, np.int64)
+    lib().or_fastq_parse(_p(a), n, int(final), cap, stride, mode, _p(rows), _p(conv), _p(qual), _p(lens), _p(qlens),
+                         _p(names), names_cap, _p(off), _p(info))
+    r = int(info[0])
+    return {"n": r, "consumed": int(info[1]), "max_len": int(info[2]), "status": int(info[3]),
+            "name_bytes": int(info[4]), "rows": rows[:r], "conv": conv[:r], "qual": qual[:r], "lens": lens[:r],
+            "qual_lens": qlens[:r], "names": names[:int(info[4])], "name_off": off[:r + 1]}
