@@ -14,9 +14,10 @@ Prints one JSON line. `roofline` is for the dominant kernel (the map kernel);
 science, OpenMP over reads + the stable-sort markdup) on a bounded sample of
 the same reads on this host. Extra sections: `serving` (the served request
 loop against plain launches), `sort_stress` (200M sort + markdup records),
-`genome_scale` (the 3.1 Gbp reference) and `bam_emit` (the step's records as
+`genome_scale` (the 3.1 Gbp reference), `bam_emit` (the step's records as
 coordinate-ordered BAM with duplicate flags: device pack, BGZF on the device
-or on host threads, with its own roofline, parity sample and CPU baseline).
+or on host threads) and `fastq_ingest` (the reads' FASTQ text parsed on the
+device), each with its own roofline, parity check and CPU baseline.
 """
 
 from __future__ import annotations
@@ -305,6 +306,108 @@ def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
     return blob[src], new_off
 
 
+def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int, parity_records: int) -> dict:
+    """FASTQ ingest of the step's reads: their FASTQ text (names, bases, '+',
+    qualities) resident in HBM, parsed on the device into base, C2T-converted
+    and quality rows, lengths and names (ma_fastq_parse_dev); timed with HIP
+    events. Algorithmic bytes = the text read once + the three row arrays, the
+    two length arrays, the names and their offsets written."""
+    import mialign
+    from mialign import linear_fastq as lf
+    from mialign import synth
+    from mialign._lib import check, lib
+    from mialign.serve import pinned_empty
+
+    n, L = reads.lens.size, reads.bases.shape[1]
+    t0 = time.time()
+    text = synth.write_fastq(reads.bases, qual, reads.lens, blob, name_off)
+    nbytes = int(text.size)
+    d_text = mialign.DeviceArray((nbytes + 16,), np.uint8)
+    check(lib().ma_memcpy_h2d(d_text.ptr, text.ctypes.data, nbytes, None), "h2d")
+    batch = lf.DeviceFastqBatch(n, L, nbytes)
+    parser = lf.FastqParser(nbytes, n)
+    st = mialign.Stream(0)
+    log(f"fastq: {nbytes / 1e9:.2f} GB of FASTQ text for {n} reads written and uploaded in {time.time() - t0:.1f}s")
+    info = parser.parse(st, d_text.ptr, nbytes, True, batch, 1)
+    ev0, ev1 = mialign.Event(), mialign.Event()
+    ms = []
+    for _ in range(5):
+        ev0.record(st)
+        check(lib().ma_fastq_parse_dev(st.handle, d_text.ptr, nbytes, 1, n, L, 1, batch.rows.ptr, batch.conv.ptr,
+                                       batch.qual.ptr, batch.lens.ptr, batch.qual_lens.ptr, batch.names.ptr,
+                                       batch.names_cap, batch.name_off.ptr, batch.info.ptr, parser.scratch.ptr,
+                                       parser.scratch_bytes), "ma_fastq_parse_dev")
+        ev1.record(st)
+        ev1.synchronize()
+        ms.append(ev0.elapsed_ms(ev1))
+    dev_ms = float(np.median(ms))
+    name_bytes = int(name_off[-1])
+    alg = nbytes + 3 * n * L + 8 * n + name_bytes + 8 * (n + 1)
+    achieved = alg / (dev_ms / 1000.0) / 1e9
+    log(f"fastq: parse {dev_ms:.2f} ms = {n / dev_ms / 1e3:.1f}M records/s, {achieved:.0f} GB/s algorithmic")
+    # size-independent round trip: the parsed batch equals the arrays the text was written from
+    got = batch.download()
+    conv = np.where((reads.bases == ord("C")) | (reads.bases == ord("c")), np.uint8(ord("T")), reads.bases)
+    ok_all = bool(info.records == n and info.status == 0 and info.consumed == nbytes
+                  and np.array_equal(got["rows"], reads.bases) and np.array_equal(got["conv"], conv)
+                  and np.array_equal(got["qual"], qual) and np.array_equal(got["lens"], reads.lens)
+                  and got["names"].tobytes() == blob[:name_bytes].tobytes()
+                  and np.array_equal(got["name_off"], name_off))
+    del got
+    # the oracle on the text of the first records
+    from oracle import oracle as orc
+
+    m = min(parity_records, n)
+    rec_end = _fastq_record_end(text, m)
+    want = orc.fastq_parse(text[:rec_end].tobytes(), L, 1)
+    ok_oracle = bool(want["n"] == m and np.array_equal(want["conv"], conv[:m]) and np.array_equal(want["qual"], qual[:m]))
+    del want
+    log(f"fastq: round trip of all {n} records exact: {ok_all}; first {m} equal the oracle's: {ok_oracle}")
+    # PCIe-inclusive: pinned text copied in, then parsed
+    pin = pinned_empty((nbytes,), np.uint8)
+    pin[:] = text
+    t_start = time.perf_counter()
+    check(lib().ma_memcpy_h2d(d_text.ptr, pin.ctypes.data, nbytes, st.handle), "h2d")
+    parser.parse(st, d_text.ptr, nbytes, True, batch, 1)
+    pcie_s = time.perf_counter() - t_start
+    cpu = None
+    if with_cpu:
+        ns = min(cpu_records, n)
+        sample = text[: _fastq_record_end(text, ns)].tobytes()
+        t0 = time.perf_counter()
+        orc.fastq_parse(sample, L, 1)
+        cs = time.perf_counter() - t0
+        cpu = {"value": round(ns / cs, 1), "unit": "records/s", "cores": 1, "kind": "port",
+               "sample": f"oracle FASTQ parse (the reference's one-thread reader restated) of the first {ns} "
+                         f"records ({len(sample) / 1e6:.0f} MB), {cs:.2f} s"}
+        log(f"fastq: cpu baseline {ns / cs:.0f} records/s")
+    del pin, d_text, batch, parser, text
+    return {
+        "workload": f"FASTQ text of the step's {n} reads ({L} bp, Illumina-style names and qualities), "
+                    "resident in HBM -> base / C2T / quality rows, lengths, names",
+        "fastq_bytes": nbytes,
+        "parse_ms": round(dev_ms, 3),
+        "records_per_s": round(n / (dev_ms / 1000.0), 1),
+        "GBps_text": round(nbytes / (dev_ms / 1000.0) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "fq_* (count, lines, measure, copy)",
+                     "algorithmic_bytes_per_read": round(alg / n, 1)},
+        "pcie_inclusive_records_per_s": round(n / pcie_s, 1),
+        "parity": {"all_records_round_trip": ok_all, "oracle_records": m, "oracle_equal": ok_oracle},
+        "cpu_baseline": cpu,
+    }
+
+
+def _fastq_record_end(text: np.ndarray, m: int) -> int:
+    """Byte offset just past the m-th record of 4-line FASTQ text."""
+    if m <= 0:
+        return 0
+    nl = np.flatnonzero(text[: min(text.size, 400 * m + 1024)] == 10)
+    if nl.size < 4 * m:
+        nl = np.flatnonzero(text == 10)
+    return int(nl[4 * m - 1]) + 1
+
+
 def _check_members(z: np.ndarray, stream: np.ndarray, k: int) -> bool:
     """The first k BGZF members of z inflate (zlib) to the stream's first k
     65280-byte blocks with matching CRC32 / ISIZE, and z ends with the EOF member."""
@@ -325,7 +428,8 @@ def _check_members(z: np.ndarray, stream: np.ndarray, k: int) -> bool:
     return bool(ok and bytes(z[-28:]) == eof)
 
 
-def bam_section(sr, reads, tid, n_tid: int, with_cpu: bool, cpu_records: int, parity_records: int) -> dict:
+def bam_section(sr, reads, tid, n_tid: int, blob, name_off, qual, with_cpu: bool, cpu_records: int,
+                parity_records: int) -> dict:
     """BAM emit of the step's records, resident in HBM: the device pack in
     coordinate order with the duplicate flag (ma_bam_pack_se_dev: sizes + scan +
     one wave per record), the stream's copy to pinned host memory and BGZF at
@@ -342,7 +446,6 @@ def bam_section(sr, reads, tid, n_tid: int, with_cpu: bool, cpu_records: int, pa
 
     n, L = reads.lens.size, reads.bases.shape[1]
     t0 = time.time()
-    blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
     dreads = bam_emit.DeviceReads(blob, name_off, reads.bases, reads.lens, qual)
     rg = b"mojo1"
     cap = dreads.out_bound(len(rg))
@@ -589,6 +692,8 @@ def main() -> int:
                     help="50-250 bp reads on the 3.1 Gbp index (one launch)")
     ap.add_argument("--gs-cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--no-bam", action="store_true", help="skip the BAM emit section")
+    ap.add_argument("--no-fastq", action="store_true", help="skip the FASTQ ingest section")
+    ap.add_argument("--fastq-cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--bam-cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--bam-parity-sample", type=int, default=1_000_000)
     args = ap.parse_args()
@@ -631,9 +736,17 @@ def main() -> int:
     merged = sr.recs.download()
     is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
-    bam = None
-    if not args.no_bam:
-        bam = bam_section(sr, reads, tid, tid.size, not args.no_cpu, args.bam_cpu_sample, args.bam_parity_sample)
+    bam = fastq = None
+    if not (args.no_bam and args.no_fastq):
+        blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
+        if not args.no_bam:
+            bam = bam_section(sr, reads, tid, tid.size, blob, name_off, qual, not args.no_cpu, args.bam_cpu_sample,
+                              args.bam_parity_sample)
+        sr.close()
+        if not args.no_fastq:
+            fastq = fastq_section(reads, blob, name_off, qual, not args.no_cpu, args.fastq_cpu_sample,
+                                  args.bam_parity_sample)
+        del blob, name_off, qual
     sr.close()
 
     # ---- served request loop vs plain one-batch-per-launch (same batches)
@@ -707,6 +820,7 @@ def main() -> int:
         "sort_stress": sort_stress,
         "genome_scale": genome_scale,
         "bam_emit": bam,
+        "fastq_ingest": fastq,
     }
     print(json.dumps(line), flush=True)
     return 0

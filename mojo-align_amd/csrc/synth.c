@@ -314,3 +314,36 @@ int64_t ms_gen_names_quals(uint64_t seed, int64_t n, const uint32_t *lens, int64
     }
     return off;
 }
+
+/* FASTQ text of n reads: "@name\nbases\n+\nqualities\n" per read (bases and
+ * qualities are rows of stride bytes, lens[i] used). out_off[n+1] receives
+ * each record's offset; returns the text length, or -1 past cap. */
+int64_t ms_write_fastq(int64_t n, const uint8_t *bases, const uint8_t *qual, int64_t stride,
+                       const uint32_t *lens, const char *names, const int64_t *name_off, char *out, int64_t cap,
+                       int64_t *out_off) {
+    int64_t off = 0;
+    for (int64_t i = 0; i < n; i++) {
+        out_off[i] = off;
+        off += 1 + (name_off[i + 1] - name_off[i]) + 1 + 2 * (int64_t)lens[i] + 4;
+    }
+    out_off[n] = off;
+    if (off > cap) return -1;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+        char *p = out + out_off[i];
+        const int64_t nl = name_off[i + 1] - name_off[i], L = lens[i];
+        *p++ = '@';
+        memcpy(p, names + name_off[i], (size_t)nl);
+        p += nl;
+        *p++ = '\n';
+        memcpy(p, bases + i * stride, (size_t)L);
+        p += L;
+        *p++ = '\n';
+        *p++ = '+';
+        *p++ = '\n';
+        memcpy(p, qual + i * stride, (size_t)L);
+        p += L;
+        *p++ = '\n';
+    }
+    return off;
+}

@@ -35,6 +35,8 @@ def _synth():
         L.ms_pack_2bit.argtypes = [P, P, I64, I64, I64, P]
         L.ms_gen_sort_records.restype = None
         L.ms_gen_sort_records.argtypes = [U64, I64, I64, I64, D, P, P, P]
+        L.ms_write_fastq.restype = I64
+        L.ms_write_fastq.argtypes = [I64, P, P, I64, P, P, P, P, I64, P]
         L.ms_gen_names_quals.restype = I64
         L.ms_gen_names_quals.argtypes = [U64, I64, P, I64, P, P, I64, P]
         _lib = L
@@ -164,3 +166,21 @@ def gen_names_quals(seed: int, lens: np.ndarray, max_len: int):
     if got < 0:
         raise ValueError("ms_gen_names_quals: name buffer too small")
     return names[:max(int(got), 1)], off, qual
+
+
+def write_fastq(bases: np.ndarray, qual: np.ndarray, lens: np.ndarray, names: np.ndarray,
+                name_off: np.ndarray) -> np.ndarray:
+    """FASTQ text (uint8 array) of the reads: "@name", bases, "+", qualities."""
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    qual = np.ascontiguousarray(qual, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    name_off = np.ascontiguousarray(name_off, dtype=np.int64)
+    n, stride = bases.shape
+    cap = int(n * 6 + 2 * int(lens.astype(np.int64).sum()) + int(name_off[-1]))
+    out = np.empty(max(cap, 1), dtype=np.uint8)
+    off = np.empty(n + 1, dtype=np.int64)
+    got = _synth().ms_write_fastq(n, _p(bases), _p(qual), stride, _p(lens), _p(np.ascontiguousarray(names)),
+                                  _p(name_off), _p(out), cap, _p(off))
+    if got < 0:
+        raise ValueError("ms_write_fastq: buffer too small")
+    return out[:got]
