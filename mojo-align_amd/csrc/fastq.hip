@@ -14,7 +14,7 @@
 // thread): one sweep counts the tile's newlines, a scan gives every tile its
 // first line number, a second sweep writes each newline's position at its line
 // number; one thread then settles the record count, the bytes consumed and the
-// status; a thread per record measures its four lines and name; a scan of the
+// status; a wave per record measures its four lines and name; a scan of the
 // name lengths places the names; a wave per record copies bases, converted
 // bases, qualities and the name.
 #include "ma_sort.hpp"
@@ -84,12 +84,31 @@ __global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restri
         const uint32_t c = count_nl(buf, n, p0);
         uint64_t tot;
         uint64_t L = tile_base[tile] + block_excl_scan_256(c, sh, tot);
-        if (c && L < (uint64_t)cap)
-            for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
-                if (buf[p] == '\n') {
-                    if (L < (uint64_t)cap) nl[L] = (uint64_t)p;
-                    L++;
+        if (c && L < (uint64_t)cap) {
+            if (p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0) {
+                // the same 64 bytes again (cached), scanned in registers
+                const uint4 *v = reinterpret_cast<const uint4 *>(buf + p0);
+#pragma unroll
+                for (int k = 0; k < FQ_BYTES / 16; k++) {
+                    const uint4 w = v[k];
+                    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+#pragma unroll
+                        for (int b = 0; b < 4; b++)
+                            if (((ws[q] >> (8 * b)) & 0xFF) == '\n') {
+                                if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 16 * k + 4 * q + b);
+                                L++;
+                            }
                 }
+            } else {
+                for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
+                    if (buf[p] == '\n') {
+                        if (L < (uint64_t)cap) nl[L] = (uint64_t)p;
+                        L++;
+                    }
+            }
+        }
     }
 }
 
@@ -136,34 +155,44 @@ __global__ void fq_settle_kernel(const uint8_t *__restrict__ buf, int64_t n, int
     info[7] = ~0ull;
 }
 
-// Per record: lengths and name span; the first read longer than stride.
-__global__ void fq_measure_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
-                                  int64_t max_records, int64_t stride, uint64_t *__restrict__ info,
-                                  uint32_t *__restrict__ lens, uint32_t *__restrict__ qual_lens,
-                                  uint32_t *__restrict__ name_len, uint64_t *__restrict__ name_src) {
+// A wave per record: lengths and name span (the first ' ' or '/' found by a
+// ballot over 64 header bytes at a time); the first read longer than stride.
+__global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
+                                                         int64_t max_records, int64_t stride,
+                                                         uint64_t *__restrict__ info, uint32_t *__restrict__ lens,
+                                                         uint32_t *__restrict__ qual_lens,
+                                                         uint32_t *__restrict__ name_len,
+                                                         uint64_t *__restrict__ name_src) {
     const int64_t recs = (int64_t)info[0];
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < max_records;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        if (r >= recs) {
-            name_len[r] = 0;
-            continue;
-        }
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int64_t n_waves = (int64_t)gridDim.x * 4;
+    // records past the batch: name length 0 (the offset scan reads them)
+    for (int64_t r = recs + wave * 64 + lane; r < max_records; r += n_waves * 64) name_len[r] = 0;
+    for (int64_t r = wave; r < recs; r += n_waves) {
         int64_t hs, he, ss, se, qs, qe;
         line_span(buf, nl, 4 * r, hs, he);
         line_span(buf, nl, 4 * r + 1, ss, se);
         line_span(buf, nl, 4 * r + 3, qs, qe);
-        int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs, e0 = he;
-        for (int64_t i = s0; i < he; i++)
-            if (buf[i] == ' ' || buf[i] == '/') {
-                e0 = i;
+        const int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs;
+        int64_t e0 = he;
+        for (int64_t c = s0; c < he; c += 64) {
+            const int64_t p = c + lane;
+            const bool stop = p < he && (buf[p] == ' ' || buf[p] == '/');
+            const uint64_t m = __ballot(stop);
+            if (m) {
+                e0 = c + __builtin_ctzll(m);
                 break;
             }
-        const int64_t slen = se - ss;
-        lens[r] = (uint32_t)slen;
-        qual_lens[r] = (uint32_t)(qe - qs);
-        name_len[r] = (uint32_t)(e0 - s0);
-        name_src[r] = (uint64_t)s0;
-        if (slen > stride) atomicMin((unsigned long long *)&info[7], (unsigned long long)r);
+        }
+        if (lane == 0) {
+            const int64_t slen = se - ss;
+            lens[r] = (uint32_t)slen;
+            qual_lens[r] = (uint32_t)(qe - qs);
+            name_len[r] = (uint32_t)(e0 - s0);
+            name_src[r] = (uint64_t)s0;
+            if (slen > stride) atomicMin((unsigned long long *)&info[7], (unsigned long long)r);
+        }
     }
 }
 
@@ -318,8 +347,9 @@ int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32
     uint32_t *name_len = (uint32_t *)(s + L.name_len);
     uint64_t *name_src = (uint64_t *)(s + L.name_src);
     const unsigned g = (unsigned)grid_for(max_records);
-    fq_measure_kernel<<<g, 256, 0, st>>>(d_buf, nl, max_records, stride, d_info, d_lens, d_qual_lens, name_len,
-                                         name_src);
+    const unsigned gm = (unsigned)((max_records + 3) / 4 < 65536 ? (max_records + 3) / 4 : 65536);
+    fq_measure_kernel<<<gm, 256, 0, st>>>(d_buf, nl, max_records, stride, d_info, d_lens, d_qual_lens, name_len,
+                                          name_src);
     MA_LAUNCHED();
     MA_TRY(exclusive_scan_u32(st, max_records, name_len, (uint64_t *)d_name_off, (uint64_t *)(s + L.name_part),
                               (uint64_t *)d_name_off + max_records));
