@@ -225,6 +225,19 @@ class DeviceReads:
                                self.d_qual_lens.ptr if self.d_qual_lens else None, self.stride)
 
     @classmethod
+    def on_device(cls, names: DeviceArray, name_off: DeviceArray, seq: DeviceArray, lens: DeviceArray, n: int,
+                  stride: int, name_bytes: int, qual: Optional[DeviceArray] = None,
+                  qual_lens: Optional[DeviceArray] = None) -> "DeviceReads":
+        """Wrap reads already in HBM (the layout ma_bam_reads describes)."""
+        self = cls.__new__(cls)
+        self.n, self.stride, self.name_bytes = int(n), int(stride), int(name_bytes)
+        self.d_names, self.d_name_off, self.d_seq, self.d_lens = names, name_off, seq, lens
+        self.d_qual, self.d_qual_lens = qual, qual_lens
+        self.struct = BamReads(names.ptr, name_off.ptr, seq.ptr, qual.ptr if qual is not None else None, lens.ptr,
+                               qual_lens.ptr if qual_lens is not None else None, self.stride)
+        return self
+
+    @classmethod
     def from_names(cls, names: Sequence[bytes], seq_rows, lens, qual_rows=None, qual_lens=None, device: int = 0,
                    stream=None) -> "DeviceReads":
         blob, off = names_blob(names)

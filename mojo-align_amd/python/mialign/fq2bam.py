@@ -1,0 +1,173 @@
+"""FASTQ -> coordinate-sorted, duplicate-marked BAM with every stage on the GPU.
+
+The end-to-end product of the reference's native-BAM path
+(fq2bam-meth/src/linear_gpu_fm.mojo: FASTQ arena -> GPU map -> _emit_bam_from_batch
+keys -> gpu_sort_markdup -> _gather_perm_to_writer -> BamWriter, :2140-3290),
+with this build's science (the hash-seeded parity mapper) and every step on
+the device: ma_fastq_parse_dev, ma_map_batch_dev, ma_sort_keys_se_dev,
+ma_sort_markdup_dev, ma_bam_pack_se_dev and ma_bgzf_compress_dev (or the host
+zlib writer, whose members are the reference's byte for byte). The reads,
+records and keys of the whole run stay in HBM (an MI355X holds ~700M reads of
+150 bp this way); single-end reads only.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+import numpy as np
+
+from ._lib import BS_MODES, MA_ENC_ASCII, REC_FIELDS, MialignError, Recs, check, default_params, lib
+from .bam_emit import BamPacker, BamWriter, DeviceBgzf, DeviceReads, bam_header, bgzf_compress, tid_table
+from .core import DeviceArray, DeviceRecs, Event, Stream, Workspace, map_batch_dev
+from .linear_fastq import DeviceFastqReader
+
+
+class _Grow:
+    """A device buffer that doubles when an append would not fit."""
+
+    def __init__(self, itemsize: int, cap: int, device: int):
+        self.itemsize, self.device = itemsize, device
+        self.cap = max(int(cap), 1)
+        self.buf = DeviceArray((self.cap * itemsize,), np.uint8, device)
+        self.n = 0
+
+    def append_dev(self, src_ptr: int, count: int, stream) -> None:
+        if self.n + count > self.cap:
+            new_cap = max(self.cap * 2, self.n + count)
+            nb = DeviceArray((new_cap * self.itemsize,), np.uint8, self.device)
+            if self.n:
+                check(lib().ma_memcpy_d2d(nb.ptr, self.buf.ptr, self.n * self.itemsize, stream.handle), "d2d")
+            stream.sync()
+            self.buf, self.cap = nb, new_cap
+        if count:
+            check(lib().ma_memcpy_d2d(self.buf.ptr + self.n * self.itemsize, src_ptr, count * self.itemsize,
+                                      stream.handle), "d2d")
+        self.n += count
+
+    def view(self, dtype) -> DeviceArray:
+        """A non-owning DeviceArray over the first n items."""
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape, v.dtype = (self.n * self.itemsize // np.dtype(dtype).itemsize,), np.dtype(dtype)
+        v.nbytes, v.device, v.ptr = self.n * self.itemsize, self.device, self.buf.ptr
+        v.free = lambda: None
+        return v
+
+
+def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_reads: int = 1 << 20,
+                 stride: int = 256, do_markdup: bool = True, device_bgzf: bool = True, level: int = 1,
+                 rg_id: str = "mojo1", sm: str = "sample", lb: str = "lib1", pl: str = "ILLUMINA",
+                 pg_vn: str = "0.1.0-mojo-fm", device: int = 0) -> dict:
+    """Single-end FASTQ -> BAM (coordinate order, 0x400 on duplicates). `index`
+    is a LinearIndex with its GPU index loaded. Returns counts and stage times."""
+    if bs_r1 not in BS_MODES:
+        raise MialignError(f"unknown conversion mode {bs_r1!r} (C2T, G2A or none)")
+    if stride > 256:
+        raise MialignError("reads longer than 256 bases are not supported")
+    gix = index.gpu
+    st = Stream(device)
+    t0 = time.perf_counter()
+    reader = DeviceFastqReader(fq1, "", bs_r1, batch_reads=batch_reads, stride=stride, device=device,
+                               with_conv=False)
+    ws = Workspace(gix, batch_reads, stride)
+    prm = default_params(k=gix.k)
+    recs_b = DeviceRecs(batch_reads, device)
+    rows, quals = _Grow(stride, batch_reads, device), _Grow(stride, batch_reads, device)
+    lens, qlens = _Grow(4, batch_reads, device), _Grow(4, batch_reads, device)
+    names = _Grow(1, 32 * batch_reads, device)
+    fields = {f: _Grow(4, batch_reads, device) for f in ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")}
+    name_off = [np.zeros(1, np.int64)]
+    t_map = 0.0
+    ev0, ev1 = Event(), Event()
+    try:
+        while True:
+            b, _ = reader.read_batch()
+            n = b.n
+            if n == 0:
+                break
+            max_len = min(stride, max(b.max_len, gix.k))
+            ev0.record(st)
+            map_batch_dev(gix, ws, st, b.rows, stride, b.lens, n, max_len, recs_b, encoding=MA_ENC_ASCII,
+                          bs_r1=BS_MODES[bs_r1], params=prm)
+            ev1.record(st)
+            base_names = names.n
+            rows.append_dev(b.rows.ptr, n, st)
+            quals.append_dev(b.qual.ptr, n, st)
+            lens.append_dev(b.lens.ptr, n, st)
+            qlens.append_dev(b.qual_lens.ptr, n, st)
+            off = b.name_off.download()[: n + 1]
+            names.append_dev(b.names.ptr, int(off[n]), st)
+            name_off.append(off[1:] + base_names)
+            for f, g in fields.items():
+                g.append_dev(recs_b.field_ptr(f), n, st)
+            st.sync()  # the reader reuses its batch buffers
+            t_map += ev0.elapsed_ms(ev1)
+    finally:
+        reader.close()
+    N = lens.n
+    t_ingest = time.perf_counter() - t0
+    if N == 0:
+        raise MialignError("no reads in " + fq1)
+    # sort keys, sort + markdup over the whole run
+    t1 = time.perf_counter()
+    sq_names, first, tid = tid_table(index.contig_names)
+    sq_lens = [index.contig_lengths[i] for i in first]
+    d_tid = DeviceArray.from_host(tid, device)
+    recs = Recs()
+    for f in REC_FIELDS:
+        setattr(recs, f, fields[f].buf.ptr if f in fields else None)
+    keys = {f: DeviceArray((N,), dt, device) for f, dt in (("coord", np.uint64), ("dhi", np.uint64),
+                                                          ("dlo", np.uint64), ("score", np.uint32),
+                                                          ("pair", np.uint32))}
+    check(lib().ma_sort_keys_se_dev(st.handle, N, C.byref(recs), lens.buf.ptr, d_tid.ptr, 0, keys["coord"].ptr,
+                                    keys["dhi"].ptr, keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr),
+          "ma_sort_keys_se_dev")
+    perm = DeviceArray((N,), np.uint32, device)
+    is_dup = DeviceArray((N,), np.uint32, device)
+    sb = int(lib().ma_sort_scratch_bytes(N))
+    scratch = DeviceArray((sb,), np.uint8, device)
+    check(lib().ma_sort_markdup_dev(st.handle, N, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
+                                    keys["score"].ptr, keys["pair"].ptr, is_dup.ptr, perm.ptr, int(do_markdup), None,
+                                    scratch.ptr, sb), "ma_sort_markdup_dev")
+    del keys, scratch
+    # BAM records in coordinate order, then BGZF
+    offs = np.concatenate(name_off)
+    d_off = DeviceArray.from_host(offs, device)
+    reads = DeviceReads.on_device(names.view(np.uint8), d_off, rows.view(np.uint8), lens.view(np.uint32), N, stride,
+                                  int(offs[-1]), quals.view(np.uint8), qlens.view(np.uint32))
+    rg = rg_id.encode("ascii")
+    packer = BamPacker(N, reads.out_bound(len(rg)), device)
+    packer.pack_se(st, N, _recs_view(recs, N), d_tid, tid.size, reads, rg, perm, is_dup, bool(do_markdup))
+    total = packer.total(N, st)
+    t_sort_pack = time.perf_counter() - t1
+    t2 = time.perf_counter()
+    head = bam_header(sq_names, sq_lens, rg_id, sm, lb, pl, pg_vn, "coordinate")
+    if device_bgzf:
+        dz = DeviceBgzf(total, device)
+        dz.compress(st, packer.d_out.ptr, total, level, True)
+        body = dz.download(st)
+        with open(out_bam, "wb") as fh:
+            fh.write(bgzf_compress(head, level, eof=False))
+            fh.write(body)
+        size = len(body)
+    else:
+        stream_bytes = packer.download(N, st)
+        w = BamWriter(out_bam, sq_names, sq_lens, rg_id, sm, lb, pl, pg_vn, level, "coordinate")
+        w.write_bytes(stream_bytes)
+        size = w.close()
+    t_bgzf = time.perf_counter() - t2
+    dups = int(is_dup.download().sum()) if do_markdup else 0
+    mapped = int((fields["cid"].view(np.int32).download() >= 0).sum())
+    return {"reads": N, "mapped": mapped, "duplicates": dups, "bam_stream_bytes": total, "bgzf_bytes": size,
+            "ingest_and_map_s": round(t_ingest, 3), "map_kernel_ms": round(t_map, 3),
+            "sort_and_pack_s": round(t_sort_pack, 3), "bgzf_s": round(t_bgzf, 3)}
+
+
+class _recs_view:
+    """The accumulated records as the DeviceRecs interface BamPacker takes."""
+
+    def __init__(self, recs: Recs, n: int):
+        self.struct, self.n = recs, n
+
+
+__all__ = ["fastq_to_bam"]

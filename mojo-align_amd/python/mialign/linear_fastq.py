@@ -142,7 +142,7 @@ class DeviceFastqReader:
     early)" as in the reference)."""
 
     def __init__(self, path1: str, path2: str = "", bs_r1: str = "", bs_r2: str = "", batch_reads: int = 262144,
-                 stride: int = 256, chunk_bytes: int = 64 << 20, device: int = 0):
+                 stride: int = 256, chunk_bytes: int = 64 << 20, device: int = 0, with_conv: bool = True):
         self.stride = int(stride)
         self.batch_reads = int(batch_reads)
         self.sides = [_Side(path1, bs_r1, chunk_bytes, device)]
@@ -151,7 +151,8 @@ class DeviceFastqReader:
         self.stream = Stream(device)
         self.parser = FastqParser(2 * chunk_bytes + 16, self.batch_reads, device)
         names_cap = 2 * chunk_bytes
-        self.batches = [DeviceFastqBatch(self.batch_reads, self.stride, names_cap, device) for _ in self.sides]
+        self.batches = [DeviceFastqBatch(self.batch_reads, self.stride, names_cap, device, with_conv=with_conv)
+                        for _ in self.sides]
 
     def _parse_side(self, k: int, max_records: int) -> ParseInfo:
         """Parse up to max_records records of side k, reading more input until

@@ -209,3 +209,45 @@ def test_device_bgzf_members_decode_to_the_stream(level):
         assert dz.total(st) <= 150_000 + 3 * (18 + 5 + 8)
         dz.compress(st, DeviceArray.from_host(np.frombuffer(bam, np.uint8)).ptr, len(bam), level, False)
         assert dz.total(st) < 0.8 * len(bam)
+
+
+@pytest.mark.parametrize("device_bgzf", [True, False])
+def test_fastq_to_bam_end_to_end(tmp_path, device_bgzf):
+    """FASTQ file -> device parse -> map -> keys -> sort + markdup -> pack ->
+    BGZF: the BAM's records equal the oracle's pack of the oracle's map of the
+    same reads, gathered by the oracle's sort; the header names the canonical
+    contigs; host BGZF members equal zlib's."""
+    say(f"test_fastq_to_bam_end_to_end device_bgzf={device_bgzf}")
+    from mialign.fq2bam import fastq_to_bam
+    from mialign.linear_index import LinearIndex
+
+    ref = synth.gen_reference(71, 3, 400_000)
+    dual = synth.dual_convert(ref)
+    gix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21)
+    lengths = [int(dual.contig_off[i + 1] - dual.contig_off[i]) for i in range(len(dual.names))]
+    index = LinearIndex.from_gpu(gix, dual.names, lengths)
+    reads = synth.gen_reads(72, ref, 20_000, 60, 150, max_len=150, dup_frac=0.1)
+    blob, name_off, qual = synth.gen_names_quals(73, reads.lens, 150)
+    fq = tmp_path / "r1.fq"
+    fq.write_bytes(synth.write_fastq(reads.bases, qual, reads.lens, blob, name_off).tobytes())
+    out = tmp_path / "out.bam"
+    stats = fastq_to_bam(index, str(fq), str(out), "C2T", batch_reads=7000, device_bgzf=device_bgzf)
+    assert stats["reads"] == 20_000
+    # the oracle: map, keys, sort + markdup, pack
+    km, of, po = orc.build_dense(dual.seq, dual.contig_off, 21)
+    oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+    want, _, _, _ = orc.map_batch(oix, orc.convert_reads(reads.bases, 1), reads.lens, nthreads=8)
+    tid = synth.tid_of_cid(dual.names)
+    d, p, _ = orc.sort_markdup(*orc.sort_keys_se(want, reads.lens, tid))
+    stream = orc.bam_pack_se(want, tid, blob, name_off, reads.bases, reads.lens, qual, b"mojo1", perm=p,
+                             is_dup=d, apply_dup=True)
+    raw = out.read_bytes()
+    data = b"".join(pl for _, pl in bd.bgzf_members(raw))
+    assert data.startswith(b"BAM\x01")
+    (lt,) = np.frombuffer(data[4:8], np.int32)
+    text = data[8:8 + lt].decode()
+    assert "@HD\tVN:1.6\tSO:coordinate" in text and "@RG\tID:mojo1" in text
+    assert data.endswith(stream)
+    assert stats["duplicates"] == int(d.sum()) and stats["mapped"] == int((want["cid"] >= 0).sum())
+    if not device_bgzf:
+        assert raw == orc.bgzf_stream(data, 1)
