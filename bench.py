@@ -398,6 +398,45 @@ def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int,
     }
 
 
+def fq2bam_section(ix, dual, reads, blob, name_off, qual, n: int) -> dict:
+    """The whole product path on the first n reads: their FASTQ file (page
+    cache) -> fastq_to_bam (device parse, map, keys, sort + markdup, pack,
+    BGZF) -> a BAM file; wall time including the host's file reads and writes."""
+    import tempfile
+
+    from mialign import synth
+    from mialign.fq2bam import fastq_to_bam
+    from mialign.linear_index import LinearIndex
+
+    n = min(n, reads.lens.size)
+    d = tempfile.mkdtemp(prefix="mialign_fq2bam_")
+    fq, out = os.path.join(d, "r1.fq"), os.path.join(d, "out.bam")
+    try:
+        sub_off = name_off[: n + 1]
+        text = synth.write_fastq(reads.bases[:n], qual[:n], reads.lens[:n], blob, sub_off)
+        with open(fq, "wb") as fh:
+            fh.write(memoryview(text))
+        fq_bytes = int(text.size)
+        del text
+        lengths = [int(dual.contig_off[i + 1] - dual.contig_off[i]) for i in range(len(dual.names))]
+        index = LinearIndex.from_gpu(ix, dual.names, lengths)
+        fastq_to_bam(index, fq, out, "C2T", batch_reads=1 << 20)  # warm: allocations, file in page cache
+        t0 = time.perf_counter()
+        stats = fastq_to_bam(index, fq, out, "C2T", batch_reads=1 << 20)
+        wall = time.perf_counter() - t0
+        bam_bytes = os.path.getsize(out)
+    finally:
+        for f in (fq, out):
+            if os.path.exists(f):
+                os.remove(f)
+        os.rmdir(d)
+    log(f"fq2bam: {n} reads FASTQ ({fq_bytes / 1e9:.2f} GB) -> BAM ({bam_bytes / 1e9:.2f} GB) in {wall:.2f} s "
+        f"= {n / wall / 1e6:.1f}M reads/s ({stats})")
+    return {"workload": f"the first {n} reads as a FASTQ file -> coordinate-sorted, duplicate-marked BAM file",
+            "reads": n, "fastq_bytes": fq_bytes, "bam_bytes": bam_bytes, "wall_s": round(wall, 3),
+            "reads_per_s": round(n / wall, 1), "stages": stats}
+
+
 def _fastq_record_end(text: np.ndarray, m: int) -> int:
     """Byte offset just past the m-th record of 4-line FASTQ text."""
     if m <= 0:
@@ -694,6 +733,8 @@ def main() -> int:
     ap.add_argument("--no-bam", action="store_true", help="skip the BAM emit section")
     ap.add_argument("--no-fastq", action="store_true", help="skip the FASTQ ingest section")
     ap.add_argument("--fastq-cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--fq2bam-reads", type=int, default=4_000_000,
+                    help="reads through the whole FASTQ file -> BAM file path (0 = skip)")
     ap.add_argument("--bam-cpu-sample", type=int, default=2_000_000)
     ap.add_argument("--bam-parity-sample", type=int, default=1_000_000)
     args = ap.parse_args()
@@ -736,8 +777,8 @@ def main() -> int:
     merged = sr.recs.download()
     is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
-    bam = fastq = None
-    if not (args.no_bam and args.no_fastq):
+    bam = fastq = f2b = None
+    if not (args.no_bam and args.no_fastq) or args.fq2bam_reads > 0:
         blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
         if not args.no_bam:
             bam = bam_section(sr, reads, tid, tid.size, blob, name_off, qual, not args.no_cpu, args.bam_cpu_sample,
@@ -746,6 +787,8 @@ def main() -> int:
         if not args.no_fastq:
             fastq = fastq_section(reads, blob, name_off, qual, not args.no_cpu, args.fastq_cpu_sample,
                                   args.bam_parity_sample)
+        if args.fq2bam_reads > 0:
+            f2b = fq2bam_section(ix, dual, reads, blob, name_off, qual, args.fq2bam_reads)
         del blob, name_off, qual
     sr.close()
 
@@ -821,6 +864,7 @@ def main() -> int:
         "genome_scale": genome_scale,
         "bam_emit": bam,
         "fastq_ingest": fastq,
+        "fq2bam": f2b,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -145,11 +145,14 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_rea
     if device_bgzf:
         dz = DeviceBgzf(total, device)
         dz.compress(st, packer.d_out.ptr, total, level, True)
-        body = dz.download(st)
+        zt = dz.total(st)
+        body = np.empty(max(zt, 1), np.uint8)
+        check(lib().ma_memcpy_d2h(body.ctypes.data, dz.d_dst.ptr, zt, st.handle), "ma_memcpy_d2h")
+        st.sync()
         with open(out_bam, "wb") as fh:
             fh.write(bgzf_compress(head, level, eof=False))
-            fh.write(body)
-        size = len(body)
+            fh.write(memoryview(body)[:zt])
+        size = zt
     else:
         stream_bytes = packer.download(N, st)
         w = BamWriter(out_bam, sq_names, sq_lens, rg_id, sm, lb, pl, pg_vn, level, "coordinate")

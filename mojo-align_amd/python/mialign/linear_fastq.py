@@ -109,27 +109,47 @@ def _open_stream(path: str):
 
 
 class _Side:
-    """One FASTQ file: host chunks, a carried tail, a device staging buffer."""
+    """One FASTQ file: a host buffer (the unparsed tail carried at its front,
+    the next chunk read in after it; plain pages, since pinning a buffer this
+    size costs more than the copies it would speed up in a one-pass run) and a
+    device staging buffer."""
 
     def __init__(self, path: str, bs: str, chunk: int, device: int):
         if bs not in BS_CODES:
             raise MialignError(f"unknown conversion mode {bs!r} (C2T, G2A or none)")
+        self._alloc = lambda shape, dtype: np.empty(shape, dtype)
         self.fh = _open_stream(path)
         self.mode = BS_CODES[bs]
         self.chunk = int(chunk)
-        self.carry = b""
+        self.buf = self._alloc((2 * self.chunk,), np.uint8)
+        self.filled = 0
         self.eof = False
+        self.device = device
         self.dbuf = DeviceArray((2 * self.chunk + 16,), np.uint8, device)
 
-    def fill(self) -> bytes:
-        """Carried tail + the next chunk (the buffer the parser sees)."""
-        while not self.eof and len(self.carry) < self.chunk:
-            data = self.fh.read(self.chunk)
-            if not data:
+    def fill(self) -> int:
+        """Read until `chunk` bytes are buffered (or the end); returns the count."""
+        if self.buf.size < self.chunk:
+            nb = self._alloc((2 * self.chunk,), np.uint8)
+            nb[: self.filled] = self.buf[: self.filled]
+            self.buf = nb
+        mv = memoryview(self.buf)
+        while not self.eof and self.filled < self.chunk:
+            got = self.fh.readinto(mv[self.filled:])
+            if not got:
                 self.eof = True
                 break
-            self.carry += data
-        return self.carry
+            self.filled += got
+        if self.filled + 16 > self.dbuf.nbytes:
+            self.dbuf = DeviceArray((2 * self.filled + 16,), np.uint8, self.device)
+        return self.filled
+
+    def consume(self, k: int) -> None:
+        """Drop the first k bytes (the records just parsed)."""
+        rest = self.filled - k
+        if rest > 0:
+            self.buf[:rest] = self.buf[k:self.filled].copy()
+        self.filled = max(rest, 0)
 
     def close(self):
         self.fh.close()
@@ -142,8 +162,10 @@ class DeviceFastqReader:
     early)" as in the reference)."""
 
     def __init__(self, path1: str, path2: str = "", bs_r1: str = "", bs_r2: str = "", batch_reads: int = 262144,
-                 stride: int = 256, chunk_bytes: int = 64 << 20, device: int = 0, with_conv: bool = True):
+                 stride: int = 256, chunk_bytes: int = 0, device: int = 0, with_conv: bool = True):
         self.stride = int(stride)
+        if chunk_bytes <= 0:  # a batch's worth of text at ~1.75 bytes per base per read
+            chunk_bytes = max(1 << 20, int(batch_reads) * (2 * self.stride + 96))
         self.batch_reads = int(batch_reads)
         self.sides = [_Side(path1, bs_r1, chunk_bytes, device)]
         if path2:
@@ -159,17 +181,16 @@ class DeviceFastqReader:
         that many are complete or the file ends."""
         side, batch = self.sides[k], self.batches[k]
         while True:
-            buf = side.fill()
-            if len(buf) + 16 > side.dbuf.nbytes:
-                side.dbuf = DeviceArray((2 * len(buf) + 16,), np.uint8, side.dbuf.device)
-            if len(buf) > self.parser.max_bytes:
-                self.parser = FastqParser(2 * len(buf), self.batch_reads, side.dbuf.device)
-            if len(buf) > batch.names_cap:  # names never take more bytes than the buffer
-                batch.names_cap = 2 * len(buf)
-                batch.names = DeviceArray((batch.names_cap,), np.uint8, side.dbuf.device)
-            if buf:
-                check(lib().ma_memcpy_h2d(side.dbuf.ptr, buf, len(buf), None), "ma_memcpy_h2d")
-            info = self.parser.parse(self.stream, side.dbuf.ptr, len(buf), side.eof, batch, side.mode, max_records)
+            nbuf = side.fill()
+            if nbuf > self.parser.max_bytes:
+                self.parser = FastqParser(2 * nbuf, self.batch_reads, side.device)
+            if nbuf > batch.names_cap:  # names never take more bytes than the buffer
+                batch.names_cap = 2 * nbuf
+                batch.names = DeviceArray((batch.names_cap,), np.uint8, side.device)
+            if nbuf:
+                check(lib().ma_memcpy_h2d(side.dbuf.ptr, side.buf.ctypes.data, nbuf, self.stream.handle),
+                      "ma_memcpy_h2d")
+            info = self.parser.parse(self.stream, side.dbuf.ptr, nbuf, side.eof, batch, side.mode, max_records)
             if info.status in FQ_ERRORS:
                 raise MialignError(FQ_ERRORS[info.status])
             if info.status == 4:
@@ -189,10 +210,10 @@ class DeviceFastqReader:
                 info2 = self._parse_side(1, n)
                 if info2.records < n:
                     raise MialignError("paired FASTQ length mismatch (R2 ended early)")
-                self.sides[1].carry = self.sides[1].carry[info2.consumed:]
+                self.sides[1].consume(info2.consumed)
             else:
                 self.batches[1].n = 0
-        self.sides[0].carry = self.sides[0].carry[info1.consumed:]
+        self.sides[0].consume(info1.consumed)
         return self.batches[0], (self.batches[1] if len(self.sides) == 2 else None)
 
     def close(self):
