@@ -306,6 +306,12 @@ def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
     return blob[src], new_off
 
 
+# HBM bytes per 10M-record launch (150 bp reads, 100 Mbp workload) from FETCH_SIZE x 2 + WRITE_SIZE,
+# profiles/bam_emit_and_fastq_ingest_10Mx150.md
+BAM_TRAFFIC_PROFILED = {"reads": 10_000_000, "read_len": 150, "pack": 10.93e9, "bgzf_deflate": 43.9e9,
+                        "fastq": 23.5e9, "source": "profiles/bam_emit_and_fastq_ingest_10Mx150.md"}
+
+
 def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int, parity_records: int) -> dict:
     """FASTQ ingest of the step's reads: their FASTQ text (names, bases, '+',
     qualities) resident in HBM, parsed on the device into base, C2T-converted
@@ -391,7 +397,8 @@ def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int,
         "GBps_text": round(nbytes / (dev_ms / 1000.0) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "fq_* (count, lines, measure, copy)",
-                     "algorithmic_bytes_per_read": round(alg / n, 1)},
+                     "algorithmic_bytes_per_read": round(alg / n, 1),
+                     "traffic": _profiled(n, L, "fastq")},
         "pcie_inclusive_records_per_s": round(n / pcie_s, 1),
         "parity": {"all_records_round_trip": ok_all, "oracle_records": m, "oracle_equal": ok_oracle},
         "cpu_baseline": cpu,
@@ -435,6 +442,12 @@ def fq2bam_section(ix, dual, reads, blob, name_off, qual, n: int) -> dict:
     return {"workload": f"the first {n} reads as a FASTQ file -> coordinate-sorted, duplicate-marked BAM file",
             "reads": n, "fastq_bytes": fq_bytes, "bam_bytes": bam_bytes, "wall_s": round(wall, 3),
             "reads_per_s": round(n / wall, 1), "stages": stats}
+
+
+def _profiled(n: int, read_len: int, what: str):
+    """Profiled HBM bytes per launch when the run is the profiled workload, else None."""
+    tp = BAM_TRAFFIC_PROFILED
+    return tp[what] if (tp["reads"], tp["read_len"]) == (n, read_len) else None
 
 
 def _fastq_record_end(text: np.ndarray, m: int) -> int:
@@ -601,13 +614,15 @@ def bam_section(sr, reads, tid, n_tid: int, blob, name_off, qual, with_cpu: bool
         "pack_ms": round(dev_ms, 3),
         "pack_records_per_s": round(n / (dev_ms / 1000.0), 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "bam_pack_kernel (+ size, scan)",
-                     "algorithmic_bytes_per_read": round((total + in_bytes) / n, 1)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "bam_pack_kernel (+ layout, sizes, scan)",
+                     "algorithmic_bytes_per_read": round((total + in_bytes) / n, 1),
+                     "traffic": _profiled(n, L, "pack")},
         "emit_s": round(emit_dev_s, 3),
         "emit_reads_per_s": round(n / emit_dev_s, 1),
         "emit": "pack + BGZF on the device + copy of the members to pinned host memory + write (/dev/null)",
         "bgzf_device": {"ms": round(dz_ms, 3), "ratio": round(gz_total / total, 4),
                         "GBps_in": round(total / (dz_ms / 1000.0) / 1e9, 1),
+                        "traffic": _profiled(n, L, "bgzf_deflate"),
                         "first_members_inflate_to_stream": bool(dev_ok)},
         "emit_host_bgzf_s": round(emit_s, 3),
         "emit_host_bgzf_reads_per_s": round(n / emit_s, 1),

@@ -270,14 +270,21 @@ __global__ void fq_finish_kernel(uint64_t *__restrict__ info, const uint64_t *__
     info[4] = info[0] ? (uint64_t)name_off[info[0]] : 0;
 }
 
-__global__ void fq_maxlen_kernel(const uint32_t *__restrict__ lens, uint64_t *__restrict__ info) {
+// Longest read: a block-wide maximum, then one atomic per block (a small grid).
+__global__ __launch_bounds__(256) void fq_maxlen_kernel(const uint32_t *__restrict__ lens, uint64_t *__restrict__ info) {
+    __shared__ uint32_t sh[4];
     const int64_t recs = (int64_t)info[0];
     uint32_t m = 0;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < recs; r += (int64_t)gridDim.x * blockDim.x)
         m = max(m, lens[r]);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = max(m, (uint32_t)__shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && m) atomicMax((unsigned long long *)&info[2], (unsigned long long)m);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+        if (m) atomicMax((unsigned long long *)&info[2], (unsigned long long)m);
+    }
 }
 
 struct FqScratch {
@@ -355,7 +362,7 @@ int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32
                               (uint64_t *)d_name_off + max_records));
     fq_finish_kernel<<<1, 64, 0, st>>>(d_info, nl, d_name_off, names_cap, nbytes);
     MA_LAUNCHED();
-    fq_maxlen_kernel<<<g, 256, 0, st>>>(d_lens, d_info);
+    fq_maxlen_kernel<<<g < 1024u ? g : 1024u, 256, 0, st>>>(d_lens, d_info);
     MA_LAUNCHED();
     const int64_t blocks = (max_records + 3) / 4 < 65536 ? (max_records + 3) / 4 : 65536;
     fq_copy_kernel<<<(unsigned)blocks, 256, 0, st>>>(d_buf, nl, stride, bs_mode, d_info, name_len, name_src,
