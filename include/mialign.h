@@ -17,6 +17,7 @@
  *                             linear_gpu_locate.mojo:1836-2720 (pack, FW/RC
  *                             encode+locate+vote_extend passes, FW/RC merge :2101-2113)
  *   ma_sort_keys_se_dev ..... SE key builders, linear_gpu_fm.mojo:371-405, 800-823
+ *   ma_sort_keys_pe_dev ..... PE key builders, linear_gpu_fm.mojo:936-977
  *   ma_sort_markdup(_dev) ... gpu_sort_markdup, fq2bam-meth/src/linear_gpu_sort.mojo:24-41
  *   ma_seed_kmers ........... seed_kmers_on_device (pack + k-mer + Wang hash),
  *                             gpu-common/src/gpu_kernels.mojo:81-281
@@ -26,6 +27,8 @@
  *   ma_bam_pack_se_dev ...... _pack_bam_aln over the SE branch of _emit_bam_from_batch
  *                             + _gather_perm_to_writer / _or_dup_flag,
  *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 755-800, 177-225, 408-420
+ *   ma_bam_pack_pe_dev ...... _pack_bam_aln over the PE branch of _emit_bam_from_batch
+ *                             (flags, mate fields, tlen), linear_gpu_fm.mojo:826-934
  *   ma_bam_pack_dev ......... _pack_bam_aln from explicit per-record fields (:497-665)
  *   ma_bgzf_* ............... _bgzf_block / _BgzfWriter / BamWriter.write_raw (host, zlib;
  *                             ma_bgzf_compress_dev: the same members compressed on the GPU),
@@ -200,6 +203,15 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
 int ma_sort_keys_se_dev(void *stream, int64_t n, const ma_recs *d_merged, const uint32_t *d_lens,
                         const int32_t *d_tid_of_cid, int64_t pair_base, uint64_t *d_coord,
                         uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair);
+/* PE keys of n_pairs pairs (device pointers): the merged records and lens hold
+ * R1 block then R2 block (2 n_pairs each); the 2 n_pairs keys come out in
+ * emission order, pair p's R1 at 2p and R2 at 2p + 1. coord per record; dhi /
+ * dlo (the pair's unclipped 5' ends, ordered), score ((mapq1 + mapq2) << 8 |
+ * 255 - min(nm1 + nm2, 255)) and pair ((pair_base + 2p) / 2) shared by the pair.
+ * Index contigs outside [0, n_tid) are unmapped. */
+int ma_sort_keys_pe_dev(void *stream, int64_t n_pairs, const ma_recs *d_merged, const uint32_t *d_lens,
+                        const int32_t *d_tid_of_cid, int32_t n_tid, int64_t pair_base, uint64_t *d_coord,
+                        uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair);
 /* gpu_sort_markdup contract: is_dup (original order), perm (stable coord
  * order), dup_perm (stable (dhi, dlo) order, identity when !do_markdup; may be
  * NULL). Host pointers. */
@@ -252,6 +264,18 @@ int64_t ma_bam_record_bound(int32_t name_len, int32_t len, int32_t rg_n);
  * (the caller compares d_rec_off[n] with out_cap). d_out 4-byte aligned;
  * rg_n <= 240. Enqueued on stream. */
 int ma_bam_pack_se_dev(void *stream, int64_t n, const ma_recs *d_merged, const int32_t *d_tid_of_cid,
+                       int32_t n_tid, const ma_bam_reads *reads, const char *rg, int32_t rg_n,
+                       const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
+                       uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,
+                       int64_t scratch_bytes);
+/* Paired-end records of n_pairs pairs: merged records and reads R1 block then
+ * R2 block (2 n_pairs each); the 2 n_pairs records in emission order (pair p's
+ * R1 at 2p, R2 at 2p + 1) are what d_perm / d_is_dup index. Fields as the
+ * reference's PE batch emit: flag | 0x41 (R1) or | 0x81 (R2), 0x8 when the mate
+ * has no target, both on one target: 0x2, mate tid / pos and tlen = |pos2 -
+ * pos1| + len(R2), positive on the leftmost mate; else a mapped mate's tid /
+ * pos in the other's mate fields. Otherwise as ma_bam_pack_se_dev. */
+int ma_bam_pack_pe_dev(void *stream, int64_t n_pairs, const ma_recs *d_merged, const int32_t *d_tid_of_cid,
                        int32_t n_tid, const ma_bam_reads *reads, const char *rg, int32_t rg_n,
                        const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
                        uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,

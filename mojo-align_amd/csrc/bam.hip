@@ -22,6 +22,7 @@
 #include <thread>
 #include <vector>
 
+#include "ma_pe.hpp"
 #include "ma_sort.hpp"
 
 using namespace ma;
@@ -41,9 +42,14 @@ struct RgArg {
 // Where a record's fields come from: SE mode derives them from merged map
 // records (+ contig -> target table); otherwise ten int32 arrays (the last
 // three may be null: mate tid -1, mate pos -1, tlen 0).
+// se: 0 = explicit field arrays, 1 = SE merged records, 2 = PE merged records
+// (2 np of them, R1 block then R2 block; output-order record i is mate i & 1 of
+// pair i >> 1 and reads row (i & 1) ? np + (i >> 1) : i >> 1).
 struct FieldSrc {
     int32_t se;
     int32_t n_tid;
+    int64_t np;
+    const uint32_t *lens;
     const int32_t *cid;
     const uint32_t *pos, *mapq;
     const int32_t *rflag;
@@ -66,9 +72,29 @@ struct RecF {
     int64_t flag, tid, pos, mapq, sl, sr, nm, ntid, npos, tlen;
 };
 
+__device__ __forceinline__ int64_t src_row(const FieldSrc &s, int64_t i) {
+    return s.se == 2 ? ((i & 1) ? s.np + (i >> 1) : (i >> 1)) : i;
+}
+
 __device__ __forceinline__ RecF load_fields(const FieldSrc &s, int64_t i) {
     RecF r;
-    if (s.se) {
+    if (s.se == 2) {
+        // _emit_bam_from_batch PE (linear_gpu_fm.mojo:826-870)
+        PeMate m[2];
+        pe_pair(s.np, i >> 1, s.cid, s.pos, s.rflag, s.lens, s.tid_of_cid, s.n_tid, m);
+        const PeMate &q = m[i & 1];
+        const int64_t row = src_row(s, i);
+        r.flag = q.flag;
+        r.tid = q.tid;
+        r.pos = q.pos;
+        r.mapq = s.mapq[row];
+        r.sl = s.sl[row];
+        r.sr = s.sr[row];
+        r.nm = s.nm[row];
+        r.ntid = q.ntid;
+        r.npos = q.npos;
+        r.tlen = q.tlen;
+    } else if (s.se) {
         // _emit_bam_from_batch SE: flag | 1 | 64; unknown contig -> tid -1, flag | 4.
         const int32_t c = s.cid[i];
         r.flag = (int64_t)s.rflag[i] | 1 | 64;
@@ -265,17 +291,18 @@ __global__ void bam_layout_kernel(int64_t n, FieldSrc fs, ReadSrc rs, int rg_n, 
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         RecF f = load_fields(fs, i);
         if (apply_dup && is_dup[i] != 0) f.flag |= 1024;
-        const int64_t qlen = rs.lens[i];
-        const int64_t ql = rs.qual_lens ? rs.qual_lens[i] : qlen;
-        const int64_t nb = rs.name_off[i];
-        const RecLay L = lay_out(f, rs.name_off[i + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
+        const int64_t row = src_row(fs, i);
+        const int64_t qlen = rs.lens[row];
+        const int64_t ql = rs.qual_lens ? rs.qual_lens[row] : qlen;
+        const int64_t nb = rs.name_off[row];
+        const RecLay L = lay_out(f, rs.name_off[row + 1] - nb, qlen, ql, rs.qual != nullptr, rg_n);
         RecLayout o;
 #pragma unroll
         for (int k = 0; k < 9; k++) o.w[k] = L.hdr[k];
 #pragma unroll
         for (int k = 0; k < 3; k++) o.w[9 + k] = L.cig[k];
         o.name = (uint64_t)nb;
-        o.row = (uint32_t)i;
+        o.row = (uint32_t)row;
         o.flags = (uint32_t)L.name_len | (uint32_t)L.rev << 16 | (uint32_t)L.has_qual << 17 |
                   (uint32_t)((L.o_seq - L.o_cig) / 4) << 18 | (uint32_t)L.nm8 << 20;
         const uint32_t j = inv ? inv[i] : (uint32_t)i;
@@ -615,6 +642,31 @@ int ma_bam_pack_se_dev(void *stream, int64_t n, const ma_recs *m, const int32_t 
     fs.tid_of_cid = d_tid_of_cid;
     return bam_pack(stream, n, fs, r, rg, rg_n, d_perm, d_is_dup, apply_dup, d_out, out_cap, d_rec_off,
                     d_scratch, scratch_bytes, "ma_bam_pack_se_dev");
+}
+
+int ma_bam_pack_pe_dev(void *stream, int64_t n_pairs, const ma_recs *m, const int32_t *d_tid_of_cid,
+                       int32_t n_tid, const ma_bam_reads *r, const char *rg, int32_t rg_n,
+                       const uint32_t *d_perm, const uint32_t *d_is_dup, int32_t apply_dup,
+                       uint8_t *d_out, int64_t out_cap, uint64_t *d_rec_off, void *d_scratch,
+                       int64_t scratch_bytes) {
+    MA_REQUIRE(m && r && n_pairs >= 0 && (n_tid == 0 || d_tid_of_cid) && n_tid >= 0,
+               "ma_bam_pack_pe_dev: bad arguments");
+    MA_REQUIRE(n_pairs == 0 || r->lens, "ma_bam_pack_pe_dev: null read lengths");
+    FieldSrc fs{};
+    fs.se = 2;
+    fs.n_tid = n_tid;
+    fs.np = n_pairs;
+    fs.lens = r->lens;
+    fs.cid = m->cid;
+    fs.pos = m->pos;
+    fs.mapq = m->mapq;
+    fs.rflag = m->flag;
+    fs.sl = m->sl;
+    fs.sr = m->sr;
+    fs.nm = m->nm;
+    fs.tid_of_cid = d_tid_of_cid;
+    return bam_pack(stream, 2 * n_pairs, fs, r, rg, rg_n, d_perm, d_is_dup, apply_dup, d_out, out_cap, d_rec_off,
+                    d_scratch, scratch_bytes, "ma_bam_pack_pe_dev");
 }
 
 int64_t ma_bgzf_bound(int64_t n) {

@@ -1,4 +1,4 @@
-// Sort keys (SE), stable coordinate sort and Picard-style duplicate marking.
+// Sort keys (SE and PE), stable coordinate sort and Picard-style duplicate marking.
 //
 // Semantics follow gpu_sort_markdup (fq2bam-meth/src/linear_gpu_sort.mojo:24-551):
 //   dup order  = stable sort by (dhi, dlo)   (LSD: by dlo, then by dhi)
@@ -9,6 +9,7 @@
 // Design: everything stays on the device; the group best is one 64-bit
 // atomicMax per record on (score << 32 | ~pair) keyed by the group index from
 // an exclusive scan of head flags (no host scan, no host merge).
+#include "ma_pe.hpp"
 #include "ma_sort.hpp"
 
 using namespace ma;
@@ -87,8 +88,30 @@ __global__ void md_mark_kernel(const uint64_t *__restrict__ dhi_s, const uint32_
     }
 }
 
-// SE sort keys; follows _unclipped5 / _coord_key / _pack_end and the SE
-// builder in linear_gpu_fm.mojo:371-405, 800-823.
+// _unclipped5 (linear_gpu_fm.mojo:371-387): the 5' end on the reference,
+// clips counted in; 0 for an unmapped record or an empty read.
+__device__ __forceinline__ int64_t unclipped5(int64_t fl, int64_t p0, int64_t a, int64_t b, int64_t ql) {
+    if ((fl & 4) != 0 || ql <= 0) return 0;
+    if (a + b >= ql) a = b = 0;
+    const int64_t mid = ql - a - b;
+    return (fl & 16) ? p0 + mid + a - 1 : p0 - a;
+}
+
+// _pack_end (:398-405): (tid + 1) << 48 | strand << 47 | (u5 + 2^30) in 47 bits.
+__device__ __forceinline__ uint64_t pack_end(int64_t tid, int64_t u5, int64_t strand) {
+    int64_t u = u5 + 1073741824;
+    if (u < 0) u = 0;
+    return ((uint64_t)(tid >= 0 ? tid + 1 : 0) << 48) | ((uint64_t)(strand & 1) << 47) |
+           ((uint64_t)u & 140737488355327ull);
+}
+
+// _coord_key (:390-396): (tid << 32 | pos), unmapped (0xFFFFFFFF << 32 | orig).
+__device__ __forceinline__ uint64_t coord_key(int64_t tid, int64_t p0, int64_t orig) {
+    if (tid < 0) return (0xFFFFFFFFull << 32) | ((uint64_t)orig & 0xFFFFFFFFull);
+    return ((uint64_t)tid << 32) | (uint64_t)(p0 < 0 ? 0 : p0);
+}
+
+// SE sort keys; follows the SE builder in linear_gpu_fm.mojo:800-823.
 __global__ void sort_keys_se_kernel(int64_t n, const int32_t *__restrict__ cid,
                                     const uint32_t *__restrict__ pos, const int32_t *__restrict__ flag,
                                     const uint32_t *__restrict__ sl, const uint32_t *__restrict__ sr,
@@ -104,29 +127,69 @@ __global__ void sort_keys_se_kernel(int64_t n, const int32_t *__restrict__ cid,
         int64_t tid = c >= 0 ? (int64_t)tid_of_cid[c] : -1;
         int64_t fl = flag[i];
         int64_t p0 = (int64_t)pos[i];
+        coord[i] = coord_key(tid, p0, i);
         if (tid < 0) {
-            coord[i] = (0xFFFFFFFFull << 32) | ((uint64_t)i & 0xFFFFFFFFull);
             dhi[i] = ~0ull;
             dlo[i] = (uint64_t)i;
         } else {
-            coord[i] = ((uint64_t)tid << 32) | (uint64_t)(p0 < 0 ? 0 : p0);
-            int64_t u5 = 0;
-            int64_t ql = (int64_t)lens[i];
-            if ((fl & 4) == 0 && ql > 0) {
-                int64_t a = sl[i], b = sr[i];
-                if (a + b >= ql) a = b = 0;
-                int64_t mid = ql - a - b;
-                u5 = (fl & 16) ? p0 + mid + a - 1 : p0 - a;
-            }
-            int64_t u = u5 + 1073741824;
-            if (u < 0) u = 0;
-            dhi[i] = ((uint64_t)(tid + 1) << 48) | ((uint64_t)((fl >> 4) & 1) << 47) |
-                     ((uint64_t)u & 140737488355327ull);
+            dhi[i] = pack_end(tid, unclipped5(fl, p0, sl[i], sr[i], lens[i]), (fl >> 4) & 1);
             dlo[i] = 0;
         }
         uint32_t nmc = nm[i] > 255u ? 255u : nm[i];
         score[i] = (mapq[i] << 8) | (255u - nmc);
         pairo[i] = (uint32_t)(pair_base + i);
+    }
+}
+
+// PE sort keys, a thread per pair writing output records 2p (R1) and 2p + 1
+// (R2); follows the PE builder in linear_gpu_fm.mojo:936-977: coord per record,
+// the pair's two ends ordered into (dhi, dlo) (one end when a mate has no
+// target; ~0 and the R1 index when neither has), score (mapq1 + mapq2) << 8 |
+// 255 - min(nm1 + nm2, 255) and pair id (pair_base + 2p) / 2 on both records.
+__global__ void sort_keys_pe_kernel(int64_t np, const int32_t *__restrict__ cid,
+                                    const uint32_t *__restrict__ pos, const int32_t *__restrict__ flag,
+                                    const uint32_t *__restrict__ sl, const uint32_t *__restrict__ sr,
+                                    const uint32_t *__restrict__ nm, const uint32_t *__restrict__ mapq,
+                                    const uint32_t *__restrict__ lens,
+                                    const int32_t *__restrict__ tid_of_cid, int64_t n_tid, int64_t pair_base,
+                                    uint64_t *__restrict__ coord, uint64_t *__restrict__ dhi,
+                                    uint64_t *__restrict__ dlo, uint32_t *__restrict__ score,
+                                    uint32_t *__restrict__ pairo) {
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < np;
+         p += (int64_t)gridDim.x * blockDim.x) {
+        PeMate m[2];
+        pe_pair(np, p, cid, pos, flag, lens, tid_of_cid, n_tid, m);
+        const int64_t r1 = p, r2 = np + p;
+        uint64_t h = ~0ull, l = (uint64_t)(2 * p);
+        if (m[0].tid >= 0 || m[1].tid >= 0) {
+            const uint64_t e1 = pack_end(m[0].tid, unclipped5(m[0].flag, m[0].pos, sl[r1], sr[r1], lens[r1]),
+                                         (m[0].flag >> 4) & 1);
+            const uint64_t e2 = pack_end(m[1].tid, unclipped5(m[1].flag, m[1].pos, sl[r2], sr[r2], lens[r2]),
+                                         (m[1].flag >> 4) & 1);
+            if (m[0].tid < 0) {
+                h = e2;
+                l = 0;
+            } else if (m[1].tid < 0) {
+                h = e1;
+                l = 0;
+            } else {
+                h = e1 <= e2 ? e1 : e2;
+                l = e1 <= e2 ? e2 : e1;
+            }
+        }
+        const uint64_t nms = (uint64_t)nm[r1] + nm[r2];
+        const uint32_t sc =
+            (uint32_t)((((uint64_t)mapq[r1] + mapq[r2]) << 8) | (255u - (nms > 255 ? 255u : (uint32_t)nms)));
+        const uint32_t pid = (uint32_t)((pair_base + 2 * p) / 2);
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int64_t i = 2 * p + k;
+            coord[i] = coord_key(m[k].tid, m[k].pos, i);
+            dhi[i] = h;
+            dlo[i] = l;
+            score[i] = sc;
+            pairo[i] = pid;
+        }
     }
 }
 
@@ -192,6 +255,19 @@ int ma_sort_keys_se_dev(void *stream, int64_t n, const ma_recs *m, const uint32_
     if (n == 0) return MA_OK;
     sort_keys_se_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(
         n, m->cid, m->pos, m->flag, m->sl, m->sr, m->nm, m->mapq, d_lens, d_tid_of_cid, pair_base,
+        d_coord, d_dhi, d_dlo, d_score, d_pair);
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
+int ma_sort_keys_pe_dev(void *stream, int64_t n_pairs, const ma_recs *m, const uint32_t *d_lens,
+                        const int32_t *d_tid_of_cid, int32_t n_tid, int64_t pair_base, uint64_t *d_coord,
+                        uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair) {
+    MA_REQUIRE(n_pairs >= 0 && m && n_tid >= 0, "ma_sort_keys_pe_dev: bad arguments");
+    MA_REQUIRE(2 * n_pairs <= 0xFFFFFFFFll, "ma_sort_keys_pe_dev: records exceed u32 ids");
+    if (n_pairs == 0) return MA_OK;
+    sort_keys_pe_kernel<<<grid_for(n_pairs), 256, 0, (hipStream_t)stream>>>(
+        n_pairs, m->cid, m->pos, m->flag, m->sl, m->sr, m->nm, m->mapq, d_lens, d_tid_of_cid, n_tid, pair_base,
         d_coord, d_dhi, d_dlo, d_score, d_pair);
     MA_LAUNCHED();
     return MA_OK;

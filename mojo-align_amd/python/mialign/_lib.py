@@ -147,12 +147,15 @@ SIGNATURES = {
     "ma_sort_keys_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I64, P, P, P, P, P]),
     "ma_sort_markdup": (C.c_int, [C.c_int, I64, P, P, P, P, P, P, P, I32, P]),
     "ma_sort_scratch_bytes": (I64, [I64]),
+    "ma_sort_keys_pe_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
     "ma_sort_markdup_dev": (C.c_int, [P, I64, P, P, P, P, P, P, P, I32, P, P, I64]),
     "ma_radix_sort_u64_dev": (C.c_int, [P, I64, P, P, P, I64]),
     "ma_seed_kmers": (C.c_int, [C.c_int, P, I64, I32, I32, P]),
     "ma_bam_scratch_bytes": (I64, [I64]),
     "ma_bam_record_bound": (I64, [I32, I32, I32]),
     "ma_bam_pack_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, I32, C.POINTER(BamReads), C.c_char_p,
+                                     I32, P, P, I32, P, I64, P, P, I64]),
+    "ma_bam_pack_pe_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, I32, C.POINTER(BamReads), C.c_char_p,
                                      I32, P, P, I32, P, I64, P, P, I64]),
     "ma_bam_pack_dev": (C.c_int, [P, I64, C.POINTER(BamFields), C.POINTER(BamReads), C.c_char_p, I32, P,
                                   P, I32, P, I64, P, P, I64]),

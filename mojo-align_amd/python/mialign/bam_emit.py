@@ -283,6 +283,22 @@ class BamPacker:
                                        self.out_cap, self.d_rec_off.ptr, self.d_scratch.ptr, self.scratch_bytes),
               "ma_bam_pack_se_dev")
 
+    def pack_pe(self, stream, n_pairs: int, recs, d_tid_of_cid: DeviceArray, n_tid: int, reads: DeviceReads,
+                rg: bytes, d_perm: Optional[DeviceArray] = None, d_is_dup: Optional[DeviceArray] = None,
+                apply_dup: bool = False) -> None:
+        """Enqueue the pack of n_pairs pairs, 2 n_pairs records (ma_bam_pack_pe_dev):
+        records and reads R1 block then R2 block, records out R1, R2 per pair."""
+        if 2 * n_pairs > self.cap_reads:
+            raise MialignError("BamPacker: batch larger than its capacity")
+        if len(rg) > RG_MAX:
+            raise MialignError(f"read-group id longer than {RG_MAX} bytes")
+        h = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().ma_bam_pack_pe_dev(h, n_pairs, C.byref(recs.struct), d_tid_of_cid.ptr, int(n_tid),
+                                       C.byref(reads.struct), rg, len(rg), d_perm.ptr if d_perm else None,
+                                       d_is_dup.ptr if d_is_dup else None, int(bool(apply_dup)), self.d_out.ptr,
+                                       self.out_cap, self.d_rec_off.ptr, self.d_scratch.ptr, self.scratch_bytes),
+              "ma_bam_pack_pe_dev")
+
     def pack_fields(self, stream, n: int, fields: dict, reads: DeviceReads, rg: bytes,
                     d_perm: Optional[DeviceArray] = None, d_is_dup: Optional[DeviceArray] = None,
                     apply_dup: bool = False) -> None:

@@ -1,4 +1,4 @@
-"""FASTQ -> coordinate-sorted, duplicate-marked BAM with every stage on the GPU.
+"""FASTQ (SE or PE) -> coordinate-sorted, duplicate-marked BAM with every stage on the GPU.
 
 The end-to-end product of the reference's native-BAM path
 (fq2bam-meth/src/linear_gpu_fm.mojo: FASTQ arena -> GPU map -> _emit_bam_from_batch
@@ -8,7 +8,8 @@ the device: ma_fastq_parse_dev, ma_map_batch_dev, ma_sort_keys_se_dev,
 ma_sort_markdup_dev, ma_bam_pack_se_dev and ma_bgzf_compress_dev (or the host
 zlib writer, whose members are the reference's byte for byte). The reads,
 records and keys of the whole run stay in HBM (an MI355X holds ~700M reads of
-150 bp this way); single-end reads only.
+150 bp this way). Pairs follow the PE branch of the same emit (:826-977):
+ma_sort_keys_pe_dev and ma_bam_pack_pe_dev.
 """
 
 from __future__ import annotations
@@ -54,57 +55,93 @@ class _Grow:
         return v
 
 
-def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_reads: int = 1 << 20,
-                 stride: int = 256, do_markdup: bool = True, device_bgzf: bool = True, level: int = 1,
-                 rg_id: str = "mojo1", sm: str = "sample", lb: str = "lib1", pl: str = "ILLUMINA",
+class _Acc:
+    """The reads and merged records of one FASTQ side, appended batch by batch in HBM."""
+
+    FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
+
+    def __init__(self, batch_reads: int, stride: int, device: int):
+        self.rows, self.quals = _Grow(stride, batch_reads, device), _Grow(stride, batch_reads, device)
+        self.lens, self.qlens = _Grow(4, batch_reads, device), _Grow(4, batch_reads, device)
+        self.names = _Grow(1, 32 * batch_reads, device)
+        self.fields = {f: _Grow(4, batch_reads, device) for f in self.FIELDS}
+        self.name_off = [np.zeros(1, np.int64)]
+
+    def append(self, b, recs_b: DeviceRecs, st) -> None:
+        n = b.n
+        base = self.names.n
+        self.rows.append_dev(b.rows.ptr, n, st)
+        self.quals.append_dev(b.qual.ptr, n, st)
+        self.lens.append_dev(b.lens.ptr, n, st)
+        self.qlens.append_dev(b.qual_lens.ptr, n, st)
+        off = b.name_off.download()[: n + 1]
+        self.names.append_dev(b.names.ptr, int(off[n]), st)
+        self.name_off.append(off[1:] + base)
+        for f, g in self.fields.items():
+            g.append_dev(recs_b.field_ptr(f), n, st)
+
+    def extend(self, other: "_Acc", st) -> None:
+        """Append another side's reads and records after these (R2 after R1)."""
+        base = self.names.n
+        for a, b in ((self.rows, other.rows), (self.quals, other.quals), (self.lens, other.lens),
+                     (self.qlens, other.qlens), (self.names, other.names)):
+            a.append_dev(b.buf.ptr, b.n, st)
+        for f in self.FIELDS:
+            self.fields[f].append_dev(other.fields[f].buf.ptr, other.fields[f].n, st)
+        self.name_off.append(np.concatenate(other.name_off)[1:] + base)
+
+
+def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str = "", bs_r2: str = "G2A",
+                 batch_reads: int = 1 << 20, stride: int = 256, do_markdup: bool = True, device_bgzf: bool = True,
+                 level: int = 1, rg_id: str = "mojo1", sm: str = "sample", lb: str = "lib1", pl: str = "ILLUMINA",
                  pg_vn: str = "0.1.0-mojo-fm", device: int = 0) -> dict:
-    """Single-end FASTQ -> BAM (coordinate order, 0x400 on duplicates). `index`
-    is a LinearIndex with its GPU index loaded. Returns counts and stage times."""
-    if bs_r1 not in BS_MODES:
-        raise MialignError(f"unknown conversion mode {bs_r1!r} (C2T, G2A or none)")
+    """FASTQ (single-end, or paired with fq2) -> BAM in coordinate order, 0x400
+    on duplicates. `index` is a LinearIndex with its GPU index loaded. Pairs are
+    mapped R1 with bs_r1 and R2 with bs_r2, emitted R1, R2 per pair with the PE
+    flags and mate fields, and marked as pairs. Returns counts and stage times."""
+    paired = bool(fq2)
+    for m in (bs_r1, bs_r2) if paired else (bs_r1,):
+        if m not in BS_MODES:
+            raise MialignError(f"unknown conversion mode {m!r} (C2T, G2A or none)")
     if stride > 256:
         raise MialignError("reads longer than 256 bases are not supported")
     gix = index.gpu
     st = Stream(device)
     t0 = time.perf_counter()
-    reader = DeviceFastqReader(fq1, "", bs_r1, batch_reads=batch_reads, stride=stride, device=device,
-                               with_conv=False)
+    reader = DeviceFastqReader(fq1, fq2, bs_r1, bs_r2 if paired else "", batch_reads=batch_reads, stride=stride,
+                               device=device, with_conv=False)
     ws = Workspace(gix, batch_reads, stride)
     prm = default_params(k=gix.k)
-    recs_b = DeviceRecs(batch_reads, device)
-    rows, quals = _Grow(stride, batch_reads, device), _Grow(stride, batch_reads, device)
-    lens, qlens = _Grow(4, batch_reads, device), _Grow(4, batch_reads, device)
-    names = _Grow(1, 32 * batch_reads, device)
-    fields = {f: _Grow(4, batch_reads, device) for f in ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")}
-    name_off = [np.zeros(1, np.int64)]
+    sides = [_Acc(batch_reads, stride, device) for _ in range(2 if paired else 1)]
+    recs_b = [DeviceRecs(batch_reads, device) for _ in sides]
+    modes = [BS_MODES[bs_r1]] + ([BS_MODES[bs_r2]] if paired else [])
     t_map = 0.0
     ev0, ev1 = Event(), Event()
     try:
         while True:
-            b, _ = reader.read_batch()
-            n = b.n
+            batches = reader.read_batch()
+            n = batches[0].n
             if n == 0:
                 break
-            max_len = min(stride, max(b.max_len, gix.k))
             ev0.record(st)
-            map_batch_dev(gix, ws, st, b.rows, stride, b.lens, n, max_len, recs_b, encoding=MA_ENC_ASCII,
-                          bs_r1=BS_MODES[bs_r1], params=prm)
+            for k in range(len(sides)):
+                b = batches[k]
+                max_len = min(stride, max(b.max_len, gix.k))
+                map_batch_dev(gix, ws, st, b.rows, stride, b.lens, n, max_len, recs_b[k], encoding=MA_ENC_ASCII,
+                              bs_r1=modes[k], params=prm)
             ev1.record(st)
-            base_names = names.n
-            rows.append_dev(b.rows.ptr, n, st)
-            quals.append_dev(b.qual.ptr, n, st)
-            lens.append_dev(b.lens.ptr, n, st)
-            qlens.append_dev(b.qual_lens.ptr, n, st)
-            off = b.name_off.download()[: n + 1]
-            names.append_dev(b.names.ptr, int(off[n]), st)
-            name_off.append(off[1:] + base_names)
-            for f, g in fields.items():
-                g.append_dev(recs_b.field_ptr(f), n, st)
+            for k in range(len(sides)):
+                sides[k].append(batches[k], recs_b[k], st)
             st.sync()  # the reader reuses its batch buffers
             t_map += ev0.elapsed_ms(ev1)
     finally:
         reader.close()
-    N = lens.n
+    acc = sides[0]
+    if paired:
+        acc.extend(sides[1], st)
+        st.sync()
+        del sides[1:]
+    N = acc.lens.n
     t_ingest = time.perf_counter() - t0
     if N == 0:
         raise MialignError("no reads in " + fq1)
@@ -115,13 +152,17 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_rea
     d_tid = DeviceArray.from_host(tid, device)
     recs = Recs()
     for f in REC_FIELDS:
-        setattr(recs, f, fields[f].buf.ptr if f in fields else None)
+        setattr(recs, f, acc.fields[f].buf.ptr if f in acc.fields else None)
     keys = {f: DeviceArray((N,), dt, device) for f, dt in (("coord", np.uint64), ("dhi", np.uint64),
                                                           ("dlo", np.uint64), ("score", np.uint32),
                                                           ("pair", np.uint32))}
-    check(lib().ma_sort_keys_se_dev(st.handle, N, C.byref(recs), lens.buf.ptr, d_tid.ptr, 0, keys["coord"].ptr,
-                                    keys["dhi"].ptr, keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr),
-          "ma_sort_keys_se_dev")
+    kargs = [keys[f].ptr for f in ("coord", "dhi", "dlo", "score", "pair")]
+    if paired:
+        check(lib().ma_sort_keys_pe_dev(st.handle, N // 2, C.byref(recs), acc.lens.buf.ptr, d_tid.ptr, tid.size, 0,
+                                        *kargs), "ma_sort_keys_pe_dev")
+    else:
+        check(lib().ma_sort_keys_se_dev(st.handle, N, C.byref(recs), acc.lens.buf.ptr, d_tid.ptr, 0, *kargs),
+              "ma_sort_keys_se_dev")
     perm = DeviceArray((N,), np.uint32, device)
     is_dup = DeviceArray((N,), np.uint32, device)
     sb = int(lib().ma_sort_scratch_bytes(N))
@@ -131,13 +172,17 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_rea
                                     scratch.ptr, sb), "ma_sort_markdup_dev")
     del keys, scratch
     # BAM records in coordinate order, then BGZF
-    offs = np.concatenate(name_off)
+    offs = np.concatenate(acc.name_off)
     d_off = DeviceArray.from_host(offs, device)
-    reads = DeviceReads.on_device(names.view(np.uint8), d_off, rows.view(np.uint8), lens.view(np.uint32), N, stride,
-                                  int(offs[-1]), quals.view(np.uint8), qlens.view(np.uint32))
+    reads = DeviceReads.on_device(acc.names.view(np.uint8), d_off, acc.rows.view(np.uint8),
+                                  acc.lens.view(np.uint32), N, stride, int(offs[-1]), acc.quals.view(np.uint8),
+                                  acc.qlens.view(np.uint32))
     rg = rg_id.encode("ascii")
     packer = BamPacker(N, reads.out_bound(len(rg)), device)
-    packer.pack_se(st, N, _recs_view(recs, N), d_tid, tid.size, reads, rg, perm, is_dup, bool(do_markdup))
+    if paired:
+        packer.pack_pe(st, N // 2, _recs_view(recs, N), d_tid, tid.size, reads, rg, perm, is_dup, bool(do_markdup))
+    else:
+        packer.pack_se(st, N, _recs_view(recs, N), d_tid, tid.size, reads, rg, perm, is_dup, bool(do_markdup))
     total = packer.total(N, st)
     t_sort_pack = time.perf_counter() - t1
     t2 = time.perf_counter()
@@ -160,10 +205,10 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, batch_rea
         size = w.close()
     t_bgzf = time.perf_counter() - t2
     dups = int(is_dup.download().sum()) if do_markdup else 0
-    mapped = int((fields["cid"].view(np.int32).download() >= 0).sum())
-    return {"reads": N, "mapped": mapped, "duplicates": dups, "bam_stream_bytes": total, "bgzf_bytes": size,
-            "ingest_and_map_s": round(t_ingest, 3), "map_kernel_ms": round(t_map, 3),
-            "sort_and_pack_s": round(t_sort_pack, 3), "bgzf_s": round(t_bgzf, 3)}
+    mapped = int((acc.fields["cid"].view(np.int32).download() >= 0).sum())
+    return {"reads": N, "pairs": N // 2 if paired else 0, "mapped": mapped, "duplicates": dups,
+            "bam_stream_bytes": total, "bgzf_bytes": size, "ingest_and_map_s": round(t_ingest, 3),
+            "map_kernel_ms": round(t_map, 3), "sort_and_pack_s": round(t_sort_pack, 3), "bgzf_s": round(t_bgzf, 3)}
 
 
 class _recs_view:

@@ -57,13 +57,12 @@ def map_fastq_to_sam(ref_fasta: str, fq1: str, out_sam: str, device: str = "auto
               f"postings={index.n_postings} device={device}", flush=True)
         if out_sam.lower().endswith(".bam"):
             # native BAM like the reference's fm driver (_path_is_bam, linear_gpu_fm.mojo:84-88):
-            # coordinate-sorted, duplicates marked, every stage on the GPU
-            if fq2:
-                raise MialignError("BAM output covers single-end reads in this build")
+            # coordinate-sorted, duplicates marked, every stage on the GPU; pairs as R1, R2
             from .fq2bam import fastq_to_bam
             from .linear_gpu_locate import env_int, env_str
 
-            stats = fastq_to_bam(index, fq1, out_sam, bs_r1, level=env_int("METHYLGRAPHER_BAM_LEVEL", 1),
+            stats = fastq_to_bam(index, fq1, out_sam, bs_r1, fq2=fq2, bs_r2=bs_r2,
+                                 level=env_int("METHYLGRAPHER_BAM_LEVEL", 1),
                                  device_bgzf=os.environ.get("MIALIGN_BGZF", "device").lower() != "host",
                                  rg_id=env_str("METHYLGRAPHER_RG_ID", "mojo1"),
                                  sm=env_str("METHYLGRAPHER_RG_SM", "sample"),

@@ -118,6 +118,29 @@ def gen_reads(seed: int, ref: Reference, n: int, len_min: int = 150, len_max: in
     return Reads(bases, lens, contig, pos, strand, dup_of)
 
 
+_COMP = np.arange(256, dtype=np.uint8)
+_COMP[np.frombuffer(b"ACGTacgt", np.uint8)] = np.frombuffer(b"TGCAtgca", np.uint8)
+
+
+def gen_pairs(seed: int, ref: Reference, n_pairs: int, frag_len: int = 300, len1: int = 150, len2: int = 150,
+              **kw) -> tuple[Reads, Reads]:
+    """Directional bisulfite pairs: fragments of frag_len bases drawn as
+    gen_reads draws reads (conversion, errors, duplicate fragments); R1 = a
+    fragment's first len1 bases, R2 = the reverse complement of its last len2.
+    Truth fields describe the fragment."""
+    if not (1 <= len1 <= frag_len and 1 <= len2 <= frag_len):
+        raise ValueError("gen_pairs: mate lengths must lie in [1, frag_len]")
+    fr = gen_reads(seed, ref, n_pairs, frag_len, frag_len, max_len=frag_len, **kw)
+    m = max(len1, len2)
+    b1 = np.full((n_pairs, m), ord("N"), np.uint8)
+    b1[:, :len1] = fr.bases[:, :len1]
+    b2 = np.full((n_pairs, m), ord("N"), np.uint8)
+    b2[:, :len2] = _COMP[fr.bases[:, frag_len - len2:frag_len]][:, ::-1]
+    r1 = Reads(b1, np.full(n_pairs, len1, np.uint32), fr.contig, fr.pos, fr.strand, fr.dup_of)
+    r2 = Reads(b2, np.full(n_pairs, len2, np.uint32), fr.contig, fr.pos, fr.strand, fr.dup_of)
+    return r1, r2
+
+
 def pack_2bit(bases: np.ndarray, lens: np.ndarray, stride_bytes: int | None = None) -> np.ndarray:
     """[n, max_len] ASCII -> [n, stride] 2-bit (base 4j in bits 7-6 of byte j)."""
     n, max_len = bases.shape

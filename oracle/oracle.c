@@ -31,6 +31,11 @@
  *        answers for the 4-bit SEQ and the bin + an independent BAM decoder
  *        (tests/bam_decode.py) checked field by field against the SAM text of
  *        the same records + planted mistakes (swapped clips, unreversed QUAL)
+ *   or_pe_fields / or_bam_pack_pe ..................... hand-worked pairs (one
+ *        contig in both orders, two contigs, one or both mates unmapped), the
+ *        parity path's own pairing (pair_hits, linear_extend.mojo:263-294) on
+ *        flags, mate fields and |tlen|, and decoded records whose mate fields
+ *        name their mate's tid / pos
  *   or_fq_cvt / or_fastq_parse ........................ reference known answers
  *        (test_linear_fastq.mojo, test_fastq_batch.py, the reference's FASTQ
  *        fixture) + the host reader of the parity path (read_one) on random
@@ -1336,6 +1341,128 @@ int64_t or_bam_pack_se(int64_t n, const int32_t *cid, const uint32_t *pos, const
                                    seq, stride, lens, qual, qual_lens, rg, rg_n, perm, is_dup,
                                    apply_dup, out, cap);
     free(f);
+    return g;
+}
+
+/* Paired-end flags, mate fields and sort keys of np pairs whose merged map
+ * records lie R1 first (records [0, np)) then R2 ([np, 2 np)); lens are the
+ * original read lengths, same order. Output record 2p is pair p's R1 and 2p + 1
+ * its R2, the order the reference emits them; row[i] is output record i's
+ * record / read row. Any output pointer may be NULL. Follows the PE branch of
+ * _emit_bam_from_batch, linear_gpu_fm.mojo:826-870 (flags, mate fields, tlen
+ * signed by position order) and :936-977 (coord per record; dhi / dlo, score
+ * and pair id shared by the pair; pair id = (pair_base + 2p) / 2). */
+void or_pe_fields(int64_t np, const int32_t *cid, const uint32_t *pos, const uint32_t *mapq,
+                  const int32_t *rflag, const uint32_t *sl, const uint32_t *sr, const uint32_t *nm,
+                  const uint32_t *lens, const int32_t *tid_of_cid, int64_t n_tid, int64_t pair_base,
+                  int32_t *o_flag, int32_t *o_tid, int32_t *o_pos, int32_t *o_mapq, int32_t *o_sl,
+                  int32_t *o_sr, int32_t *o_nm, int32_t *o_ntid, int32_t *o_npos, int32_t *o_tlen,
+                  int32_t *o_row, uint64_t *coord, uint64_t *dhi, uint64_t *dlo, uint32_t *score,
+                  uint32_t *pair) {
+    for (int64_t p = 0; p < np; p++) {
+        int64_t r[2] = {p, np + p}, fl[2], tid[2], ps[2], ntid[2] = {-1, -1}, npos[2] = {-1, -1},
+                tlen[2] = {0, 0};
+        for (int m = 0; m < 2; m++) {
+            int64_t c = cid[r[m]];
+            fl[m] = (int64_t)rflag[r[m]] | 1 | (m ? 128 : 64);
+            tid[m] = -1;
+            if (c >= 0 && c < n_tid) tid[m] = tid_of_cid[c];
+            else fl[m] |= 4;
+            ps[m] = (int64_t)pos[r[m]];
+        }
+        if (tid[0] < 0) fl[1] |= 8;
+        if (tid[1] < 0) fl[0] |= 8;
+        if (tid[0] >= 0 && tid[1] >= 0 && tid[0] == tid[1]) {
+            fl[0] |= 2;
+            fl[1] |= 2;
+            ntid[0] = tid[0];
+            ntid[1] = tid[1];
+            npos[0] = ps[1];
+            npos[1] = ps[0];
+            int64_t span = ps[1] - ps[0];
+            if (span < 0) span = -span;
+            span += (int64_t)lens[r[1]];
+            tlen[0] = ps[0] <= ps[1] ? span : -span;
+            tlen[1] = -tlen[0];
+        } else {
+            if (tid[0] >= 0) { ntid[1] = tid[0]; npos[1] = ps[0]; }
+            if (tid[1] >= 0) { ntid[0] = tid[1]; npos[0] = ps[1]; }
+        }
+        uint64_t h = ~0ull, l = (uint64_t)(2 * p);
+        if (tid[0] >= 0 || tid[1] >= 0) {
+            uint64_t e[2];
+            for (int m = 0; m < 2; m++)
+                e[m] = or_pack_end(tid[m], or_unclipped5(fl[m], ps[m], sl[r[m]], sr[r[m]], lens[r[m]]),
+                                   (fl[m] >> 4) & 1);
+            if (tid[0] < 0) { h = e[1]; l = 0; }
+            else if (tid[1] < 0) { h = e[0]; l = 0; }
+            else if (e[0] <= e[1]) { h = e[0]; l = e[1]; }
+            else { h = e[1]; l = e[0]; }
+        }
+        int64_t nms = (int64_t)nm[r[0]] + (int64_t)nm[r[1]];
+        if (nms > 255) nms = 255;
+        uint32_t sc = (uint32_t)((((int64_t)mapq[r[0]] + (int64_t)mapq[r[1]]) << 8) | (255 - nms));
+        for (int m = 0; m < 2; m++) {
+            int64_t i = 2 * p + m;
+            if (o_flag) o_flag[i] = (int32_t)fl[m];
+            if (o_tid) o_tid[i] = (int32_t)tid[m];
+            if (o_pos) o_pos[i] = (int32_t)ps[m];
+            if (o_mapq) o_mapq[i] = (int32_t)mapq[r[m]];
+            if (o_sl) o_sl[i] = (int32_t)sl[r[m]];
+            if (o_sr) o_sr[i] = (int32_t)sr[r[m]];
+            if (o_nm) o_nm[i] = (int32_t)nm[r[m]];
+            if (o_ntid) o_ntid[i] = (int32_t)ntid[m];
+            if (o_npos) o_npos[i] = (int32_t)npos[m];
+            if (o_tlen) o_tlen[i] = (int32_t)tlen[m];
+            if (o_row) o_row[i] = (int32_t)r[m];
+            if (coord) coord[i] = or_coord_key(tid[m], ps[m], i);
+            if (dhi) dhi[i] = h;
+            if (dlo) dlo[i] = l;
+            if (score) score[i] = sc;
+            if (pair) pair[i] = (uint32_t)((pair_base + 2 * p) / 2);
+        }
+    }
+}
+
+/* Paired-end BAM records: or_pe_fields, the reads taken in output order
+ * (record i reads row row[i]), then or_bam_pack_fields. perm / is_dup index
+ * the 2 np output records. */
+int64_t or_bam_pack_pe(int64_t np, const int32_t *cid, const uint32_t *pos, const uint32_t *mapq,
+                       const int32_t *rflag, const uint32_t *sl, const uint32_t *sr,
+                       const uint32_t *nm, const int32_t *tid_of_cid, int64_t n_tid,
+                       const uint8_t *names, const int64_t *name_off, const uint8_t *seq,
+                       int64_t stride, const uint32_t *lens, const uint8_t *qual,
+                       const uint32_t *qual_lens, const uint8_t *rg, int64_t rg_n,
+                       const uint32_t *perm, const uint32_t *is_dup, int apply_dup, uint8_t *out,
+                       int64_t cap) {
+    const int64_t n = 2 * np, nn = n > 0 ? n : 1;
+    int32_t *f = (int32_t *)malloc((size_t)nn * 11 * sizeof(int32_t));
+    int64_t *noff = (int64_t *)malloc((size_t)(nn + 1) * sizeof(int64_t));
+    uint8_t *nb = (uint8_t *)malloc((size_t)(name_off[n] - name_off[0] + 1));
+    uint8_t *sq = (uint8_t *)malloc((size_t)(nn * (stride > 0 ? stride : 1)));
+    uint8_t *qu = qual ? (uint8_t *)malloc((size_t)(nn * (stride > 0 ? stride : 1))) : NULL;
+    uint32_t *ln = (uint32_t *)malloc((size_t)nn * 2 * sizeof(uint32_t));
+    int64_t g = -1;
+    if (f && noff && nb && sq && ln && (qu || !qual)) {
+        int32_t *c[11];
+        for (int k = 0; k < 11; k++) c[k] = f + k * n;
+        or_pe_fields(np, cid, pos, mapq, rflag, sl, sr, nm, lens, tid_of_cid, n_tid, 0, c[0], c[1], c[2],
+                     c[3], c[4], c[5], c[6], c[7], c[8], c[9], c[10], NULL, NULL, NULL, NULL, NULL);
+        uint32_t *ql = ln + n;
+        noff[0] = 0;
+        for (int64_t i = 0; i < n; i++) {
+            int64_t r = c[10][i], a = name_off[r], b = name_off[r + 1];
+            memcpy(nb + noff[i], names + a, (size_t)(b - a));
+            noff[i + 1] = noff[i] + (b - a);
+            memcpy(sq + i * stride, seq + r * stride, (size_t)stride);
+            if (qu) memcpy(qu + i * stride, qual + r * stride, (size_t)stride);
+            ln[i] = lens[r];
+            ql[i] = qual_lens ? qual_lens[r] : lens[r];
+        }
+        g = or_bam_pack_fields(n, c[0], c[1], c[2], c[3], c[4], c[5], c[6], c[7], c[8], c[9], nb, noff, sq,
+                               stride, ln, qu, ql, rg, rg_n, perm, is_dup, apply_dup, out, cap);
+    }
+    free(f); free(noff); free(nb); free(sq); free(qu); free(ln);
     return g;
 }
 

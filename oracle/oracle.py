@@ -116,6 +116,11 @@ def lib():
         L.or_bam_pack_se.restype = I64
         L.or_bam_pack_se.argtypes = [I64] + [P] * 8 + [I64, P, P, P, I64, P, P, P, P, I64, P, P,
                                                        C.c_int, P, I64]
+        L.or_pe_fields.restype = None
+        L.or_pe_fields.argtypes = [I64] + [P] * 9 + [I64, I64] + [P] * 16
+        L.or_bam_pack_pe.restype = I64
+        L.or_bam_pack_pe.argtypes = [I64] + [P] * 8 + [I64, P, P, P, I64, P, P, P, P, I64, P, P,
+                                                       C.c_int, P, I64]
         L.or_recount_nm.argtypes = [P, P, I64, P, I64, P, I64, P, P, P, P, P, P]
         _lib = L
     return _lib
@@ -282,6 +287,58 @@ def bam_pack_se(rec: dict, tid_of_cid: np.ndarray, names: np.ndarray, name_off: 
                              _p(rgb), len(rg), _p(pm), _p(dp), int(bool(apply_dup)), _p(out), cap)
     if g < 0:
         raise RuntimeError("or_bam_pack_se: BAM batch buffer overflow")
+    return out[:g].tobytes()
+
+
+PE_FIELDS = ("flag", "tid", "pos", "mapq", "sl", "sr", "nm", "ntid", "npos", "tlen", "row")
+
+
+def pe_fields(rec: dict, lens: np.ndarray, tid_of_cid: np.ndarray, pair_base: int = 0) -> dict:
+    """Paired-end fields and keys (or_pe_fields) of 2 np merged records, R1
+    block then R2 block; output record 2p is pair p's R1, 2p + 1 its R2."""
+    n = int(lens.size)
+    assert n % 2 == 0
+    out = {f: np.empty(n, np.int32) for f in PE_FIELDS}
+    for f, dt in (("coord", np.uint64), ("dhi", np.uint64), ("dlo", np.uint64), ("score", np.uint32),
+                  ("pair", np.uint32)):
+        out[f] = np.empty(n, dt)
+    cols = {f: np.ascontiguousarray(rec[f]) for f in ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")}
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    tid = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
+    lib().or_pe_fields(n // 2, _p(cols["cid"]), _p(cols["pos"]), _p(cols["mapq"]), _p(cols["flag"]),
+                       _p(cols["sl"]), _p(cols["sr"]), _p(cols["nm"]), _p(lens), _p(tid), tid.size, pair_base,
+                       *[_p(out[f]) for f in PE_FIELDS + ("coord", "dhi", "dlo", "score", "pair")])
+    return out
+
+
+def bam_pack_pe(rec: dict, tid_of_cid: np.ndarray, names: np.ndarray, name_off: np.ndarray,
+                seq: np.ndarray, lens: np.ndarray, qual: np.ndarray | None, rg: bytes,
+                perm: np.ndarray | None = None, is_dup: np.ndarray | None = None,
+                apply_dup: bool = False, qual_lens: np.ndarray | None = None) -> bytes:
+    """Uncompressed BAM records of paired-end map records (or_bam_pack_pe);
+    reads and records R1 block then R2 block, records out as R1, R2 per pair."""
+    n = int(lens.size)
+    assert n % 2 == 0
+    seq = np.ascontiguousarray(seq, dtype=np.uint8)
+    stride = seq.shape[1] if seq.ndim == 2 else 0
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    names = np.ascontiguousarray(names, dtype=np.uint8)
+    name_off = np.ascontiguousarray(name_off, dtype=np.int64)
+    tid = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
+    q = None if qual is None else np.ascontiguousarray(qual, dtype=np.uint8)
+    ql = None if qual_lens is None else np.ascontiguousarray(qual_lens, dtype=np.uint32)
+    pm = None if perm is None else np.ascontiguousarray(perm, dtype=np.uint32)
+    dp = None if is_dup is None else np.ascontiguousarray(is_dup, dtype=np.uint32)
+    rgb = np.frombuffer(rg, dtype=np.uint8) if rg else np.zeros(1, np.uint8)
+    cols = {f: np.ascontiguousarray(rec[f]) for f in ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")}
+    cap = bam_record_bound(lens, name_off, len(rg))
+    out = np.empty(max(cap, 1), np.uint8)
+    g = lib().or_bam_pack_pe(n // 2, _p(cols["cid"]), _p(cols["pos"]), _p(cols["mapq"]), _p(cols["flag"]),
+                             _p(cols["sl"]), _p(cols["sr"]), _p(cols["nm"]), _p(tid), tid.size,
+                             _p(names), _p(name_off), _p(seq), stride, _p(lens), _p(q), _p(ql),
+                             _p(rgb), len(rg), _p(pm), _p(dp), int(bool(apply_dup)), _p(out), cap)
+    if g < 0:
+        raise RuntimeError("or_bam_pack_pe: BAM batch buffer overflow")
     return out[:g].tobytes()
 
 
