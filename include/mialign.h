@@ -61,8 +61,11 @@ typedef struct ma_workspace ma_workspace;
 typedef struct ma_server ma_server;
 
 /* Mapping knobs; ma_params_default gives the reference defaults
- * (linear_gpu_locate.mojo:1729-1751, linear_index.mojo:425-434). max_indel must
- * be 0 here: the gapless contract (indel branches are not built). */
+ * (linear_gpu_locate.mojo:1729-1751, linear_index.mojo:425-434) except
+ * max_indel, 0 (gapless extension). max_indel 1..4 (larger values clamp to 4)
+ * turns on the single-indel branch (:1015-1024, 1142-1235, 1408-1511), the
+ * reference's own default being 4: records then carry iop (1 insertion, 2
+ * deletion), iat (query bases before the indel) and ilen. */
 typedef struct {
     int32_t k;         /* k-mer length (index k) */
     int32_t stride;    /* seed stride (METHYLGRAPHER_LINEAR_SEED_STRIDE, 5) */
@@ -71,7 +74,7 @@ typedef struct {
     int32_t vote_fast; /* pass-0 vote cap (VOTE_OCC_FAST, = vote_occ) */
     int32_t max_diff;  /* mismatch budget (MAX_DIFF, 6) */
     int32_t max_soft;  /* soft-clip cap (MAX_SOFT, 8) */
-    int32_t max_indel; /* 0 = gapless */
+    int32_t max_indel; /* 0 = gapless; 1..4 = single-indel cap */
     int32_t do_rc;     /* reverse-complement pass (LINEAR_RC, 1) */
 } ma_params;
 
@@ -86,9 +89,9 @@ typedef struct {
     uint32_t *sl;   /* soft clip, left (reference orientation) */
     uint32_t *sr;   /* soft clip, right */
     uint32_t *nm;   /* mismatches */
-    uint32_t *iop;  /* indel op (always 0 in the gapless contract) */
-    uint32_t *iat;
-    uint32_t *ilen;
+    uint32_t *iop;  /* indel op: 0 none (always with max_indel 0), 1 insertion, 2 deletion */
+    uint32_t *iat;  /* query bases before the indel */
+    uint32_t *ilen; /* indel length (1..4) */
 } ma_recs;
 
 enum { MA_ENC_ASCII = 0, MA_ENC_2BIT = 1 };

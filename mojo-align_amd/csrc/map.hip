@@ -50,15 +50,21 @@ constexpr int MAX_CLAIM = 16;  // most reads claimed per work-counter atomic
 constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kernels' register budget
 
 struct Knobs {
-    int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc;
+    int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc, max_indel;
 };
 
+// ind packs the single indel of a hit: iop (bits 0-1: 0 none, 1 insertion,
+// 2 deletion) | ilen << 2 | iat << 8 (query bases before the indel).
 struct Rec {
     int32_t cid;
     uint32_t pos, mapq;
     int32_t flag;
-    uint32_t sl, sr, nm;
+    uint32_t sl, sr, nm, ind;
 };
+
+__host__ __device__ constexpr uint32_t pack_ind(uint32_t iop, uint32_t ilen, uint32_t iat) {
+    return iop | ilen << 2 | iat << 8;
+}
 
 struct RecOut {
     int32_t *cid;
@@ -80,7 +86,7 @@ struct ReadsIn {
 
 #define DEV __device__ __forceinline__
 
-DEV Rec unmapped_rec() { return Rec{-1, 0u, 0u, 4, 0u, 0u, 0u}; }
+DEV Rec unmapped_rec() { return Rec{-1, 0u, 0u, 4, 0u, 0u, 0u, 0u}; }
 
 DEV Rec sel_rec(bool c, const Rec &a, const Rec &b) {
     Rec r;
@@ -91,6 +97,7 @@ DEV Rec sel_rec(bool c, const Rec &a, const Rec &b) {
     r.sl = c ? a.sl : b.sl;
     r.sr = c ? a.sr : b.sr;
     r.nm = c ? a.nm : b.nm;
+    r.ind = c ? a.ind : b.ind;
     return r;
 }
 
@@ -185,7 +192,7 @@ DEV uint64_t groups_mask2(int n) {
 }
 
 struct Caps {
-    int budget, clip_cap, shift_cap;
+    int budget, clip_cap, shift_cap, indel_cap;
 };
 
 DEV Caps make_caps(const Knobs &kn, int L) {
@@ -195,9 +202,13 @@ DEV Caps make_caps(const Knobs &kn, int L) {
     if (budget > L / 2) budget = L / 2;
     int clip = kn.max_soft;
     if (clip > L / 4) clip = L / 4;
+    // indel_cap = clamp(max_indel, 0, 4), shift_cap = clamp(indel_cap, 2, 6)
+    // (linear_gpu_locate.mojo:1015-1024)
+    int indel = kn.max_indel < 0 ? 0 : (kn.max_indel > 4 ? 4 : kn.max_indel);
     c.budget = budget;
     c.clip_cap = clip;
-    c.shift_cap = 2;  // clamp(indel_cap = 0, 2, 6)
+    c.indel_cap = indel;
+    c.shift_cap = indel < 2 ? 2 : indel;
     return c;
 }
 
@@ -206,13 +217,16 @@ DEV uint64_t load_posting(const IndexView &ix, int64_t pi) {
     return *(const uint64_t *)(ix.postings + 2 * pi);
 }
 
+// MAPQ 20, 40 at votes >= 3, 60 at votes >= 5; 20 at most when cost > 2 or
+// the hit has an indel (linear_gpu_locate.mojo:1587-1597).
 DEV Rec finish_rec(int64_t cid, int64_t pos, int votes, int cost, int sl, int sr,
-                   int32_t strand_flag) {
+                   int32_t strand_flag, uint32_t ind = 0) {
     Rec r;
     uint32_t mq = 20;
     if (votes >= 3) mq = 40;
     if (votes >= 5) mq = 60;
     if (cost > 2 && mq > 20) mq = 20;
+    if ((ind & 3u) != 0 && mq > 20) mq = 20;
     r.cid = (int32_t)cid;
     r.pos = (uint32_t)pos;
     r.mapq = mq;
@@ -220,6 +234,7 @@ DEV Rec finish_rec(int64_t cid, int64_t pos, int votes, int cost, int sl, int sr
     r.sl = (uint32_t)sl;
     r.sr = (uint32_t)sr;
     r.nm = (uint32_t)cost;
+    r.ind = ind;
     return r;
 }
 
@@ -239,9 +254,9 @@ DEV uint8_t ascii_code(uint8_t b) {
 // and stores a read's records with one lane per field, so the 30 pointers hold
 // no scalar registers across the kernel.
 constexpr int REC_FIELDS = 10;
-constexpr int REC_WORDS = 7;  // fields kept in Rec; iop, iat and ilen are 0 (gapless)
+constexpr int REC_WORDS = 8;  // Rec's words: seven fields, then iop / ilen / iat packed
 constexpr int N_REC_PTRS = 3 * REC_FIELDS;
-static_assert(sizeof(Rec) == 4 * REC_WORDS, "Rec is seven 32-bit fields");
+static_assert(sizeof(Rec) == 4 * REC_WORDS, "Rec is eight 32-bit words");
 
 DEV uint32_t *rec_field(const RecOut &o, int f) {
     if (!o.cid) return nullptr;
@@ -267,6 +282,7 @@ constexpr int MAXW = 4;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
 constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace lds
@@ -279,6 +295,7 @@ constexpr int MAXW = 4;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
 constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace glb
@@ -291,6 +308,7 @@ constexpr int MAXW = 5;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
 constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace lds
@@ -303,6 +321,7 @@ constexpr int MAXW = 5;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 4;
 constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace glb
@@ -315,6 +334,7 @@ constexpr int MAXW = 8;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 3;
 constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace lds
@@ -327,9 +347,92 @@ constexpr int MAXW = 8;
 constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 3;
 constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = false;
 #include "map_body.inc"
 }  // namespace
 }  // namespace glb
+}  // namespace w8
+
+// The single-indel branch (max_indel 1..4): its own instantiations (contig
+// offsets in LDS or HBM, as for the gapless kernels), so the gapless kernels
+// carry none of its code or registers; two waves per SIMD leave its extra
+// window masks in registers.
+namespace w4 {
+namespace ind_l {
+namespace {
+constexpr int MAXW = 4;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_l
+}  // namespace w4
+
+namespace w4 {
+namespace ind_g {
+namespace {
+constexpr int MAXW = 4;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_g
+}  // namespace w4
+
+namespace w5 {
+namespace ind_l {
+namespace {
+constexpr int MAXW = 5;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_l
+}  // namespace w5
+
+namespace w5 {
+namespace ind_g {
+namespace {
+constexpr int MAXW = 5;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_g
+}  // namespace w5
+
+namespace w8 {
+namespace ind_l {
+namespace {
+constexpr int MAXW = 8;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = true;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_l
+}  // namespace w8
+
+namespace w8 {
+namespace ind_g {
+namespace {
+constexpr int MAXW = 8;
+constexpr int MAX_READ = MAXW * 32;
+constexpr int WAVES_PER_EU = 2;
+constexpr bool COFF_IN_LDS = false;
+constexpr bool INDEL = true;
+#include "map_body.inc"
+}  // namespace
+}  // namespace ind_g
 }  // namespace w8
 
 namespace {
@@ -388,8 +491,6 @@ int check_map_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
     MA_REQUIRE(p, "map: null params");
     MA_REQUIRE(p->k == ix->k, "map: params.k differs from the index k");
     MA_REQUIRE(p->stride >= 1, "map: stride must be >= 1");
-    MA_REQUIRE(p->max_indel == 0,
-               "map: max_indel must be 0 (gapless extension; indel branches are not built)");
     MA_REQUIRE(p->max_occ >= 1 && p->max_occ < (1 << 17), "map: max_occ must be in 1..131071");
     MA_REQUIRE(max_len >= 0 && max_len <= 256, "map: reads longer than 256 bases are not supported");
     int32_t ml = max_len < p->k ? p->k : max_len;
@@ -457,7 +558,8 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     RecOut mo = rec_out(d_merged);
     MA_REQUIRE(mo.cid, "ma_map_batch_dev: merged record arrays must be non-null");
     hipStream_t st = (hipStream_t)stream;
-    Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc};
+    Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc,
+             p->max_indel};
     ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
     // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
     // the resident waves all get `chunks_per_wave` chunks (more, smaller chunks
@@ -484,7 +586,18 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     // contig count where the contig offsets live
     const unsigned g = (unsigned)blocks;
     const bool lds = ix->n_contigs < COFF_LDS;
-    if (max_len <= 4 * 32) {
+    if (p->max_indel > 0) {
+        if (max_len <= 4 * 32) {
+            if (lds) w4::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w4::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        } else if (max_len <= 5 * 32) {
+            if (lds) w5::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w5::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        } else {
+            if (lds) w8::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w8::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        }
+    } else if (max_len <= 4 * 32) {
         if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
         else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
     } else if (max_len <= 5 * 32) {

@@ -146,15 +146,29 @@ class LinearHit:
 
 
 def hit_from_gpu(names: List[str], name: str, align_seq: bytes, original_seq: bytes, cid: int,
-                 start0: int, mapq: int, flag: int, soft_l: int, soft_r: int) -> LinearHit:
-    """CIGAR slS mM srS, SEQ = original read (RC for flag 16), canonical contig;
-    unmapped when cid < 0 or nothing is left between the clips (:217-284)."""
+                 start0: int, mapq: int, flag: int, soft_l: int, soft_r: int, indel_op: int = 0,
+                 indel_at: int = 0, indel_len: int = 0) -> LinearHit:
+    """CIGAR slS mM srS, or with the hit's single indel inside the middle m
+    bases (insertion: atM lenI (m-at-len)M, unmapped if that is negative;
+    deletion: atM lenD (m-at)M; zero-length M runs left out); SEQ = original
+    read (RC for flag 16), canonical contig; unmapped when cid < 0 or nothing
+    is left between the clips (:217-284)."""
     qlen = len(align_seq)
     emit = original_seq if original_seq else align_seq
     m = qlen - soft_l - soft_r
+    unmapped = LinearHit(name, 4, "*", 0, 0, "*", emit.decode(), "*")
     if cid < 0 or m < 1:
-        return LinearHit(name, 4, "*", 0, 0, "*", emit.decode(), "*")
-    cigar = (f"{soft_l}S" if soft_l > 0 else "") + f"{m}M" + (f"{soft_r}S" if soft_r > 0 else "")
+        return unmapped
+    mid = f"{m}M"
+    if indel_op == 1 and indel_len > 0 and 0 <= indel_at <= m:
+        right = m - indel_at - indel_len
+        if right < 0:
+            return unmapped
+        mid = (f"{indel_at}M" if indel_at > 0 else "") + f"{indel_len}I" + (f"{right}M" if right > 0 else "")
+    elif indel_op == 2 and indel_len > 0 and 0 <= indel_at <= m:
+        right = m - indel_at
+        mid = (f"{indel_at}M" if indel_at > 0 else "") + f"{indel_len}D" + (f"{right}M" if right > 0 else "")
+    cigar = (f"{soft_l}S" if soft_l > 0 else "") + mid + (f"{soft_r}S" if soft_r > 0 else "")
     if flag & 16:
         emit = reverse_complement(emit)
     return LinearHit(name, flag, canonical_contig(names[cid]), start0 + 1, mapq, cigar, emit.decode(), "*")
@@ -212,8 +226,10 @@ def sam_header(names: List[str], lengths: List[int]) -> str:
 
 
 def knobs_from_env(k: int, max_occ: int):
-    """METHYLGRAPHER_LINEAR_* knobs with the reference defaults (:1729-1751).
-    The extension is gapless: MAX_INDEL is not honoured (indel_cap = 0)."""
+    """METHYLGRAPHER_LINEAR_* knobs with the reference defaults (:1729-1751),
+    except MAX_INDEL: unset (or below 1) keeps the extension gapless
+    (indel_cap 0, this build's default); 1..4 turns on the single-indel branch
+    at that cap, as the reference's default of 4 does."""
     vote_occ = env_int("METHYLGRAPHER_LINEAR_VOTE_OCC", 256)
     rc_raw = env_str("METHYLGRAPHER_LINEAR_RC", "1").lower()
     return default_params(
@@ -224,7 +240,7 @@ def knobs_from_env(k: int, max_occ: int):
         vote_fast=env_int("METHYLGRAPHER_LINEAR_VOTE_OCC_FAST", vote_occ),
         max_diff=env_int("METHYLGRAPHER_LINEAR_MAX_DIFF", 6),
         max_soft=env_int("METHYLGRAPHER_LINEAR_MAX_SOFT", 8),
-        max_indel=0,
+        max_indel=env_int("METHYLGRAPHER_LINEAR_MAX_INDEL", 0),
         do_rc=0 if rc_raw in ("0", "false", "no") else 1,
     )
 
@@ -262,7 +278,8 @@ def map_fastq_dense_gpu_locate(index, fq1: str, out_sam: str, device: str = "aut
             hits = [
                 hit_from_gpu(index.contig_names, r.name, r.seq, r.original_seq, int(got["cid"][i]),
                              int(got["pos"][i]), int(got["mapq"][i]), int(got["flag"][i]),
-                             int(got["sl"][i]), int(got["sr"][i]))
+                             int(got["sl"][i]), int(got["sr"][i]), int(got["iop"][i]), int(got["iat"][i]),
+                             int(got["ilen"][i]))
                 for i, r in enumerate(recs)
             ]
             for h, r in zip(hits, recs):
