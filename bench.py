@@ -17,7 +17,9 @@ loop against plain launches), `sort_stress` (200M sort + markdup records),
 `genome_scale` (the 3.1 Gbp reference), `bam_emit` (the step's records as
 coordinate-ordered BAM with duplicate flags: device pack, BGZF on the device
 or on host threads) and `fastq_ingest` (the reads' FASTQ text parsed on the
-device), each with its own roofline, parity check and CPU baseline.
+device), each with its own roofline, parity check and CPU baseline; `fq2bam`
+(FASTQ file to BAM file end to end) and `indel_mode` (the same step with the
+single-indel branch on, max_indel 4, parity-checked on a sample).
 """
 
 from __future__ import annotations
@@ -295,6 +297,44 @@ def cpu_map_baseline(ix, dual, bases, lens, tid, ns, what):
 
 
 FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
+ALL_FIELDS = FIELDS + ("iop", "iat", "ilen")
+
+
+def indel_section(sr, ix, dual, reads, with_cpu: bool, parity_reads: int) -> dict:
+    """The reference's default extension (max_indel 4, the single-indel branch
+    on) over the same step: reads/s of map + keys + sort + markdup, the map
+    kernel's time, indel hits, and GPU == oracle (all ten record fields) on
+    the first reads."""
+    import mialign
+    from oracle import oracle as orc
+
+    gapless = sr.prm
+    sr.prm = mialign.default_params(k=21, max_indel=4)
+    try:
+        value, ms, map_ms = sr.run(3, 1, "indel cap 4")
+        merged = sr.recs.download()
+    finally:
+        sr.prm = gapless
+        sr.step(False)  # the gapless records and order again, for the sections after this one
+        sr.stream.sync()
+    n = sr.n
+    out = {"workload": f"the step's {n} reads with max_indel 4 (the reference default)", "max_indel": 4,
+           "reads_per_s": round(value, 1), "ms_per_step": round(ms, 3), "map_kernel_ms": round(map_ms, 3),
+           "indel_hits": int((merged["iop"] != 0).sum()), "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4)}
+    log(f"indel cap 4: {ms:.1f} ms/step, map {map_ms:.1f} ms, {value / 1e6:.1f} M reads/s, "
+        f"{out['indel_hits']} indel hits")
+    if with_cpu:
+        ns = min(parity_reads, n)
+        km, of, po = ix.export_dense()
+        oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        want, _, _, _ = orc.map_batch(oix, orc.convert_reads(reads.bases[:ns], 1), reads.lens[:ns],
+                                      orc.params(max_indel=4), nthreads=threads)
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in ALL_FIELDS)
+        out["parity"] = {"reads": ns, "records_bit_exact": bool(ok), "oracle_indel_hits": int((want["iop"] != 0).sum())}
+        log(f"indel cap 4: first {ns} reads equal the oracle's: {ok}")
+    return out
+
 
 
 def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
@@ -751,6 +791,8 @@ def main() -> int:
     ap.add_argument("--fq2bam-reads", type=int, default=4_000_000,
                     help="reads through the whole FASTQ file -> BAM file path (0 = skip)")
     ap.add_argument("--bam-cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--no-indel", action="store_true", help="skip the max_indel 4 section")
+    ap.add_argument("--indel-parity-sample", type=int, default=200_000)
     ap.add_argument("--bam-parity-sample", type=int, default=1_000_000)
     args = ap.parse_args()
 
@@ -792,6 +834,7 @@ def main() -> int:
     merged = sr.recs.download()
     is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
+    indel = None if args.no_indel else indel_section(sr, ix, dual, reads, not args.no_cpu, args.indel_parity_sample)
     bam = fastq = f2b = None
     if not (args.no_bam and args.no_fastq) or args.fq2bam_reads > 0:
         blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
@@ -880,6 +923,7 @@ def main() -> int:
         "bam_emit": bam,
         "fastq_ingest": fastq,
         "fq2bam": f2b,
+        "indel_mode": indel,
     }
     print(json.dumps(line), flush=True)
     return 0
