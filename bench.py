@@ -18,7 +18,8 @@ loop against plain launches), `sort_stress` (200M sort + markdup records),
 coordinate-ordered BAM with duplicate flags: device pack, BGZF on the device
 or on host threads) and `fastq_ingest` (the reads' FASTQ text parsed on the
 device), each with its own roofline, parity check and CPU baseline; `fq2bam`
-(FASTQ file to BAM file end to end) and `indel_mode` (the same step with the
+(FASTQ file to BAM file end to end), `pe_emit` (the step's records as pairs: PE
+keys, sort + markdup, PE pack) and `indel_mode` (the same step with the
 single-indel branch on, max_indel 4, parity-checked on a sample).
 """
 
@@ -298,6 +299,67 @@ def cpu_map_baseline(ix, dual, bases, lens, tid, ns, what):
 
 FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
 ALL_FIELDS = FIELDS + ("iop", "iat", "ilen")
+
+
+def pe_section(sr, reads, tid, blob, name_off, qual, with_cpu: bool) -> dict:
+    """The step's records as pairs (read i and read n/2 + i): PE keys, sort +
+    markdup and the PE record pack (flags, mate fields, tlen, pair-wise 0x400)
+    on the device, timed with HIP events on their stream; the keys and the
+    sort's order and duplicate flags equal the oracle's for every pair."""
+    import ctypes as C
+
+    import mialign
+    from mialign import bam_emit
+    from mialign._lib import check, lib
+    from oracle import oracle as orc
+
+    n = sr.n - (sr.n & 1)
+    np_ = n // 2
+    st = sr.stream
+    keys = {f: mialign.DeviceArray((n,), dt) for f, dt in (("coord", np.uint64), ("dhi", np.uint64),
+                                                          ("dlo", np.uint64), ("score", np.uint32),
+                                                          ("pair", np.uint32))}
+    perm, is_dup = mialign.DeviceArray((n,), np.uint32), mialign.DeviceArray((n,), np.uint32)
+    sb = int(lib().ma_sort_scratch_bytes(n))
+    scratch = mialign.DeviceArray((sb,), np.uint8)
+    dreads = bam_emit.DeviceReads(blob, name_off, reads.bases[:n], reads.lens[:n], qual[:n])
+    rg = b"mojo1"
+    pk = bam_emit.BamPacker(n, dreads.out_bound(len(rg)))
+    ka = [keys[f].ptr for f in ("coord", "dhi", "dlo", "score", "pair")]
+    ev = [mialign.Event() for _ in range(4)]
+    times = []
+    for it in range(4):
+        ev[0].record(st)
+        check(lib().ma_sort_keys_pe_dev(st.handle, np_, C.byref(sr.recs.struct), sr.d_lens.ptr, sr.d_tid.ptr,
+                                        int(tid.size), 0, *ka), "ma_sort_keys_pe_dev")
+        ev[1].record(st)
+        check(lib().ma_sort_markdup_dev(st.handle, n, *ka, is_dup.ptr, perm.ptr, 1, None, scratch.ptr, sb),
+              "ma_sort_markdup_dev")
+        ev[2].record(st)
+        pk.pack_pe(st, np_, sr.recs, sr.d_tid, int(tid.size), dreads, rg, perm, is_dup, True)
+        ev[3].record(st)
+        ev[3].synchronize()
+        if it:
+            times.append([ev[0].elapsed_ms(ev[1]), ev[1].elapsed_ms(ev[2]), ev[2].elapsed_ms(ev[3])])
+    total = pk.total(n, st)
+    k_ms, s_ms, p_ms = (float(np.median([t[i] for t in times])) for i in range(3))
+    all_ms = k_ms + s_ms + p_ms
+    out = {"workload": f"the step's {n} records as {np_} pairs (read i with read {np_} + i)",
+           "pairs_per_s": round(np_ / (all_ms / 1000.0), 1), "keys_ms": round(k_ms, 3),
+           "sort_markdup_ms": round(s_ms, 3), "pack_ms": round(p_ms, 3), "bam_stream_bytes": total,
+           "pack_GBps_out": round(total / (p_ms / 1000.0) / 1e9, 1)}
+    log(f"pe: {np_} pairs keys {k_ms:.2f} ms, sort+markdup {s_ms:.2f} ms, pack {p_ms:.2f} ms "
+        f"({total / 1e9:.2f} GB of records)")
+    if with_cpu:
+        rec = {f: v[:n] for f, v in sr.recs.download().items()}
+        want = orc.pe_fields(rec, reads.lens[:n], tid)
+        same_keys = all(np.array_equal(keys[f].download(), want[f]) for f in ("coord", "dhi", "dlo", "score", "pair"))
+        wd, wp, _ = orc.sort_markdup(want["coord"], want["dhi"], want["dlo"], want["score"], want["pair"])
+        same_sort = bool(np.array_equal(perm.download(), wp) and np.array_equal(is_dup.download(), wd))
+        out["parity"] = {"pairs": np_, "keys_equal": bool(same_keys), "order_and_dup_flags_equal": same_sort,
+                         "duplicate_records": int(wd.sum())}
+        log(f"pe: keys equal the oracle's {same_keys}; order and duplicate flags {same_sort}")
+    return out
 
 
 def indel_section(sr, ix, dual, reads, with_cpu: bool, parity_reads: int) -> dict:
@@ -835,12 +897,13 @@ def main() -> int:
     is_dup = sr.outs["is_dup"].download()
     mapped_frac = float((merged["cid"] >= 0).mean())
     indel = None if args.no_indel else indel_section(sr, ix, dual, reads, not args.no_cpu, args.indel_parity_sample)
-    bam = fastq = f2b = None
+    bam = fastq = f2b = pe = None
     if not (args.no_bam and args.no_fastq) or args.fq2bam_reads > 0:
         blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
         if not args.no_bam:
             bam = bam_section(sr, reads, tid, tid.size, blob, name_off, qual, not args.no_cpu, args.bam_cpu_sample,
                               args.bam_parity_sample)
+            pe = pe_section(sr, reads, tid, blob, name_off, qual, not args.no_cpu)
         sr.close()
         if not args.no_fastq:
             fastq = fastq_section(reads, blob, name_off, qual, not args.no_cpu, args.fastq_cpu_sample,
@@ -924,6 +987,7 @@ def main() -> int:
         "fastq_ingest": fastq,
         "fq2bam": f2b,
         "indel_mode": indel,
+        "pe_emit": pe,
     }
     print(json.dumps(line), flush=True)
     return 0

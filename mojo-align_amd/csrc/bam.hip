@@ -82,7 +82,9 @@ __device__ __forceinline__ RecF load_fields(const FieldSrc &s, int64_t i) {
         // _emit_bam_from_batch PE (linear_gpu_fm.mojo:826-870)
         PeMate m[2];
         pe_pair(s.np, i >> 1, s.cid, s.pos, s.rflag, s.lens, s.tid_of_cid, s.n_tid, m);
-        const PeMate &q = m[i & 1];
+        const bool r2 = (i & 1) != 0;  // selects, not an index: the pair stays in registers
+        const PeMate q{r2 ? m[1].flag : m[0].flag, r2 ? m[1].tid : m[0].tid,   r2 ? m[1].pos : m[0].pos,
+                       r2 ? m[1].ntid : m[0].ntid, r2 ? m[1].npos : m[0].npos, r2 ? m[1].tlen : m[0].tlen};
         const int64_t row = src_row(s, i);
         r.flag = q.flag;
         r.tid = q.tid;
