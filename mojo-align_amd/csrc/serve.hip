@@ -41,7 +41,7 @@ constexpr int64_t RESERVE_READS_MAX = 1 << 20;  // larger launches grow the slot
 constexpr double EST_FIXED_MS = 0.25;
 constexpr double EST_MS_PER_READ = 1.0e-5;  // 100M reads/s
 constexpr double DEFAULT_SIZE_WEIGHT = 0.5;
-constexpr int REC_FIELDS = 7;  // cid pos mapq flag sl sr nm
+constexpr int REC_FIELDS = 10;  // cid pos mapq flag sl sr nm iop iat ilen
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
@@ -101,7 +101,8 @@ bool compatible(const Req &a, int32_t mode_a, const Req &b, int32_t mode_b) {
 int64_t record_pitch(const ma_recs &o, int64_t n) {
     const char *f[REC_FIELDS] = {(const char *)o.cid, (const char *)o.pos, (const char *)o.mapq,
                                  (const char *)o.flag, (const char *)o.sl, (const char *)o.sr,
-                                 (const char *)o.nm};
+                                 (const char *)o.nm, (const char *)o.iop, (const char *)o.iat,
+                                 (const char *)o.ilen};
     for (int i = 0; i < REC_FIELDS; i++)
         if (!f[i]) return 0;
     const int64_t pitch = f[1] - f[0];
@@ -191,7 +192,9 @@ int pump(ma_server *s) {
         dr.sl = (uint32_t *)(rec + 4 * total * 4);
         dr.sr = (uint32_t *)(rec + 5 * total * 4);
         dr.nm = (uint32_t *)(rec + 6 * total * 4);
-        dr.iop = dr.iat = dr.ilen = nullptr;
+        dr.iop = (uint32_t *)(rec + 7 * total * 4);
+        dr.iat = (uint32_t *)(rec + 8 * total * 4);
+        dr.ilen = (uint32_t *)(rec + 9 * total * 4);
         int rc = ma_map_batch_dev(s->ix, s->ws[slot], st, d_bases, head->stride, d_lens, total, head->max_len,
                                   head->encoding, mode, mode, total, &head->p, &dr, nullptr, nullptr);
         if (rc != MA_OK) return rc;
@@ -202,7 +205,7 @@ int pump(ma_server *s) {
                                         (size_t)total * 4, (size_t)pc.cnt * 4, REC_FIELDS,
                                         hipMemcpyDeviceToHost, st));
             } else {
-                void *dst[REC_FIELDS] = {o.cid, o.pos, o.mapq, o.flag, o.sl, o.sr, o.nm};
+                void *dst[REC_FIELDS] = {o.cid, o.pos, o.mapq, o.flag, o.sl, o.sr, o.nm, o.iop, o.iat, o.ilen};
                 for (int f = 0; f < REC_FIELDS; f++)
                     if (dst[f])
                         MA_HIP(hipMemcpyAsync((char *)dst[f] + pc.off * 4, rec + ((size_t)f * total + pc.at) * 4,

@@ -96,3 +96,29 @@ def test_indels_with_offsets_in_hbm():
     want, _, _, _ = orc.map_batch(oix, orc.convert_reads(reads.bases, 1), reads.lens, orc.params(max_indel=4),
                                   nthreads=8)
     assert_all_fields_equal(got, want, "noisy, 600 contigs")
+
+
+def test_served_loop_with_indels(genome):
+    """The served request loop with max_indel 4: requests of mixed sizes (one
+    of planted indels), merged and cut into 1024-read launches, give the same
+    records as each batch mapped alone, iop / iat / ilen included."""
+    say("test_served_loop_with_indels: 6 requests, cap 4")
+    from mialign import serve
+
+    ref, dual, gix, _oix = genome
+    rng = np.random.default_rng(17)
+    batches = [planted_reads(dual, 40, 500)[:2]]
+    for i, n in enumerate((1, 900, 3000, 64, 2500)):
+        rd = synth.gen_reads(41 + i, ref, n, 50, 150, max_len=150, p_err=0.02)
+        batches.append((orc.convert_reads(rd.bases, 1), rd.lens))
+    reqs = [serve.Request(serve.pinned_copy(b), serve.pinned_copy(l), 150, 0,
+                          arrival=float(rng.uniform(0, 0.01))) for b, l in batches]
+    prm = default_params(k=21, max_indel=4)
+    srv = serve.Server(gix, slots=3, hbm_budget=2 << 20, launch_reads=1024)
+    st, outs = serve.run_request_loop(srv, reqs, params=prm, keep_outputs=True)
+    srv.close()
+    assert len(outs) == len(reqs)
+    for i, (b, l) in enumerate(batches):
+        alone = mialign.map_batch(gix, b, l, 150, params=prm)
+        assert_all_fields_equal(outs[i], alone, f"request {i} served vs alone")
+    assert (outs[0]["iop"] != 0).sum() >= 400
