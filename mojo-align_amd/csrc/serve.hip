@@ -198,15 +198,17 @@ int pump(ma_server *s) {
         int rc = ma_map_batch_dev(s->ix, s->ws[slot], st, d_bases, head->stride, d_lens, total, head->max_len,
                                   head->encoding, mode, mode, total, &head->p, &dr, nullptr, nullptr);
         if (rc != MA_OK) return rc;
+        // gapless launches leave iop / iat / ilen zero: those three are not copied
+        const int n_fields = head->p.max_indel > 0 ? REC_FIELDS : 7;
         for (const Piece &pc : pieces) {
             const ma_recs &o = pc.r->out;
             if (pc.r->out_pitch) {
                 MA_HIP(hipMemcpy2DAsync((char *)o.cid + pc.off * 4, (size_t)pc.r->out_pitch, rec + pc.at * 4,
-                                        (size_t)total * 4, (size_t)pc.cnt * 4, REC_FIELDS,
+                                        (size_t)total * 4, (size_t)pc.cnt * 4, (size_t)n_fields,
                                         hipMemcpyDeviceToHost, st));
             } else {
                 void *dst[REC_FIELDS] = {o.cid, o.pos, o.mapq, o.flag, o.sl, o.sr, o.nm, o.iop, o.iat, o.ilen};
-                for (int f = 0; f < REC_FIELDS; f++)
+                for (int f = 0; f < n_fields; f++)
                     if (dst[f])
                         MA_HIP(hipMemcpyAsync((char *)dst[f] + pc.off * 4, rec + ((size_t)f * total + pc.at) * 4,
                                               (size_t)pc.cnt * 4, hipMemcpyDeviceToHost, st));
