@@ -1,26 +1,26 @@
 """Benchmark: reads/s of the seed -> extend -> sort path on one MI355X.
 
-Workload (default): synthetic 100 Mbp reference in 10 contigs (seeded ACGT,
-CpG depleted to ~1% of dinucleotides), dual C->T / G->A conversion, k=21
-dense-v1 index resident in HBM (hash table over u64 keys, u32 postings);
-10M single-end 150 bp bisulfite reads (non-CpG C->T p=0.99, CpG kept p=0.7,
-0.5% substitutions, 5% planted duplicates), 2-bit packed u8 input resident in
-HBM. One step = map every read (in-silico C->T, seeds, vote, extend, rescue,
-high-occ tally, FW + RC, merge) + SE sort keys + stable coordinate sort +
-duplicate marking, all on the device.
+Headline workload: 150 bp WGBS reads against a 3.1 Gbp synthetic reference
+(24 contigs, CpG depleted to ~1% of dinucleotides, 5% low-complexity repeat
+blocks), dual C->T / G->A conversion (6.2 Gbp indexed), k=21 index built on
+the device and resident in HBM; 10M single-end bisulfite reads (non-CpG C->T
+p=0.99, CpG kept p=0.7, 0.5% substitutions, 5% planted duplicates), 2-bit
+packed u8 input resident in HBM. One step = map every read (in-silico C->T,
+seeds, vote, extend, rescue, high-occ tally, FW + RC, merge) + SE sort keys +
+stable coordinate sort + duplicate marking, all on the device.
 
-Prints one JSON line. `roofline` is for the dominant kernel (the map kernel);
+Prints one JSON line. `roofline` is for the dominant kernel (the map kernel),
+priced with the algorithmic bytes of the oracle's counts on a sample;
 `cpu_baseline` times the oracle (the C restatement of the reference's CPU
 science, OpenMP over reads + the stable-sort markdup) on a bounded sample of
-the same reads on this host. Extra sections: `serving` (the served request
-loop against plain launches), `sort_stress` (200M sort + markdup records),
-`genome_scale` (the 3.1 Gbp reference), `bam_emit` (the step's records as
-coordinate-ordered BAM with duplicate flags: device pack, BGZF on the device
-or on host threads) and `fastq_ingest` (the reads' FASTQ text parsed on the
-device), each with its own roofline, parity check and CPU baseline; `fq2bam`
-(FASTQ file to BAM file end to end), `pe_emit` (the step's records as pairs: PE
-keys, sort + markdup, PE pack) and `indel_mode` (the same step with the
-single-indel branch on, max_indel 4, parity-checked on a sample).
+the same reads on this host, on every thread and on one. `parity_sample`
+compares that sample's records; `sort_parity` the step's keys, coordinate
+order and duplicate flags of every record with the oracle's. Extra sections:
+`uneven_50_250bp` (50M reads of 50-250 bp in one persistent launch),
+`serving` (the served request loop on the same index against plain
+launches), `sort_stress` (200M sort + markdup records) and `config2_100mbp`
+(a 100 Mbp reference, 10M x 150 bp, with the sections built on its records:
+`indel_mode`, `bam_emit`, `pe_emit`, `fastq_ingest`, `fq2bam`).
 """
 
 from __future__ import annotations
@@ -44,17 +44,6 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def algorithmic_bytes_per_read(cnt, n_sample: int, read_len: int) -> float:
-    """Bytes the algorithm must move per read: 2-bit read in, merged record out
-    (7 x u32), one 16-byte hash slot per seed probe, one 8-byte posting per
-    posting visited, one 2-bit reference window per window compared (counts
-    from the oracle's sequential walk of the same reads)."""
-    w = (read_len + 3) // 4
-    per = w + 28
-    per += (16 * cnt.probes + 8 * cnt.postings + w * cnt.windows) / n_sample
-    return per
 
 
 def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int, size_weight: float,
@@ -222,6 +211,7 @@ class StepRunner:
         self.d_bases = mialign.DeviceArray.from_host(packed, dev)
         self.d_lens = mialign.DeviceArray.from_host(lens, dev)
         self.d_tid = mialign.DeviceArray.from_host(tid, dev)
+        self.n_tid = int(tid.size)
         self.recs = mialign.DeviceRecs(self.n, dev)
         self.ws = mialign.Workspace(ix, self.n, L, persistent=persistent)
         self.prm = mialign.default_params(k=21)
@@ -244,7 +234,7 @@ class StepRunner:
         if timed_kernel:
             self.ev_b.record(st)
         check(lib().ma_sort_keys_se_dev(st.handle, n, C.byref(self.recs.struct), self.d_lens.ptr, self.d_tid.ptr,
-                                        0, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
+                                        self.n_tid, 0, keys["coord"].ptr, keys["dhi"].ptr, keys["dlo"].ptr,
                                         keys["score"].ptr, keys["pair"].ptr), "sort keys")
         check(lib().ma_sort_markdup_dev(st.handle, n, keys["coord"].ptr, keys["dhi"].ptr,
                                         keys["dlo"].ptr, keys["score"].ptr, keys["pair"].ptr,
@@ -271,30 +261,6 @@ class StepRunner:
     def close(self) -> None:
         for a in ("d_bases", "d_lens", "d_tid", "recs", "ws", "keys", "outs", "scratch"):
             setattr(self, a, None)
-
-
-def cpu_map_baseline(ix, dual, bases, lens, tid, ns, what):
-    """Oracle on the first ns reads (OpenMP over reads + SE keys + stable-sort
-    markdup) on the host's threads; returns (baseline dict, oracle records,
-    oracle counters)."""
-    from oracle import oracle as orc
-
-    t0 = time.time()
-    km, of, po = ix.export_dense()
-    oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
-    del km, of, po
-    conv = orc.convert_reads(bases[:ns], 1)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    log(f"cpu baseline ({what}): oracle index ready in {time.time() - t0:.1f}s; mapping {ns} reads on {threads} threads")
-    t0 = time.perf_counter()
-    want, _, _, cnt = orc.map_batch(oix, conv, lens[:ns], nthreads=threads)
-    kk = orc.sort_keys_se(want, lens[:ns], tid)
-    orc.sort_markdup(*kk)
-    cpu_s = time.perf_counter() - t0
-    cpu = {"value": ns / cpu_s, "unit": "reads/s", "cores": threads, "kind": "port",
-           "sample": f"first {ns} reads of {what} (oracle map with OpenMP over reads + SE keys + "
-                     f"stable-sort markdup), {cpu_s:.1f} s"}
-    return cpu, want, cnt
 
 
 FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
@@ -737,85 +703,286 @@ def bam_section(sr, reads, tid, n_tid: int, blob, name_off, qual, with_cpu: bool
     return out
 
 
-def genome_scale_section(args) -> dict:
-    """The 3.1 Gbp reference of the uneven-length and served configurations
-    (24 contigs, 5% low-complexity repeat blocks, both conversions indexed:
-    6.2 Gbp): (1) 150 bp reads, map + sort + markdup, next to the oracle on the
-    host's threads on a sample of the same reads (records compared); (2) reads
-    of 50-250 bp in one persistent launch per step; (3) the served loop over
-    those uneven reads with the index resident."""
+def cpu_info() -> dict:
+    """CPU model name and logical CPU count of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count()}
+
+
+def map_bytes_per_read(cnt, n_sample: int, lens: np.ndarray) -> float:
+    """Algorithmic bytes of one read's map: its 2-bit bases in, a 16-byte
+    record out, and per pass (FW, plus RC where it runs) 24 bytes per seed
+    probe (u64 key + u64 range), 4 bytes per posting visited (u32 position)
+    and one 2-bit window of the read's length per reference window compared.
+    probes / postings / windows are the oracle's counts over its sequential
+    walk of the same reads; for uneven lengths the window term uses the
+    sample's mean 2-bit read size."""
+    w = float(np.mean((lens[:n_sample].astype(np.int64) + 3) // 4))
+    return w + 16 + (24 * cnt.probes + 4 * cnt.postings + w * cnt.windows) / n_sample
+
+
+def oracle_index(ix, dual):
+    from oracle import oracle as orc
+
+    t0 = time.time()
+    km, of, po = ix.export_dense()
+    oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+    log(f"oracle index ready in {time.time() - t0:.1f}s")
+    return oix
+
+
+def cpu_map_baseline(oix, bases, lens, tid, ns, ns1, what):
+    """Oracle on the first ns reads (OpenMP over reads on every host thread +
+    SE keys + stable-sort markdup), and on the first ns1 reads with one thread;
+    returns (baseline dict, oracle records, oracle counters)."""
+    from oracle import oracle as orc
+
+    conv = orc.convert_reads(bases[:ns], 1)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    log(f"cpu baseline ({what}): mapping {ns} reads on {threads} threads")
+    t0 = time.perf_counter()
+    want, _, _, cnt = orc.map_batch(oix, conv, lens[:ns], nthreads=threads)
+    orc.sort_markdup(*orc.sort_keys_se(want, lens[:ns], tid))
+    cpu_s = time.perf_counter() - t0
+    ns1 = min(ns1, ns)
+    t0 = time.perf_counter()
+    w1, _, _, _ = orc.map_batch(oix, conv[:ns1], lens[:ns1], nthreads=1)
+    orc.sort_markdup(*orc.sort_keys_se(w1, lens[:ns1], tid))
+    cpu1_s = time.perf_counter() - t0
+    cpu = {"value": round(ns / cpu_s, 1), "unit": "reads/s", "cores": threads, "kind": "port",
+           "value_1core": round(ns1 / cpu1_s, 1), **cpu_info(),
+           "sample": f"first {ns} reads of {what} (oracle map with OpenMP over reads on {threads} threads + SE "
+                     f"keys + stable-sort markdup), {cpu_s:.1f} s; one thread: first {ns1} reads, {cpu1_s:.1f} s"}
+    log(f"cpu baseline ({what}): {cpu['value']:.0f} reads/s on {threads} threads, "
+        f"{cpu['value_1core']:.0f} reads/s on one")
+    return cpu, want, cnt
+
+
+def sort_parity(sr, merged, lens, tid) -> dict:
+    """The step's keys, coordinate order and duplicate flags against the
+    oracle's keys of the same records and its stable sorts + markdup scan,
+    over every record of the step."""
+    from oracle import oracle as orc
+
+    t0 = time.time()
+    keys = orc.sort_keys_se(merged, lens, tid)
+    got = [sr.keys[f].download() for f in ("coord", "dhi", "dlo", "score", "pair")]
+    keys_ok = all(np.array_equal(a, b) for a, b in zip(keys, got))
+    d, p, _ = orc.sort_markdup(*keys)
+    perm_ok = bool(np.array_equal(p, sr.outs["perm"].download()))
+    dup_ok = bool(np.array_equal(d, sr.outs["is_dup"].download()))
+    log(f"sort parity over {lens.size} records: keys {keys_ok}, perm {perm_ok}, is_dup {dup_ok} "
+        f"({time.time() - t0:.1f}s)")
+    return {"records": int(lens.size), "keys_bit_exact": bool(keys_ok), "perm_bit_exact": perm_ok,
+            "is_dup_bit_exact": dup_ok, "duplicates": int(d.sum())}
+
+
+def roofline_of(bpr: float, n: int, kernel_ms: float, traffic=None, **extra) -> dict:
+    achieved = bpr * n / (kernel_ms / 1000.0) / 1e9
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "map_reads_kernel",
+           "kernel_ms": round(kernel_ms, 3), "algorithmic_bytes_per_read": round(bpr, 1)}
+    out.update(extra)
+    return out
+
+
+class GenomeScale:
+    """The 3.1 Gbp reference of the north-star, uneven-length and served
+    workloads: 24 contigs, 1% CpG, 5% low-complexity repeat blocks, both
+    conversions indexed (6.2 Gbp), k=21 index built on the device."""
+
+    NAME = "3.1 Gbp synthetic (24 contigs, 1% CpG, 5% low-complexity repeat blocks) x2 conversions"
+
+    def __init__(self, keep_dense: bool, genome_bp: int = 3_100_000_000):
+        import mialign
+        from mialign import synth
+
+        t0 = time.time()
+        self.ref = synth.gen_reference(31, 24, genome_bp, repeat_frac=0.05)
+        self.dual = synth.dual_convert(self.ref)
+        self.tid = synth.tid_of_cid(self.dual.names)
+        log(f"genome scale: reference 3.1 Gbp x2 in {time.time() - t0:.1f}s")
+        t0 = time.time()
+        self.ix = mialign.GpuIndex.build(self.dual.seq, self.dual.contig_off, 21, 0, keep_dense=keep_dense)
+        tb = time.time() - t0
+        self.index = {"keys": self.ix.n_keys, "postings": self.ix.n_postings, "build_s": round(tb, 1),
+                      "table": self.ix.hash_stats()}
+        log(f"genome scale: index {self.ix.n_keys} keys, {self.ix.n_postings} postings in {tb:.1f}s, "
+            f"{self.index['table']}")
+        self._oix = None
+
+    def oracle(self):
+        if self._oix is None:
+            self._oix = oracle_index(self.ix, self.dual)
+        return self._oix
+
+    def close(self) -> None:
+        self._oix = None
+        self.ix.close()
+
+
+# HBM bytes one map launch moves on the headline workload (3.1 Gbp x2
+# reference, 10M x 150 bp reads), measured as FETCH_SIZE x the copy kernel's
+# factor + WRITE_SIZE x its factor; None until profiled.
+MAP_TRAFFIC_PROFILED_GS = None
+
+
+def headline_section(args, gs: GenomeScale) -> dict:
+    """North-star workload: 10M single-end 150 bp bisulfite reads against the
+    3.1 Gbp x2 index; one step = map (FW + RC + merge) + SE keys + stable
+    coordinate sort + duplicate marking, on one stream. Records of a sample
+    equal the oracle's; keys, order and duplicate flags of every record equal
+    the oracle's sort of the same records."""
+    from mialign import synth
+
+    n, L = args.reads, 150
+    t0 = time.time()
+    reads = synth.gen_reads(1102, gs.ref, n, L, L)
+    packed = synth.pack_2bit(reads.bases, reads.lens)
+    log(f"headline: {n} x {L} bp reads generated + 2-bit packed in {time.time() - t0:.1f}s")
+    sr = StepRunner(gs.ix, packed, reads.lens, L, gs.tid)
+    v, ms, map_ms = sr.run(args.steps, args.warmup, "headline 3.1 Gbp 150 bp")
+    log(f"headline: {args.steps} steps, {ms:.1f} ms/step, map kernel {map_ms:.1f} ms, {v / 1e6:.2f} M reads/s")
+    merged = sr.recs.download()
+    out = {"value": v, "ms_per_step": ms, "map_ms": map_ms, "reads": n, "read_len": L,
+           "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
+           "duplicates_marked": int(sr.outs["is_dup"].download().sum()),
+           "cpu_baseline": None, "roofline": None, "parity_sample": None, "sort_parity": None}
+    if args.headline_only:
+        sr.close()
+        return out
+    out["sort_parity"] = sort_parity(sr, merged, reads.lens, gs.tid)
+    sr.close()
+    del packed
+    if not args.no_cpu:
+        ns = min(args.gs_cpu_sample, n)
+        cpu, want, cnt = cpu_map_baseline(gs.oracle(), reads.bases, reads.lens, gs.tid, ns, args.cpu_1core_sample,
+                                          "the 3.1 Gbp 150 bp workload")
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
+        bpr = map_bytes_per_read(cnt, ns, reads.lens)
+        tp = MAP_TRAFFIC_PROFILED_GS
+        traffic = tp["bytes"] if tp and (tp["reads"], tp["read_len"]) == (n, L) else None
+        detail = None
+        if traffic:
+            detail = {"per_read": round(traffic / n, 1), "x_algorithmic": round(traffic / (bpr * n), 2),
+                      "profiled_kernel_ms": tp["kernel_ms"], "source": tp["source"]}
+        out["cpu_baseline"] = cpu
+        out["parity_sample"] = {"reads": ns, "records_bit_exact": bool(ok), "fields": list(FIELDS)}
+        out["roofline"] = roofline_of(bpr, n, map_ms, traffic, traffic_detail=detail,
+                                      oracle_counts_per_read={"probes": round(cnt.probes / ns, 2),
+                                                              "postings": round(cnt.postings / ns, 2),
+                                                              "windows": round(cnt.windows / ns, 2),
+                                                              "rc_pass_frac": round(cnt.reads_rc / ns, 3)})
+        log(f"headline: sample parity {ok}, {bpr:.0f} B/read algorithmic, x{v / cpu['value']:.0f} the host")
+    return out
+
+
+def uneven_section(args, gs: GenomeScale) -> dict:
+    """50M reads of 50-250 bp (uniform, as after adapter trimming) against the
+    3.1 Gbp x2 index in one persistent map launch per step (+ keys, sort,
+    markdup); a sample of the records against the oracle."""
+    from mialign import synth
+
+    n2 = args.gs_uneven_reads
+    t0 = time.time()
+    ur = synth.gen_reads(1103, gs.ref, n2, 50, 250, max_len=250)
+    upacked = synth.pack_2bit(ur.bases, ur.lens)
+    log(f"uneven: {n2} reads generated + packed in {time.time() - t0:.1f}s")
+    sr = StepRunner(gs.ix, upacked, ur.lens, 250, gs.tid)
+    v, ms, map_ms = sr.run(max(1, args.steps // 2), 1, "uneven 50-250 bp")
+    merged = sr.recs.download()
+    sr.close()
+    res = {"reads": n2, "read_len": "uniform 50-250", "mean_len": round(float(ur.lens.mean()), 1),
+           "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3), "map_kernel_ms": round(map_ms, 3),
+           "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
+           "launch": "one persistent map launch over all reads per step"}
+    log(f"uneven 50-250 bp: {n2} reads, {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
+    if not args.no_cpu:
+        ns = min(args.gs_uneven_cpu_sample, n2)
+        cpu, want, cnt = cpu_map_baseline(gs.oracle(), ur.bases, ur.lens, gs.tid, ns,
+                                          args.cpu_1core_sample // 2, "the 3.1 Gbp 50-250 bp workload")
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
+        bpr = map_bytes_per_read(cnt, ns, ur.lens)
+        res["cpu_baseline"] = cpu
+        res["x_cpu"] = round(v / cpu["value"], 1)
+        res["parity_sample"] = {"reads": ns, "records_bit_exact": bool(ok), "fields": list(FIELDS)}
+        res["roofline"] = roofline_of(bpr, n2, map_ms)
+        log(f"uneven: sample parity {ok}, {bpr:.0f} B/read algorithmic")
+        del want
+    res["_packed"], res["_lens"] = upacked, ur.lens
+    return res
+
+
+def config2_section(args) -> dict:
+    """The single-large-batch workload: 100 Mbp x2 reference (10 contigs),
+    10M x 150 bp reads, the same step; then the sections that start from its
+    records (indel mode, BAM emit, PE emit, FASTQ ingest, FASTQ -> BAM)."""
     import mialign
     from mialign import synth
 
-    res = {"reference": "3.1 Gbp synthetic (24 contigs, 1% CpG, 5% low-complexity repeat blocks) x2 conversions"}
     t0 = time.time()
-    ref = synth.gen_reference(31, 24, 3_100_000_000, repeat_frac=0.05)
+    ref = synth.gen_reference(1001, args.contigs, args.genome)
     dual = synth.dual_convert(ref)
-    tid = synth.tid_of_cid(dual.names)
-    log(f"genome scale: reference 3.1 Gbp x2 in {time.time() - t0:.1f}s")
+    log(f"config 2: reference {args.genome} bp x2 conversions in {time.time() - t0:.1f}s")
     t0 = time.time()
-    ix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, 0, keep_dense=not args.no_cpu)
-    tb = time.time() - t0
-    res["index"] = {"keys": ix.n_keys, "postings": ix.n_postings, "build_s": round(tb, 1),
-                    "table": ix.hash_stats()}
-    log(f"genome scale: index {ix.n_keys} keys, {ix.n_postings} postings in {tb:.1f}s, {res['index']['table']}")
-
-    # (1) 150 bp reads
-    n, L = args.gs_reads, 150
-    reads = synth.gen_reads(1102, ref, n, L, L)
+    ix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, 0)
+    log(f"config 2: index {ix.n_keys} keys, {ix.n_postings} postings in {time.time() - t0:.1f}s")
+    n, L = args.reads, args.read_len
+    reads = synth.gen_reads(1002, ref, n, L, L)
     packed = synth.pack_2bit(reads.bases, reads.lens)
-    sr = StepRunner(ix, packed, reads.lens, L, tid)
-    v, ms, map_ms = sr.run(args.steps, args.warmup, "genome scale 150 bp")
+    tid = synth.tid_of_cid(dual.names)
+    sr = StepRunner(ix, packed, reads.lens, L, tid, persistent=args.launch == "persistent")
+    value, ms_per_step, map_ms = sr.run(args.steps, args.warmup, "config 2")
+    log(f"config 2: {ms_per_step:.1f} ms/step, map kernel {map_ms:.1f} ms, {value / 1e6:.2f} M reads/s")
     merged = sr.recs.download()
-    sr.close()
-    del packed
-    r150 = {"reads": n, "read_len": L, "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3),
-            "map_kernel_ms": round(map_ms, 3), "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4)}
-    log(f"genome scale 150 bp: {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
+    out = {"workload": f"single large batch: {args.genome // 1_000_000} Mbp reference ({args.contigs} contigs, "
+                       f"1% CpG) x2 conversions, {n} SE {L} bp bisulfite reads, 2-bit packed u8 input",
+           "reads": n, "read_len": L, "reads_per_s": round(value, 1), "ms_per_step": round(ms_per_step, 3),
+           "map_kernel_ms": round(map_ms, 3), "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4)}
     if not args.no_cpu:
-        ns = min(args.gs_cpu_sample, n)
-        cpu, want, cnt = cpu_map_baseline(ix, dual, reads.bases, reads.lens, tid, ns,
-                                          "the 3.1 Gbp 150 bp workload")
-        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
-        bpr = algorithmic_bytes_per_read(cnt, ns, L)
-        achieved = bpr * n / (map_ms / 1000.0) / 1e9
-        r150["cpu_baseline"] = cpu
-        r150["x_cpu"] = round(v / cpu["value"], 1)
-        r150["parity_sample"] = {"reads": ns, "records_bit_exact": bool(ok)}
-        r150["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "map_reads_kernel",
-                            "kernel_ms": round(map_ms, 3), "algorithmic_bytes_per_read": round(bpr, 1)}
-        log(f"genome scale 150 bp: cpu {cpu['value']:.0f} reads/s ({r150['x_cpu']}x), sample parity {ok}, "
-            f"{bpr:.0f} B/read algorithmic")
-        del want
-    del reads, merged
-    res["reads_150bp"] = r150
-
-    # (2) uneven lengths, one persistent launch per step
-    n2 = args.gs_uneven_reads
-    ur = synth.gen_reads(1103, ref, n2, 50, 250, max_len=250)
-    upacked = synth.pack_2bit(ur.bases, ur.lens)
-    ulens = ur.lens
-    del ur
-    sr = StepRunner(ix, upacked, ulens, 250, tid)
-    v, ms, map_ms = sr.run(max(1, args.steps // 2), 1, "genome scale 50-250 bp")
-    merged = sr.recs.download()
+        out["sort_parity"] = sort_parity(sr, merged, reads.lens, tid)
+    out["indel_mode"] = None if args.no_indel else indel_section(sr, ix, dual, reads, not args.no_cpu,
+                                                                 args.indel_parity_sample)
+    if not (args.no_bam and args.no_fastq) or args.fq2bam_reads > 0:
+        blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
+        if not args.no_bam:
+            out["bam_emit"] = bam_section(sr, reads, tid, tid.size, blob, name_off, qual, not args.no_cpu,
+                                          args.bam_cpu_sample, args.bam_parity_sample)
+            out["pe_emit"] = pe_section(sr, reads, tid, blob, name_off, qual, not args.no_cpu)
+        sr.close()
+        if not args.no_fastq:
+            out["fastq_ingest"] = fastq_section(reads, blob, name_off, qual, not args.no_cpu,
+                                                args.fastq_cpu_sample, args.bam_parity_sample)
+        if args.fq2bam_reads > 0:
+            out["fq2bam"] = fq2bam_section(ix, dual, reads, blob, name_off, qual, args.fq2bam_reads)
+        del blob, name_off, qual
     sr.close()
-    res["reads_50_250bp"] = {"reads": n2, "read_len": "uniform 50-250", "mean_len": round(float(ulens.mean()), 1),
-                             "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3),
-                             "map_kernel_ms": round(map_ms, 3),
-                             "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
-                             "launch": "one persistent map launch over all reads per step"}
-    log(f"genome scale 50-250 bp: {n2} reads, {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
-    del merged
-
-    # (3) served loop over the uneven reads, index resident
-    if not args.no_serve:
-        res["serving"] = serve_section(ix, upacked, ulens, 250, args.serve_requests, args.serve_slots,
+    if args.serve_100mbp and not args.no_serve:
+        out["serving"] = serve_section(ix, packed, reads.lens, L, args.serve_requests, args.serve_slots,
                                        args.serve_launch_reads, args.serve_size_weight)
-        log(f"genome scale serving: {res['serving']}")
+    if not args.no_cpu:
+        oix = oracle_index(ix, dual)
+        ns = min(args.cpu_sample, n)
+        cpu, want, cnt = cpu_map_baseline(oix, reads.bases, reads.lens, tid, ns, args.cpu_1core_sample,
+                                          "the 100 Mbp workload")
+        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
+        bpr = map_bytes_per_read(cnt, ns, reads.lens)
+        tp = MAP_TRAFFIC_PROFILED
+        traffic = tp["bytes"] if (tp["genome_bp"], tp["reads"], tp["read_len"]) == (args.genome, n, L) else None
+        out["cpu_baseline"] = cpu
+        out["parity_sample"] = {"reads": ns, "records_bit_exact": bool(ok)}
+        out["roofline"] = roofline_of(bpr, n, map_ms, traffic)
+        del want, oix
     ix.close()
-    return res
+    return out
 
 
 def main() -> int:
@@ -823,13 +990,19 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--genome", type=int, default=100_000_000)
+    ap.add_argument("--reads", type=int, default=10_000_000, help="150 bp reads of the headline (3.1 Gbp) step")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the headline step (no checks, baselines or other sections; for profiling)")
+    ap.add_argument("--genome", type=int, default=100_000_000, help="reference size of the config-2 section")
     ap.add_argument("--contigs", type=int, default=10)
-    ap.add_argument("--reads", type=int, default=10_000_000)
+    ap.add_argument("--c2-reads", type=int, default=10_000_000, help="150 bp reads of the config-2 section")
     ap.add_argument("--read-len", type=int, default=150)
     ap.add_argument("--cpu-sample", type=int, default=5_000_000)
+    ap.add_argument("--cpu-1core-sample", type=int, default=100_000,
+                    help="reads the one-thread CPU baseline maps")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-100mbp", action="store_true", help="also serve on the 100 Mbp index")
     ap.add_argument("--serve-requests", type=int, default=512)
     ap.add_argument("--sort-records", type=int, default=200_000_000,
                     help="sort + markdup stress records (0 = skip)")
@@ -838,15 +1011,16 @@ def main() -> int:
     ap.add_argument("--serve-launch-reads", type=int, default=128 << 10)
     ap.add_argument("--serve-size-weight", type=float, default=0.5,
                     help="served order: arrival + weight x estimated stand-alone time (0 = arrival order)")
-    ap.add_argument("--serve-only", action="store_true", help="skip the timed batch steps")
     ap.add_argument("--launch", choices=("persistent", "batch"), default="persistent",
                     help="map-kernel grid: persistent (8 blocks/CU + work counter) or sized to the batch")
-    ap.add_argument("--no-genome-scale", action="store_true",
-                    help="skip the 3.1 Gbp reference section (150 bp + 50-250 bp reads, served loop)")
-    ap.add_argument("--gs-reads", type=int, default=10_000_000, help="150 bp reads on the 3.1 Gbp index")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 100 Mbp section and what builds on it")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 50-250 bp launch (and serving on it)")
+    ap.add_argument("--gs-genome", type=int, default=3_100_000_000,
+                    help="reference size of the headline, uneven and served workloads")
     ap.add_argument("--gs-uneven-reads", type=int, default=50_000_000,
                     help="50-250 bp reads on the 3.1 Gbp index (one launch)")
     ap.add_argument("--gs-cpu-sample", type=int, default=2_000_000)
+    ap.add_argument("--gs-uneven-cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-bam", action="store_true", help="skip the BAM emit section")
     ap.add_argument("--no-fastq", action="store_true", help="skip the FASTQ ingest section")
     ap.add_argument("--fastq-cpu-sample", type=int, default=2_000_000)
@@ -863,131 +1037,76 @@ def main() -> int:
               "(reads shard across GPUs with a replicated index; no multi-GPU run is made here)",
               file=sys.stderr)
         return 2
+    if args.headline_only:
+        args.no_cpu = args.no_serve = args.no_config2 = args.no_uneven = True
+        args.sort_records = 0
 
     import mialign
-    from mialign import synth
 
-    dev = 0
-    info = mialign.device_info(dev)
+    info = mialign.device_info(0)
     log(f"device {info['name']} cu={info['n_cu']} hbm={info['hbm_total'] / 2**30:.0f} GiB")
 
-    t0 = time.time()
-    ref = synth.gen_reference(1001, args.contigs, args.genome)
-    dual = synth.dual_convert(ref)
-    log(f"reference {args.genome} bp x2 conversions in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    ix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, dev)
-    log(f"GPU index build: {ix.n_keys} keys, {ix.n_postings} postings in {time.time() - t0:.1f}s")
+    gs = GenomeScale(keep_dense=not args.no_cpu, genome_bp=args.gs_genome)
+    head = headline_section(args, gs)
+    uneven = serving = None
+    if not args.no_uneven:
+        uneven = uneven_section(args, gs)
+        upacked, ulens = uneven.pop("_packed"), uneven.pop("_lens")
+        if not args.no_serve:
+            serving = serve_section(gs.ix, upacked, ulens, 250, args.serve_requests, args.serve_slots,
+                                    args.serve_launch_reads, args.serve_size_weight)
+            log(f"serving (3.1 Gbp, 50-250 bp): {serving}")
+        del upacked, ulens
+    index_info = gs.index
+    gs.close()
+    del gs
 
-    t0 = time.time()
-    n, L = args.reads, args.read_len
-    reads = synth.gen_reads(1002, ref, n, L, L)
-    packed = synth.pack_2bit(reads.bases, reads.lens)
-    log(f"reads {n} x {L} bp generated + 2-bit packed in {time.time() - t0:.1f}s")
-
-    tid = synth.tid_of_cid(dual.names)
-    sr = StepRunner(ix, packed, reads.lens, L, tid, persistent=args.launch == "persistent")
-    if args.serve_only:
-        args.warmup, args.steps = 0, 1
-    value, ms_per_step, map_ms = sr.run(args.steps, args.warmup, "config 2")
-    log(f"timed {args.steps} steps: {ms_per_step:.1f} ms/step, map kernel {map_ms:.1f} ms, "
-        f"{value / 1e6:.2f} M reads/s")
-
-    merged = sr.recs.download()
-    is_dup = sr.outs["is_dup"].download()
-    mapped_frac = float((merged["cid"] >= 0).mean())
-    indel = None if args.no_indel else indel_section(sr, ix, dual, reads, not args.no_cpu, args.indel_parity_sample)
-    bam = fastq = f2b = pe = None
-    if not (args.no_bam and args.no_fastq) or args.fq2bam_reads > 0:
-        blob, name_off, qual = synth.gen_names_quals(1004, reads.lens, L)
-        if not args.no_bam:
-            bam = bam_section(sr, reads, tid, tid.size, blob, name_off, qual, not args.no_cpu, args.bam_cpu_sample,
-                              args.bam_parity_sample)
-            pe = pe_section(sr, reads, tid, blob, name_off, qual, not args.no_cpu)
-        sr.close()
-        if not args.no_fastq:
-            fastq = fastq_section(reads, blob, name_off, qual, not args.no_cpu, args.fastq_cpu_sample,
-                                  args.bam_parity_sample)
-        if args.fq2bam_reads > 0:
-            f2b = fq2bam_section(ix, dual, reads, blob, name_off, qual, args.fq2bam_reads)
-        del blob, name_off, qual
-    sr.close()
-
-    # ---- served request loop vs plain one-batch-per-launch (same batches)
-    serving = None
-    if not args.no_serve:
-        serving = serve_section(ix, packed, reads.lens, L, args.serve_requests, args.serve_slots,
-                                args.serve_launch_reads, args.serve_size_weight)
-        log(f"serving: {serving}")
-
-    # ---- CPU baseline (oracle) on a bounded sample + sample parity + bytes/read
-    cpu = None
-    roofline = None
-    parity_sample = None
-    if not args.no_cpu:
-        ns = min(args.cpu_sample, n)
-        cpu, want, cnt = cpu_map_baseline(ix, dual, reads.bases, reads.lens, tid, ns, "the same workload")
-        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in FIELDS)
-        parity_sample = {"reads": ns, "records_bit_exact": bool(ok)}
-        bpr = algorithmic_bytes_per_read(cnt, ns, L)
-        achieved = bpr * n / (map_ms / 1000.0) / 1e9
-        tp = MAP_TRAFFIC_PROFILED
-        traffic, traffic_detail = None, None
-        if (tp["genome_bp"], tp["reads"], tp["read_len"]) == (args.genome, n, L):
-            traffic = tp["bytes"]
-            traffic_detail = {"per_read": round(tp["bytes"] / n, 1), "x_algorithmic": round(tp["bytes"] / (bpr * n), 2),
-                              "profiled_kernel_ms": tp["kernel_ms"], "source": tp["source"]}
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                    "traffic_detail": traffic_detail,
-                    "kernel": "map_reads_kernel", "kernel_ms": round(map_ms, 3),
-                    "algorithmic_bytes_per_read": round(bpr, 1)}
-        log(f"cpu baseline {cpu['value']:.0f} reads/s; sample parity {ok}; {bpr:.0f} B/read algorithmic")
-        del want
-    ix.close()
-    del reads, packed, dual, ref
+    c2 = None
+    if not args.no_config2:
+        a2 = argparse.Namespace(**vars(args))
+        a2.reads = args.c2_reads
+        c2 = config2_section(a2)
 
     sort_stress = None
     if args.sort_records > 0:
         sort_stress = sort_section(args.sort_records, args.sort_cpu_sample, not args.no_cpu)
 
-    genome_scale = None if args.no_genome_scale else genome_scale_section(args)
-
     line = {
         "metric": METRIC,
-        "value": round(value, 1),
+        "value": round(head["value"], 1),
         "unit": "reads/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
+        "ms_per_step": round(head["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64",
         "data": "synthetic",
         "config": {
-            "workload": f"single large batch: {args.genome // 1_000_000} Mbp reference "
-                        f"({args.contigs} contigs, 1% CpG) x2 conversions, {n} SE {L} bp bisulfite "
-                        "reads, 2-bit packed u8 input, k=21 index resident in HBM",
-            "reads": n,
-            "read_len": L,
-            "genome_bp": args.genome,
+            "workload": f"150 bp WGBS reads against a 3.1 Gbp synthetic reference: {GenomeScale.NAME}, "
+                        f"{head['reads']} SE {head['read_len']} bp bisulfite reads (non-CpG C->T 0.99, CpG kept "
+                        "0.7, 0.5% substitutions, 5% planted duplicates), 2-bit packed u8 input, k=21 index "
+                        "resident in HBM; step = map + SE keys + coordinate sort + duplicate marking",
+            "reads": head["reads"],
+            "read_len": head["read_len"],
+            "genome_bp": args.gs_genome,
             "k": 21,
-            "mapped_frac": round(mapped_frac, 4),
-            "duplicates_marked": int(is_dup.sum()),
+            "index": index_info,
+            "map_kernel_ms": round(head["map_ms"], 3),
+            "mapped_frac": head["mapped_frac"],
+            "duplicates_marked": head["duplicates_marked"],
         },
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "parity_sample": parity_sample,
+        "roofline": head["roofline"],
+        "cpu_baseline": head["cpu_baseline"],
+        "x_cpu": round(head["value"] / head["cpu_baseline"]["value"], 1) if head["cpu_baseline"] else None,
+        "parity_sample": head["parity_sample"],
+        "sort_parity": head["sort_parity"],
+        "uneven_50_250bp": uneven,
         "serving": serving,
         "sort_stress": sort_stress,
-        "genome_scale": genome_scale,
-        "bam_emit": bam,
-        "fastq_ingest": fastq,
-        "fq2bam": f2b,
-        "indel_mode": indel,
-        "pe_emit": pe,
+        "config2_100mbp": c2,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -202,9 +202,10 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
 
 /* ---- sort keys, sort, duplicate marking ---- */
 /* SE keys from merged records (device pointers): coord, dhi, dlo, score, pair
- * (pair = pair_base + i). d_tid_of_cid maps index contig -> canonical target. */
+ * (pair = pair_base + i). d_tid_of_cid[n_tid] maps index contig -> canonical
+ * target; index contigs outside [0, n_tid) are unmapped. */
 int ma_sort_keys_se_dev(void *stream, int64_t n, const ma_recs *d_merged, const uint32_t *d_lens,
-                        const int32_t *d_tid_of_cid, int64_t pair_base, uint64_t *d_coord,
+                        const int32_t *d_tid_of_cid, int32_t n_tid, int64_t pair_base, uint64_t *d_coord,
                         uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair);
 /* PE keys of n_pairs pairs (device pointers): the merged records and lens hold
  * R1 block then R2 block (2 n_pairs each); the 2 n_pairs keys come out in

@@ -198,8 +198,9 @@ int pump(ma_server *s) {
         int rc = ma_map_batch_dev(s->ix, s->ws[slot], st, d_bases, head->stride, d_lens, total, head->max_len,
                                   head->encoding, mode, mode, total, &head->p, &dr, nullptr, nullptr);
         if (rc != MA_OK) return rc;
-        // gapless launches leave iop / iat / ilen zero: those three are not copied
-        const int n_fields = head->p.max_indel > 0 ? REC_FIELDS : 7;
+        // all ten fields go back, so a caller's iop / iat / ilen arrays hold the
+        // launch's values (zero for gapless launches) whatever they held before
+        const int n_fields = REC_FIELDS;
         for (const Piece &pc : pieces) {
             const ma_recs &o = pc.r->out;
             if (pc.r->out_pitch) {

@@ -117,14 +117,16 @@ __global__ void sort_keys_se_kernel(int64_t n, const int32_t *__restrict__ cid,
                                     const uint32_t *__restrict__ sl, const uint32_t *__restrict__ sr,
                                     const uint32_t *__restrict__ nm, const uint32_t *__restrict__ mapq,
                                     const uint32_t *__restrict__ lens,
-                                    const int32_t *__restrict__ tid_of_cid, int64_t pair_base,
+                                    const int32_t *__restrict__ tid_of_cid, int64_t n_tid, int64_t pair_base,
                                     uint64_t *__restrict__ coord, uint64_t *__restrict__ dhi,
                                     uint64_t *__restrict__ dlo, uint32_t *__restrict__ score,
                                     uint32_t *__restrict__ pairo) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t c = cid[i];
-        int64_t tid = c >= 0 ? (int64_t)tid_of_cid[c] : -1;
+        // a contig outside the target table has no target (rid1 < len(rid_to_tid),
+        // linear_gpu_fm.mojo:764-765)
+        int64_t tid = c >= 0 && c < n_tid ? (int64_t)tid_of_cid[c] : -1;
         int64_t fl = flag[i];
         int64_t p0 = (int64_t)pos[i];
         coord[i] = coord_key(tid, p0, i);
@@ -249,12 +251,12 @@ int ma_radix_sort_u64_dev(void *stream, int64_t n, uint64_t *d_keys, uint32_t *d
 }
 
 int ma_sort_keys_se_dev(void *stream, int64_t n, const ma_recs *m, const uint32_t *d_lens,
-                        const int32_t *d_tid_of_cid, int64_t pair_base, uint64_t *d_coord,
+                        const int32_t *d_tid_of_cid, int32_t n_tid, int64_t pair_base, uint64_t *d_coord,
                         uint64_t *d_dhi, uint64_t *d_dlo, uint32_t *d_score, uint32_t *d_pair) {
-    MA_REQUIRE(n >= 0 && m, "ma_sort_keys_se_dev: bad arguments");
+    MA_REQUIRE(n >= 0 && m && n_tid >= 0, "ma_sort_keys_se_dev: bad arguments");
     if (n == 0) return MA_OK;
     sort_keys_se_kernel<<<grid_for(n), 256, 0, (hipStream_t)stream>>>(
-        n, m->cid, m->pos, m->flag, m->sl, m->sr, m->nm, m->mapq, d_lens, d_tid_of_cid, pair_base,
+        n, m->cid, m->pos, m->flag, m->sl, m->sr, m->nm, m->mapq, d_lens, d_tid_of_cid, n_tid, pair_base,
         d_coord, d_dhi, d_dlo, d_score, d_pair);
     MA_LAUNCHED();
     return MA_OK;

@@ -94,24 +94,31 @@ class _Acc:
 def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str = "", bs_r2: str = "G2A",
                  batch_reads: int = 1 << 20, stride: int = 256, do_markdup: bool = True, device_bgzf: bool = True,
                  level: int = 1, rg_id: str = "mojo1", sm: str = "sample", lb: str = "lib1", pl: str = "ILLUMINA",
-                 pg_vn: str = "0.1.0-mojo-fm", device: int = 0) -> dict:
+                 pg_vn: str = "0.1.0-mojo-fm", device: int = 0, params=None) -> dict:
     """FASTQ (single-end, or paired with fq2) -> BAM in coordinate order, 0x400
     on duplicates. `index` is a LinearIndex with its GPU index loaded. Pairs are
     mapped R1 with bs_r1 and R2 with bs_r2, emitted R1, R2 per pair with the PE
-    flags and mate fields, and marked as pairs. Returns counts and stage times."""
+    flags and mate fields, and marked as pairs. `params` are the mapping knobs
+    (default: the reference defaults for the index's k); the BAM records carry
+    slS mM srS CIGARs only, as the reference's native-BAM records do, so
+    params.max_indel must be 0. Returns counts and stage times."""
     paired = bool(fq2)
+    gix = index.gpu
+    prm = default_params(k=gix.k) if params is None else params
+    if prm.k != gix.k:
+        raise MialignError(f"params.k {prm.k} differs from the index k {gix.k}")
+    if prm.max_indel != 0:
+        raise MialignError("BAM output carries gapless records only (slS mM srS): params.max_indel must be 0")
     for m in (bs_r1, bs_r2) if paired else (bs_r1,):
         if m not in BS_MODES:
             raise MialignError(f"unknown conversion mode {m!r} (C2T, G2A or none)")
     if stride > 256:
         raise MialignError("reads longer than 256 bases are not supported")
-    gix = index.gpu
     st = Stream(device)
     t0 = time.perf_counter()
     reader = DeviceFastqReader(fq1, fq2, bs_r1, bs_r2 if paired else "", batch_reads=batch_reads, stride=stride,
                                device=device, with_conv=False)
     ws = Workspace(gix, batch_reads, stride)
-    prm = default_params(k=gix.k)
     sides = [_Acc(batch_reads, stride, device) for _ in range(2 if paired else 1)]
     recs_b = [DeviceRecs(batch_reads, device) for _ in sides]
     modes = [BS_MODES[bs_r1]] + ([BS_MODES[bs_r2]] if paired else [])
@@ -161,7 +168,8 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
         check(lib().ma_sort_keys_pe_dev(st.handle, N // 2, C.byref(recs), acc.lens.buf.ptr, d_tid.ptr, tid.size, 0,
                                         *kargs), "ma_sort_keys_pe_dev")
     else:
-        check(lib().ma_sort_keys_se_dev(st.handle, N, C.byref(recs), acc.lens.buf.ptr, d_tid.ptr, 0, *kargs),
+        check(lib().ma_sort_keys_se_dev(st.handle, N, C.byref(recs), acc.lens.buf.ptr, d_tid.ptr, tid.size, 0,
+                                        *kargs),
               "ma_sort_keys_se_dev")
     perm = DeviceArray((N,), np.uint32, device)
     is_dup = DeviceArray((N,), np.uint32, device)

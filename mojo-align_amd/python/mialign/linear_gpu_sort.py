@@ -51,7 +51,8 @@ def sort_keys_se(recs: dict, lens: np.ndarray, tid_of_cid: np.ndarray, pair_base
     d_lens = DeviceArray.from_host(np.ascontiguousarray(lens, dtype=np.uint32), device)
     d_tid = DeviceArray.from_host(np.ascontiguousarray(tid_of_cid, dtype=np.int32), device)
     outs = [DeviceArray((n1,), dt, device) for dt in (np.uint64, np.uint64, np.uint64, np.uint32, np.uint32)]
-    check(lib().ma_sort_keys_se_dev(st.handle, n, C.byref(d.struct), d_lens.ptr, d_tid.ptr, pair_base,
+    check(lib().ma_sort_keys_se_dev(st.handle, n, C.byref(d.struct), d_lens.ptr, d_tid.ptr, d_tid.shape[0],
+                                    pair_base,
                                     *[o.ptr for o in outs]), "ma_sort_keys_se_dev")
     st.sync()
     return tuple(o.download()[:n] for o in outs)

@@ -59,9 +59,16 @@ def map_fastq_to_sam(ref_fasta: str, fq1: str, out_sam: str, device: str = "auto
             # native BAM like the reference's fm driver (_path_is_bam, linear_gpu_fm.mojo:84-88):
             # coordinate-sorted, duplicates marked, every stage on the GPU; pairs as R1, R2
             from .fq2bam import fastq_to_bam
-            from .linear_gpu_locate import env_int, env_str
+            from .linear_gpu_locate import env_int, env_str, knobs_from_env
 
-            stats = fastq_to_bam(index, fq1, out_sam, bs_r1, fq2=fq2, bs_r2=bs_r2,
+            prm = knobs_from_env(k, index.max_occ())
+            if prm.max_indel != 0:
+                # BAM records are written with slS mM srS CIGARs only (the reference's
+                # native-BAM records carry no indel); indel hits need SAM output
+                print(f"mialign: METHYLGRAPHER_LINEAR_MAX_INDEL={prm.max_indel} ignored for BAM output "
+                      "(gapless extension); write SAM for the single-indel branch", file=sys.stderr, flush=True)
+                prm.max_indel = 0
+            stats = fastq_to_bam(index, fq1, out_sam, bs_r1, fq2=fq2, bs_r2=bs_r2, params=prm,
                                  level=env_int("METHYLGRAPHER_BAM_LEVEL", 1),
                                  device_bgzf=os.environ.get("MIALIGN_BGZF", "device").lower() != "host",
                                  rg_id=env_str("METHYLGRAPHER_RG_ID", "mojo1"),

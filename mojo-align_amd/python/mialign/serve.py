@@ -52,10 +52,7 @@ def pinned_recs(n: int) -> Dict[str, np.ndarray]:
     device-to-host copy per request)."""
     proto = host_recs(0)
     block = pinned_empty((len(proto) * max(n, 1) * 4,), np.uint8)
-    out = {f: block[i * n * 4 : (i + 1) * n * 4].view(a.dtype) for i, (f, a) in enumerate(proto.items())}
-    for f in ("iop", "iat", "ilen"):  # a gapless launch does not copy them back
-        out[f][:] = 0
-    return out
+    return {f: block[i * n * 4 : (i + 1) * n * 4].view(a.dtype) for i, (f, a) in enumerate(proto.items())}
 
 
 @dataclass

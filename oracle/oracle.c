@@ -1000,16 +1000,18 @@ uint64_t or_pack_end(int64_t tid, int64_t u5, int64_t strand) {
     return (t << 48) | ((uint64_t)(strand & 1) << 47) | ((uint64_t)u & 140737488355327ull);
 }
 
-/* Single-end sort keys per merged record. tid_of_cid maps index contig ids to
- * canonical (f/r-stripped) target ids. pair = pair_base + record index.
+/* Single-end sort keys per merged record. tid_of_cid[n_tid] maps index contig
+ * ids to canonical (f/r-stripped) target ids; a contig outside the table has no
+ * target (tid -1, unmapped keys), as rid1 < len(rid_to_tid) guards it at
+ * linear_gpu_fm.mojo:764-765. pair = pair_base + record index.
  * Follows the SE builder, linear_gpu_fm.mojo:800-823. */
 void or_sort_keys_se(int64_t n, const int32_t *cid, const uint32_t *pos, const int32_t *flag,
                      const uint32_t *sl, const uint32_t *sr, const uint32_t *nm,
                      const uint32_t *mapq, const uint32_t *lens, const int32_t *tid_of_cid,
-                     int64_t pair_base, uint64_t *coord, uint64_t *dhi, uint64_t *dlo,
+                     int64_t n_tid, int64_t pair_base, uint64_t *coord, uint64_t *dhi, uint64_t *dlo,
                      uint32_t *score, uint32_t *pair) {
     for (int64_t i = 0; i < n; i++) {
-        int64_t tid = cid[i] >= 0 ? tid_of_cid[cid[i]] : -1;
+        int64_t tid = cid[i] >= 0 && cid[i] < n_tid ? tid_of_cid[cid[i]] : -1;
         int64_t fl = flag[i];
         coord[i] = or_coord_key(tid, (int64_t)pos[i], i);
         if (tid >= 0) {

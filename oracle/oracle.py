@@ -95,7 +95,7 @@ def lib():
         L.or_coord_key.argtypes = [I64, I64, I64]
         L.or_pack_end.restype = C.c_uint64
         L.or_pack_end.argtypes = [I64, I64, I64]
-        L.or_sort_keys_se.argtypes = [I64, P, P, P, P, P, P, P, P, P, I64, P, P, P, P, P]
+        L.or_sort_keys_se.argtypes = [I64, P, P, P, P, P, P, P, P, P, I64, I64, P, P, P, P, P]
         L.or_sort_markdup.restype = C.c_int
         L.or_sort_markdup.argtypes = [I64, P, P, P, P, P, P, P, C.c_int, P]
         L.or_num_threads.restype = C.c_int
@@ -231,7 +231,7 @@ def sort_keys_se(rec: dict, lens: np.ndarray, tid_of_cid: np.ndarray, pair_base:
     lens = np.ascontiguousarray(lens, dtype=np.uint32)
     tid = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
     lib().or_sort_keys_se(n, _p(rec["cid"]), _p(rec["pos"]), _p(rec["flag"]), _p(rec["sl"]),
-                          _p(rec["sr"]), _p(rec["nm"]), _p(rec["mapq"]), _p(lens), _p(tid),
+                          _p(rec["sr"]), _p(rec["nm"]), _p(rec["mapq"]), _p(lens), _p(tid), tid.size,
                           pair_base, _p(coord), _p(dhi), _p(dlo), _p(score), _p(pair))
     return coord, dhi, dlo, score, pair
 

@@ -1,11 +1,20 @@
-"""Uneven-length batched launch at full size: a 3.1 Gbp reference in 24
-contigs with 5% low-complexity repeat blocks, converted both ways (6.2 Gbp
-indexed), reads of 50-250 bp in one launch. The oracle cannot build this
-index in reasonable time, so parity here rests on properties that do not
-depend on size: each mapped record's nm equals a recount at its locus (the
-oracle's checker), the MAPQ / nm / aligned-length rules hold, error-free
-reads map exactly, and reads mapped alone equal the same reads in the batch.
+"""Genome scale at full size: a 3.1 Gbp reference in 24 contigs with 5%
+low-complexity repeat blocks, converted both ways (6.2 Gbp indexed), the k=21
+index built on the device and resident in HBM.
+
+Uneven-length batched launch: 2M reads of 50-250 bp in one launch; the first
+200k records equal the oracle's bit for bit (the oracle maps on the device
+index's own dense-v1 arrays, exported), every mapped record's nm equals a
+recount at its locus, the MAPQ / nm / aligned-length rules hold, reads mapped
+alone equal the same reads in the batch, and error-free reads map exactly.
+
+Served request loop: mixed requests of 50-250 bp reads (sizes lognormal,
+1k-200k reads) arrive while others run; every request's records equal the
+same reads mapped alone and the same reads in the one big launch, and a
+sample equals the oracle's.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -14,26 +23,75 @@ from mialign import synth
 from mialign._lib import MA_BS_C2T, MA_ENC_2BIT
 from oracle import oracle as orc
 
-pytestmark = pytest.mark.gpu
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(900)]
 
 FIELDS = ("cid", "pos", "mapq", "flag", "sl", "sr", "nm")
+THREADS = min(16, os.cpu_count() or 1)
 
 
 def say(msg):
     print(f"[test] {msg}", flush=True)
 
 
-def test_config3_full_size_properties():
-    say("test_config3_full_size_properties: 3.1 Gbp x 2, lean index, 2M reads of 50-250 bp")
-    ref = synth.gen_reference(31, 24, 3_100_000_000, repeat_frac=0.05)
-    dual = synth.dual_convert(ref)
-    gix = mialign.GpuIndex.build(dual.seq, dual.contig_off, 21, keep_dense=False)
-    say(f"index: {gix.n_keys} keys, {gix.n_postings} postings, {gix.hash_stats()}")
-    reads = synth.gen_reads(32, ref, 2_000_000, 50, 250, max_len=250)
-    packed = synth.pack_2bit(reads.bases, reads.lens)
-    got = mialign.map_batch(gix, packed, reads.lens, 250, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
-    conv = orc.convert_reads(reads.bases, 1)
-    nm = orc.recount_nm(dual.seq, dual.contig_off, conv, reads.lens, got)
+def first_diff(got, want, fields=FIELDS):
+    for f in fields:
+        bad = np.nonzero(got[f] != want[f])[0]
+        if bad.size:
+            i = int(bad[0])
+            return f"field {f} differs on {bad.size} reads; first {i}: " + str(
+                {g: (int(got[g][i]), int(want[g][i])) for g in fields})
+    return None
+
+
+class Genome:
+    def __init__(self):
+        say("fixture: 3.1 Gbp x 2 reference, dense index on the device")
+        self.ref = synth.gen_reference(31, 24, 3_100_000_000, repeat_frac=0.05)
+        self.dual = synth.dual_convert(self.ref)
+        self.gix = mialign.GpuIndex.build(self.dual.seq, self.dual.contig_off, 21, keep_dense=True)
+        say(f"index: {self.gix.n_keys} keys, {self.gix.n_postings} postings, {self.gix.hash_stats()}")
+        self.reads = synth.gen_reads(32, self.ref, 2_000_000, 50, 250, max_len=250)
+        self.packed = synth.pack_2bit(self.reads.bases, self.reads.lens)
+        self.got = mialign.map_batch(self.gix, self.packed, self.reads.lens, 250, encoding=MA_ENC_2BIT,
+                                     bs_r1=MA_BS_C2T)
+        self._oix = None
+
+    def oracle(self):
+        if self._oix is None:
+            say("exporting the dense-v1 arrays for the oracle")
+            km, of, po = self.gix.export_dense()
+            self._oix = orc.OracleIndex(km, of, po, self.dual.seq, self.dual.contig_off)
+        return self._oix
+
+
+@pytest.fixture(scope="module")
+def genome():
+    g = Genome()
+    yield g
+    g._oix = None
+    g.gix.close()
+
+
+def test_config3_sample_equals_oracle(genome):
+    ns = 200_000
+    say(f"test_config3_sample_equals_oracle: first {ns} of 2M reads of 50-250 bp in one launch")
+    r = genome.reads
+    want, _, _, cnt = orc.map_batch(genome.oracle(), orc.convert_reads(r.bases[:ns], 1), r.lens[:ns],
+                                    nthreads=THREADS)
+    sub = {f: genome.got[f][:ns] for f in FIELDS}
+    msg = first_diff(sub, want)
+    assert msg is None, msg
+    say(f"oracle: {cnt.postings / ns:.1f} postings / read, {cnt.overflow} vote-table overflow replacements, "
+        f"{cnt.tally} high-occ tally placements")
+    # the repeat blocks exercise the high-occ tally the sample is meant to cover
+    assert cnt.tally > 0
+
+
+def test_config3_full_size_properties(genome):
+    say("test_config3_full_size_properties: 2M reads of 50-250 bp")
+    r, got, dual = genome.reads, genome.got, genome.dual
+    conv = orc.convert_reads(r.bases, 1)
+    nm = orc.recount_nm(dual.seq, dual.contig_off, conv, r.lens, got)
     m = got["cid"] >= 0
     say(f"mapped {m.mean():.4f}")
     assert m.mean() > 0.9
@@ -41,22 +99,25 @@ def test_config3_full_size_properties():
     assert np.isin(got["mapq"][m], (20, 40, 60)).all()
     assert (got["mapq"][m & (got["nm"] > 2)] == 20).all()
     assert (got["nm"][m] <= 6).all()
-    alen = reads.lens.astype(np.int64) - got["sl"] - got["sr"]
+    alen = r.lens.astype(np.int64) - got["sl"] - got["sr"]
     assert (alen[m] >= 24).all()
     # the same reads mapped alone (a 2-bit batch of 20k) equal their records in the big launch
     sub = slice(1_000_000, 1_020_000)
-    alone = mialign.map_batch(gix, packed[sub], reads.lens[sub], 250, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
+    alone = mialign.map_batch(genome.gix, genome.packed[sub], r.lens[sub], 250, encoding=MA_ENC_2BIT,
+                              bs_r1=MA_BS_C2T)
     for f in FIELDS:
         assert np.array_equal(alone[f], got[f][sub]), f
     # error-free reads: almost all exact, mostly at their origin. Not all: in
     # the low-complexity repeat blocks the vote can rank other loci above the
     # origin and the extend then settles on the best of those, as the
-    # reference's own scan does; their nm still equals the recount.
-    clean = synth.gen_reads(33, ref, 200_000, 50, 250, p_err=0.0, dup_frac=0.0, max_len=250)
+    # reference's own scan does (the first 20k of them are checked against the
+    # oracle just below); their nm still equals the recount.
+    clean = synth.gen_reads(33, genome.ref, 200_000, 50, 250, p_err=0.0, dup_frac=0.0, max_len=250)
     cp = synth.pack_2bit(clean.bases, clean.lens)
-    g2 = mialign.map_batch(gix, cp, clean.lens, 250, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
+    g2 = mialign.map_batch(genome.gix, cp, clean.lens, 250, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
     m2 = g2["cid"] >= 0
-    nm2 = orc.recount_nm(dual.seq, dual.contig_off, orc.convert_reads(clean.bases, 1), clean.lens, g2)
+    cconv = orc.convert_reads(clean.bases, 1)
+    nm2 = orc.recount_nm(dual.seq, dual.contig_off, cconv, clean.lens, g2)
     assert np.array_equal(nm2[m2], g2["nm"][m2].astype(np.int32))
     exact = (g2["nm"][m2] == 0).mean()
     at_origin = (g2["cid"] == 2 * clean.contig + clean.strand) & (g2["pos"] == clean.pos)
@@ -64,3 +125,55 @@ def test_config3_full_size_properties():
     assert m2.mean() > 0.95
     assert exact > 0.99
     assert at_origin.mean() > 0.9
+    nc = 20_000
+    want, _, _, _ = orc.map_batch(genome.oracle(), cconv[:nc], clean.lens[:nc], nthreads=THREADS)
+    msg = first_diff({f: g2[f][:nc] for f in FIELDS}, want)
+    assert msg is None, "error-free reads: " + msg
+
+
+def test_served_loop_genome_scale(genome):
+    """Config-5 shape on the resident 3.1 Gbp index: 48 requests of 50-250 bp
+    reads, sizes lognormal (median 20k, clipped to [1k, 200k]), Poisson
+    arrivals over ~40 ms, two slots, 128k-read launches (big requests are cut
+    into pieces, small ones merged); some requests take records into one
+    pinned block, others into separate arrays."""
+    say("test_served_loop_genome_scale")
+    from mialign import serve
+    from mialign._lib import host_recs
+
+    rng = np.random.default_rng(11)
+    pool_b, pool_l = serve.pinned_copy(genome.packed), serve.pinned_copy(genome.reads.lens)
+    n_pool = pool_l.size
+    sizes = np.clip(np.exp(rng.normal(np.log(20_000), 1.0, 48)), 1_000, 200_000).astype(int)
+    starts = rng.integers(0, n_pool - sizes + 1)
+    t = 0.0
+    reqs = []
+    for i, (a, sz) in enumerate(zip(starts, sizes)):
+        t += float(rng.exponential(0.04 / 48))
+        reqs.append(serve.Request(pool_b[a:a + sz], pool_l[a:a + sz], 250, MA_ENC_2BIT, bs_r1=MA_BS_C2T,
+                                  arrival=t, out=host_recs(int(sz)) if i % 3 == 1 else None))
+    srv = serve.Server(genome.gix, slots=2, hbm_budget=1 << 30, launch_reads=128 << 10)
+    st, outs = serve.run_request_loop(srv, reqs, keep_outputs=True)
+    srv.close()
+    assert len(outs) == len(reqs) and len(st.latencies_ms) == len(reqs)
+    say(f"served {int(sizes.sum())} reads in {len(reqs)} requests: p50 {st.pct(50):.1f} ms, "
+        f"p99 {st.pct(99):.1f} ms")
+    for i, (a, sz) in enumerate(zip(starts, sizes)):
+        # in the mix == in the one big launch of the same reads
+        big = {f: genome.got[f][a:a + sz] for f in FIELDS}
+        msg = first_diff(outs[i], big)
+        assert msg is None, f"request {i} (reads {a}..{a + sz}) served vs the big launch: {msg}"
+    for i in (0, 1, int(np.argmax(sizes))):
+        a, sz = int(starts[i]), int(sizes[i])
+        alone = mialign.map_batch(genome.gix, genome.packed[a:a + sz], genome.reads.lens[a:a + sz], 250,
+                                  encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T)
+        msg = first_diff(outs[i], alone)
+        assert msg is None, f"request {i} served vs alone: {msg}"
+    # a sample of served records against the oracle
+    i = int(np.argmin(starts))
+    a, sz = int(starts[i]), min(int(sizes[i]), 20_000)
+    r = genome.reads
+    want, _, _, _ = orc.map_batch(genome.oracle(), orc.convert_reads(r.bases[a:a + sz], 1), r.lens[a:a + sz],
+                                  nthreads=THREADS)
+    msg = first_diff({f: outs[i][f][:sz] for f in FIELDS}, want)
+    assert msg is None, f"request {i} served vs oracle: {msg}"

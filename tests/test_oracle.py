@@ -331,6 +331,11 @@ def test_sort_keys_se_records():
     assert dhi[2] == 0xFFFFFFFFFFFFFFFF and dlo[2] == 2 and dlo[0] == 0
     assert list(score) == [(60 << 8) | 252, (20 << 8) | 0, 255]
     assert list(pair) == [10, 11, 12]
+    # a target table shorter than the contig count: contig 2 has no target, so
+    # its record keys as unmapped (rid1 < len(rid_to_tid), linear_gpu_fm.mojo:764-765)
+    coord, dhi, dlo, _, _ = orc.sort_keys_se(rec, lens, tid[:2], pair_base=10)
+    assert coord[1] == (0xFFFFFFFF << 32) | 1 and dhi[1] == 0xFFFFFFFFFFFFFFFF and dlo[1] == 1
+    assert coord[0] == (0 << 32) | 100 and dhi[0] == (1 << 48) | (1 << 47) | (246 + 2**30)
 
 
 # --------------------------------------------------------- A11 / A12
