@@ -830,9 +830,11 @@ class GenomeScale:
 
 
 # HBM bytes one map launch moves on the headline workload (3.1 Gbp x2
-# reference, 10M x 150 bp reads), measured as FETCH_SIZE x the copy kernel's
-# factor + WRITE_SIZE x its factor; None until profiled.
-MAP_TRAFFIC_PROFILED_GS = None
+# reference, 10M x 150 bp reads): rocprofv3 --pmc FETCH_SIZE x 2.00 +
+# WRITE_SIZE x 1.00, the factors read off copy_u64_kernel (a copy of known
+# size) in the same runs.
+MAP_TRAFFIC_PROFILED_GS = {"reads": 10_000_000, "read_len": 150, "bytes": 452.2e9, "kernel_ms": 169.5,
+                           "source": "profiles/map_reads_kernel_3100Mbp_10Mx150.md"}
 
 
 def headline_section(args, gs: GenomeScale) -> dict:
