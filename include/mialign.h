@@ -200,6 +200,21 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
                      int32_t encoding, int32_t bs_r1, int32_t bs_r2, int64_t n_r1,
                      const ma_params *p, ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc);
 
+/* ma_map_batch_dev without a workspace, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops that forbid
+ * hipMalloc). No counterpart in the reference. Its one scratch buffer, the
+ * persistent grid's work counter, is a static 256-byte device array of the
+ * library (one per device, held until the process ends), so the call makes no
+ * allocation of any kind; the counter is cleared by a kernel at each call.
+ * Calls that overlap on one device share that counter and race: serialize
+ * them (one stream, or events). Limits: n < 2^32 reads per call (the
+ * counter's range), device ids 0..63; otherwise as ma_map_batch_dev with a
+ * persistent workspace (same records). */
+int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases, int64_t stride,
+                             const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t encoding,
+                             int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p,
+                             ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc);
+
 /* ---- sort keys, sort, duplicate marking ---- */
 /* SE keys from merged records (device pointers): coord, dhi, dlo, score, pair
  * (pair = pair_base + i). d_tid_of_cid[n_tid] maps index contig -> canonical

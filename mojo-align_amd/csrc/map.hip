@@ -26,6 +26,8 @@
 // passes and the FW/RC merge all run inside the kernel.
 #include <string.h>
 
+#include <mutex>
+
 #include "ma_index.hpp"
 #include "ma_sort.hpp"
 
@@ -500,6 +502,116 @@ int check_map_params(const ma_index *ix, const ma_params *p, int32_t max_len) {
 
 }  // namespace ma
 
+// The scratch of ma_map_batch_dev_noalloc: the persistent grid's work counter
+// (4 of these 256 bytes). Each device holds its own copy for the life of the
+// process; host code reaches it only through the address hipGetSymbolAddress
+// returns for that device, taken once per device with the device current.
+// Calls of the no-alloc entry point that overlap on one device share the
+// counter and race.
+constexpr int NOALLOC_SCRATCH_BYTES = 256;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned int scratch[NOALLOC_SCRATCH_BYTES / 4];
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned int *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+int noalloc_scratch(int device, unsigned int **out) {
+    MA_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, "ma_map_batch_dev_noalloc: device id out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        MA_HIP(hipGetDevice(&cur));
+        if (cur != device) MA_HIP(hipSetDevice(device));
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !ptr) {
+            set_error(std::string("ma_map_batch_dev_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return MA_ERR_HIP;
+        }
+        g_scratch_ptr[device] = (unsigned int *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return MA_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// Validates one batch and enqueues its map launch on st. persistent: a grid of
+// at most 8 blocks per CU taking read chunks from the counter at ctr_buf (zeroed
+// here by a one-thread kernel, not a memset, so the launch can be captured);
+// otherwise a grid sized to the batch, one chunk per wave (ctr unused).
+int map_launch(ma_index *ix, hipStream_t st, const uint8_t *d_bases, int64_t stride, const uint32_t *d_lens,
+               int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2, int64_t n_r1,
+               const ma_params *p, ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc, int32_t persistent,
+               int32_t chunks_per_wave, unsigned int *ctr_buf, const char *who) {
+    MA_REQUIRE(n >= 0 && n < (1ll << 32), std::string(who) + ": n out of range");
+    MA_REQUIRE(encoding == MA_ENC_ASCII || encoding == MA_ENC_2BIT, std::string(who) + ": bad encoding");
+    MA_REQUIRE(encoding == MA_ENC_ASCII ? stride >= max_len : stride * 4 >= max_len,
+               std::string(who) + ": stride smaller than a read row");
+    MA_TRY(check_map_params(ix, p, max_len));
+    if (n == 0) return MA_OK;
+    RecOut mo = rec_out(d_merged);
+    MA_REQUIRE(mo.cid, std::string(who) + ": merged record arrays must be non-null");
+    Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc,
+             p->max_indel};
+    ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
+    // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
+    // the resident waves all get `chunks_per_wave` chunks (more, smaller chunks
+    // shorten the tail of a small batch). Persistent mode: at most 8 blocks per
+    // CU loop over chunks taken from a counter. Batch-sized mode: one chunk per
+    // wave, blocks retire as they finish (concurrent launches interleave).
+    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU * chunks_per_wave;
+    int64_t claim = n / waves;
+    if (claim < 1) claim = 1;
+    if (claim > MAX_CLAIM) claim = MAX_CLAIM;
+    int64_t blocks = (n + claim * WPB - 1) / (claim * WPB);
+    unsigned int *ctr = nullptr;
+    if (persistent) {
+        int64_t cap = (int64_t)ix->n_cu * 8;
+        if (blocks > cap) blocks = cap;
+        ctr = ctr_buf;
+        zero_u32_kernel<<<1, 1, 0, st>>>(ctr);
+        MA_LAUNCHED();
+    }
+    MA_REQUIRE(blocks < (1ll << 31), std::string(who) + ": batch too large for one launch");
+    const IndexView v = index_view(ix);
+    const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
+    // the read-word class of the batch's longest read picks the kernel, the
+    // contig count where the contig offsets live
+    const unsigned g = (unsigned)blocks;
+    const bool lds = ix->n_contigs < COFF_LDS;
+    if (p->max_indel > 0) {
+        if (max_len <= 4 * 32) {
+            if (lds) w4::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w4::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        } else if (max_len <= 5 * 32) {
+            if (lds) w5::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w5::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        } else {
+            if (lds) w8::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+            else w8::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        }
+    } else if (max_len <= 4 * 32) {
+        if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    } else if (max_len <= 5 * 32) {
+        if (lds) w5::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w5::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    } else {
+        if (lds) w8::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        else w8::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    }
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int64_t ma_workspace_bytes(int64_t max_reads, int32_t max_len) {
@@ -548,67 +660,21 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
                      int32_t encoding, int32_t bs_r1, int32_t bs_r2, int64_t n_r1,
                      const ma_params *p, ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc) {
     MA_REQUIRE(ix && ws && d_merged, "ma_map_batch_dev: null index/workspace/output");
-    MA_REQUIRE(n >= 0 && n < (1ll << 32), "ma_map_batch_dev: n out of range");
     MA_REQUIRE(n <= ws->max_reads || ws->max_reads == 0, "ma_map_batch_dev: n exceeds workspace");
-    MA_REQUIRE(encoding == MA_ENC_ASCII || encoding == MA_ENC_2BIT, "ma_map_batch_dev: bad encoding");
-    MA_REQUIRE(encoding == MA_ENC_ASCII ? stride >= max_len : stride * 4 >= max_len,
-               "ma_map_batch_dev: stride smaller than a read row");
-    MA_TRY(check_map_params(ix, p, max_len));
-    if (n == 0) return MA_OK;
-    RecOut mo = rec_out(d_merged);
-    MA_REQUIRE(mo.cid, "ma_map_batch_dev: merged record arrays must be non-null");
-    hipStream_t st = (hipStream_t)stream;
-    Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc,
-             p->max_indel};
-    ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
-    // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
-    // the resident waves all get `chunks_per_wave` chunks (more, smaller chunks
-    // shorten the tail of a small batch). Persistent mode: at most 8 blocks per
-    // CU loop over chunks taken from a counter. Batch-sized mode: one chunk per
-    // wave, blocks retire as they finish (concurrent launches interleave).
-    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU * ws->chunks_per_wave;
-    int64_t claim = n / waves;
-    if (claim < 1) claim = 1;
-    if (claim > MAX_CLAIM) claim = MAX_CLAIM;
-    int64_t blocks = (n + claim * WPB - 1) / (claim * WPB);
+    return map_launch(ix, (hipStream_t)stream, d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1,
+                      p, d_merged, d_fw, d_rc, ws->persistent, ws->chunks_per_wave,
+                      ws->counter.as<unsigned int>(), "ma_map_batch_dev");
+}
+
+int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases, int64_t stride,
+                             const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t encoding,
+                             int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p,
+                             ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc) {
+    MA_REQUIRE(ix && d_merged, "ma_map_batch_dev_noalloc: null index/output");
     unsigned int *ctr = nullptr;
-    if (ws->persistent) {
-        int64_t cap = (int64_t)ix->n_cu * 8;
-        if (blocks > cap) blocks = cap;
-        ctr = ws->counter.as<unsigned int>();
-        zero_u32_kernel<<<1, 1, 0, st>>>(ctr);
-        MA_LAUNCHED();
-    }
-    MA_REQUIRE(blocks < (1ll << 31), "ma_map_batch_dev: batch too large for one launch");
-    const IndexView v = index_view(ix);
-    const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
-    // the read-word class of the batch's longest read picks the kernel, the
-    // contig count where the contig offsets live
-    const unsigned g = (unsigned)blocks;
-    const bool lds = ix->n_contigs < COFF_LDS;
-    if (p->max_indel > 0) {
-        if (max_len <= 4 * 32) {
-            if (lds) w4::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w4::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        } else if (max_len <= 5 * 32) {
-            if (lds) w5::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w5::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        } else {
-            if (lds) w8::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w8::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        }
-    } else if (max_len <= 4 * 32) {
-        if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    } else if (max_len <= 5 * 32) {
-        if (lds) w5::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w5::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    } else {
-        if (lds) w8::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w8::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    }
-    MA_LAUNCHED();
-    return MA_OK;
+    MA_TRY(noalloc_scratch(ix->device, &ctr));
+    return map_launch(ix, (hipStream_t)stream, d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1,
+                      p, d_merged, d_fw, d_rc, 1, 1, ctr, "ma_map_batch_dev_noalloc");
 }
 
 int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint32_t *lens,

@@ -144,6 +144,9 @@ SIGNATURES = {
     "ma_map_batch_dev": (C.c_int, [P, P, P, P, I64, P, I64, I32, I32, I32, I32, I64,
                                    C.POINTER(Params), C.POINTER(Recs), C.POINTER(Recs),
                                    C.POINTER(Recs)]),
+    "ma_map_batch_dev_noalloc": (C.c_int, [P, P, P, I64, P, I64, I32, I32, I32, I32, I64,
+                                           C.POINTER(Params), C.POINTER(Recs), C.POINTER(Recs),
+                                           C.POINTER(Recs)]),
     "ma_sort_keys_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
     "ma_sort_markdup": (C.c_int, [C.c_int, I64, P, P, P, P, P, P, P, I32, P]),
     "ma_sort_scratch_bytes": (I64, [I64]),
