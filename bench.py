@@ -1009,6 +1009,7 @@ def main() -> int:
     ap.add_argument("--sort-records", type=int, default=200_000_000,
                     help="sort + markdup stress records (0 = skip)")
     ap.add_argument("--sort-cpu-sample", type=int, default=20_000_000)
+    ap.add_argument("--sort-only", action="store_true", help="only the sort + markdup stress section")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-launch-reads", type=int, default=128 << 10)
     ap.add_argument("--serve-size-weight", type=float, default=0.5,
@@ -1047,6 +1048,10 @@ def main() -> int:
 
     info = mialign.device_info(0)
     log(f"device {info['name']} cu={info['n_cu']} hbm={info['hbm_total'] / 2**30:.0f} GiB")
+    if args.sort_only:
+        print(json.dumps({"sort_stress": sort_section(args.sort_records, args.sort_cpu_sample, not args.no_cpu)}),
+              flush=True)
+        return 0
 
     gs = GenomeScale(keep_dense=not args.no_cpu, genome_bp=args.gs_genome)
     head = headline_section(args, gs)

@@ -243,6 +243,11 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
                         const uint64_t *d_dlo, const uint32_t *d_score, const uint32_t *d_pair,
                         uint32_t *d_is_dup, uint32_t *d_perm, int32_t do_markdup,
                         uint32_t *d_dup_perm, void *d_scratch, int64_t scratch_bytes);
+/* After ma_sort_markdup_dev on `stream` (same n and scratch): synchronizes the
+ * stream and reports MA_ERR_HIP if a radix pass gave up waiting on a preceding
+ * tile's published counts (a bounded wait that never occurs in a correct run;
+ * its results would then be invalid), MA_OK otherwise. */
+int ma_sort_dev_status(void *stream, int64_t n, const void *d_scratch, int64_t scratch_bytes);
 /* Stable LSD radix sort of u64 keys carrying u32 values (device pointers). */
 int ma_radix_sort_u64_dev(void *stream, int64_t n, uint64_t *d_keys, uint32_t *d_vals,
                           void *d_scratch, int64_t scratch_bytes);

@@ -6,9 +6,15 @@
 //   best       = max score, ties -> min pair id
 //   is_dup     = pair != best pair           (written in original order)
 //   perm       = stable sort by coord
-// Design: everything stays on the device; the group best is one 64-bit
-// atomicMax per record on (score << 32 | ~pair) keyed by the group index from
-// an exclusive scan of head flags (no host scan, no host merge).
+// Design: both orders come from the one-sweep radix sort (ma_onesweep.hpp), a
+// fixed chain of kernels with no host round trip. Duplicates are marked in the
+// (dhi, dlo) order: a tile's head flags are scanned into group ids, each
+// record's (score << 32 | ~pair) is gathered once and folded into its group's
+// best with a 64-bit atomicMax (groups are contiguous, so the atomics are
+// too), and a second sweep writes is_dup back in original order.
+#include <stddef.h>
+
+#include "ma_onesweep.hpp"
 #include "ma_pe.hpp"
 #include "ma_sort.hpp"
 
@@ -43,48 +49,98 @@ __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
         p[i] = v;
 }
 
-// heads[i] = record i of the (dhi, dlo) order starts a new run.
-__global__ void md_heads_kernel(const uint64_t *__restrict__ dhi_s, const uint32_t *__restrict__ idx,
-                                const uint64_t *__restrict__ dlo, uint32_t *__restrict__ heads,
-                                int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t h = 1;
-        if (i > 0 && dhi_s[i] == dhi_s[i - 1] && dlo[idx[i]] == dlo[idx[i - 1]]) h = 0;
-        heads[i] = h;
-    }
+// ---- markdup in the (dhi, dlo) order
+constexpr int MD_ITEMS = 16;
+constexpr int MD_TILE = 256 * MD_ITEMS;
+
+// The sorted dup keys and permutation as the dup sort's plan left them.
+struct MdBufs {
+    uint64_t *lo[2], *hi[2];
+    uint32_t *v[2];
+    uint32_t *out_v;  // caller's dup_perm (null: v[fin])
+};
+
+struct MdView {
+    const uint64_t *hi, *lo;  // lo null when it did not move (implied by hi, or constant)
+    const uint32_t *idx;
+    uint64_t *best;           // per group: max (score << 32 | ~pair); the free lo buffer
+    uint64_t *vs;             // per position: its record's (score << 32 | ~pair); the free hi buffer
+};
+
+__device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B) {
+    const uint32_t f = plan->fin;
+    MdView m;
+    m.hi = B.hi[f];
+    m.lo = plan->carry ? B.lo[f] : nullptr;
+    m.idx = B.out_v ? B.out_v : B.v[f];
+    m.best = B.lo[f ^ 1u];
+    m.vs = B.hi[f ^ 1u];
+    return m;
 }
 
-__global__ void md_best_kernel(const uint64_t *__restrict__ dhi_s, const uint32_t *__restrict__ idx,
-                               const uint32_t *__restrict__ heads, const uint64_t *__restrict__ excl,
-                               const uint32_t *__restrict__ score, const uint32_t *__restrict__ pair,
-                               unsigned long long *__restrict__ best, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        if (dhi_s[i] == ~0ull) continue;
-        uint64_t seg = excl[i] + heads[i] - 1;
-        uint32_t o = idx[i];
-        unsigned long long packed =
-            ((unsigned long long)score[o] << 32) | (unsigned long long)(0xFFFFFFFFu - pair[o]);
-        atomicMax(&best[seg], packed);
-    }
+__device__ __forceinline__ uint32_t md_head(const MdView &m, int64_t i) {
+    if (i == 0) return 1u;
+    if (m.hi[i] != m.hi[i - 1]) return 1u;
+    return (m.lo && m.lo[i] != m.lo[i - 1]) ? 1u : 0u;
 }
 
-__global__ void md_mark_kernel(const uint64_t *__restrict__ dhi_s, const uint32_t *__restrict__ idx,
-                               const uint32_t *__restrict__ heads, const uint64_t *__restrict__ excl,
-                               const uint32_t *__restrict__ pair,
-                               const unsigned long long *__restrict__ best,
-                               uint32_t *__restrict__ is_dup, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t o = idx[i];
-        uint32_t d = 0;
-        if (dhi_s[i] != ~0ull) {
-            uint64_t seg = excl[i] + heads[i] - 1;
-            uint32_t best_pair = 0xFFFFFFFFu - (uint32_t)(best[seg] & 0xFFFFFFFFull);
-            d = pair[o] != best_pair ? 1u : 0u;
+// Group heads per tile of the sorted order.
+__global__ void __launch_bounds__(256) md_tile_heads_kernel(const SortPlan *__restrict__ plan, MdBufs B,
+                                                             int64_t n, uint64_t *__restrict__ partial) {
+    __shared__ uint64_t sh[4];
+    const MdView m = md_view(plan, B);
+    const int64_t base = (int64_t)blockIdx.x * MD_TILE;
+    uint64_t c = 0;
+    for (int r = 0; r < MD_ITEMS; r++) {
+        const int64_t i = base + (int64_t)r * 256 + threadIdx.x;
+        if (i < n) c += md_head(m, i);
+    }
+    uint64_t tot;
+    (void)block_excl_scan_256(c, sh, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// best[g] = 0 for the groups' slots (the free lo buffer, chosen by the plan).
+__global__ void md_clear_best_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n) {
+    uint64_t *best = B.lo[plan->fin ^ 1u];
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        best[i] = 0ull;
+}
+
+// MARK = false: gather each record's (score << 32 | ~pair) into the sorted
+// order and fold it into its group's best; MARK = true: is_dup in original
+// order = a mapped record whose pair is not its group's best pair.
+template <bool MARK>
+__global__ void __launch_bounds__(256) md_groups_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                                         const uint64_t *__restrict__ partial,
+                                                         const uint32_t *__restrict__ score,
+                                                         const uint32_t *__restrict__ pair,
+                                                         uint32_t *__restrict__ is_dup) {
+    __shared__ uint64_t sh[4];
+    const MdView m = md_view(plan, B);
+    const int64_t base = (int64_t)blockIdx.x * MD_TILE;
+    uint64_t run = partial[blockIdx.x];
+    for (int r = 0; r < MD_ITEMS; r++) {
+        const int64_t r0 = base + (int64_t)r * 256;
+        if (r0 >= n) break;  // block-uniform
+        const int64_t i = r0 + threadIdx.x;
+        const bool valid = i < n;
+        const uint32_t h = valid ? md_head(m, i) : 0u;
+        uint64_t tot;
+        const uint64_t ex = block_excl_scan_256(h, sh, tot);
+        const uint32_t o = valid ? m.idx[i] : 0u;
+        if (valid && (int64_t)o < n) {  // o >= n only after a radix pass's timeout (plan->error)
+            const uint64_t g = run + ex + h - 1;
+            const bool mapped = m.hi[i] != ~0ull;
+            if (!MARK) {
+                const uint64_t v = ((uint64_t)score[o] << 32) | (uint64_t)(0xFFFFFFFFu - pair[o]);
+                m.vs[i] = v;
+                if (mapped) atomicMax((unsigned long long *)&m.best[g], (unsigned long long)v);
+            } else {
+                is_dup[o] = (mapped && (uint32_t)m.vs[i] != (uint32_t)m.best[g]) ? 1u : 0u;
+            }
         }
-        is_dup[o] = d;
+        run += tot;
     }
 }
 
@@ -223,11 +279,64 @@ SortLayout sort_layout(int64_t n) {
     return L;
 }
 
+// Scratch of ma_sort_markdup_dev: the one-sweep buffers (two of each key word
+// and of the permutation), the look-back status arrays, histograms, the two
+// sorts' plans, tile counters and the markdup tile partials.
+struct MdLayout {
+    size_t lo0, lo1, hi0, hi1, v0, v1, status, hist, plan_dup, plan_coord, tile_ctr, partial, total;
+    int64_t status_stride;
+};
+
+MdLayout md_layout(int64_t n) {
+    MdLayout L{};
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t nn = (size_t)(n < 1 ? 1 : n);
+    L.lo0 = take(nn * 8);
+    L.lo1 = take(nn * 8);
+    L.hi0 = take(nn * 8);
+    L.hi1 = take(nn * 8);
+    L.v0 = take(nn * 4);
+    L.v1 = take(nn * 4);
+    L.status_stride = os_status_entries(n);
+    L.status = take((size_t)L.status_stride * 2 * 8);
+    L.hist = take(OS_PASSES * OS_RADIX * 8);
+    L.plan_dup = take(sizeof(SortPlan));
+    L.plan_coord = take(sizeof(SortPlan));
+    L.tile_ctr = take(OS_PASSES * 4);
+    L.partial = take((size_t)((n + MD_TILE - 1) / MD_TILE + 1) * 8);
+    L.total = off;
+    return L;
+}
+
+OsScratch os_scratch(char *s, const MdLayout &L, size_t plan_off) {
+    OsScratch S;
+    S.lo[0] = (uint64_t *)(s + L.lo0);
+    S.lo[1] = (uint64_t *)(s + L.lo1);
+    S.hi[0] = (uint64_t *)(s + L.hi0);
+    S.hi[1] = (uint64_t *)(s + L.hi1);
+    S.v[0] = (uint32_t *)(s + L.v0);
+    S.v[1] = (uint32_t *)(s + L.v1);
+    S.status = (uint64_t *)(s + L.status);
+    S.status_stride = L.status_stride;
+    S.hist = (unsigned long long *)(s + L.hist);
+    S.plan = (SortPlan *)(s + plan_off);
+    S.tile_ctr = (uint32_t *)(s + L.tile_ctr);
+    return S;
+}
+
 }  // namespace
 
 extern "C" {
 
-int64_t ma_sort_scratch_bytes(int64_t n) { return (int64_t)sort_layout(n < 1 ? 1 : n).total; }
+int64_t ma_sort_scratch_bytes(int64_t n) {
+    const size_t a = sort_layout(n < 1 ? 1 : n).total, b = md_layout(n < 1 ? 1 : n).total;
+    return (int64_t)(a > b ? a : b);
+}
 
 int ma_radix_sort_u64_dev(void *stream, int64_t n, uint64_t *d_keys, uint32_t *d_vals,
                           void *d_scratch, int64_t scratch_bytes) {
@@ -282,43 +391,27 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
     MA_REQUIRE(n >= 0, "ma_sort_markdup_dev: n < 0");
     MA_REQUIRE(n <= 0xFFFFFFFFll, "ma_sort_markdup_dev: n exceeds u32 record ids");
     if (n == 0) return MA_OK;
-    SortLayout L = sort_layout(n);
-    MA_REQUIRE(scratch_bytes >= (int64_t)L.total, "ma_sort_markdup_dev: scratch too small");
+    MA_REQUIRE(scratch_bytes >= ma_sort_scratch_bytes(n), "ma_sort_markdup_dev: scratch too small");
     hipStream_t st = (hipStream_t)stream;
+    const MdLayout L = md_layout(n);
     char *s = (char *)d_scratch;
-    RadixScratch rs{(uint32_t *)(s + L.bh), (uint64_t *)(s + L.bo), (unsigned long long *)(s + L.h8)};
-    uint32_t *I0 = (uint32_t *)(s + L.idx_a), *I1 = (uint32_t *)(s + L.idx_b);
-    uint64_t *K0 = (uint64_t *)(s + L.key_a), *K1 = (uint64_t *)(s + L.key_b);
-    uint32_t *heads = (uint32_t *)(s + L.heads);
-    uint64_t *excl = (uint64_t *)(s + L.excl);
-    unsigned long long *best = (unsigned long long *)(s + L.best);
-    uint64_t *partial = (uint64_t *)(s + L.partial);
-    int g = grid_for(n);
+    const OsScratch S = os_scratch(s, L, L.plan_dup);
+    const int g = grid_for(n);
     if (do_markdup) {
-        iota_u32_kernel<<<g, 256, 0, st>>>(I0, n);
+        MA_TRY(os_sort(st, n, d_dlo, d_dhi, d_dup_perm, 1, 1, S));
+        const MdBufs B{{S.lo[0], S.lo[1]}, {S.hi[0], S.hi[1]}, {S.v[0], S.v[1]}, d_dup_perm};
+        const int64_t nt = (n + MD_TILE - 1) / MD_TILE;
+        uint64_t *partial = (uint64_t *)(s + L.partial);
+        md_tile_heads_kernel<<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial);
         MA_LAUNCHED();
-        copy_u64_kernel<<<g, 256, 0, st>>>(d_dlo, K0, n);
+        scan_top_kernel<<<1, RS_BLOCK, 0, st>>>(partial, nt, nullptr);
         MA_LAUNCHED();
-        uint64_t *kp = K0, *ka = K1;
-        uint32_t *ip = I0, *ia = I1;
-        MA_TRY(radix_sort_pairs<uint32_t>(st, n, kp, ip, ka, ia, rs));
-        gather_u64_kernel<<<g, 256, 0, st>>>(d_dhi, ip, ka, n);
+        md_clear_best_kernel<<<g, 256, 0, st>>>(S.plan, B, n);
         MA_LAUNCHED();
-        uint64_t *kd = ka, *kda = kp;
-        uint32_t *id = ip, *ida = ia;
-        MA_TRY(radix_sort_pairs<uint32_t>(st, n, kd, id, kda, ida, rs));
-        md_heads_kernel<<<g, 256, 0, st>>>(kd, id, d_dlo, heads, n);
+        md_groups_kernel<false><<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial, d_score, d_pair, nullptr);
         MA_LAUNCHED();
-        MA_TRY(exclusive_scan_u32(st, n, heads, excl, partial, nullptr));
-        MA_TRY(fill_u64(st, (uint64_t *)best, n, 0));
-        md_best_kernel<<<g, 256, 0, st>>>(kd, id, heads, excl, d_score, d_pair, best, n);
+        md_groups_kernel<true><<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial, d_score, d_pair, d_is_dup);
         MA_LAUNCHED();
-        md_mark_kernel<<<g, 256, 0, st>>>(kd, id, heads, excl, d_pair, best, d_is_dup, n);
-        MA_LAUNCHED();
-        if (d_dup_perm) {
-            copy_u32_kernel<<<g, 256, 0, st>>>(id, d_dup_perm, n);
-            MA_LAUNCHED();
-        }
     } else {
         fill_u32_kernel<<<g, 256, 0, st>>>(d_is_dup, n, 0u);
         MA_LAUNCHED();
@@ -327,15 +420,26 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
             MA_LAUNCHED();
         }
     }
-    iota_u32_kernel<<<g, 256, 0, st>>>(I0, n);
-    MA_LAUNCHED();
-    copy_u64_kernel<<<g, 256, 0, st>>>(d_coord, K0, n);
-    MA_LAUNCHED();
-    uint64_t *kc = K0, *kca = K1;
-    uint32_t *ic = I0, *ica = I1;
-    MA_TRY(radix_sort_pairs<uint32_t>(st, n, kc, ic, kca, ica, rs));
-    copy_u32_kernel<<<g, 256, 0, st>>>(ic, d_perm, n);
-    MA_LAUNCHED();
+    const OsScratch S2 = os_scratch(s, L, L.plan_coord);
+    MA_TRY(os_sort(st, n, d_coord, nullptr, d_perm, 0, 0, S2));
+    return MA_OK;
+}
+
+int ma_sort_dev_status(void *stream, int64_t n, const void *d_scratch, int64_t scratch_bytes) {
+    MA_REQUIRE(n >= 0 && d_scratch && scratch_bytes >= ma_sort_scratch_bytes(n), "ma_sort_dev_status: bad arguments");
+    if (n == 0) return MA_OK;
+    const MdLayout L = md_layout(n);
+    uint32_t err[2] = {0, 0};
+    hipStream_t st = (hipStream_t)stream;
+    MA_HIP(hipMemcpyAsync(&err[0], (const char *)d_scratch + L.plan_dup + offsetof(SortPlan, error), 4,
+                          hipMemcpyDeviceToHost, st));
+    MA_HIP(hipMemcpyAsync(&err[1], (const char *)d_scratch + L.plan_coord + offsetof(SortPlan, error), 4,
+                          hipMemcpyDeviceToHost, st));
+    MA_HIP(hipStreamSynchronize(st));
+    if (err[0] || err[1]) {
+        set_error("ma_sort_markdup: a radix pass timed out waiting for a preceding tile (results invalid)");
+        return MA_ERR_HIP;
+    }
     return MA_OK;
 }
 
@@ -375,6 +479,8 @@ int ma_sort_markdup(int device, int64_t n, const uint64_t *coord, const uint64_t
                                  b_dlo.as<uint64_t>(), b_score.as<uint32_t>(), b_pair.as<uint32_t>(),
                                  b_dup.as<uint32_t>(), b_perm.as<uint32_t>(), do_markdup,
                                  b_dperm.as<uint32_t>(), b_scr.p, sb);
+        if (rc != MA_OK) break;
+        rc = ma_sort_dev_status(st, n, b_scr.p, sb);
         if (rc != MA_OK) break;
         if (hipMemcpyAsync(is_dup, b_dup.p, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipMemcpyAsync(perm, b_perm.p, (size_t)n * 4, hipMemcpyDeviceToHost, st) != hipSuccess ||

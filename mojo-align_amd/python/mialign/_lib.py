@@ -148,6 +148,7 @@ SIGNATURES = {
                                            C.POINTER(Params), C.POINTER(Recs), C.POINTER(Recs),
                                            C.POINTER(Recs)]),
     "ma_sort_keys_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
+    "ma_sort_dev_status": (C.c_int, [P, I64, P, I64]),
     "ma_sort_markdup": (C.c_int, [C.c_int, I64, P, P, P, P, P, P, P, I32, P]),
     "ma_sort_scratch_bytes": (I64, [I64]),
     "ma_sort_keys_pe_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
