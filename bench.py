@@ -241,6 +241,45 @@ class StepRunner:
                                         outs["is_dup"].ptr, outs["perm"].ptr, 1, outs["dup_perm"].ptr,
                                         self.scratch.ptr, self.scratch_bytes), "sort markdup")
 
+    def capture(self):
+        """One step (map, keys, sort + markdup) captured as a HIP graph."""
+        with self.mialign.Graph.capture(self.stream) as cap:
+            self.step(False)
+        return cap.graph
+
+    def run_graph(self, graph, steps: int, warmup: int, label: str):
+        """Steps as graph replays: (reads/s, ms per step)."""
+        for _ in range(warmup):
+            graph.launch(self.stream)
+        self.stream.sync()
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            graph.launch(self.stream)
+        self.stream.sync()
+        elapsed = time.perf_counter() - t_start
+        log(f"{label}: {steps} graph replays, {1000.0 * elapsed / steps:.2f} ms/step")
+        return self.n * steps / elapsed, 1000.0 * elapsed / steps
+
+    def outputs(self):
+        """The step's records and sort outputs (host copies), to compare runs."""
+        self.stream.sync()
+        r = self.recs.download()
+        return [r[f] for f in FIELDS] + [self.outs[f].download() for f in ("is_dup", "perm", "dup_perm")]
+
+    def tune(self, names, repeats: int = 3) -> dict:
+        """Map knobs on this step's map launch and the radix tile on its sort."""
+        from mialign import autotune
+        from mialign._lib import MA_BS_C2T, MA_ENC_2BIT
+
+        mp = autotune.tune_map(self.ix, self.ws, self.stream, self.d_bases, self.stride, self.d_lens, self.n, self.L,
+                               self.recs, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T, params=self.prm, repeats=repeats,
+                               names=names)
+        k = self.keys
+        so = autotune.tune_sort(self.stream, self.n, k["coord"], k["dhi"], k["dlo"], k["score"], k["pair"],
+                                self.outs["is_dup"], self.outs["perm"], self.outs["dup_perm"], self.scratch,
+                                self.scratch_bytes, repeats=repeats)
+        return {"map": mp, "sort": so}
+
     def run(self, steps: int, warmup: int, label: str):
         """Returns (reads/s over the timed steps, ms per step, mean map-kernel ms)."""
         for i in range(warmup):
@@ -703,6 +742,18 @@ def bam_section(sr, reads, tid, n_tid: int, blob, name_off, qual, with_cpu: bool
     return out
 
 
+def sr_enc():
+    from mialign._lib import MA_ENC_2BIT
+
+    return MA_ENC_2BIT
+
+
+def sr_c2t():
+    from mialign._lib import MA_BS_C2T
+
+    return MA_BS_C2T
+
+
 def cpu_info() -> dict:
     """CPU model name and logical CPU count of this host."""
     model = ""
@@ -851,10 +902,43 @@ def headline_section(args, gs: GenomeScale) -> dict:
     packed = synth.pack_2bit(reads.bases, reads.lens)
     log(f"headline: {n} x {L} bp reads generated + 2-bit packed in {time.time() - t0:.1f}s")
     sr = StepRunner(gs.ix, packed, reads.lens, L, gs.tid)
+    tuning = None
+    if not args.no_tune:
+        from mialign import autotune
+
+        autotune.reset()
+        t0 = time.time()
+        tuning = sr.tune(("map_max_claim", "map_blocks_per_cu", "map_chunks_per_wave"))
+        tuning["search_s"] = round(time.time() - t0, 1)
+        tuning["cache"] = str(autotune.save(tuning))
+        log(f"headline: tuned {autotune.current()} in {tuning['search_s']} s: map {tuning['map']['default_ms']} -> "
+            f"{tuning['map']['tuned_ms']} ms, sort {tuning['sort']['default_ms']} -> {tuning['sort']['tuned_ms']} ms")
     v, ms, map_ms = sr.run(args.steps, args.warmup, "headline 3.1 Gbp 150 bp")
     log(f"headline: {args.steps} steps, {ms:.1f} ms/step, map kernel {map_ms:.1f} ms, {v / 1e6:.2f} M reads/s")
+    ref_out = sr.outputs()
+    graph = None
+    if not args.no_graph:
+        g = sr.capture()
+        gv, gms = sr.run_graph(g, args.steps, args.warmup, "headline graph")
+        same = all(np.array_equal(a, b) for a, b in zip(sr.outputs(), ref_out))
+        graph = {"nodes": g.n_nodes, "ms_per_step_graph": round(gms, 3), "ms_per_step_stream": round(ms, 3),
+                 "reads_per_s_graph": round(gv, 1), "outputs_equal_stream": bool(same)}
+        del g
+    if tuning is not None:
+        from mialign import autotune
+
+        tuned = autotune.current()
+        autotune.reset()
+        dv, dms, dmap = sr.run(args.steps, 1, "headline default knobs")
+        same = all(np.array_equal(a, b) for a, b in zip(sr.outputs(), ref_out))
+        autotune.apply(tuned)
+        tuning.update({"choices": tuned, "ms_per_step_default": round(dms, 3), "ms_per_step_tuned": round(ms, 3),
+                       "map_kernel_ms_default": round(dmap, 3), "map_kernel_ms_tuned": round(map_ms, 3),
+                       "outputs_equal_default": bool(same)})
+        sr.step(False)  # the tuned step's outputs again
     merged = sr.recs.download()
-    out = {"value": v, "ms_per_step": ms, "map_ms": map_ms, "reads": n, "read_len": L,
+    out = {"value": v, "ms_per_step": ms, "map_ms": map_ms, "reads": n, "read_len": L, "tuning": tuning,
+           "graph": graph,
            "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
            "duplicates_marked": int(sr.outs["is_dup"].download().sum()),
            "cpu_baseline": None, "roofline": None, "parity_sample": None, "sort_parity": None}
@@ -899,13 +983,37 @@ def uneven_section(args, gs: GenomeScale) -> dict:
     upacked = synth.pack_2bit(ur.bases, ur.lens)
     log(f"uneven: {n2} reads generated + packed in {time.time() - t0:.1f}s")
     sr = StepRunner(gs.ix, upacked, ur.lens, 250, gs.tid)
+    tuning = None
+    if not args.no_tune:
+        from mialign import autotune
+
+        before = autotune.current()
+        t0 = time.time()
+        mp = autotune.tune_map(sr.ix, sr.ws, sr.stream, sr.d_bases, sr.stride, sr.d_lens, sr.n, sr.L, sr.recs,
+                               encoding=sr_enc(), bs_r1=sr_c2t(), params=sr.prm, repeats=2,
+                               names=("map_class_split", "map_max_claim"))
+        tuning = {"map": mp, "search_s": round(time.time() - t0, 1), "start": before}
+        log(f"uneven: tuned {mp['choices']}: map {mp['default_ms']} -> {mp['tuned_ms']} ms")
     v, ms, map_ms = sr.run(max(1, args.steps // 2), 1, "uneven 50-250 bp")
     merged = sr.recs.download()
+    if tuning is not None:
+        from mialign import autotune
+
+        tuned = autotune.current()
+        autotune.apply(tuning["start"])
+        dv, dms, dmap = sr.run(max(1, args.steps // 4), 1, "uneven default split/claim")
+        same = all(np.array_equal(sr.recs.download()[f], merged[f]) for f in FIELDS)
+        autotune.apply(tuned)
+        tuning.update({"choices": tuned, "ms_per_step_default": round(dms, 3), "ms_per_step_tuned": round(ms, 3),
+                       "map_kernel_ms_default": round(dmap, 3), "map_kernel_ms_tuned": round(map_ms, 3),
+                       "records_equal_default": bool(same)})
+        autotune.save({"uneven": tuning})
     sr.close()
     res = {"reads": n2, "read_len": "uniform 50-250", "mean_len": round(float(ur.lens.mean()), 1),
            "reads_per_s": round(v, 1), "ms_per_step": round(ms, 3), "map_kernel_ms": round(map_ms, 3),
            "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
-           "launch": "one persistent map launch over all reads per step"}
+           "launch": "one persistent map launch over all reads per step (per read-length class when the "
+                     "tuned class split is on)", "tuning": tuning}
     log(f"uneven 50-250 bp: {n2} reads, {ms:.1f} ms/step, map {map_ms:.1f} ms, {v / 1e6:.1f} M reads/s")
     if not args.no_cpu:
         ns = min(args.gs_uneven_cpu_sample, n2)
@@ -1010,6 +1118,9 @@ def main() -> int:
                     help="sort + markdup stress records (0 = skip)")
     ap.add_argument("--sort-cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--sort-only", action="store_true", help="only the sort + markdup stress section")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch knobs (no autotuning)")
+    ap.add_argument("--knobs", default="", help='fixed launch knobs as JSON (e.g. {"map_max_claim": 8}); no tuning')
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing of the headline step")
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-launch-reads", type=int, default=128 << 10)
     ap.add_argument("--serve-size-weight", type=float, default=0.5,
@@ -1041,8 +1152,15 @@ def main() -> int:
               file=sys.stderr)
         return 2
     if args.headline_only:
-        args.no_cpu = args.no_serve = args.no_config2 = args.no_uneven = True
+        args.no_cpu = args.no_serve = args.no_config2 = args.no_uneven = args.no_graph = True
         args.sort_records = 0
+        args.no_tune = True
+    if args.knobs:
+        from mialign import autotune
+
+        autotune.apply(json.loads(args.knobs))
+        args.no_tune = True
+        log(f"launch knobs {autotune.current()}")
 
     import mialign
 
@@ -1110,6 +1228,8 @@ def main() -> int:
         "x_cpu": round(head["value"] / head["cpu_baseline"]["value"], 1) if head["cpu_baseline"] else None,
         "parity_sample": head["parity_sample"],
         "sort_parity": head["sort_parity"],
+        "tuning": head["tuning"],
+        "graph": head["graph"],
         "uneven_50_250bp": uneven,
         "serving": serving,
         "sort_stress": sort_stress,

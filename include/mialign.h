@@ -105,6 +105,23 @@ int ma_device_count(int *n);
 int ma_device_info(int device, char *name, int name_len, int64_t *hbm_total, int64_t *hbm_free,
                    int32_t *n_cu, int32_t *lds_per_block);
 
+/* Launch knobs, process-wide; every value gives bit-identical results (they
+ * change tile sizes and launch shapes only). No reference counterpart: the
+ * reference's batch knobs are hand-set (linear_gpu_locate.mojo:1729-1751);
+ * here the autotuner (mialign.autotune) times candidates on the device and
+ * sets these. ma_tune_get also returns the built-in default. */
+enum {
+    MA_TUNE_SORT_TILE_ITEMS = 0,     /* keys per thread of a radix tile: 8 or 16 (tiles of 2048 / 4096) */
+    MA_TUNE_MAP_MAX_CLAIM = 1,       /* most reads a map wave claims per work-counter atomic, 1..64 */
+    MA_TUNE_MAP_BLOCKS_PER_CU = 2,   /* persistent map grid: blocks (4 waves) per CU, 1..16 */
+    MA_TUNE_MAP_CHUNKS_PER_WAVE = 3, /* default read chunks per resident wave of new workspaces, 1..64 */
+    MA_TUNE_MAP_CLASS_SPLIT = 4,     /* 1: a batch of uneven reads maps each read-length class in its
+                                        own launch (reads of <= 128, <= 160, <= 256 bases) */
+    MA_TUNE_COUNT = 5
+};
+int ma_tune_set(int32_t knob, int64_t value);
+int ma_tune_get(int32_t knob, int64_t *value, int64_t *default_value);
+
 /* ---- device memory, streams and events (so hosts need no other GPU runtime) ---- */
 int ma_dev_alloc(int device, int64_t bytes, void **ptr);
 int ma_dev_free(void *ptr);
@@ -120,6 +137,17 @@ int ma_event_record(void *ev, void *stream);
 int ma_event_sync(void *ev);
 int ma_event_elapsed_ms(void *start, void *stop, float *ms);
 int ma_event_query(void *ev); /* 1 = complete, 0 = pending, <0 error */
+/* Stage chains as HIP graphs: ma_graph_begin starts capturing `stream` (this
+ * thread's calls only); the library's *_dev calls enqueued on it until
+ * ma_graph_end become the graph's nodes (kernels only: the device-pointer
+ * calls make no allocation, copy, memset or synchronisation once their
+ * buffers exist), which ma_graph_end instantiates (n_nodes may be NULL);
+ * ma_graph_launch replays it on a stream. No reference counterpart (the
+ * reference enqueues each kernel of a batch, linear_gpu_locate.mojo:1836-2753). */
+int ma_graph_begin(void *stream);
+int ma_graph_end(void *stream, void **graph_exec, int64_t *n_nodes);
+int ma_graph_launch(void *graph_exec, void *stream);
+int ma_graph_destroy(void *graph_exec);
 /* Pinned host memory (page-locked; lets copies overlap kernels). */
 int ma_host_alloc(int64_t bytes, void **ptr);
 int ma_host_free(void *ptr);
@@ -177,9 +205,10 @@ int ma_workspace_destroy(ma_workspace *ws);
  * grid sized to the batch, one chunk per wave, so concurrent launches on other
  * streams interleave (what the served loop uses). Results are identical. */
 int ma_workspace_set_persistent(ma_workspace *ws, int32_t persistent);
-/* Launch shape: persistent as above; chunks_per_wave (1..64, default 1) =
- * read chunks per resident wave, so claim = clamp(n / (resident waves x
- * chunks_per_wave), 1, 16) reads. More, smaller chunks shorten the tail of a
+/* Launch shape: persistent as above; chunks_per_wave (1..64; until set, the
+ * MA_TUNE_MAP_CHUNKS_PER_WAVE knob at each launch) = read chunks per resident
+ * wave, so claim = clamp(n / (resident waves x chunks_per_wave), 1,
+ * MA_TUNE_MAP_MAX_CLAIM) reads. More, smaller chunks shorten the tail of a
  * small batch. Results are identical for every setting. */
 int ma_workspace_tune(ma_workspace *ws, int32_t persistent, int32_t chunks_per_wave);
 /* Bytes a workspace of this shape holds on the device (for admission). */

@@ -13,6 +13,8 @@ namespace ma {
 
 // ---------------------------------------------------------------- errors
 void set_error(const std::string &msg);
+// Current value of a launch knob (ma_tune_set / ma_tune_get).
+int64_t tune_get(int knob);
 
 #define MA_HIP(call)                                                                   \
     do {                                                                               \

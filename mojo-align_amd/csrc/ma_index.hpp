@@ -37,8 +37,9 @@ struct ma_workspace {
     int64_t max_reads = 0;  // 0 = no limit
     int32_t max_len = 0;
     int32_t persistent = 1;  // 1: persistent grid + work counter; 0: grid sized to the batch
-    int32_t chunks_per_wave = 1;  // work chunks per resident wave (sets the claim size)
+    int32_t chunks_per_wave = 0;  // work chunks per resident wave (sets the claim size); 0 = the tuned knob
     ma::DevBuf counter;
+    ma::DevBuf split;  // read-length class sub-batches (MA_TUNE_MAP_CLASS_SPLIT), grow-only
 };
 
 namespace ma {

@@ -369,7 +369,9 @@ inline int os_sort(hipStream_t st, int64_t n, const uint64_t *in_lo, const uint6
     OsBufs B{in_lo, in_hi, {s.lo[0], s.lo[1]}, {s.hi[0], s.hi[1]}, {s.v[0], s.v[1]}, out_v,
              s.status, s.tile_ctr, s.status_stride};
     const int np = in_hi ? OS_PASSES : 8;
-    const int g_plain = os_grid((n + OS_BLOCK * OS_ITEMS_PLAIN - 1) / (OS_BLOCK * OS_ITEMS_PLAIN));
+    // tile of the passes that move one key word: 4096 keys (16 per thread) or 2048 (8), as tuned
+    const int plain_items = tune_get(MA_TUNE_SORT_TILE_ITEMS) == 8 ? 8 : OS_ITEMS_PLAIN;
+    const int g_plain = os_grid((n + OS_BLOCK * plain_items - 1) / (OS_BLOCK * plain_items));
     const int g_carry = os_grid((n + OS_BLOCK * OS_ITEMS_CARRY - 1) / (OS_BLOCK * OS_ITEMS_CARRY));
     for (int p = 0; p < np && n > 1; p++) {
         if (in_hi) {
@@ -377,7 +379,10 @@ inline int os_sort(hipStream_t st, int64_t n, const uint64_t *in_lo, const uint6
             MA_LAUNCHED();
         }
         if (!in_hi || p >= 8) {
-            os_pass_kernel<false, OS_ITEMS_PLAIN><<<g_plain, OS_BLOCK, 0, st>>>(s.plan, p, n, B);
+            if (plain_items == 8)
+                os_pass_kernel<false, 8><<<g_plain, OS_BLOCK, 0, st>>>(s.plan, p, n, B);
+            else
+                os_pass_kernel<false, OS_ITEMS_PLAIN><<<g_plain, OS_BLOCK, 0, st>>>(s.plan, p, n, B);
             MA_LAUNCHED();
         }
     }

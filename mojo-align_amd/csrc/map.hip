@@ -48,7 +48,6 @@ constexpr int BLOCK = WPB * 64;
 constexpr int MAX_HIGH_TRIED = 96;  // :1277-1281
 
 constexpr int PBUF = 256;  // postings prefetched into LDS per wave
-constexpr int MAX_CLAIM = 16;  // most reads claimed per work-counter atomic
 constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kernels' register budget
 
 struct Knobs {
@@ -79,11 +78,18 @@ struct ReadsIn {
     const uint8_t *bases;
     int64_t stride;
     const uint32_t *lens;
-    int64_t n;
+    int64_t n;  // reads (with n_dev: the capacity of the arrays)
     int32_t max_len;
     int32_t encoding;
     int32_t bs_r1, bs_r2;
     int64_t n_r1;
+    // Read from device memory by the kernel when set (a launch whose shape
+    // does not depend on a host-known count): n_dev the number of reads,
+    // base_dev the first read's row in bases / lens / out_idx; out_idx[r] the
+    // slot read r's records go to (null: r).
+    const uint32_t *n_dev;
+    const uint32_t *base_dev;
+    const uint32_t *out_idx;
 };
 
 #define DEV __device__ __forceinline__
@@ -541,6 +547,99 @@ int noalloc_scratch(int device, unsigned int **out) {
 
 namespace {
 
+// Read-length classes of the map kernels: 0 = at most 4 read words (128
+// bases), 1 = 5 words (160), 2 = 8 words (256).
+constexpr int SPLIT_MIN_READS = 65536;  // smaller uneven batches map in one launch
+
+__host__ __device__ inline int len_class(int64_t L) { return L <= 4 * 32 ? 0 : (L <= 5 * 32 ? 1 : 2); }
+
+__global__ void class_zero_kernel(uint32_t *c) {
+    if (threadIdx.x < 12) c[threadIdx.x] = 0u;
+}
+
+// Reads per length class; one atomic per class and wave.
+__global__ void class_count_kernel(const uint32_t *__restrict__ lens, int64_t n, uint32_t *__restrict__ cnt) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + lane;
+        const int c = i < n ? len_class(lens[i]) : -1;
+        for (int k = 0; k < 3; k++) {
+            const uint64_t m = __ballot(c == k);
+            if (lane == 0 && m) atomicAdd(&cnt[k], (uint32_t)__popcll(m));
+        }
+    }
+}
+
+__global__ void class_offsets_kernel(const uint32_t *__restrict__ cnt, uint32_t *__restrict__ off,
+                                     uint32_t *__restrict__ cur) {
+    off[0] = 0u;
+    off[1] = cnt[0];
+    off[2] = cnt[0] + cnt[1];
+    cur[0] = cur[1] = cur[2] = 0u;
+}
+
+// Each read's row, length and own index into its class's region of the
+// sub-batch (order within a class is free: a read's records do not depend on
+// its neighbours).
+__global__ void class_place_kernel(const uint8_t *__restrict__ bases, int64_t stride, const uint32_t *__restrict__ lens,
+                                   int64_t n, const uint32_t *__restrict__ off, uint32_t *__restrict__ cur,
+                                   uint8_t *__restrict__ sb, uint32_t *__restrict__ sl, uint32_t *__restrict__ sidx) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + lane;
+        const uint32_t L = i < n ? lens[i] : 0u;
+        const int c = i < n ? len_class(L) : -1;
+        int64_t slot = -1;
+        for (int k = 0; k < 3; k++) {
+            const uint64_t m = __ballot(c == k);
+            if (!m) continue;
+            uint32_t b = 0;
+            if (lane == (int)__builtin_ctzll(m)) b = atomicAdd(&cur[k], (uint32_t)__popcll(m));
+            b = (uint32_t)__shfl(b, (int)__builtin_ctzll(m), 64);
+            if (c == k) slot = (int64_t)off[k] + b + __popcll(m & lt);
+        }
+        if (slot >= 0) {
+            const uint8_t *src = bases + i * stride;
+            uint8_t *dst = sb + slot * stride;
+            for (int64_t j = 0; j < stride; j++) dst[j] = src[j];
+            sl[slot] = L;
+            sidx[slot] = (uint32_t)i;
+        }
+    }
+}
+
+// The map kernel of a read-length class (and indel mode, contig-offset place).
+int dispatch_map(int cls, bool indel, bool lds, unsigned g, hipStream_t st, const IndexView &v, const Knobs &kn,
+                 const ReadsIn &in, const RecOut &mo, const RecOut &fo, const RecOut &ro, unsigned int *ctr,
+                 int claim) {
+    if (indel) {
+        if (cls == 0) {
+            if (lds) w4::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+            else w4::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        } else if (cls == 1) {
+            if (lds) w5::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+            else w5::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        } else {
+            if (lds) w8::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+            else w8::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        }
+    } else if (cls == 0) {
+        if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+    } else if (cls == 1) {
+        if (lds) w5::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        else w5::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+    } else {
+        if (lds) w8::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+        else w8::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, claim);
+    }
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
 // Validates one batch and enqueues its map launch on st. persistent: a grid of
 // at most 8 blocks per CU taking read chunks from the counter at ctr_buf (zeroed
 // here by a one-thread kernel, not a memset, so the launch can be captured);
@@ -548,7 +647,7 @@ namespace {
 int map_launch(ma_index *ix, hipStream_t st, const uint8_t *d_bases, int64_t stride, const uint32_t *d_lens,
                int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2, int64_t n_r1,
                const ma_params *p, ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc, int32_t persistent,
-               int32_t chunks_per_wave, unsigned int *ctr_buf, const char *who) {
+               int32_t chunks_per_wave, unsigned int *ctr_buf, ma::DevBuf *split_buf, const char *who) {
     MA_REQUIRE(n >= 0 && n < (1ll << 32), std::string(who) + ": n out of range");
     MA_REQUIRE(encoding == MA_ENC_ASCII || encoding == MA_ENC_2BIT, std::string(who) + ": bad encoding");
     MA_REQUIRE(encoding == MA_ENC_ASCII ? stride >= max_len : stride * 4 >= max_len,
@@ -559,54 +658,74 @@ int map_launch(ma_index *ix, hipStream_t st, const uint8_t *d_bases, int64_t str
     MA_REQUIRE(mo.cid, std::string(who) + ": merged record arrays must be non-null");
     Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc,
              p->max_indel};
-    ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1};
-    // Size the launch to the batch: claim = reads per wave chunk (<= 16) so that
-    // the resident waves all get `chunks_per_wave` chunks (more, smaller chunks
-    // shorten the tail of a small batch). Persistent mode: at most 8 blocks per
-    // CU loop over chunks taken from a counter. Batch-sized mode: one chunk per
-    // wave, blocks retire as they finish (concurrent launches interleave).
-    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU * chunks_per_wave;
+    ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1, nullptr, nullptr, nullptr};
+    // Size the launch to the batch: claim = reads per wave chunk (at most the
+    // tuned cap) so that the resident waves all get `chunks_per_wave` chunks
+    // (more, smaller chunks shorten the tail of a small batch). Persistent
+    // mode: at most the tuned blocks per CU loop over chunks taken from a
+    // counter. Batch-sized mode: one chunk per wave, blocks retire as they
+    // finish (concurrent launches interleave).
+    const int64_t cpw = chunks_per_wave > 0 ? chunks_per_wave : tune_get(MA_TUNE_MAP_CHUNKS_PER_WAVE);
+    const int64_t waves = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU * cpw;
+    const int64_t max_claim = tune_get(MA_TUNE_MAP_MAX_CLAIM);
     int64_t claim = n / waves;
     if (claim < 1) claim = 1;
-    if (claim > MAX_CLAIM) claim = MAX_CLAIM;
+    if (claim > max_claim) claim = max_claim;
     int64_t blocks = (n + claim * WPB - 1) / (claim * WPB);
     unsigned int *ctr = nullptr;
     if (persistent) {
-        int64_t cap = (int64_t)ix->n_cu * 8;
+        int64_t cap = (int64_t)ix->n_cu * tune_get(MA_TUNE_MAP_BLOCKS_PER_CU);
         if (blocks > cap) blocks = cap;
         ctr = ctr_buf;
-        zero_u32_kernel<<<1, 1, 0, st>>>(ctr);
-        MA_LAUNCHED();
     }
     MA_REQUIRE(blocks < (1ll << 31), std::string(who) + ": batch too large for one launch");
     const IndexView v = index_view(ix);
     const RecOut fo = rec_out(d_fw), ro = rec_out(d_rc);
-    // the read-word class of the batch's longest read picks the kernel, the
-    // contig count where the contig offsets live
-    const unsigned g = (unsigned)blocks;
     const bool lds = ix->n_contigs < COFF_LDS;
-    if (p->max_indel > 0) {
-        if (max_len <= 4 * 32) {
-            if (lds) w4::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w4::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        } else if (max_len <= 5 * 32) {
-            if (lds) w5::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w5::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        } else {
-            if (lds) w8::ind_l::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-            else w8::ind_g::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+    const int top = len_class(max_len);
+    // Uneven reads that span length classes: each class maps with its own
+    // kernel (register arrays sized to its reads) from a sub-batch gathered on
+    // the device; records go straight to the reads' own slots.
+    const bool split = split_buf && tune_get(MA_TUNE_MAP_CLASS_SPLIT) != 0 && top > 0 && n >= SPLIT_MIN_READS &&
+                       (n_r1 >= n || bs_r1 == bs_r2);
+    if (!split) {
+        if (ctr) {
+            zero_u32_kernel<<<1, 1, 0, st>>>(ctr);
+            MA_LAUNCHED();
         }
-    } else if (max_len <= 4 * 32) {
-        if (lds) w4::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w4::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    } else if (max_len <= 5 * 32) {
-        if (lds) w5::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w5::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-    } else {
-        if (lds) w8::lds::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
-        else w8::glb::map_reads_kernel<<<g, BLOCK, 0, st>>>(v, kn, in, mo, fo, ro, ctr, (int)claim);
+        // the read-word class of the batch's longest read picks the kernel, the
+        // contig count where the contig offsets live
+        return dispatch_map(top, p->max_indel > 0, lds, (unsigned)blocks, st, v, kn, in, mo, fo, ro, ctr, (int)claim);
     }
+    const size_t row = (size_t)stride;
+    MA_TRY(split_buf->reserve(256 + (size_t)n * (row + 8)));
+    char *sb = (char *)split_buf->p;
+    uint32_t *cnt = (uint32_t *)sb, *off = cnt + 4, *cur = cnt + 8;
+    uint8_t *s_bases = (uint8_t *)(sb + 256);
+    uint32_t *s_lens = (uint32_t *)(s_bases + (size_t)n * row);
+    uint32_t *s_idx = s_lens + n;
+    const int gs = grid_for(n);
+    class_zero_kernel<<<1, 64, 0, st>>>(cnt);
     MA_LAUNCHED();
+    class_count_kernel<<<gs, 256, 0, st>>>(d_lens, n, cnt);
+    MA_LAUNCHED();
+    class_offsets_kernel<<<1, 1, 0, st>>>(cnt, off, cur);
+    MA_LAUNCHED();
+    class_place_kernel<<<gs, 256, 0, st>>>(d_bases, stride, d_lens, n, off, cur, s_bases, s_lens, s_idx);
+    MA_LAUNCHED();
+    const int class_max[3] = {4 * 32, 5 * 32, 8 * 32};
+    for (int c = 0; c <= top; c++) {
+        ReadsIn ic{s_bases, stride, s_lens, n, max_len < class_max[c] ? max_len : class_max[c], encoding, bs_r1,
+                   bs_r1, n, cnt + c, off + c, s_idx};
+        unsigned int *cc = nullptr;
+        unsigned g = (unsigned)blocks;
+        if (ctr) {
+            cc = ctr;
+            zero_u32_kernel<<<1, 1, 0, st>>>(cc);
+            MA_LAUNCHED();
+        }
+        MA_TRY(dispatch_map(c, p->max_indel > 0, lds, g, st, v, kn, ic, mo, fo, ro, cc, (int)claim));
+    }
     return MA_OK;
 }
 
@@ -627,6 +746,7 @@ int ma_workspace_create(ma_index *ix, int64_t max_reads, int32_t max_len, ma_wor
     ws->ix = ix;
     ws->max_reads = max_reads;
     ws->max_len = max_len;
+    ws->chunks_per_wave = 0;  // the tuned value at each launch, until ma_workspace_tune sets one
     int rc = ws->counter.alloc(256);
     if (rc != MA_OK) {
         delete ws;
@@ -663,7 +783,7 @@ int ma_map_batch_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t
     MA_REQUIRE(n <= ws->max_reads || ws->max_reads == 0, "ma_map_batch_dev: n exceeds workspace");
     return map_launch(ix, (hipStream_t)stream, d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1,
                       p, d_merged, d_fw, d_rc, ws->persistent, ws->chunks_per_wave,
-                      ws->counter.as<unsigned int>(), "ma_map_batch_dev");
+                      ws->counter.as<unsigned int>(), &ws->split, "ma_map_batch_dev");
 }
 
 int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases, int64_t stride,
@@ -674,7 +794,7 @@ int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases,
     unsigned int *ctr = nullptr;
     MA_TRY(noalloc_scratch(ix->device, &ctr));
     return map_launch(ix, (hipStream_t)stream, d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1,
-                      p, d_merged, d_fw, d_rc, 1, 1, ctr, "ma_map_batch_dev_noalloc");
+                      p, d_merged, d_fw, d_rc, 1, 0, ctr, nullptr, "ma_map_batch_dev_noalloc");
 }
 
 int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint32_t *lens,
@@ -717,6 +837,7 @@ int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint3
     ws.ix = ix;
     ws.max_reads = n;
     ws.max_len = max_len;
+    ws.chunks_per_wave = 0;
     MA_TRY(ws.counter.alloc(256));
     MA_TRY(ma_map_batch_dev(ix, &ws, st, d_in.as<uint8_t>(), stride, d_lens.as<uint32_t>(), n, max_len,
                             encoding, bs_r1, bs_r2, n_r1, p, &dev_recs[0],

@@ -1,6 +1,7 @@
 // Library-level entry points: errors, device info, memory, streams, events.
 #include <string.h>
 
+#include <atomic>
 #include <string>
 
 #include "ma_common.hpp"
@@ -8,6 +9,24 @@
 namespace ma {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
+
+// Launch knobs the autotuner sets (process-wide; every value gives the same
+// results). Defaults are the hand-set values the kernels were first tuned at.
+namespace {
+struct TuneSpec {
+    int64_t def, lo, hi;
+};
+constexpr TuneSpec TUNE_SPEC[MA_TUNE_COUNT] = {
+    {16, 8, 16},  // MA_TUNE_SORT_TILE_ITEMS: keys per thread of a radix tile (8 or 16)
+    {16, 1, 64},  // MA_TUNE_MAP_MAX_CLAIM: most reads a wave claims per counter atomic
+    {8, 1, 16},   // MA_TUNE_MAP_BLOCKS_PER_CU: persistent map grid, blocks per CU
+    {1, 1, 64},   // MA_TUNE_MAP_CHUNKS_PER_WAVE: default read chunks per resident wave
+    {0, 0, 1},    // MA_TUNE_MAP_CLASS_SPLIT: map uneven batches per read-length class
+};
+std::atomic<int64_t> g_tune[MA_TUNE_COUNT] = {{16}, {16}, {8}, {1}, {0}};
+}  // namespace
+
+int64_t tune_get(int knob) { return g_tune[knob].load(std::memory_order_relaxed); }
 }  // namespace ma
 
 using namespace ma;
@@ -17,6 +36,22 @@ extern "C" {
 const char *ma_last_error(void) { return g_last_error.c_str(); }
 
 const char *ma_version(void) { return "mialign 0.1 gfx950"; }
+
+int ma_tune_set(int32_t knob, int64_t value) {
+    MA_REQUIRE(knob >= 0 && knob < MA_TUNE_COUNT, "ma_tune_set: unknown knob");
+    const TuneSpec &t = TUNE_SPEC[knob];
+    MA_REQUIRE(value >= t.lo && value <= t.hi, "ma_tune_set: value out of range");
+    MA_REQUIRE(knob != MA_TUNE_SORT_TILE_ITEMS || value == 8 || value == 16, "ma_tune_set: sort tile items are 8 or 16");
+    g_tune[knob].store(value, std::memory_order_relaxed);
+    return MA_OK;
+}
+
+int ma_tune_get(int32_t knob, int64_t *value, int64_t *default_value) {
+    MA_REQUIRE(knob >= 0 && knob < MA_TUNE_COUNT, "ma_tune_get: unknown knob");
+    if (value) *value = g_tune[knob].load(std::memory_order_relaxed);
+    if (default_value) *default_value = TUNE_SPEC[knob].def;
+    return MA_OK;
+}
 
 void ma_params_default(ma_params *p) {
     p->k = 21;
@@ -129,6 +164,40 @@ int ma_event_sync(void *ev) {
 
 int ma_event_elapsed_ms(void *start, void *stop, float *ms) {
     MA_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return MA_OK;
+}
+
+int ma_graph_begin(void *stream) {
+    MA_HIP(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+    return MA_OK;
+}
+
+int ma_graph_end(void *stream, void **graph_exec, int64_t *n_nodes) {
+    MA_REQUIRE(graph_exec, "ma_graph_end: null output");
+    hipGraph_t g = nullptr;
+    MA_HIP(hipStreamEndCapture((hipStream_t)stream, &g));
+    size_t nn = 0;
+    hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+    hipGraphExec_t x = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_error(std::string("ma_graph_end: ") + hipGetErrorString(e));
+        return MA_ERR_HIP;
+    }
+    if (n_nodes) *n_nodes = (int64_t)nn;
+    *graph_exec = (void *)x;
+    return MA_OK;
+}
+
+int ma_graph_launch(void *graph_exec, void *stream) {
+    MA_REQUIRE(graph_exec, "ma_graph_launch: null graph");
+    MA_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream));
+    return MA_OK;
+}
+
+int ma_graph_destroy(void *graph_exec) {
+    if (graph_exec) MA_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
     return MA_OK;
 }
 

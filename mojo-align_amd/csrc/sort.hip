@@ -62,9 +62,12 @@ struct MdBufs {
 
 struct MdView {
     const uint64_t *hi, *lo;  // lo null when it did not move (implied by hi, or constant)
-    const uint32_t *idx;
-    uint64_t *best;           // per group: max (score << 32 | ~pair); the free lo buffer
-    uint64_t *vs;             // per position: its record's (score << 32 | ~pair); the free hi buffer
+    const uint32_t *idx;      // the (dhi, dlo) order: original record of each sorted position
+    uint32_t *gid;            // group of each sorted position (the free permutation buffer)
+    uint64_t *vorig;          // (score << 32 | ~pair) in original order (the final lo buffer,
+                              // free once the group ids are made)
+    uint64_t *best;           // per group: the largest of those (the other lo buffer)
+    uint64_t *vs;             // per sorted position: its record's value (the free hi buffer)
 };
 
 __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B) {
@@ -73,6 +76,8 @@ __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B)
     m.hi = B.hi[f];
     m.lo = plan->carry ? B.lo[f] : nullptr;
     m.idx = B.out_v ? B.out_v : B.v[f];
+    m.gid = B.v[f ^ 1u];
+    m.vorig = B.lo[f];
     m.best = B.lo[f ^ 1u];
     m.vs = B.hi[f ^ 1u];
     return m;
@@ -100,22 +105,10 @@ __global__ void __launch_bounds__(256) md_tile_heads_kernel(const SortPlan *__re
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// best[g] = 0 for the groups' slots (the free lo buffer, chosen by the plan).
-__global__ void md_clear_best_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n) {
-    uint64_t *best = B.lo[plan->fin ^ 1u];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        best[i] = 0ull;
-}
-
-// MARK = false: gather each record's (score << 32 | ~pair) into the sorted
-// order and fold it into its group's best; MARK = true: is_dup in original
-// order = a mapped record whose pair is not its group's best pair.
-template <bool MARK>
-__global__ void __launch_bounds__(256) md_groups_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
-                                                         const uint64_t *__restrict__ partial,
-                                                         const uint32_t *__restrict__ score,
-                                                         const uint32_t *__restrict__ pair,
-                                                         uint32_t *__restrict__ is_dup) {
+// gid[i] = the group of sorted position i (heads scanned within the tile on
+// top of the tile's exclusive head count).
+__global__ void __launch_bounds__(256) md_gid_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                                      const uint64_t *__restrict__ partial) {
     __shared__ uint64_t sh[4];
     const MdView m = md_view(plan, B);
     const int64_t base = (int64_t)blockIdx.x * MD_TILE;
@@ -128,19 +121,43 @@ __global__ void __launch_bounds__(256) md_groups_kernel(const SortPlan *__restri
         const uint32_t h = valid ? md_head(m, i) : 0u;
         uint64_t tot;
         const uint64_t ex = block_excl_scan_256(h, sh, tot);
-        const uint32_t o = valid ? m.idx[i] : 0u;
-        if (valid && (int64_t)o < n) {  // o >= n only after a radix pass's timeout (plan->error)
-            const uint64_t g = run + ex + h - 1;
-            const bool mapped = m.hi[i] != ~0ull;
-            if (!MARK) {
-                const uint64_t v = ((uint64_t)score[o] << 32) | (uint64_t)(0xFFFFFFFFu - pair[o]);
-                m.vs[i] = v;
-                if (mapped) atomicMax((unsigned long long *)&m.best[g], (unsigned long long)v);
-            } else {
-                is_dup[o] = (mapped && (uint32_t)m.vs[i] != (uint32_t)m.best[g]) ? 1u : 0u;
-            }
-        }
+        if (valid) m.gid[i] = (uint32_t)(run + ex + h - 1);
         run += tot;
+    }
+}
+
+// Per record, in original order: (score << 32 | ~pair), so the best of a
+// group is the largest (highest score, then lowest pair id); best cleared.
+__global__ void md_pack_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                               const uint32_t *__restrict__ score, const uint32_t *__restrict__ pair) {
+    const MdView m = md_view(plan, B);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        m.vorig[i] = ((uint64_t)score[i] << 32) | (uint64_t)(0xFFFFFFFFu - pair[i]);
+        m.best[i] = 0ull;
+    }
+}
+
+// One sorted position per thread (many gathers in flight): its record's value
+// into the sorted order, folded into its group's best (mapped records only).
+__global__ void md_best_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n) {
+    const MdView m = md_view(plan, B);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = m.idx[i];
+        if ((int64_t)o >= n) continue;  // only after a radix pass's timeout (plan->error)
+        const uint64_t v = m.vorig[o];
+        m.vs[i] = v;
+        if (m.hi[i] != ~0ull) atomicMax((unsigned long long *)&m.best[m.gid[i]], (unsigned long long)v);
+    }
+}
+
+// is_dup in original order: a mapped record whose pair is not its group's best pair.
+__global__ void md_mark_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n, uint32_t *__restrict__ is_dup) {
+    const MdView m = md_view(plan, B);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t o = m.idx[i];
+        if ((int64_t)o >= n) continue;
+        const bool dup = m.hi[i] != ~0ull && (uint32_t)m.vs[i] != (uint32_t)m.best[m.gid[i]];
+        is_dup[o] = dup ? 1u : 0u;
     }
 }
 
@@ -406,11 +423,14 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
         MA_LAUNCHED();
         scan_top_kernel<<<1, RS_BLOCK, 0, st>>>(partial, nt, nullptr);
         MA_LAUNCHED();
-        md_clear_best_kernel<<<g, 256, 0, st>>>(S.plan, B, n);
+        md_gid_kernel<<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial);
         MA_LAUNCHED();
-        md_groups_kernel<false><<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial, d_score, d_pair, nullptr);
+        const int gw = (int)((n + 255) / 256 < (1 << 20) ? (n + 255) / 256 : (1 << 20));
+        md_pack_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_score, d_pair);
         MA_LAUNCHED();
-        md_groups_kernel<true><<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial, d_score, d_pair, d_is_dup);
+        md_best_kernel<<<gw, 256, 0, st>>>(S.plan, B, n);
+        MA_LAUNCHED();
+        md_mark_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_is_dup);
         MA_LAUNCHED();
     } else {
         fill_u32_kernel<<<g, 256, 0, st>>>(d_is_dup, n, 0u);

@@ -11,6 +11,7 @@ from .core import (  # noqa: F401
     DeviceRecs,
     Event,
     GpuIndex,
+    Graph,
     Stream,
     Workspace,
     device_info,

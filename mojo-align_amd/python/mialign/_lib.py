@@ -149,6 +149,12 @@ SIGNATURES = {
                                            C.POINTER(Recs)]),
     "ma_sort_keys_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
     "ma_sort_dev_status": (C.c_int, [P, I64, P, I64]),
+    "ma_tune_set": (C.c_int, [I32, I64]),
+    "ma_graph_begin": (C.c_int, [P]),
+    "ma_graph_end": (C.c_int, [P, C.POINTER(P), C.POINTER(I64)]),
+    "ma_graph_launch": (C.c_int, [P, P]),
+    "ma_graph_destroy": (C.c_int, [P]),
+    "ma_tune_get": (C.c_int, [I32, C.POINTER(I64), C.POINTER(I64)]),
     "ma_sort_markdup": (C.c_int, [C.c_int, I64, P, P, P, P, P, P, P, I32, P]),
     "ma_sort_scratch_bytes": (I64, [I64]),
     "ma_sort_keys_pe_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),

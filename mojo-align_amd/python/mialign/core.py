@@ -88,6 +88,52 @@ class Stream:
             pass
 
 
+class Graph:
+    """A stage chain captured from a stream and replayed as one HIP graph.
+
+    with Graph.capture(stream) as g:   # the *_dev calls made here are recorded
+        ...
+    g.launch(stream)                   # replays them (same buffers, same shapes)
+    """
+
+    def __init__(self, handle: int, n_nodes: int):
+        self.handle, self.n_nodes = handle, n_nodes
+
+    @classmethod
+    def capture(cls, stream):
+        return _Capture(stream)
+
+    def launch(self, stream) -> None:
+        h = stream.handle if isinstance(stream, Stream) else stream
+        check(lib().ma_graph_launch(self.handle, h), "ma_graph_launch")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().ma_graph_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class _Capture:
+    def __init__(self, stream):
+        self.h = stream.handle if isinstance(stream, Stream) else stream
+        self.graph: Optional[Graph] = None
+
+    def __enter__(self) -> "_Capture":
+        check(lib().ma_graph_begin(self.h), "ma_graph_begin")
+        return self
+
+    def __exit__(self, et, ev, tb) -> None:
+        x, nn = C.c_void_p(), C.c_int64(0)
+        rc = lib().ma_graph_end(self.h, C.byref(x), C.byref(nn))
+        if et is None:
+            check(rc, "ma_graph_end")
+            self.graph = Graph(x.value, int(nn.value))
+        elif rc == 0 and x.value:
+            lib().ma_graph_destroy(x.value)
+
+
 def sync(stream) -> None:
     h = stream.handle if isinstance(stream, Stream) else stream
     check(lib().ma_stream_sync(h), "ma_stream_sync")
