@@ -271,6 +271,8 @@ class StepRunner:
         from mialign import autotune
         from mialign._lib import MA_BS_C2T, MA_ENC_2BIT
 
+        self.step(False)  # the sort is timed on this step's own keys
+        self.stream.sync()
         mp = autotune.tune_map(self.ix, self.ws, self.stream, self.d_bases, self.stride, self.d_lens, self.n, self.L,
                                self.recs, encoding=MA_ENC_2BIT, bs_r1=MA_BS_C2T, params=self.prm, repeats=repeats,
                                names=names)

@@ -137,16 +137,38 @@ __global__ void md_pack_kernel(const SortPlan *__restrict__ plan, MdBufs B, int6
     }
 }
 
-// One sorted position per thread (many gathers in flight): its record's value
+// One sorted position per lane (many gathers in flight): its record's value
 // into the sorted order, folded into its group's best (mapped records only).
+// A wave covers 64 consecutive positions, whose group ids only grow, so each
+// run of one group is first reduced across its lanes and only the run's first
+// lane updates the group's slot: a group of any size costs one atomic per
+// wave it spans.
 __global__ void md_best_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n) {
     const MdView m = md_view(plan, B);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t o = m.idx[i];
-        if ((int64_t)o >= n) continue;  // only after a radix pass's timeout (plan->error)
-        const uint64_t v = m.vorig[o];
-        m.vs[i] = v;
-        if (m.hi[i] != ~0ull) atomicMax((unsigned long long *)&m.best[m.gid[i]], (unsigned long long)v);
+    const int lane = threadIdx.x & 63;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
+         base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + lane;
+        uint64_t v = 0;
+        uint32_t g = 0xFFFFFFFFu;  // lanes past n: a group of their own
+        if (i < n) {
+            const uint32_t o = m.idx[i];
+            if ((int64_t)o < n) {  // o >= n only after a radix pass's timeout (plan->error)
+                const uint64_t x = m.vorig[o];
+                m.vs[i] = x;
+                if (m.hi[i] != ~0ull) v = x;
+            }
+            g = m.gid[i];
+        }
+        // suffix max over the lanes of the same group (contiguous lanes)
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t ov = __shfl_down(v, off, 64);
+            const uint32_t og = (uint32_t)__shfl_down((int)g, off, 64);
+            if (lane + off < 64 && og == g && ov > v) v = ov;
+        }
+        const uint32_t gp = (uint32_t)__shfl_up((int)g, 1, 64);
+        if (i < n && v != 0 && (lane == 0 || gp != g)) atomicMax((unsigned long long *)&m.best[g], (unsigned long long)v);
     }
 }
 

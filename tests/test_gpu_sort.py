@@ -77,8 +77,10 @@ def test_sort_markdup_se_keys_implied_low_word(n):
 
 
 def test_sort_markdup_all_equal_keys():
+    """One duplicate group of 2M records (every key equal): no radix pass runs,
+    and the group's best is folded per wave, not per record."""
     say("test_sort_markdup_all_equal_keys")
-    n = 10_000
+    n = 2_000_000
     k = np.full(n, 12345, np.uint64)
     score = np.random.default_rng(3).integers(0, 8, n).astype(np.uint32)
     want = _both(k, k, np.zeros(n, np.uint64), score, np.arange(n, dtype=np.uint32))
