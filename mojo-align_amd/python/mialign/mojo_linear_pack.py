@@ -13,12 +13,13 @@ packs load unchanged — fq2bam-meth/python/mojo_linear_pack.py:1-11):
 
 build_dense_pack() builds on the GPU (GpuIndex.build: one key per window,
 stable radix sort, CSR by scan) instead of numpy's host mergesort; the bytes
-are the same. ensure_dense_pack() keeps the reference's lock-then-recheck
-contract (:98-157).
+are the same. ensure_dense_pack() keeps the reference's build-once contract
+for concurrent workers (:98-157).
 """
 
 from __future__ import annotations
 
+import contextlib
 import fcntl
 import json
 import os
@@ -38,23 +39,20 @@ def pack_is_complete(cache_dir: Path) -> bool:
 
 
 def read_fasta_contigs(path: Path) -> List[Tuple[str, bytes]]:
-    """(name, upper-cased bases); name = first header word (:160-176)."""
-    contigs: List[Tuple[str, bytes]] = []
-    name = ""
-    chunks: List[bytes] = []
-    with Path(path).open("rb") as fh:
-        for raw in fh:
-            if raw.startswith(b">"):
-                if name:
-                    contigs.append((name, b"".join(chunks).upper()))
-                header = raw[1:].decode("ascii", errors="replace").strip()
-                name = header.split()[0] if header else "unnamed"
-                chunks = []
-            else:
-                chunks.append(raw.strip())
-    if name:
-        contigs.append((name, b"".join(chunks).upper()))
-    return contigs
+    """Contigs of a FASTA file as (name, bases), in file order. A record starts
+    at a line beginning with '>'; its name is the header's first word
+    ("unnamed" for a blank header); its bases are its lines with surrounding
+    whitespace removed, joined and upper-cased. Lines before the first header
+    belong to no record. (The contract of the reference's reader,
+    fq2bam-meth/python/mojo_linear_pack.py:160-176.)"""
+    records: List[Tuple[str, List[bytes]]] = []
+    for line in Path(path).read_bytes().split(b"\n"):
+        if line.startswith(b">"):
+            words = line[1:].decode("ascii", errors="replace").split()
+            records.append((words[0] if words else "unnamed", []))
+        elif records:
+            records[-1][1].append(line.strip())
+    return [(name, b"".join(parts).upper()) for name, parts in records]
 
 
 def contigs_to_arrays(contigs: Sequence[Tuple[str, bytes]]):
@@ -124,30 +122,41 @@ def build_dense_pack(c2t_fasta: Path, out_dir: Path, k: int = 15, device: int = 
                            str(c2t_fasta))
 
 
+@contextlib.contextmanager
+def _exclusive(lock_path: Path):
+    """Hold an exclusive flock on lock_path (workers building the same pack
+    queue here; flock, the lock the reference's workers take on the same file)."""
+    fd = os.open(lock_path, os.O_RDWR | os.O_CREAT | os.O_APPEND, 0o644)
+    try:
+        fcntl.flock(fd, fcntl.LOCK_EX)
+        yield fd
+    finally:
+        os.close(fd)  # closing the descriptor releases the lock
+
+
 def ensure_dense_pack(*, c2t_fasta: Path, cache_dir: Path, k: int = 15,
                       log: Optional[Callable[[str], None]] = None) -> Path:
-    """Return a complete pack, building under an exclusive flock when missing.
-    METHYLGRAPHER_LINEAR_CACHE_BUILD=0 fails instead of building."""
+    """A complete dense-v1 pack at cache_dir, built from c2t_fasta when missing
+    (the contract of fq2bam-meth/python/mojo_linear_pack.py:98-157): workers
+    serialize on `<cache_dir>.lock` and whoever gets the lock first builds;
+    the others find the pack complete once they hold it.
+    METHYLGRAPHER_LINEAR_CACHE_BUILD=0 makes a missing pack an error."""
     cache_dir = Path(cache_dir)
     say = log or (lambda m: print(m, flush=True))
     if pack_is_complete(cache_dir):
         return cache_dir
-    auto = os.environ.get("METHYLGRAPHER_LINEAR_CACHE_BUILD", "1").strip().lower()
-    if auto in ("0", "false", "no", "off"):
-        raise RuntimeError(f"dense-v1 pack missing at {cache_dir} (METHYLGRAPHER_LINEAR_CACHE_BUILD={auto})")
+    policy = os.environ.get("METHYLGRAPHER_LINEAR_CACHE_BUILD", "1").strip().lower()
+    if policy in ("0", "false", "no", "off"):
+        raise RuntimeError(f"dense-v1 pack missing at {cache_dir} (METHYLGRAPHER_LINEAR_CACHE_BUILD={policy})")
     if not Path(c2t_fasta).is_file():
         raise FileNotFoundError(f"C2T FASTA required to build pack: {c2t_fasta}")
     cache_dir.parent.mkdir(parents=True, exist_ok=True)
-    lock_path = Path(str(cache_dir) + ".lock")
-    t0 = time.time()
-    with lock_path.open("a+", encoding="utf-8") as lock_fh:
-        fcntl.flock(lock_fh.fileno(), fcntl.LOCK_EX)
-        say(f"mialign pack: acquired lock after {time.time() - t0:.1f}s")
-        if pack_is_complete(cache_dir):
-            return cache_dir
-        build_dense_pack(Path(c2t_fasta), cache_dir, k=k)
-        lock_fh.write(f"built k={k} t={time.time():.0f}\n")
-        lock_fh.flush()
+    started = time.time()
+    with _exclusive(Path(f"{cache_dir}.lock")) as fd:
+        say(f"mialign pack: lock held after {time.time() - started:.1f}s")
+        if not pack_is_complete(cache_dir):  # nobody built it while we waited
+            build_dense_pack(Path(c2t_fasta), cache_dir, k=k)
+            os.write(fd, f"built k={k} t={time.time():.0f}\n".encode())
     return cache_dir
 
 

@@ -339,9 +339,43 @@ def test_seed_kmers_wang_hash():
 
 
 def test_index_open_dense_v1_dir(w1, tmp_path):
+    """A pack the reference's own builder wrote (tests/golden/dense_v1_pack)
+    opens unchanged: its arrays come back byte for byte, the device build of
+    its sequences equals them, and reads map through it as the oracle maps
+    them on the same arrays; a pack this package writes opens the same way."""
     say("test_index_open_dense_v1_dir")
+    from pathlib import Path
+
     from mialign import mojo_linear_pack as pack
 
+    gold = Path(__file__).resolve().parent / "golden" / "dense_v1_pack"
+    g = mialign.GpuIndex.open(str(gold))
+    km = np.fromfile(gold / "kmers.bin", "<u8")
+    of = np.fromfile(gold / "offsets.bin", "<u8")
+    po = np.fromfile(gold / "postings.bin", "<u4")
+    seq = np.fromfile(gold / "sequences.bin", np.uint8)
+    coff = np.fromfile(gold / "contig_offsets.bin", "<u8")
+    gk, go, gp = g.export_dense()
+    assert np.array_equal(gk, km) and np.array_equal(go, of) and np.array_equal(gp, po)
+    b = mialign.GpuIndex.build(seq, coff, 11)
+    bk, bo, bp = b.export_dense()
+    assert np.array_equal(bk, km) and np.array_equal(bo, of) and np.array_equal(bp, po)
+    b.close()
+    rng = np.random.default_rng(17)
+    rows = np.full((400, 90), ord("N"), np.uint8)
+    lens = rng.integers(40, 91, 400).astype(np.uint32)
+    for i in range(400):
+        c = int(rng.integers(0, coff.size - 1))
+        a0, a1 = int(coff[c]), int(coff[c + 1])
+        st = int(rng.integers(a0, a1 - int(lens[i])))
+        rows[i, : lens[i]] = seq[st : st + int(lens[i])]
+    prm = mialign.default_params(k=11)
+    got = mialign.map_batch(g, rows, lens, 90, params=prm)
+    oix = orc.OracleIndex(km, of, po, seq, coff)
+    want, _, _, _ = orc.map_batch(oix, rows, lens, orc.params(k=11), nthreads=8)
+    assert_recs_equal(got, want, "index opened from the reference's dense-v1 pack")
+    assert (got["cid"] >= 0).mean() > 0.9
+    g.close()
     out = tmp_path / "pack"
     pack.write_dense_dir(out, 21, *w1.dense, w1.dual.seq, w1.dual.contig_off, w1.dual.names)
     g = mialign.GpuIndex.open(str(out))
