@@ -24,6 +24,7 @@
  *   ma_device_info .......... select_device / probe_device_context + hbm_info,
  *                             gpu-common/src/gpu_device.mojo:19-63, gpu_kernels.mojo:42-65,
  *                             gpu-common/python/gpu_mem.py:88-121
+ *   ma_mate_rescue_dev ...... fm_mate_rescue_kernel, fq2bam-meth/src/linear_gpu_fm.mojo:1839-2011
  *   ma_bam_pack_se_dev ...... _pack_bam_aln over the SE branch of _emit_bam_from_batch
  *                             + _gather_perm_to_writer / _or_dup_flag,
  *                             fq2bam-meth/src/linear_gpu_fm.mojo:497-665, 755-800, 177-225, 408-420
@@ -243,6 +244,23 @@ int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases,
                              const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t encoding,
                              int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p,
                              ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc);
+
+/* Mate rescue of a paired batch, in place on its merged records (device
+ * pointers; R1 block [0, n_pairs) with bs_r1, R2 block [n_pairs, 2 n_pairs)
+ * with bs_r2, rows as ma_map_batch_dev's). For a pair with exactly one mate
+ * mapped, the other is scanned gapless within +-window (clamped to 32..1024)
+ * of the mate on the mate's contig, the opposite strand first; the first
+ * least mismatch count within clamp(max_diff, 2, len / 2) wins (counting stops
+ * past budget + 4); with none, a one-base deletion at the first least window,
+ * split points every 4 bases. A rescued mate: the contig, the window start,
+ * flag 16 if reverse, no clips, nm, MAPQ 60 / 40 / 20 (nm 0 / <= 2 / more),
+ * iop / iat / ilen 0; reads shorter than 24 stay unmapped; window <= 0 does
+ * nothing. Replaces fm_mate_rescue_kernel of the native-BAM driver,
+ * fq2bam-meth/src/linear_gpu_fm.mojo:1839-2011 (METHYLGRAPHER_FM_RESCUE_WIN,
+ * default 512, :2082). Enqueued on stream. */
+int ma_mate_rescue_dev(ma_index *ix, void *stream, const uint8_t *d_bases, int64_t stride, const uint32_t *d_lens,
+                       int64_t n_pairs, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
+                       int32_t window, int32_t max_diff, ma_recs *d_merged);
 
 /* ---- sort keys, sort, duplicate marking ---- */
 /* SE keys from merged records (device pointers): coord, dhi, dlo, score, pair

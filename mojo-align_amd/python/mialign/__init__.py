@@ -17,6 +17,8 @@ from .core import (  # noqa: F401
     device_info,
     map_batch,
     map_batch_dev,
+    mate_rescue,
+    mate_rescue_dev,
     seed_kmers,
     sort_markdup,
     sync,

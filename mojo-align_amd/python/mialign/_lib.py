@@ -148,6 +148,7 @@ SIGNATURES = {
                                            C.POINTER(Params), C.POINTER(Recs), C.POINTER(Recs),
                                            C.POINTER(Recs)]),
     "ma_sort_keys_se_dev": (C.c_int, [P, I64, C.POINTER(Recs), P, P, I32, I64, P, P, P, P, P]),
+    "ma_mate_rescue_dev": (C.c_int, [P, P, P, I64, P, I64, I32, I32, I32, I32, I32, I32, C.POINTER(Recs)]),
     "ma_sort_dev_status": (C.c_int, [P, I64, P, I64]),
     "ma_tune_set": (C.c_int, [I32, I64]),
     "ma_graph_begin": (C.c_int, [P]),

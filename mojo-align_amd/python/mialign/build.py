@@ -20,7 +20,8 @@ CSRC = PKG_ROOT / "csrc"
 INCLUDE = PKG_ROOT.parent / "include"
 BUILD_DIR = PKG_ROOT / "_build"
 
-HIP_SOURCES = ["runtime.hip", "index.hip", "sort.hip", "map.hip", "serve.hip", "bam.hip", "bgzf.hip", "fastq.hip"]
+HIP_SOURCES = ["runtime.hip", "index.hip", "sort.hip", "map.hip", "serve.hip", "bam.hip", "bgzf.hip", "fastq.hip",
+               "rescue.hip"]
 HIP_DEPS = ["ma_common.hpp", "ma_index.hpp", "ma_sort.hpp", "ma_onesweep.hpp", "map_body.inc"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -381,6 +381,34 @@ def map_batch_dev(index: GpuIndex, ws: Workspace, stream, d_bases: DeviceArray, 
     )
 
 
+def mate_rescue_dev(index: GpuIndex, stream, d_bases: DeviceArray, stride: int, d_lens: DeviceArray, n_pairs: int,
+                    max_len: int, recs: DeviceRecs, *, encoding: int = MA_ENC_2BIT, bs_r1: int = 0, bs_r2: int = 0,
+                    window: int = 512, max_diff: int = 6) -> None:
+    """Mate rescue of a paired batch in place on its device records (R1 block
+    then R2 block of n_pairs each); ma_mate_rescue_dev."""
+    h = stream.handle if isinstance(stream, Stream) else stream
+    check(lib().ma_mate_rescue_dev(index.handle, h, d_bases.ptr, stride, d_lens.ptr, n_pairs, int(max_len), encoding,
+                                   bs_r1, bs_r2, int(window), int(max_diff), C.byref(recs.struct)),
+          "ma_mate_rescue_dev")
+
+
+def mate_rescue(index: GpuIndex, bases: np.ndarray, lens: np.ndarray, max_len: int, recs: dict, *,
+                encoding: int = MA_ENC_ASCII, bs_r1: int = 0, bs_r2: int = 0, window: int = 512,
+                max_diff: int = 6) -> dict:
+    """Host-array form: the merged records of a paired batch with rescued
+    mates filled in (a copy)."""
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n = lens.size
+    st = Stream(index.device)
+    d_b = DeviceArray.from_host(np.ascontiguousarray(bases, dtype=np.uint8), index.device)
+    d_l = DeviceArray.from_host(lens, index.device)
+    d_r = DeviceRecs.from_host(recs, n, index.device)
+    mate_rescue_dev(index, st, d_b, bases.shape[1], d_l, n // 2, max_len, d_r, encoding=encoding, bs_r1=bs_r1,
+                    bs_r2=bs_r2, window=window, max_diff=max_diff)
+    st.sync()
+    return d_r.download()
+
+
 # -------------------------------------------------------- sort + markdup
 def sort_markdup(coord, dhi, dlo, score, pair, do_markdup: bool = True, device: int = 0):
     """gpu_sort_markdup contract on host arrays -> (is_dup, perm, dup_perm)."""

@@ -94,14 +94,16 @@ class _Acc:
 def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str = "", bs_r2: str = "G2A",
                  batch_reads: int = 1 << 20, stride: int = 256, do_markdup: bool = True, device_bgzf: bool = True,
                  level: int = 1, rg_id: str = "mojo1", sm: str = "sample", lb: str = "lib1", pl: str = "ILLUMINA",
-                 pg_vn: str = "0.1.0-mojo-fm", device: int = 0, params=None) -> dict:
+                 pg_vn: str = "0.1.0-mojo-fm", device: int = 0, params=None, rescue_window: int = 512) -> dict:
     """FASTQ (single-end, or paired with fq2) -> BAM in coordinate order, 0x400
     on duplicates. `index` is a LinearIndex with its GPU index loaded. Pairs are
     mapped R1 with bs_r1 and R2 with bs_r2, emitted R1, R2 per pair with the PE
     flags and mate fields, and marked as pairs. `params` are the mapping knobs
     (default: the reference defaults for the index's k); the BAM records carry
     slS mM srS CIGARs only, as the reference's native-BAM records do, so
-    params.max_indel must be 0. Returns counts and stage times."""
+    params.max_indel must be 0. Pairs with one mate unmapped get the native
+    BAM driver's mate rescue within +-rescue_window bases of the mapped mate
+    (0 = off; ma_mate_rescue_dev). Returns counts and stage times."""
     paired = bool(fq2)
     gix = index.gpu
     prm = default_params(k=gix.k) if params is None else params
@@ -160,6 +162,13 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
     recs = Recs()
     for f in REC_FIELDS:
         setattr(recs, f, acc.fields[f].buf.ptr if f in acc.fields else None)
+    rescued = 0
+    if paired and rescue_window > 0:
+        before = int((acc.fields["cid"].view(np.int32).download() >= 0).sum())
+        check(lib().ma_mate_rescue_dev(gix.handle, st.handle, acc.rows.buf.ptr, stride, acc.lens.buf.ptr, N // 2,
+                                       stride, MA_ENC_ASCII, modes[0], modes[1], int(rescue_window), prm.max_diff,
+                                       C.byref(recs)), "ma_mate_rescue_dev")
+        rescued = int((acc.fields["cid"].view(np.int32).download(st) >= 0).sum()) - before
     keys = {f: DeviceArray((N,), dt, device) for f, dt in (("coord", np.uint64), ("dhi", np.uint64),
                                                           ("dlo", np.uint64), ("score", np.uint32),
                                                           ("pair", np.uint32))}
@@ -214,7 +223,7 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
     t_bgzf = time.perf_counter() - t2
     dups = int(is_dup.download().sum()) if do_markdup else 0
     mapped = int((acc.fields["cid"].view(np.int32).download() >= 0).sum())
-    return {"reads": N, "pairs": N // 2 if paired else 0, "mapped": mapped, "duplicates": dups,
+    return {"reads": N, "pairs": N // 2 if paired else 0, "mapped": mapped, "rescued": rescued, "duplicates": dups,
             "bam_stream_bytes": total, "bgzf_bytes": size, "ingest_and_map_s": round(t_ingest, 3),
             "map_kernel_ms": round(t_map, 3), "sort_and_pack_s": round(t_sort_pack, 3), "bgzf_s": round(t_bgzf, 3)}
 

@@ -69,6 +69,7 @@ def map_fastq_to_sam(ref_fasta: str, fq1: str, out_sam: str, device: str = "auto
                       "(gapless extension); write SAM for the single-indel branch", file=sys.stderr, flush=True)
                 prm.max_indel = 0
             stats = fastq_to_bam(index, fq1, out_sam, bs_r1, fq2=fq2, bs_r2=bs_r2, params=prm,
+                                 rescue_window=env_int("METHYLGRAPHER_FM_RESCUE_WIN", 512),
                                  level=env_int("METHYLGRAPHER_BAM_LEVEL", 1),
                                  device_bgzf=os.environ.get("MIALIGN_BGZF", "device").lower() != "host",
                                  rg_id=env_str("METHYLGRAPHER_RG_ID", "mojo1"),

@@ -1028,6 +1028,114 @@ void or_sort_keys_se(int64_t n, const int32_t *cid, const uint32_t *pos, const i
     }
 }
 
+/* ------------------------------------------------------ F4c mate rescue */
+
+/* Mate rescue of a paired batch, in place on the merged records (R1 block at
+ * [0, n_pairs), R2 block at [n_pairs, 2 n_pairs)); codes are the converted
+ * reads' base codes [2 n_pairs x max_len] (0-3, anything else = N), lens their
+ * lengths. For a pair with exactly one mate mapped, the other mate is scanned
+ * gapless at every start within +-window (clamped to 32..1024) of the mapped
+ * mate's position on its contig copy, the strand opposite the mate's first,
+ * then the mate's own; a window's mismatches are counted until they pass
+ * budget + 4 (budget = clamp(max_diff, 2, len / 2)); the first window of least
+ * count within budget wins. With none within budget, a single-base deletion is
+ * tried at the first window of least count: split points 0, 4, 8, ... <= len,
+ * bases before the split against the window, after it against the window one
+ * base on, plus 1; the first least within budget wins. A rescued mate gets the
+ * mapped mate's contig, the window start, flag 16 for the reverse strand, no
+ * clips, nm, and MAPQ 60 (nm 0), 40 (nm <= 2) or 20. Reads shorter than 24,
+ * and windows reaching past the reference's last base, are skipped; a
+ * reference N (code > 3) counts as a mismatch. Follows fm_mate_rescue_kernel,
+ * fq2bam-meth/src/linear_gpu_fm.mojo:1839-2011. */
+static int64_t rescue_mm(const uint8_t *ref, int64_t bstart, const uint8_t *q, int64_t qlen, int rev,
+                         int64_t from, int64_t to, int64_t stop) {
+    int64_t nm = 0;
+    for (int64_t j = from; j < to && nm <= stop; j++) {
+        int64_t rb = ref[bstart + j];
+        int64_t qc = q[j];
+        if (rev) {
+            int64_t qr = q[qlen - 1 - j];
+            qc = qr <= 3 ? 3 - qr : 4;
+        }
+        if (rb > 3 || qc > 3 || rb != qc) nm++;
+    }
+    return nm;
+}
+
+void or_mate_rescue(const or_index *ix, const uint8_t *codes, const uint32_t *lens, int64_t max_len,
+                    int64_t n_pairs, int64_t window, int64_t max_diff, or_recs *r) {
+    const int64_t l_pac = (int64_t)ix->contig_off[ix->n_contigs];
+    for (int64_t pi = 0; pi < n_pairs; pi++) {
+        const int64_t i1 = pi, i2 = pi + n_pairs;
+        const int m1 = r->cid[i1] >= 0, m2 = r->cid[i2] >= 0;
+        if (m1 == m2) continue;
+        const int64_t src = m1 ? i1 : i2, dst = m1 ? i2 : i1;
+        const int64_t crid = r->cid[src];
+        if (crid < 0 || crid >= ix->n_contigs) continue;
+        const int64_t qlen = lens[dst];
+        if (qlen < 24) continue;
+        const int64_t coff = (int64_t)ix->contig_off[crid];
+        const int64_t clen = (int64_t)ix->contig_off[crid + 1] - coff;
+        const int64_t mate_pos = r->pos[src];
+        int64_t win = window < 32 ? 32 : (window > 1024 ? 1024 : window);
+        int64_t lo = mate_pos - win;
+        if (lo < 0) lo = 0;
+        int64_t hi = mate_pos + win;
+        if (hi > clen - 24) hi = clen - 24;
+        if (hi <= lo) continue;
+        int64_t budget = max_diff < 2 ? 2 : max_diff;
+        if (budget > qlen / 2) budget = qlen / 2;
+        const uint8_t *q = codes + dst * max_len;
+        int64_t best_nm = 9999, best_pos = 0, best_flag = 0, scan_nm = 9999, scan_local = lo, scan_rev = 0;
+        const int mate_rev = (r->flag[src] & 16) != 0;
+        for (int pass = 0; pass < 2; pass++) {
+            const int is_rev = pass == 0 ? !mate_rev : mate_rev;
+            for (int64_t local = lo; local <= hi; local++) {
+                const int64_t bstart = coff + local;
+                if (bstart < 0 || bstart + qlen > l_pac) continue;
+                const int64_t nm = rescue_mm(ix->ref_codes, bstart, q, qlen, is_rev, 0, qlen, budget + 4);
+                if (nm < scan_nm) {
+                    scan_nm = nm;
+                    scan_local = local;
+                    scan_rev = is_rev;
+                }
+                if (nm <= budget && nm < best_nm) {
+                    best_nm = nm;
+                    best_pos = local;
+                    best_flag = is_rev * 16;
+                }
+            }
+        }
+        if (best_nm > budget && scan_nm < 9999) {
+            const int64_t bstart = coff + scan_local;
+            if (bstart >= 0 && bstart + qlen + 1 <= l_pac) {
+                for (int64_t g = 0; g <= qlen; g += 4) {
+                    int64_t nm_d = 1 + rescue_mm(ix->ref_codes, bstart, q, qlen, (int)scan_rev, 0, g, budget - 1);
+                    if (nm_d <= budget)
+                        nm_d += rescue_mm(ix->ref_codes, bstart + 1, q, qlen, (int)scan_rev, g, qlen, budget - nm_d);
+                    if (nm_d <= budget && nm_d < best_nm) {
+                        best_nm = nm_d;
+                        best_pos = scan_local;
+                        best_flag = scan_rev * 16;
+                    }
+                }
+            }
+        }
+        if (best_nm <= budget) {
+            r->cid[dst] = (int32_t)crid;
+            r->pos[dst] = (uint32_t)best_pos;
+            r->mapq[dst] = best_nm == 0 ? 60u : (best_nm <= 2 ? 40u : 20u);
+            r->flag[dst] = (int32_t)best_flag;
+            r->sl[dst] = 0;
+            r->sr[dst] = 0;
+            r->nm[dst] = (uint32_t)best_nm;
+            r->iop[dst] = 0;
+            r->iat[dst] = 0;
+            r->ilen[dst] = 0;
+        }
+    }
+}
+
 /* ------------------------------------------------------------- A11 / A12 */
 
 typedef struct { uint64_t hi, lo; uint32_t idx; } or_k2;

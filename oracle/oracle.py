@@ -96,6 +96,8 @@ def lib():
         L.or_pack_end.restype = C.c_uint64
         L.or_pack_end.argtypes = [I64, I64, I64]
         L.or_sort_keys_se.argtypes = [I64, P, P, P, P, P, P, P, P, P, I64, I64, P, P, P, P, P]
+        L.or_mate_rescue.argtypes = [C.POINTER(Index), P, P, I64, I64, I64, I64, C.POINTER(Recs)]
+        L.or_mate_rescue.restype = None
         L.or_sort_markdup.restype = C.c_int
         L.or_sort_markdup.argtypes = [I64, P, P, P, P, P, P, P, C.c_int, P]
         L.or_num_threads.restype = C.c_int
@@ -219,6 +221,22 @@ def map_batch(ix: OracleIndex, bases: np.ndarray, lens: np.ndarray, prm: Params 
     if r != 0:
         raise RuntimeError(f"or_map_batch failed ({r})")
     return mg, fw, rc, cnt
+
+
+def mate_rescue(ix: OracleIndex, bases: np.ndarray, lens: np.ndarray, rec: dict, window: int = 512,
+                max_diff: int = 6) -> dict:
+    """Mate rescue of a paired batch (R1 block then R2 block of n_pairs each;
+    reads already in-silico converted, [2 n_pairs, max_len] ASCII): a copy of
+    the merged records with rescued mates filled in."""
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n, max_len = bases.shape
+    codes = np.empty_like(bases)
+    lib().or_pack_bases(_p(bases), _p(codes), bases.size)
+    out = {f: np.ascontiguousarray(rec[f]).copy() for f in REC_FIELDS}
+    lib().or_mate_rescue(C.byref(ix.struct), _p(codes), _p(lens), max_len, n // 2, window, max_diff,
+                         C.byref(recs_struct(out)))
+    return out
 
 
 def sort_keys_se(rec: dict, lens: np.ndarray, tid_of_cid: np.ndarray, pair_base: int = 0):

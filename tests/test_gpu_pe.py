@@ -179,6 +179,7 @@ def test_fastq_to_bam_pe_end_to_end(tmp_path):
     oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
     conv = np.concatenate([orc.convert_reads(r1.bases, 1), orc.convert_reads(r2.bases, 2)])
     want, _, _, _ = orc.map_batch(oix, conv, lens, nthreads=8)
+    want = orc.mate_rescue(oix, conv, lens, want, 512, 6)  # the native-BAM driver's mate rescue
     tid = synth.tid_of_cid(dual.names)
     wf = orc.pe_fields(want, lens, tid)
     d, p, _ = orc.sort_markdup(wf["coord"], wf["dhi"], wf["dlo"], wf["score"], wf["pair"])
