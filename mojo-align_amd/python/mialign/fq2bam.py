@@ -168,7 +168,7 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
         check(lib().ma_mate_rescue_dev(gix.handle, st.handle, acc.rows.buf.ptr, stride, acc.lens.buf.ptr, N // 2,
                                        stride, MA_ENC_ASCII, modes[0], modes[1], int(rescue_window), prm.max_diff,
                                        C.byref(recs)), "ma_mate_rescue_dev")
-        rescued = int((acc.fields["cid"].view(np.int32).download(st) >= 0).sum()) - before
+        rescued = int((acc.fields["cid"].view(np.int32).download(st.handle) >= 0).sum()) - before
     keys = {f: DeviceArray((N,), dt, device) for f, dt in (("coord", np.uint64), ("dhi", np.uint64),
                                                           ("dlo", np.uint64), ("score", np.uint32),
                                                           ("pair", np.uint32))}
