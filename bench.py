@@ -47,16 +47,22 @@ def log(msg: str) -> None:
 
 
 def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int, size_weight: float,
-                  loads=(0.85, 1.0, 1.2), seeds=3) -> dict:
+                  loads=(0.85, 1.0, 1.2), seeds=3, tid=None, samples: int = 64, tune: bool = False) -> dict:
     """Request loop: n_req batches (sizes lognormal, median 50k, clipped to
     [1k, 2M] reads) cut from the resident workload's reads (one pinned pool),
+    each from one of `samples` samples (request i from sample i % samples),
     Poisson arrivals offered at each of `loads` x the plain loop's throughput.
-    Plain = the same batches, one per launch, one after another. Each load
-    runs `seeds` arrival sequences; its p50 / p99 are the medians of the
-    sequences' own p50 / p99 and its throughput their mean (the per-sequence
-    p99s are listed too): a tail of 512 requests moves by several ms when the
-    runtime holds an async copy call for ~7 ms, which happened about once in
-    every few seconds of serving when it was timed.
+    With a target table `tid`, every batch is also sorted and duplicate-marked
+    on its own once mapped (SE keys, per sample: never over the device's mix).
+    Admission: no budget of the server's own; it takes device memory only
+    within the free HBM the driver reports at that moment.
+    Plain = the same batches, one per launch, one after another (sorted too).
+    With `tune`, launch_reads and slots are picked by autotune.tune_serve on
+    arrival sequences of their own (not the measured ones) at the first load,
+    and saved with the device's other choices.
+    Each load runs `seeds` arrival sequences; its p50 / p99 are the medians of
+    the sequences' own p50 / p99 and its throughput their mean (the
+    per-sequence p99s are listed too).
     The headline figures are the first load's."""
     from mialign import serve
     from mialign._lib import MA_BS_C2T, MA_ENC_2BIT
@@ -65,27 +71,62 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
     n_pool = lens.size
     sizes = np.clip(np.exp(rng.normal(np.log(50_000), 1.0, n_req)), 1_000, min(2_000_000, n_pool)).astype(int)
     pool_b, pool_l = serve.pinned_copy(packed), serve.pinned_copy(lens)
+    keys = 1 if tid is not None else 0
     reqs = []
     for sz in sizes:
         a = int(rng.integers(0, n_pool - sz + 1))
         reqs.append(serve.Request(pool_b[a : a + sz], pool_l[a : a + sz], L, MA_ENC_2BIT, bs_r1=MA_BS_C2T,
-                                  out=serve.pinned_recs(int(sz))))
-    budget = 8 << 30
-    serve.run_plain(ix, reqs[:4], hbm_budget=budget)  # warm-up
-    plain = serve.run_plain(ix, reqs, hbm_budget=budget)
-    srv = serve.Server(ix, slots=slots, hbm_budget=budget, launch_reads=launch_reads, size_weight=size_weight)
-    warm = [serve.Request(r.bases, r.lens, r.max_len, r.encoding, bs_r1=r.bs_r1, out=r.out) for r in reqs[:4]]
-    serve.run_request_loop(srv, warm)  # same warm-up as the plain loop got
+                                  out=serve.pinned_recs(int(sz)), keys=keys))
+    budget = 0  # live free HBM only
+    serve.run_plain(ix, reqs[:4], hbm_budget=budget, tid=tid)  # warm-up
+    plain = serve.run_plain(ix, reqs, hbm_budget=budget, tid=tid)
+    warm = [serve.Request(r.bases, r.lens, r.max_len, r.encoding, bs_r1=r.bs_r1, out=r.out, keys=r.keys)
+            for r in reqs[:4]]
+
+    def open_server(lr, sl):
+        srv = serve.Server(ix, slots=sl, hbm_budget=budget, launch_reads=lr, size_weight=size_weight)
+        if tid is not None:
+            srv.set_targets(tid)
+        serve.run_request_loop(srv, warm)  # same warm-up as the plain loop got
+        return srv
+
+    def arrivals(load, seed):
+        arr = np.random.default_rng(seed)
+        rate = load * plain.reads_per_s / float(sizes.mean())  # requests per second
+        t = 0.0
+        for r in reqs:
+            t += float(arr.exponential(1.0 / rate))
+            r.arrival = t
+
+    tuning = None
+    if tune:
+        from mialign import autotune
+
+        def trial(lr, sl):
+            """The worse of two arrival sequences (the tail of one is noisy)."""
+            srv = open_server(lr, sl)
+            got = []
+            for k in range(2):
+                arrivals(loads[0], 90_000 + 1000 * k)
+                st = serve.run_request_loop(srv, reqs)
+                got.append((st.reads_per_s, st.pct(99)))
+            srv.close()
+            thr, p99 = min(g[0] for g in got), max(g[1] for g in got)
+            log(f"serve tuning: launch_reads {lr} slots {sl}: p99 {p99:.1f} ms ({p99 / plain.pct(99):.2f} x plain)")
+            return thr, p99, plain.reads_per_s, plain.pct(99)
+
+        t0 = time.time()
+        tuning = autotune.tune_serve(ix, reqs, trial, launch_reads=(262144, 524288), slots=(2, 4))
+        tuning["search_s"] = round(time.time() - t0, 1)
+        launch_reads, slots = tuning["choice"]["launch_reads"], tuning["choice"]["slots"]
+        autotune.save({}, serve=tuning["choice"])
+        log(f"serve tuning: chose {tuning['choice']} in {tuning['search_s']} s")
+    srv = open_server(launch_reads, slots)
     sweep = []
     for load in loads:
         runs = []
         for k in range(seeds):
-            arr = np.random.default_rng(2000 + int(load * 100) + 1000 * k)
-            rate = load * plain.reads_per_s / float(sizes.mean())  # requests per second
-            t = 0.0
-            for r in reqs:
-                t += float(arr.exponential(1.0 / rate))
-                r.arrival = t
+            arrivals(load, 2000 + int(load * 100) + 1000 * k)
             runs.append(serve.run_request_loop(srv, reqs))
         p50 = float(np.median([st.pct(50) for st in runs]))
         p99 = float(np.median([st.pct(99) for st in runs]))
@@ -100,10 +141,13 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
             "p99_ms_by_sequence": [round(st.pct(99), 3) for st in runs],
         })
         log(f"serving at {load:.2f}x plain: {sweep[-1]}")
+    mem = srv.memory()
     srv.close()
     head = sweep[0]
     return {
         "requests": n_req,
+        "samples": samples,
+        "per_sample_sort_markdup": bool(keys),
         "reads": int(sizes.sum()),
         "batch_reads_median": int(np.median(sizes)),
         "offered_load": f"{loads[0]:.0%} of plain throughput, Poisson arrivals, median of {seeds} sequences",
@@ -119,7 +163,10 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
         "slots": slots,
         "launch_reads": launch_reads,
         "size_weight": size_weight,
-        "hbm_budget_bytes": budget,
+        "tuning": tuning,
+        "admission": "live free HBM (hipMemGetInfo when the server takes memory; 256 MiB headroom), no fixed budget",
+        "device_bytes_held": mem["held"],
+        "free_hbm_bytes_after": mem["free_hbm"],
     }
 
 
@@ -1079,7 +1126,8 @@ def config2_section(args) -> dict:
     sr.close()
     if args.serve_100mbp and not args.no_serve:
         out["serving"] = serve_section(ix, packed, reads.lens, L, args.serve_requests, args.serve_slots,
-                                       args.serve_launch_reads, args.serve_size_weight)
+                                       args.serve_launch_reads, args.serve_size_weight,
+                                       tid=synth.tid_of_cid(dual.names))
     if not args.no_cpu:
         oix = oracle_index(ix, dual)
         ns = min(args.cpu_sample, n)
@@ -1120,6 +1168,9 @@ def main() -> int:
                     help="sort + markdup stress records (0 = skip)")
     ap.add_argument("--sort-cpu-sample", type=int, default=20_000_000)
     ap.add_argument("--sort-only", action="store_true", help="only the sort + markdup stress section")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the served loop on the 3.1 Gbp index (a pool of --serve-pool 50-250 bp reads)")
+    ap.add_argument("--serve-pool", type=int, default=10_000_000)
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in launch knobs (no autotuning)")
     ap.add_argument("--knobs", default="", help='fixed launch knobs as JSON (e.g. {"map_max_claim": 8}); no tuning')
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay timing of the headline step")
@@ -1173,6 +1224,18 @@ def main() -> int:
               flush=True)
         return 0
 
+    if args.serve_only:
+        from mialign import synth
+
+        gs = GenomeScale(keep_dense=False, genome_bp=args.gs_genome)
+        ur = synth.gen_reads(1103, gs.ref, args.serve_pool, 50, 250, max_len=250)
+        serving = serve_section(gs.ix, synth.pack_2bit(ur.bases, ur.lens), ur.lens, 250, args.serve_requests,
+                                args.serve_slots, args.serve_launch_reads, args.serve_size_weight, tid=gs.tid,
+                                tune=not args.no_tune)
+        gs.close()
+        print(json.dumps({"serving": serving}), flush=True)
+        return 0
+
     gs = GenomeScale(keep_dense=not args.no_cpu, genome_bp=args.gs_genome)
     head = headline_section(args, gs)
     uneven = serving = None
@@ -1181,7 +1244,8 @@ def main() -> int:
         upacked, ulens = uneven.pop("_packed"), uneven.pop("_lens")
         if not args.no_serve:
             serving = serve_section(gs.ix, upacked, ulens, 250, args.serve_requests, args.serve_slots,
-                                    args.serve_launch_reads, args.serve_size_weight)
+                                    args.serve_launch_reads, args.serve_size_weight, tid=gs.tid,
+                                    tune=not args.no_tune)
             log(f"serving (3.1 Gbp, 50-250 bp): {serving}")
         del upacked, ulens
     index_info = gs.index

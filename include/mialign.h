@@ -410,12 +410,19 @@ int ma_seed_kmers(int device, const uint8_t *bases, int64_t n, int32_t max_len, 
                   uint64_t *out_hash);
 
 /* ---- served request loop ---- */
-/* A server owns `slots` streams and an HBM budget (bytes); each slot's launch
- * buffer is reserved at open. Requests queue on submit and are cut into pieces
- * of at most `launch_reads` reads (default 131072; a paired request is also
- * split where its conversion changes). Whenever a slot is free, one launch is
- * assembled from waiting pieces in deadline order (see ma_serve_tune), merging compatible pieces (same row layout, conversion
- * and knobs) while the launch's bytes fit the free budget. */
+/* A server owns `slots` streams and maps queued read batches (requests) with
+ * continuous batching: requests are cut into pieces of at most `launch_reads`
+ * reads (default 131072; a paired request is also split where its conversion
+ * changes) and, whenever a slot is free, one launch is assembled from waiting
+ * pieces in deadline order (see ma_serve_tune), merging compatible pieces (same
+ * row layout, conversion and knobs).
+ * Device memory: everything the server holds (slot buffers, map workspaces,
+ * the target table, sorted requests' arenas) stays within hbm_budget bytes
+ * (0 = no cap of its own), and every time it takes more it first reads the
+ * device's free HBM (hipMemGetInfo) and takes it only if 256 MiB stay free.
+ * Slot buffers are reserved here; steady-state launches allocate nothing.
+ * Replaces no single reference call: the admission follows
+ * gpu-common/python/gpu_mem.py:203-377 (free-HBM preflight and wait). */
 int ma_serve_open(ma_index *ix, int32_t slots, int64_t hbm_budget, int32_t max_len,
                   ma_server **out);
 /* launch_reads >= 1024 caps the reads of one launch; waiting work runs in
@@ -425,20 +432,46 @@ int ma_serve_open(ma_index *ix, int32_t slots, int64_t hbm_budget, int32_t max_l
  * flight. */
 int ma_serve_tune(ma_server *s, int64_t launch_reads, double size_weight, int32_t persistent,
                   int32_t chunks_per_wave);
+/* Target table for sorted requests (host int32 tid_of_cid[n_tid], as
+ * ma_sort_keys_se_dev takes it; default: index contig c is target c). Only
+ * while no request is queued or running. */
+int ma_serve_set_targets(ma_server *s, const int32_t *tid_of_cid, int32_t n_tid);
 /* Queue host reads (same layout and record outputs as ma_map_batch's merged
  * records); the host arrays must stay valid until the ticket completes (pinned
- * memory lets the copies overlap; seven record arrays at one constant pitch in
- * one block are copied back with one 2D copy per piece). Returns a ticket >= 0. */
+ * memory lets the copies overlap; ten record arrays at one constant pitch in
+ * one block are copied back with one 2D copy per piece). Returns a ticket >= 0,
+ * or MA_ERR_NOMEM when the request is not admitted now (nothing of it queued). */
 int64_t ma_serve_submit(ma_server *s, const uint8_t *bases, int64_t stride, const uint32_t *lens,
                         int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
                         int64_t n_r1, const ma_params *p, ma_recs *merged_out);
+/* As ma_serve_submit, and the request is one sample's batch that is sorted and
+ * duplicate-marked on its own once mapped (the reference marks duplicates per
+ * sample: fq2bam-meth/src/linear_gpu_fm.mojo:800-823 builds the keys of one
+ * sample's records, linear_gpu_sort.mojo:156-551 sorts and marks them).
+ * keys 1: SE keys over its n records; keys 2: PE keys, R1 rows [0, n/2) and R2
+ * rows [n/2, n) with n_r1 = n/2, records then numbered in emission order (pair
+ * p's R1 = 2p, R2 = 2p + 1, as ma_sort_keys_pe_dev). is_dup[n] and perm[n]
+ * (host, valid until completion) receive what ma_sort_markdup_dev gives for
+ * those keys (pair_base 0). Its device arena (~112 bytes per read) is taken at
+ * submit: MA_ERR_NOMEM when it does not fit now (wait for completions and
+ * submit again). A sort whose radix pass gave up waiting is reported as
+ * MA_ERR_HIP by the poll / reap call that completes the ticket. */
+int64_t ma_serve_submit_sorted(ma_server *s, const uint8_t *bases, int64_t stride, const uint32_t *lens,
+                               int64_t n, int32_t max_len, int32_t encoding, int32_t bs_r1, int32_t bs_r2,
+                               int64_t n_r1, const ma_params *p, ma_recs *merged_out, int32_t keys,
+                               int32_t do_markdup, uint32_t *is_dup, uint32_t *perm);
+/* Device bytes the server holds, its budget (0 = none) and the device's free
+ * HBM now. Any pointer may be NULL. */
+int ma_serve_memory(ma_server *s, int64_t *held_bytes, int64_t *budget, int64_t *free_hbm);
 /* Returns 1 when the ticket is complete (records written to its arrays) and
  * writes its latency (submit to observed completion, ms), 0 while it is queued
  * or running; wait != 0 blocks until complete. */
 int ma_serve_poll(ma_server *s, int64_t ticket, int32_t wait, double *latency_ms);
 /* Progress the server and report finished tickets: writes up to `cap` of them
  * to `tickets` and returns how many (each is reported once, by this call or by
- * ma_serve_poll). wait != 0 blocks until one finishes or nothing is in flight. */
+ * ma_serve_poll). wait != 0 blocks until one finishes or nothing is in flight.
+ * MA_ERR_NOMEM when nothing is in flight and the most urgent queued request
+ * cannot get a launch buffer. */
 int ma_serve_reap(ma_server *s, int64_t *tickets, int32_t cap, int32_t wait);
 int ma_serve_close(ma_server *s);
 

@@ -15,6 +15,9 @@ namespace ma {
 void set_error(const std::string &msg);
 // Current value of a launch knob (ma_tune_set / ma_tune_get).
 int64_t tune_get(int knob);
+// Byte offsets, inside ma_sort_markdup_dev's scratch for n records, of the two
+// u32 words that are nonzero when one of its sorts gave up waiting on a tile.
+void sort_error_words(int64_t n, size_t *dup_word, size_t *coord_word);
 
 #define MA_HIP(call)                                                                   \
     do {                                                                               \

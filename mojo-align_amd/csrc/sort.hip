@@ -370,6 +370,14 @@ OsScratch os_scratch(char *s, const MdLayout &L, size_t plan_off) {
 
 }  // namespace
 
+namespace ma {
+void sort_error_words(int64_t n, size_t *dup_word, size_t *coord_word) {
+    const MdLayout L = md_layout(n < 1 ? 1 : n);
+    *dup_word = L.plan_dup + offsetof(SortPlan, error);
+    *coord_word = L.plan_coord + offsetof(SortPlan, error);
+}
+}  // namespace ma
+
 extern "C" {
 
 int64_t ma_sort_scratch_bytes(int64_t n) {
@@ -456,6 +464,10 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
         MA_LAUNCHED();
     } else {
         fill_u32_kernel<<<g, 256, 0, st>>>(d_is_dup, n, 0u);
+        MA_LAUNCHED();
+        // no duplicate-order sort runs: clear its error word, which
+        // ma_sort_dev_status reads whatever the scratch held before
+        fill_u32_kernel<<<1, 64, 0, st>>>((uint32_t *)(s + L.plan_dup + offsetof(SortPlan, error)), 1, 0u);
         MA_LAUNCHED();
         if (d_dup_perm) {
             iota_u32_kernel<<<g, 256, 0, st>>>(d_dup_perm, n);

@@ -183,6 +183,10 @@ SIGNATURES = {
     "ma_serve_tune": (C.c_int, [P, I64, C.c_double, I32, I32]),
     "ma_serve_submit": (I64, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                               C.POINTER(Recs)]),
+    "ma_serve_submit_sorted": (I64, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
+                                     C.POINTER(Recs), I32, I32, P, P]),
+    "ma_serve_set_targets": (C.c_int, [P, P, I32]),
+    "ma_serve_memory": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64), C.POINTER(I64)]),
     "ma_serve_poll": (C.c_int, [P, I64, I32, C.POINTER(C.c_double)]),
     "ma_serve_reap": (C.c_int, [P, C.POINTER(I64), I32, I32]),
     "ma_serve_close": (C.c_int, [P]),

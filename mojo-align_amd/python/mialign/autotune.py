@@ -116,11 +116,16 @@ def cache_path(device: int = 0) -> Path:
 
 
 def save(result: Dict[str, dict], device: int = 0, serve: Optional[dict] = None) -> Path:
-    """Persist the current knob values (and served-loop choices) for this device."""
+    """Persist the current knob values (and served-loop choices) for this
+    device; results and served-loop choices saved before for the same device
+    and library are kept unless replaced."""
     p = cache_path(device)
     p.parent.mkdir(parents=True, exist_ok=True)
+    old = load(device, apply_knobs=False) or {}
+    results = dict(old.get("results", {}))
+    results.update(result)
     doc = {"device": _device_key(device), "library": lib().ma_version().decode(), "saved": time.time(),
-           "knobs": current(), "serve": serve or {}, "results": result}
+           "knobs": current(), "serve": serve if serve is not None else old.get("serve", {}), "results": results}
     tmp = p.with_suffix(".tmp")
     tmp.write_text(json.dumps(doc, indent=1))
     os.replace(tmp, p)

@@ -1,11 +1,15 @@
 """Served request loop over a resident index (continuous batching).
 
-The C side (csrc/serve.hip) owns `slots` streams and a free-HBM budget;
-`Server.submit` queues a read batch; whenever a slot is free the server
-assembles one launch from waiting work in deadline order (large batches cut
-into pieces, small ones merged) and enqueues its copy-in, map kernel and
-copy-out on that slot's stream, so requests arriving while others run
-overlap with them on the device.
+The C side (csrc/serve.hip) owns `slots` streams; `Server.submit` queues a
+read batch; whenever a slot is free the server assembles one launch from
+waiting work in deadline order (large batches cut into pieces, small ones
+merged) and enqueues its copy-in, map kernel and copy-out on that slot's
+stream, so requests arriving while others run overlap with them on the
+device. Device memory the server holds stays within `hbm_budget` (0: no cap
+of its own) and is only taken when the device's free HBM, read at that
+moment, covers it. A request with `keys` set is one sample's batch: once
+mapped it is sorted and duplicate-marked on its own (`is_dup` and `perm` in
+its outputs), never together with other samples' reads.
 `run_request_loop` drives Poisson-like arrivals through it and reports
 throughput and p50/p99 batch latency; `run_plain` is the baseline of one
 batch per launch, strictly one after another.
@@ -66,6 +70,8 @@ class Request:
     n_r1: Optional[int] = None
     arrival: float = 0.0  # seconds from the start of the run
     out: Optional[Dict[str, np.ndarray]] = None  # preallocated (pinned) merged records
+    keys: int = 0  # 0 map only; 1 sort the sample SE; 2 PE (R1 rows then R2 rows, n_r1 = n / 2)
+    markdup: bool = True
 
     @property
     def n(self) -> int:
@@ -81,7 +87,7 @@ class _Flight:
 
 
 class Server:
-    def __init__(self, index: GpuIndex, slots: int = 2, hbm_budget: int = 8 << 30, max_len: int = 256,
+    def __init__(self, index: GpuIndex, slots: int = 2, hbm_budget: int = 0, max_len: int = 256,
                  launch_reads: Optional[int] = None, size_weight: float = DEFAULT_SIZE_WEIGHT,
                  persistent: bool = False, chunks_per_wave: int = 4):
         h = C.c_void_p()
@@ -95,13 +101,23 @@ class Server:
                                   int(persistent), int(chunks_per_wave)), "ma_serve_tune")
 
     def submit(self, req: Request, params: Optional[Params] = None) -> Optional[int]:
-        """Ticket, or None when the request is not admitted now."""
+        """Ticket, or None when the request is not admitted now (its device
+        memory does not fit: submit it again once others complete). A sorted
+        request's outputs get "is_dup" and "perm" beside the records."""
         params = params or default_params(k=self.index.k)
         out = req.out if req.out is not None else pinned_recs(req.n)
         st = recs_struct(out)
-        t = lib().ma_serve_submit(self.handle, ptr(req.bases), req.bases.shape[1], ptr(req.lens), req.n,
-                                  req.max_len, req.encoding, req.bs_r1, req.bs_r2,
-                                  req.n if req.n_r1 is None else req.n_r1, C.byref(params), C.byref(st))
+        n_r1 = req.n if req.n_r1 is None else req.n_r1
+        head = (self.handle, ptr(req.bases), req.bases.shape[1], ptr(req.lens), req.n, req.max_len, req.encoding,
+                req.bs_r1, req.bs_r2, n_r1, C.byref(params), C.byref(st))
+        if req.keys:
+            for f in ("is_dup", "perm"):
+                if out.get(f) is None or out[f].size < req.n:
+                    out[f] = pinned_empty((req.n,), np.uint32)
+            t = lib().ma_serve_submit_sorted(*head, int(req.keys), int(req.markdup), ptr(out["is_dup"]),
+                                             ptr(out["perm"]))
+        else:
+            t = lib().ma_serve_submit(*head)
         if t == MA_ERR_NOMEM:
             return None
         check(int(t), "ma_serve_submit")
@@ -129,6 +145,17 @@ class Server:
                 out[int(buf[i])] = fl.out
             if n < 256:
                 return out
+
+    def set_targets(self, tid_of_cid: np.ndarray) -> None:
+        """Target of each index contig for sorted requests (-1: none)."""
+        t = np.ascontiguousarray(tid_of_cid, dtype=np.int32)
+        check(lib().ma_serve_set_targets(self.handle, ptr(t), int(t.size)), "ma_serve_set_targets")
+
+    def memory(self) -> Dict[str, int]:
+        """Device bytes held, the budget (0 = none) and the device's free HBM now."""
+        h, b, f = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        check(lib().ma_serve_memory(self.handle, C.byref(h), C.byref(b), C.byref(f)), "ma_serve_memory")
+        return {"held": h.value, "budget": b.value, "free_hbm": f.value}
 
     def close(self) -> None:
         if self.handle:
@@ -179,19 +206,26 @@ def run_request_loop(server: Server, requests: Sequence[Request], params: Option
 
 
 def _drive(server, requests, params, order, active, lat, outs, keep_outputs, t0) -> None:
-    """Submit due requests, collect finished ones, sleep briefly when idle."""
+    """Submit due requests, collect finished ones, sleep briefly when idle. A
+    request not admitted (its device memory does not fit now) is submitted
+    again after the next completion, in arrival order."""
     pi = 0
     done = 0
     while done < len(requests):
         now = time.perf_counter() - t0
+        held_back = False
         while pi < len(order) and requests[order[pi]].arrival <= now:
             i = order[pi]
             t = server.submit(requests[i], params)
             if t is None:
-                raise RuntimeError("request not admitted")
+                if not active:
+                    raise RuntimeError(f"request {i} ({requests[i].n} reads) not admitted with nothing in flight: "
+                                       f"{server.memory()}")
+                held_back = True
+                break
             active[t] = i
             pi += 1
-        finished = server.reap(wait=False)
+        finished = server.reap(wait=held_back)
         tnow = time.perf_counter() - t0
         for t, rec in finished.items():
             i = active.pop(t)
@@ -206,18 +240,20 @@ def _drive(server, requests, params, order, active, lat, outs, keep_outputs, t0)
 
 
 def run_plain(index: GpuIndex, requests: Sequence[Request], params: Optional[Params] = None,
-              hbm_budget: int = 8 << 30, **shape) -> LoopStats:
+              hbm_budget: int = 0, tid: Optional[np.ndarray] = None, **shape) -> LoopStats:
     """Baseline: one batch per launch, strictly one after another (a one-slot
     server, waiting for each batch before the next); latency = that batch's
-    own copy-in + map + copy-out time."""
+    own copy-in + map (+ sort for a sorted request) + copy-out time."""
     srv = Server(index, slots=1, hbm_budget=hbm_budget, launch_reads=1 << 30, **shape)
+    if tid is not None:
+        srv.set_targets(tid)
     lat = []
     gc_was = gc.isenabled()
     gc.disable()
     try:
         big = max(requests, key=lambda r: r.n)  # sizes the launch buffer before timing
         srv.poll(srv.submit(Request(big.bases, big.lens, big.max_len, big.encoding, big.bs_r1, big.bs_r2,
-                                    big.n_r1, out=big.out), params), wait=True)
+                                    big.n_r1, out=big.out, keys=big.keys, markdup=big.markdup), params), wait=True)
         t0 = time.perf_counter()
         for r in requests:
             t = time.perf_counter()
