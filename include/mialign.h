@@ -13,6 +13,7 @@
  *                             linear_gpu_locate.mojo:1609-1703
  *   ma_index_build .......... build_dense_pack, fq2bam-meth/python/mojo_linear_pack.py:208-287
  *   ma_index_export_dense ... the dense-v1 files written by build_dense_pack (:271-273)
+ *   ma_map_forward(_dev) .... the `-device cpu` host loop, linear_mapper.mojo:209-311
  *   ma_map_batch(_dev) ...... per-batch body of map_fastq_dense_gpu_locate,
  *                             linear_gpu_locate.mojo:1836-2720 (pack, FW/RC
  *                             encode+locate+vote_extend passes, FW/RC merge :2101-2113)
@@ -244,6 +245,34 @@ int ma_map_batch_dev_noalloc(ma_index *ix, void *stream, const uint8_t *d_bases,
                              const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t encoding,
                              int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p,
                              ma_recs *d_merged, ma_recs *d_fw, ma_recs *d_rc);
+
+/* The reference's `-device cpu` host loop on the device, for SE reads or one
+ * mate at a time (fq2bam-meth/src/linear_mapper.mojo:209-311 ->
+ * extend_read_with_seeds, linear_extend.mojo:143-211 -> vote_dense_seeds,
+ * linear_index.mojo:530-578): forward strand only, no RC pass. ASCII rows
+ * (stride >= max_len, max_len <= 256), converted as that loop does (for C2T a
+ * C or c becomes T, for G2A a G or g becomes A; bs_r1 for reads [0, n_r1),
+ * bs_r2 after). Every k-mer position i of a read (no byte outside ACGT/acgt,
+ * in the index, at most max_occ postings) votes for (contig, pos - i *
+ * p->stride) with each posting at pos >= i * p->stride; the first key whose
+ * count exceeds every count before it wins; its window of the read's length
+ * (none past the contig's end) is compared byte for byte (upper case; a
+ * non-ACGT reference base reads as N); over p->max_diff mismatches the
+ * leading and trailing mismatch runs (p->max_soft each at most) are clipped
+ * and the rest recounted. Mapped when at most p->max_diff mismatches remain
+ * over at least 32 aligned bases: pos = start + left clip, flag 0, MAPQ 60 /
+ * 40 / 20 for >= 5 / >= 3 / fewer votes, nm the count; iop / iat / ilen 0.
+ * Other p fields are not used; p->k must be the index's. Host pointers. */
+int ma_map_forward(ma_index *ix, const uint8_t *bases, int64_t stride, const uint32_t *lens, int64_t n,
+                   int32_t max_len, int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p,
+                   ma_recs *out);
+/* ma_map_forward on device pointers. Two launches on stream with a host wait
+ * between them: the first finds the batch's largest count of voting postings
+ * of one read, which sizes the vote tables (one per wave, at most 4 GiB
+ * together, grown in ws and kept). Not for graph capture. */
+int ma_map_forward_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t *d_bases, int64_t stride,
+                       const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t bs_r1, int32_t bs_r2,
+                       int64_t n_r1, const ma_params *p, ma_recs *d_out);
 
 /* Mate rescue of a paired batch, in place on its merged records (device
  * pointers; R1 block [0, n_pairs) with bs_r1, R2 block [n_pairs, 2 n_pairs)

@@ -40,6 +40,10 @@ struct ma_workspace {
     int32_t chunks_per_wave = 0;  // work chunks per resident wave (sets the claim size); 0 = the tuned knob
     ma::DevBuf counter;
     ma::DevBuf split;  // read-length class sub-batches (MA_TUNE_MAP_CLASS_SPLIT), grow-only
+    // forward host loop: per-wave vote tables (grow-only) and their tag clock
+    ma::DevBuf fwd_tab, fwd_aux;
+    int64_t fwd_entries = 0;
+    uint64_t fwd_tag = 0;
 };
 
 namespace ma {

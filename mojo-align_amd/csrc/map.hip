@@ -92,6 +92,25 @@ struct ReadsIn {
     const uint32_t *out_idx;
 };
 
+// Arguments of the forward host loop's kernel (fwd_map_kernel).
+struct FwdArgs {
+    const uint8_t *bases;  // ASCII rows
+    int64_t stride;
+    const uint32_t *lens;
+    int64_t n, n_r1;
+    int32_t max_len, bs_r1, bs_r2;
+    int32_t k, seed_stride, max_occ, max_diff, max_soft;
+    int32_t count_only;               // 1: only the largest count of voting postings of a read
+    unsigned long long *max_total;    // ... into here
+    uint32_t *tags;                   // per-wave tables of 1 << cap_log2 entries
+    uint32_t *cnts;
+    uint64_t *keys;
+    int32_t cap_log2;
+    uint32_t tag_base;                // read r's entries carry tag_base + r + 1
+    unsigned int *counter;            // work claims
+    int32_t claim;
+};
+
 #define DEV __device__ __forceinline__
 
 DEV Rec unmapped_rec() { return Rec{-1, 0u, 0u, 4, 0u, 0u, 0u, 0u}; }
@@ -343,7 +362,9 @@ constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 3;
 constexpr bool COFF_IN_LDS = true;
 constexpr bool INDEL = false;
+#define MA_WITH_FWD  // the forward host loop's kernel too (reads up to 256 bases)
 #include "map_body.inc"
+#undef MA_WITH_FWD
 }  // namespace
 }  // namespace lds
 }  // namespace w8
@@ -356,7 +377,9 @@ constexpr int MAX_READ = MAXW * 32;
 constexpr int WAVES_PER_EU = 3;
 constexpr bool COFF_IN_LDS = false;
 constexpr bool INDEL = false;
+#define MA_WITH_FWD  // the forward host loop's kernel too (reads up to 256 bases)
 #include "map_body.inc"
+#undef MA_WITH_FWD
 }  // namespace
 }  // namespace glb
 }  // namespace w8
@@ -729,6 +752,23 @@ int map_launch(ma_index *ix, hipStream_t st, const uint8_t *d_bases, int64_t str
     return MA_OK;
 }
 
+// ---- the forward host loop (`-device cpu`)
+
+// Pass 1 sizes the vote tables to the batch's largest count of voting
+// postings of one read; pass 2 maps with one table per wave, the tables of all
+// waves within this many bytes (fewer waves for batches holding repeat reads;
+// one block at least).
+constexpr int64_t FWD_TABLE_BUDGET = 4ll << 30;
+constexpr int64_t FWD_ENTRY_BYTES = 16;  // tag u32 + count u32 + key u64
+
+int fwd_launch(bool lds, unsigned g, hipStream_t st, const IndexView &v, const FwdArgs &a,
+               const RecOut &out) {
+    if (lds) w8::lds::fwd_map_kernel<<<g, BLOCK, 0, st>>>(v, a, out);
+    else w8::glb::fwd_map_kernel<<<g, BLOCK, 0, st>>>(v, a, out);
+    MA_LAUNCHED();
+    return MA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -853,6 +893,123 @@ int ma_map_batch(ma_index *ix, const uint8_t *bases, int64_t stride, const uint3
                                   hipMemcpyDeviceToHost, st));
         }
     }
+    MA_HIP(hipStreamSynchronize(st));
+    return MA_OK;
+}
+
+int ma_map_forward_dev(ma_index *ix, ma_workspace *ws, void *stream, const uint8_t *d_bases, int64_t stride,
+                       const uint32_t *d_lens, int64_t n, int32_t max_len, int32_t bs_r1, int32_t bs_r2,
+                       int64_t n_r1, const ma_params *p, ma_recs *d_out) {
+    MA_REQUIRE(ix && ws && p && d_out, "ma_map_forward_dev: null index/workspace/params/output");
+    MA_REQUIRE(n >= 0 && n < (1ll << 31), "ma_map_forward_dev: n out of range");
+    MA_REQUIRE(max_len >= 1 && max_len <= 256 && stride >= max_len,
+               "ma_map_forward_dev: max_len must be 1..256 and stride >= max_len");
+    MA_REQUIRE(p->k == ix->k, "ma_map_forward_dev: k differs from the index's");
+    MA_REQUIRE(p->stride >= 1 && p->max_occ >= 1 && p->max_diff >= 0 && p->max_soft >= 0,
+               "ma_map_forward_dev: bad knobs");
+    if (n == 0) return MA_OK;
+    MA_REQUIRE(d_bases && d_lens, "ma_map_forward_dev: null input");
+    MA_HIP(hipSetDevice(ix->device));
+    hipStream_t st = (hipStream_t)stream;
+    MA_TRY(ws->fwd_aux.reserve(256));
+    unsigned int *ctr = ws->fwd_aux.as<unsigned int>();
+    unsigned long long *mx = (unsigned long long *)((char *)ws->fwd_aux.p + 8);
+    const IndexView v = index_view(ix);
+    const RecOut out = rec_out(d_out);
+    const bool lds = ix->n_contigs < COFF_LDS;
+    FwdArgs a{};
+    a.bases = d_bases;
+    a.stride = stride;
+    a.lens = d_lens;
+    a.n = n;
+    a.n_r1 = n_r1;
+    a.max_len = max_len;
+    a.bs_r1 = bs_r1;
+    a.bs_r2 = bs_r2;
+    a.k = p->k;
+    a.seed_stride = p->stride;
+    a.max_occ = p->max_occ;
+    a.max_diff = p->max_diff;
+    a.max_soft = p->max_soft;
+    a.counter = ctr;
+    a.claim = 16;
+    a.max_total = mx;
+    // pass 1: the largest count of voting postings of one read
+    a.count_only = 1;
+    MA_HIP(hipMemsetAsync(ws->fwd_aux.p, 0, 16, st));
+    MA_TRY(fwd_launch(lds, (unsigned)(ix->n_cu * 4), st, v, a, out));
+    unsigned long long max_total = 0;
+    MA_HIP(hipMemcpyAsync(&max_total, mx, 8, hipMemcpyDeviceToHost, st));
+    MA_HIP(hipStreamSynchronize(st));
+    int cap_log2 = 6;
+    while ((1ull << cap_log2) < 2 * max_total) cap_log2++;
+    const int64_t per = 1ll << cap_log2;
+    int64_t waves = FWD_TABLE_BUDGET / (per * FWD_ENTRY_BYTES) / WPB * WPB;
+    const int64_t most = (int64_t)ix->n_cu * RESIDENT_WAVES_PER_CU;
+    if (waves > most) waves = most;
+    if (waves < WPB) waves = WPB;
+    // tables: tags, counts and keys as three arrays of the buffer's entry
+    // capacity; a wave's table is its slice. Tags only grow between clears
+    // (tag 0 = never written), so an entry left by an earlier read or launch
+    // never carries the tag of the read voting now.
+    const int64_t need = waves * per;
+    if (ws->fwd_entries < need) {
+        MA_TRY(ws->fwd_tab.alloc((size_t)(need * FWD_ENTRY_BYTES)));
+        ws->fwd_entries = need;
+        ws->fwd_tag = 0;
+        MA_HIP(hipMemsetAsync(ws->fwd_tab.p, 0, (size_t)need * 4, st));
+    } else if (ws->fwd_tag > 0xFFFFFFFFull - (uint64_t)n - 1) {
+        ws->fwd_tag = 0;
+        MA_HIP(hipMemsetAsync(ws->fwd_tab.p, 0, (size_t)ws->fwd_entries * 4, st));
+    }
+    char *tb = (char *)ws->fwd_tab.p;
+    a.tags = (uint32_t *)tb;
+    a.cnts = (uint32_t *)(tb + ws->fwd_entries * 4);
+    a.keys = (uint64_t *)(tb + ws->fwd_entries * 8);
+    a.cap_log2 = cap_log2;
+    a.tag_base = (uint32_t)ws->fwd_tag;
+    a.count_only = 0;
+    MA_HIP(hipMemsetAsync(ws->fwd_aux.p, 0, 4, st));
+    MA_TRY(fwd_launch(lds, (unsigned)(waves / WPB), st, v, a, out));
+    ws->fwd_tag += (uint64_t)n + 1;
+    return MA_OK;
+}
+
+int ma_map_forward(ma_index *ix, const uint8_t *bases, int64_t stride, const uint32_t *lens, int64_t n,
+                   int32_t max_len, int32_t bs_r1, int32_t bs_r2, int64_t n_r1, const ma_params *p, ma_recs *out) {
+    MA_REQUIRE(ix && p && out, "ma_map_forward: null index/params/output");
+    MA_REQUIRE(n >= 0, "ma_map_forward: n < 0");
+    if (n == 0) return MA_OK;
+    MA_REQUIRE(bases && lens, "ma_map_forward: null input");
+    for (int64_t i = 0; i < n; i++)
+        MA_REQUIRE((int64_t)lens[i] <= max_len, "ma_map_forward: a read is longer than max_len");
+    MA_HIP(hipSetDevice(ix->device));
+    hipStream_t st = ix->stream;
+    DevBuf d_in, d_lens, d_out;
+    MA_TRY(d_in.alloc((size_t)(n * stride)));
+    MA_TRY(d_lens.alloc((size_t)n * 4));
+    MA_TRY(d_out.alloc((size_t)n * 4 * 10));
+    char *b = (char *)d_out.p;
+    ma_recs dr;
+    dr.cid = (int32_t *)(b + 0 * n * 4);
+    dr.pos = (uint32_t *)(b + 1 * n * 4);
+    dr.mapq = (uint32_t *)(b + 2 * n * 4);
+    dr.flag = (int32_t *)(b + 3 * n * 4);
+    dr.sl = (uint32_t *)(b + 4 * n * 4);
+    dr.sr = (uint32_t *)(b + 5 * n * 4);
+    dr.nm = (uint32_t *)(b + 6 * n * 4);
+    dr.iop = (uint32_t *)(b + 7 * n * 4);
+    dr.iat = (uint32_t *)(b + 8 * n * 4);
+    dr.ilen = (uint32_t *)(b + 9 * n * 4);
+    MA_HIP(hipMemcpyAsync(d_in.p, bases, (size_t)(n * stride), hipMemcpyHostToDevice, st));
+    MA_HIP(hipMemcpyAsync(d_lens.p, lens, (size_t)n * 4, hipMemcpyHostToDevice, st));
+    ma_workspace ws;
+    ws.ix = ix;
+    MA_TRY(ma_map_forward_dev(ix, &ws, st, d_in.as<uint8_t>(), stride, d_lens.as<uint32_t>(), n, max_len, bs_r1,
+                              bs_r2, n_r1, p, &dr));
+    void *dst[10] = {out->cid, out->pos, out->mapq, out->flag, out->sl, out->sr, out->nm, out->iop, out->iat, out->ilen};
+    for (int f = 0; f < 10; f++)
+        if (dst[f]) MA_HIP(hipMemcpyAsync(dst[f], b + (size_t)f * n * 4, (size_t)n * 4, hipMemcpyDeviceToHost, st));
     MA_HIP(hipStreamSynchronize(st));
     return MA_OK;
 }

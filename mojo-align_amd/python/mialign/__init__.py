@@ -17,6 +17,7 @@ from .core import (  # noqa: F401
     device_info,
     map_batch,
     map_batch_dev,
+    map_forward,
     mate_rescue,
     mate_rescue_dev,
     seed_kmers,

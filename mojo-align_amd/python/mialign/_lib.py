@@ -139,6 +139,9 @@ SIGNATURES = {
     "ma_workspace_set_persistent": (C.c_int, [P, I32]),
     "ma_workspace_tune": (C.c_int, [P, I32, I32]),
     "ma_workspace_bytes": (I64, [I64, I32]),
+    "ma_map_forward": (C.c_int, [P, P, I64, P, I64, I32, I32, I32, I64, C.POINTER(Params), C.POINTER(Recs)]),
+    "ma_map_forward_dev": (C.c_int, [P, P, P, P, I64, P, I64, I32, I32, I32, I64, C.POINTER(Params),
+                                     C.POINTER(Recs)]),
     "ma_map_batch": (C.c_int, [P, P, I64, P, I64, I32, I32, I32, I32, I64, C.POINTER(Params),
                                C.POINTER(Recs), C.POINTER(Recs), C.POINTER(Recs)]),
     "ma_map_batch_dev": (C.c_int, [P, P, P, P, I64, P, I64, I32, I32, I32, I32, I64,

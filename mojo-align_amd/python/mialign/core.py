@@ -315,6 +315,23 @@ def map_batch(
     return merged
 
 
+def map_forward(index: GpuIndex, bases: np.ndarray, lens: np.ndarray, max_len: int, *, bs_r1: int = 0,
+                bs_r2: int = 0, n_r1: Optional[int] = None, params: Optional[Params] = None) -> dict:
+    """The reference's `-device cpu` host loop (forward strand only) on the
+    device, over ASCII rows [n, stride]: records as ma_map_forward."""
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n = lens.size
+    if bases.ndim != 2 or bases.shape[0] != n:
+        raise ValueError("bases must be [n_reads, stride]")
+    params = params or default_params(k=index.k)
+    out = host_recs(n)
+    st = recs_struct(out)
+    check(lib().ma_map_forward(index.handle, ptr(bases), bases.shape[1], ptr(lens), n, int(max_len), bs_r1, bs_r2,
+                               n if n_r1 is None else int(n_r1), C.byref(params), C.byref(st)), "ma_map_forward")
+    return out
+
+
 class DeviceRecs:
     """Merged-record arrays resident on the device (one allocation)."""
 

@@ -6,9 +6,11 @@ Mirrors fq2bam-meth/src/linear_mapper.mojo:129-354. Same argv
 dense-v1 cache directory. Exit code 0 on success; any failure exits non-zero
 (the orchestrator's signal to fall back, fq2bam_meth.py:808-838).
 
-Differences, by design: only the hash-seeded parity science runs (on the
-GPU); `-device cpu` (the reference's forward-only host loop) and the fm /
-speed engines are not part of this build and fail with a message.
+`-device cpu` runs the reference's forward-only host loop (its science, SAM
+text and pairing; linear_mapper.mojo:209-311) with the per-read vote and
+verify on the GPU (linear_host_loop.py); every other device value runs the
+hash-seeded parity science on the GPU. The fm / speed engines are not part of
+this build and fail with a message.
 """
 
 from __future__ import annotations
@@ -39,10 +41,8 @@ def map_fastq_to_sam(ref_fasta: str, fq1: str, out_sam: str, device: str = "auto
     if engine in ("fm", "bwa-mem", "bwamem", "speed", "fast"):
         raise MialignError(f"engine {engine!r} is not part of this build (parity only)")
     dev = device.lower()
-    if dev == "cpu":
-        raise MialignError("-device cpu: this build maps on the GPU only")
-    if dev in ("auto", "amd", "hip", "rocm") and device_count() < 1 and not build_cache_only:
-        raise MialignError("no GPU visible")
+    if device_count() < 1 and not build_cache_only:
+        raise MialignError("no GPU visible (every device choice maps on the GPU here)")
     if not cache_dir:
         cache_dir = str(Path(ref_fasta).with_name(Path(ref_fasta).name + f".mojo_linear_k{k}"))
     ensure_dense_pack(c2t_fasta=Path(ref_fasta), cache_dir=Path(cache_dir), k=k)
@@ -55,6 +55,10 @@ def map_fastq_to_sam(ref_fasta: str, fq1: str, out_sam: str, device: str = "auto
     try:
         print(f"mialign index contigs={index.contig_count()} bases={index.total_bases()} "
               f"postings={index.n_postings} device={device}", flush=True)
+        if dev == "cpu":
+            from .linear_host_loop import map_fastq_host_loop
+
+            return map_fastq_host_loop(index, fq1, out_sam, fq2, bs_r1, bs_r2)
         if out_sam.lower().endswith(".bam"):
             # native BAM like the reference's fm driver (_path_is_bam, linear_gpu_fm.mojo:84-88):
             # coordinate-sorted, duplicates marked, every stage on the GPU; pairs as R1, R2

@@ -43,6 +43,11 @@
  *        records, empty lines)
  *   or_recount_nm (a checker, no reference counterpart) ... an independent Python
  *                                                    recount + planted one-base shifts
+ *   or_mate_rescue .................................... hand-derived pairs (test_rescue.py)
+ *   or_map_forward (the `-device cpu` host loop) ....... hand-derived reads (unique,
+ *        mismatches, broken first k-mer, clipping, short reads, contig end, N, case
+ *        and conversion) + the vote written as the earliest max-count-th occurrence
+ *        on repeat reads over all MAPQ bands (test_forward.py)
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -1666,4 +1671,162 @@ int or_num_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------ A13: the host (-device cpu) loop */
+
+/* The reference's `-device cpu` mapping of SE reads or of one mate at a time
+ * against a dense pack (fq2bam-meth/src/linear_mapper.mojo:209-311, forward
+ * strand only, no RC pass):
+ *  - the read is converted first (_bs_convert, linear_mapper.mojo:44-61: for
+ *    C2T a C or c becomes T, for G2A a G or g becomes A, other bytes are kept);
+ *  - every k-mer position i of it is a seed (extract_kmers,
+ *    gpu-common/src/gpu_kmer.mojo:6-15), voted as if it sat at query offset
+ *    i * seed_stride (vote_dense_seeds, linear_index.mojo:530-578): a k-mer
+ *    with a byte outside ACGT/acgt (_encode_kmer_u64, :59-79), absent from the
+ *    pack or with more than max_occ postings casts nothing; each posting with
+ *    pos >= q_off votes for key (cid, pos - q_off); the winner is the first key
+ *    whose count exceeds every count reached before it;
+ *  - the winner's window (extend_read_with_seeds, linear_extend.mojo:143-211)
+ *    of the read's length from the pack's bytes (none past the contig's end,
+ *    contig_window_id, linear_index.mojo:383-394) is compared byte by byte
+ *    after upper-casing both; over max_diff mismatches the leading and the
+ *    trailing mismatch runs (max_soft each at most) are clipped and the rest
+ *    recounted; the hit stands when at most max_diff mismatches remain over at
+ *    least 32 aligned bases: flag 0, pos = start + left clip, MAPQ 60 / 40 / 20
+ *    for >= 5 / >= 3 / fewer votes.
+ * ref_bytes are the pack's sequence bytes (sequences.bin), in the contig_off
+ * coordinates of ix. Rows of `stride` bytes; read r uses bs_r1 when r < n_r1,
+ * else bs_r2. Pinned by hand-derived reads and by a vote written as the
+ * earliest max-count-th occurrence (tests/test_forward.py). */
+static inline uint8_t fw_upper(uint8_t c) { return (c >= 'a' && c <= 'z') ? (uint8_t)(c - 32) : c; }
+
+static int64_t fw_mismatches(const uint8_t *w, const uint8_t *q, int64_t from, int64_t to) {
+    int64_t nm = 0;
+    for (int64_t j = from; j < to; j++) nm += fw_upper(w[j]) != q[j];
+    return nm;
+}
+
+int or_map_forward(const or_index *ix, const uint8_t *ref_bytes, const uint8_t *reads, int64_t stride,
+                   const uint32_t *lens, int64_t n, int64_t n_r1, int bs_r1, int bs_r2, int k, int seed_stride,
+                   int64_t max_occ, int max_diff, int max_soft, or_recs *out, int nthreads) {
+    if (k < 1 || k > 31 || seed_stride < 1) return -1;
+    int status = 0;
+#pragma omp parallel num_threads(nthreads)
+    {
+        uint8_t q[4096];
+        int64_t tab_cap = 0;
+        uint64_t *tkey = NULL;
+        int32_t *tcnt = NULL;
+#pragma omp for schedule(dynamic, 64)
+        for (int64_t r = 0; r < n; r++) {
+            rec_reset(out, r);
+            int64_t L = (int64_t)lens[r];
+            if (L > stride || L > 4096) L = 0;
+            const int mode = r < n_r1 ? bs_r1 : bs_r2;
+            const uint8_t *src = reads + r * stride;
+            for (int64_t j = 0; j < L; j++) {
+                uint8_t u = fw_upper(src[j]);
+                if (mode == 1 && u == 'C') u = 'T';
+                if (mode == 2 && u == 'G') u = 'A';
+                q[j] = u;  /* converted, upper-cased: what the compare sees */
+            }
+            if (L < k) continue;
+            /* postings that can vote: sizes the per-read count table */
+            int64_t total = 0;
+            for (int64_t i = 0; i + k <= L; i++) {
+                uint64_t key = 0;
+                int ok = 1;
+                for (int j = 0; j < k && ok; j++) {
+                    int c = q[i + j] == 'A' ? 0 : q[i + j] == 'C' ? 1 : q[i + j] == 'G' ? 2 : q[i + j] == 'T' ? 3 : -1;
+                    if (c < 0) ok = 0;
+                    else key = (key << 2) | (uint64_t)c;
+                }
+                if (!ok) continue;
+                uint64_t st, en;
+                locate_one(key, ix->kmers, ix->offsets, ix->n_keys, max_occ, &st, &en);
+                total += (int64_t)(en - st);
+            }
+            if (total == 0) continue;
+            int64_t cap = 64;
+            while (cap < 2 * total) cap <<= 1;
+            if (cap > tab_cap) {
+                free(tkey);
+                free(tcnt);
+                tkey = (uint64_t *)malloc((size_t)cap * 8);
+                tcnt = (int32_t *)malloc((size_t)cap * 4);
+                tab_cap = cap;
+                if (!tkey || !tcnt) {
+#pragma omp atomic write
+                    status = -2;
+                    tab_cap = 0;
+                    continue;
+                }
+            }
+            memset(tcnt, 0, (size_t)cap * 4);  /* count 0 = empty slot */
+            int64_t best_n = 0, best_cid = -1, best_start = 0;
+            for (int64_t i = 0; i + k <= L; i++) {
+                uint64_t key = 0;
+                int ok = 1;
+                for (int j = 0; j < k && ok; j++) {
+                    int c = q[i + j] == 'A' ? 0 : q[i + j] == 'C' ? 1 : q[i + j] == 'G' ? 2 : q[i + j] == 'T' ? 3 : -1;
+                    if (c < 0) ok = 0;
+                    else key = (key << 2) | (uint64_t)c;
+                }
+                if (!ok) continue;
+                uint64_t st, en;
+                locate_one(key, ix->kmers, ix->offsets, ix->n_keys, max_occ, &st, &en);
+                const int64_t q_off = i * seed_stride;
+                for (uint64_t p = st; p < en; p++) {
+                    const int64_t cid = (int64_t)ix->postings[2 * p];
+                    const int64_t pos = (int64_t)ix->postings[2 * p + 1];
+                    if (pos < q_off) continue;
+                    const uint64_t vk = ((uint64_t)cid << 32) | (uint64_t)(pos - q_off);
+                    uint64_t h = (vk * 0x9E3779B97F4A7C15ull) >> 20;
+                    while (1) {
+                        h &= (uint64_t)(cap - 1);
+                        if (tcnt[h] == 0) {
+                            tkey[h] = vk;
+                            tcnt[h] = 1;
+                            break;
+                        }
+                        if (tkey[h] == vk) {
+                            tcnt[h]++;
+                            break;
+                        }
+                        h++;
+                    }
+                    if (tcnt[h] > best_n) {
+                        best_n = tcnt[h];
+                        best_cid = cid;
+                        best_start = pos - q_off;
+                    }
+                }
+            }
+            if (best_cid < 0 || best_n <= 0) continue;
+            const int64_t o0 = (int64_t)ix->contig_off[best_cid];
+            const int64_t clen = (int64_t)ix->contig_off[best_cid + 1] - o0;
+            if (best_start + L > clen) continue;  /* contig_window_id: no window */
+            const uint8_t *w = ref_bytes + o0 + best_start;
+            int64_t nm = fw_mismatches(w, q, 0, L), sl = 0, sr = 0;
+            if (nm > max_diff) {
+                while (sl < max_soft && sl < L && fw_upper(w[sl]) != q[sl]) sl++;
+                while (sr < max_soft && sr + sl < L && fw_upper(w[L - 1 - sr]) != q[L - 1 - sr]) sr++;
+                nm = fw_mismatches(w, q, sl, L - sr);
+            }
+            const int64_t alen = L - sl - sr;
+            if (nm <= max_diff && alen >= 32) {
+                out->cid[r] = (int32_t)best_cid;
+                out->pos[r] = (uint32_t)(best_start + sl);
+                out->mapq[r] = best_n >= 5 ? 60u : best_n >= 3 ? 40u : 20u;
+                out->flag[r] = 0;
+                out->sl[r] = (uint32_t)sl;
+                out->sr[r] = (uint32_t)sr;
+                out->nm[r] = (uint32_t)nm;
+            }
+        }
+        free(tkey);
+        free(tcnt);
+    }
+    return status;
 }

@@ -124,6 +124,9 @@ def lib():
         L.or_bam_pack_pe.argtypes = [I64] + [P] * 8 + [I64, P, P, P, I64, P, P, P, P, I64, P, P,
                                                        C.c_int, P, I64]
         L.or_recount_nm.argtypes = [P, P, I64, P, I64, P, I64, P, P, P, P, P, P]
+        L.or_map_forward.restype = C.c_int
+        L.or_map_forward.argtypes = [C.POINTER(Index), P, P, I64, P, I64, I64, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, I64, C.c_int, C.c_int, C.POINTER(Recs), C.c_int]
         _lib = L
     return _lib
 
@@ -174,6 +177,7 @@ class OracleIndex:
         self.offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
         self.postings = np.ascontiguousarray(postings, dtype=np.uint32)
         seq = np.ascontiguousarray(seq, dtype=np.uint8)
+        self.seq = seq  # the pack's bytes (the host loop compares these)
         self.ref_codes = np.empty_like(seq)
         lib().or_pack_bases(_p(seq), _p(self.ref_codes), seq.size)
         self.contig_off = np.ascontiguousarray(contig_off, dtype=np.uint64)
@@ -236,6 +240,24 @@ def mate_rescue(ix: OracleIndex, bases: np.ndarray, lens: np.ndarray, rec: dict,
     out = {f: np.ascontiguousarray(rec[f]).copy() for f in REC_FIELDS}
     lib().or_mate_rescue(C.byref(ix.struct), _p(codes), _p(lens), max_len, n // 2, window, max_diff,
                          C.byref(recs_struct(out)))
+    return out
+
+
+def map_forward(ix: OracleIndex, bases: np.ndarray, lens: np.ndarray, *, k: int, n_r1: int | None = None,
+                bs_r1: int = 0, bs_r2: int = 0, stride: int = 5, max_occ: int = 16384, max_diff: int = 6,
+                max_soft: int = 8, nthreads: int = 8) -> dict:
+    """The reference's `-device cpu` host loop (or_map_forward) over raw ASCII
+    rows [n, row bytes]: converts each read itself (bs_r1 for rows < n_r1,
+    else bs_r2) and returns the records (forward strand only)."""
+    bases = np.ascontiguousarray(bases, dtype=np.uint8)
+    lens = np.ascontiguousarray(lens, dtype=np.uint32)
+    n, row = bases.shape
+    out = new_recs(n)
+    r = lib().or_map_forward(C.byref(ix.struct), _p(ix.seq), _p(bases), row, _p(lens), n,
+                             n if n_r1 is None else n_r1, bs_r1, bs_r2, k, stride, max_occ, max_diff, max_soft,
+                             C.byref(recs_struct(out)), nthreads)
+    if r != 0:
+        raise RuntimeError(f"or_map_forward failed ({r})")
     return out
 
 

@@ -110,3 +110,94 @@ def test_cli_bam_honours_env_knobs(tmp_path, monkeypatch, capfd):
                              is_dup=d, apply_dup=True)
     data = b"".join(pl for _, pl in bd.bgzf_members(out.read_bytes()))
     assert data.startswith(b"BAM\x01") and data.endswith(stream)
+
+
+def test_cli_device_cpu_host_loop(tmp_path):
+    """`-device cpu`: the reference's forward-only host loop (linear_mapper.mojo:
+    209-311) — SAM header with the pack's own contig names, eleven fields per
+    line (no tags), SEQ the converted read, records equal the oracle's
+    or_map_forward; pairs get the pair_hits fields (linear_extend.mojo:263-294):
+    0x1 and 0x40 / 0x80, 0x2 with '=' mates, PNEXT and TLEN |pos2 - pos1| +
+    len(R2) when both map to one contig, else the other mate's contig and
+    position, 0x8 on the mate of an unmapped read. (Forward only: in a
+    directional library R2 is the reverse complement and never maps with its
+    R1, so the pairs here are two forward reads each.)"""
+    print("test_cli_device_cpu_host_loop", flush=True)
+    ref = synth.gen_reference(51, 2, 300_000)
+    dual = synth.dual_convert(ref)
+    fa = tmp_path / "ref.c2t.fa"
+    with fa.open("w") as fh:
+        for i, nm in enumerate(dual.names):
+            a, b = int(dual.contig_off[i]), int(dual.contig_off[i + 1])
+            fh.write(f">{nm}\n{dual.seq[a:b].tobytes().decode()}\n")
+    # R2: reads of their own (either strand, 120 bp), converted C2T like R1, so
+    # that both mates of a pair can map forward: to one contig (a proper pair)
+    # or to two
+    r1 = synth.gen_reads(52, ref, 1500, 150, 150)
+    r2 = synth.gen_reads(53, ref, 1500, 120, 120)
+    for nm, rd in (("R1.fq", r1), ("R2.fq", r2)):
+        with (tmp_path / nm).open("w") as fh:
+            for i in range(rd.n):
+                s = rd.bases[i, : rd.lens[i]].tobytes().decode()
+                fh.write(f"@p{i}/{nm[1]} x\n{s}\n+\n{'F' * len(s)}\n")
+    km, of, po = orc.build_dense(dual.seq, dual.contig_off, 21)
+    oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
+    w1 = orc.map_forward(oix, r1.bases, r1.lens, k=21, bs_r1=1)
+    w2 = orc.map_forward(oix, r2.bases, r2.lens, k=21, bs_r1=1)
+
+    def expect(w, rd, i, mode):
+        s = rd.bases[i, : rd.lens[i]].tobytes().replace(b"C" if mode == 1 else b"G", b"T" if mode == 1 else b"A")
+        cid = int(w["cid"][i])
+        if cid < 0:
+            return [4, "*", 0, 0, "*", s.decode()]
+        sl, sr = int(w["sl"][i]), int(w["sr"][i])
+        cig = (f"{sl}S" if sl else "") + f"{len(s) - sl - sr}M" + (f"{sr}S" if sr else "")
+        return [0, dual.names[cid], int(w["pos"][i]) + 1, int(w["mapq"][i]), cig, s.decode()]
+
+    # single end
+    out = tmp_path / "se.sam"
+    rc = linear_mapper.main(["-ref", str(fa), "-fq1", str(tmp_path / "R1.fq"), "-out_sam", str(out), "-k", "21",
+                             "-device", "cpu", "-cache_dir", str(tmp_path / "pack"), "-bs_r1", "C2T"])
+    assert rc == 0
+    text = out.read_text()
+    sq = [l.split("\t")[1][3:] for l in text.splitlines() if l.startswith("@SQ")]
+    assert sq == list(dual.names)
+    recs = sam_records(out)
+    assert len(recs) == r1.n and all(len(r) == 11 for r in recs)
+    for i, r in enumerate(recs):
+        e = expect(w1, r1, i, 1)
+        assert r[0] == f"p{i}"
+        assert [int(r[1]), r[2], int(r[3]), int(r[4]), r[5], r[9]] == e, f"read {i}"
+        assert r[6:9] == ["*", "0", "0"] and r[10] == "F" * int(r1.lens[i])
+    assert sum(1 for r in recs if r[2] != "*") > 0.3 * r1.n
+    # paired
+    out = tmp_path / "pe.sam"
+    rc = linear_mapper.main(["-ref", str(fa), "-fq1", str(tmp_path / "R1.fq"), "-fq2", str(tmp_path / "R2.fq"),
+                             "-out_sam", str(out), "-k", "21", "-device", "cpu", "-cache_dir", str(tmp_path / "pack"),
+                             "-bs_r1", "C2T", "-bs_r2", "C2T"])
+    assert rc == 0
+    recs = sam_records(out)
+    assert len(recs) == 2 * r1.n
+    n_proper = n_both = 0
+    for i in range(r1.n):
+        a, b = recs[2 * i], recs[2 * i + 1]
+        ea, eb = expect(w1, r1, i, 1), expect(w2, r2, i, 1)
+        fa_, fb_ = ea[0] | 1 | 64, eb[0] | 1 | 128
+        if ea[1] != "*" and eb[1] != "*" and ea[1] == eb[1]:
+            fa_ |= 2
+            fb_ |= 2
+            tlen = abs(eb[2] - ea[2]) + int(r2.lens[i])
+            mates = (["=", str(eb[2]), str(tlen)], ["=", str(ea[2]), str(-tlen)])
+            n_proper += 1
+        else:
+            n_both += ea[1] != "*" and eb[1] != "*"
+            if ea[1] == "*":
+                fb_ |= 8
+            if eb[1] == "*":
+                fa_ |= 8
+            mates = ([eb[1] if eb[1] != "*" else "*", str(eb[2]) if eb[1] != "*" else "0", "0"],
+                     [ea[1] if ea[1] != "*" else "*", str(ea[2]) if ea[1] != "*" else "0", "0"])
+        assert [int(a[1])] + [a[2], int(a[3]), int(a[4]), a[5], a[9]] == [fa_] + ea[1:], f"pair {i} R1"
+        assert [int(b[1])] + [b[2], int(b[3]), int(b[4]), b[5], b[9]] == [fb_] + eb[1:], f"pair {i} R2"
+        assert a[6:9] == mates[0] and b[6:9] == mates[1], f"pair {i} mate fields"
+    assert n_proper > 0.05 * r1.n and n_both > 0.05 * r1.n
