@@ -8,10 +8,12 @@
 //   perm       = stable sort by coord
 // Design: both orders come from the one-sweep radix sort (ma_onesweep.hpp), a
 // fixed chain of kernels with no host round trip. Duplicates are marked in the
-// (dhi, dlo) order: a tile's head flags are scanned into group ids, each
-// record's (score << 32 | ~pair) is gathered once and folded into its group's
-// best with a 64-bit atomicMax (groups are contiguous, so the atomics are
-// too), and a second sweep writes is_dup back in original order.
+// (dhi, dlo) order: a tile's head flags are scanned into group ids; each wave
+// then takes a range of consecutive sorted positions, gathers each record's
+// (score << 32 | ~pair) once, finds each group's best in LDS and writes is_dup
+// (in original order) for every group that lies inside the range; only groups
+// reaching past a range meet in HBM (one 64-bit atomicMax per range) and are
+// marked by a last sweep.
 #include <stddef.h>
 
 #include "ma_onesweep.hpp"
@@ -67,7 +69,6 @@ struct MdView {
     uint64_t *vorig;          // (score << 32 | ~pair) in original order (the final lo buffer,
                               // free once the group ids are made)
     uint64_t *best;           // per group: the largest of those (the other lo buffer)
-    uint64_t *vs;             // per sorted position: its record's value (the free hi buffer)
 };
 
 __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B) {
@@ -79,7 +80,6 @@ __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B)
     m.gid = B.v[f ^ 1u];
     m.vorig = B.lo[f];
     m.best = B.lo[f ^ 1u];
-    m.vs = B.hi[f ^ 1u];
     return m;
 }
 
@@ -137,48 +137,98 @@ __global__ void md_pack_kernel(const SortPlan *__restrict__ plan, MdBufs B, int6
     }
 }
 
-// One sorted position per lane (many gathers in flight): its record's value
-// into the sorted order, folded into its group's best (mapped records only).
-// A wave covers 64 consecutive positions, whose group ids only grow, so each
-// run of one group is first reduced across its lanes and only the run's first
-// lane updates the group's slot: a group of any size costs one atomic per
-// wave it spans.
-__global__ void md_best_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n) {
+// Best + mark, fused. A wave takes ranges of MD_RANGE consecutive sorted
+// positions (lane l: MD_W consecutive ones), gathers each record's value
+// (score << 32 | ~pair) and folds it into its group's slot in LDS: the group
+// ids of a range are consecutive, so group g has slot g - (the range's first
+// id), one of at most MD_RANGE. A group whose positions all lie in the range
+// is then resolved at once: a mapped record is a duplicate unless its pair is
+// the group's best. The range's first and last group, when they reach past
+// the range, fold their part into best[g] in HBM instead (one atomic each)
+// and are marked by md_mark_edges_kernel, so a group of any size costs at
+// most one atomic per range it spans and a singleton none.
+constexpr int MD_W = 8;
+constexpr int MD_RANGE = 64 * MD_W;
+
+__device__ __forceinline__ bool md_first_spills(const MdView &m, int64_t b) {
+    return b > 0 && m.gid[b - 1] == m.gid[b];
+}
+
+__device__ __forceinline__ bool md_last_spills(const MdView &m, int64_t e, int64_t n) {
+    return e < n && m.gid[e] == m.gid[e - 1];
+}
+
+__global__ void __launch_bounds__(256) md_best_mark_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                                           uint32_t *__restrict__ is_dup) {
+    __shared__ unsigned long long s_best[4][MD_RANGE];
     const MdView m = md_view(plan, B);
-    const int lane = threadIdx.x & 63;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + lane;
-        uint64_t v = 0;
-        uint32_t g = 0xFFFFFFFFu;  // lanes past n: a group of their own
-        if (i < n) {
-            const uint32_t o = m.idx[i];
-            if ((int64_t)o < n) {  // o >= n only after a radix pass's timeout (plan->error)
-                const uint64_t x = m.vorig[o];
-                m.vs[i] = x;
-                if (m.hi[i] != ~0ull) v = x;
-            }
-            g = m.gid[i];
-        }
-        // suffix max over the lanes of the same group (contiguous lanes)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    unsigned long long *sb = s_best[w];
+    for (int64_t b = ((int64_t)blockIdx.x * 4 + w) * MD_RANGE; b < n; b += (int64_t)gridDim.x * 4 * MD_RANGE) {
+        const int64_t e = b + MD_RANGE < n ? b + MD_RANGE : n;
+        const uint32_t g0 = m.gid[b];
+        const uint32_t g_first_spill = md_first_spills(m, b) ? g0 : 0xFFFFFFFFu;
+        const uint32_t g_last_spill = md_last_spills(m, e, n) ? m.gid[e - 1] : 0xFFFFFFFFu;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const uint64_t ov = __shfl_down(v, off, 64);
-            const uint32_t og = (uint32_t)__shfl_down((int)g, off, 64);
-            if (lane + off < 64 && og == g && ov > v) v = ov;
+        for (int j = 0; j < MD_W; j++) sb[lane * MD_W + j] = 0ull;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t o[MD_W], g[MD_W];
+        uint64_t x[MD_W];
+        bool mapped[MD_W];
+#pragma unroll
+        for (int j = 0; j < MD_W; j++) {
+            const int64_t i = b + lane * MD_W + j;
+            o[j] = 0xFFFFFFFFu;
+            g[j] = g0;
+            mapped[j] = false;
+            if (i < e) {
+                o[j] = m.idx[i];
+                g[j] = m.gid[i];
+                mapped[j] = m.hi[i] != ~0ull;
+            }
         }
-        const uint32_t gp = (uint32_t)__shfl_up((int)g, 1, 64);
-        if (i < n && v != 0 && (lane == 0 || gp != g)) atomicMax((unsigned long long *)&m.best[g], (unsigned long long)v);
+#pragma unroll
+        for (int j = 0; j < MD_W; j++)  // o >= n only after a radix pass's timeout (plan->error)
+            x[j] = (int64_t)o[j] < n ? m.vorig[o[j]] : 0ull;
+#pragma unroll
+        for (int j = 0; j < MD_W; j++)
+            if (mapped[j] && (int64_t)o[j] < n) atomicMax(&sb[g[j] - g0], (unsigned long long)x[j]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the spilling groups' parts go to HBM (their first lane in the range)
+#pragma unroll
+        for (int j = 0; j < MD_W; j++) {
+            const int64_t i = b + lane * MD_W + j;
+            if (i >= e) continue;
+            const bool spill = g[j] == g_first_spill || g[j] == g_last_spill;
+            const bool first_here = i == b || m.gid[i - 1] != g[j];
+            if (spill && first_here && sb[g[j] - g0] != 0ull)
+                atomicMax((unsigned long long *)&m.best[g[j]], sb[g[j] - g0]);
+            if (!spill && (int64_t)o[j] < n)
+                is_dup[o[j]] = mapped[j] && (uint32_t)x[j] != (uint32_t)sb[g[j] - g0] ? 1u : 0u;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
-// is_dup in original order: a mapped record whose pair is not its group's best pair.
-__global__ void md_mark_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n, uint32_t *__restrict__ is_dup) {
+// The records of the groups that reach past a range: marked against best[g].
+__global__ void md_mark_edges_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                     uint32_t *__restrict__ is_dup) {
     const MdView m = md_view(plan, B);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = i / MD_RANGE * MD_RANGE;
+        const int64_t e = b + MD_RANGE < n ? b + MD_RANGE : n;
+        const uint32_t gi = m.gid[i];
+        const bool spill = (gi == m.gid[b] && md_first_spills(m, b)) || (gi == m.gid[e - 1] && md_last_spills(m, e, n));
+        if (!spill) continue;
         const uint32_t o = m.idx[i];
         if ((int64_t)o >= n) continue;
-        const bool dup = m.hi[i] != ~0ull && (uint32_t)m.vs[i] != (uint32_t)m.best[m.gid[i]];
+        const bool dup = m.hi[i] != ~0ull && (uint32_t)m.vorig[o] != (uint32_t)m.best[gi];
         is_dup[o] = dup ? 1u : 0u;
     }
 }
@@ -458,9 +508,11 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
         const int gw = (int)((n + 255) / 256 < (1 << 20) ? (n + 255) / 256 : (1 << 20));
         md_pack_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_score, d_pair);
         MA_LAUNCHED();
-        md_best_kernel<<<gw, 256, 0, st>>>(S.plan, B, n);
+        const int64_t nr = (n + MD_RANGE - 1) / MD_RANGE;
+        const int gr = (int)((nr + 3) / 4 < (1 << 20) ? (nr + 3) / 4 : (1 << 20));
+        md_best_mark_kernel<<<gr, 256, 0, st>>>(S.plan, B, n, d_is_dup);
         MA_LAUNCHED();
-        md_mark_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_is_dup);
+        md_mark_edges_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_is_dup);
         MA_LAUNCHED();
     } else {
         fill_u32_kernel<<<g, 256, 0, st>>>(d_is_dup, n, 0u);
