@@ -52,7 +52,7 @@ constexpr int RESIDENT_WAVES_PER_CU = 16;  // 4 waves/SIMD x 4 SIMDs at the kern
 
 struct Knobs {
     int32_t k, stride, max_occ, vote_occ, vote_fast, max_diff, max_soft, do_rc, max_indel;
-    int32_t rc_out;  // the caller wants the per-strand RC records too
+    int32_t rc_out, fw_out;  // the caller wants the per-strand RC / FW records too
 };
 
 // ind packs the single indel of a hit: iop (bits 0-1: 0 none, 1 insertion,
@@ -681,10 +681,13 @@ int map_launch(ma_index *ix, hipStream_t st, const uint8_t *d_bases, int64_t str
     RecOut mo = rec_out(d_merged);
     MA_REQUIRE(mo.cid, std::string(who) + ": merged record arrays must be non-null");
     Knobs kn{p->k, p->stride, p->max_occ, p->vote_occ, p->vote_fast, p->max_diff, p->max_soft, p->do_rc,
-             p->max_indel, 0};
-    if (d_rc && (d_rc->cid || d_rc->pos || d_rc->mapq || d_rc->flag || d_rc->sl || d_rc->sr || d_rc->nm ||
-                 d_rc->iop || d_rc->iat || d_rc->ilen))
-        kn.rc_out = 1;
+             p->max_indel, 0, 0};
+    auto wanted = [](const ma_recs *o) {
+        return o && (o->cid || o->pos || o->mapq || o->flag || o->sl || o->sr || o->nm || o->iop || o->iat ||
+                     o->ilen);
+    };
+    kn.rc_out = wanted(d_rc) ? 1 : 0;
+    kn.fw_out = wanted(d_fw) ? 1 : 0;
     ReadsIn in{d_bases, stride, d_lens, n, max_len, encoding, bs_r1, bs_r2, n_r1, nullptr, nullptr, nullptr};
     // Size the launch to the batch: claim = reads per wave chunk (at most the
     // tuned cap) so that the resident waves all get `chunks_per_wave` chunks
