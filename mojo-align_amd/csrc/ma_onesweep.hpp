@@ -16,8 +16,9 @@
 //              counter, ranks its keys stably (wave ballots, per-wave digit
 //              counts), publishes its per-digit counts, looks back over the
 //              preceding tiles' published counts for its global offsets
-//              (decoupled look-back), stages the tile in LDS in digit order and
-//              writes it out in runs; inactive passes return at once;
+//              (decoupled look-back), stages the tile in LDS in digit order
+//              one array at a time (keys, the other key word, values) and
+//              writes each out in runs; inactive passes return at once;
 //   os_finish — the identity order when no pass is active.
 // The first active pass reads the caller's key arrays and makes the values
 // (i) on the fly; the last writes the permutation straight to the caller's
@@ -32,12 +33,18 @@ namespace {
 constexpr int OS_BLOCK = 256;
 constexpr int OS_RADIX = 256;
 constexpr int OS_PASSES = 16;
-constexpr int OS_ITEMS_PLAIN = 16;  // tile of 4096 keys (keys + values staged: 48 KB of LDS)
-constexpr int OS_ITEMS_CARRY = 8;   // tile of 2048 when both key words move (40 KB)
+constexpr int OS_ITEMS_PLAIN = 16;  // tile of 4096 keys (staged one array at a time: 32 KB of LDS)
+constexpr int OS_ITEMS_CARRY = 8;   // tile of 2048 when both key words move (16 KB)
 constexpr int OS_MIN_TILE = OS_BLOCK * OS_ITEMS_CARRY;
 constexpr uint64_t OS_FLAG_AGG = 1ull << 62;   // tile count published
 constexpr uint64_t OS_FLAG_INCL = 2ull << 62;  // inclusive prefix published
 constexpr uint64_t OS_COUNT_MASK = (1ull << 62) - 1;
+// Tiles go to blocks in chunks per XCD: the launch dispatches block b to XCD
+// b % 8, so block group b % OS_GROUPS takes chunks of consecutive tiles from a
+// counter of its own, and one XCD's L2 sees the neighbouring runs of a digit
+// that its tiles write (whole lines written back instead of partial ones).
+constexpr int OS_GROUPS = 8;
+constexpr uint32_t OS_NO_POS = 0xFFFFFFFFu;  // no output position (sorts hold at most 2^32 - 1 keys)
 constexpr uint32_t OS_SPIN_LIMIT = 1u << 22;   // look-back wait bound (~0.1 s): then error, never a hang
 
 struct SortPlan {
@@ -63,14 +70,17 @@ struct OsBufs {
     uint32_t *v[2];
     uint32_t *out_v;  // permutation destination (null: v[fin])
     uint64_t *status; // 2 x tiles x 256 (cleared before the sort)
-    uint32_t *tile_ctr;  // one counter per pass (cleared by os_plan)
+    uint32_t *tile_ctr;  // OS_GROUPS counters per pass (cleared by os_plan)
     int64_t status_stride;  // entries per status array
 };
 
 __device__ __forceinline__ uint32_t os_digit(uint64_t k, int shift) { return (uint32_t)(k >> shift) & 255u; }
 
 // Histograms of the digits of lo (passes 0..7) and hi (8..15); a digit that is
-// the same on every valid lane of a wave is counted with one LDS add.
+// the same on every valid lane of a wave is counted with one LDS add. A wave
+// takes OS_HIST_U chunks of 64 keys per step, their loads issued together.
+constexpr int OS_HIST_U = 4;
+
 __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint64_t *__restrict__ lo,
                                                             const uint64_t *__restrict__ hi,
                                                             unsigned long long *__restrict__ hist,
@@ -81,23 +91,32 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
     const int lane = threadIdx.x & 63;
     const int np = hi ? 16 : 8;
     uint32_t viol = 0;
-    for (int64_t base = (int64_t)blockIdx.x * OS_BLOCK + (threadIdx.x & ~63); base < n;
-         base += (int64_t)gridDim.x * OS_BLOCK) {
-        const int64_t i = base + lane;
-        const bool valid = i < n;
-        const uint64_t vmask = __ballot(valid);
-        const uint64_t a = valid ? lo[i] : 0ull;
-        const uint64_t b = (valid && hi) ? hi[i] : 0ull;
-        if (valid && hi && a != 0ull && !(b == ~0ull && a == (uint64_t)i)) viol = 1;
-        for (int p = 0; p < np; p++) {
-            const uint64_t k = p < 8 ? a : b;
-            const uint32_t d = os_digit(k, 8 * (p & 7));
-            const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane(d);
-            const uint64_t same = __ballot(valid && d == d0);
-            if (same == vmask) {
-                if (lane == 0) atomicAdd(&h[p * OS_RADIX + d0], (uint32_t)__popcll(vmask));
-            } else if (valid) {
-                atomicAdd(&h[p * OS_RADIX + d], 1u);
+    for (int64_t base = ((int64_t)blockIdx.x * OS_BLOCK + (threadIdx.x & ~63)) * OS_HIST_U; base < n;
+         base += (int64_t)gridDim.x * OS_BLOCK * OS_HIST_U) {
+        uint64_t a[OS_HIST_U], b[OS_HIST_U];
+#pragma unroll
+        for (int u = 0; u < OS_HIST_U; u++) {
+            const int64_t i = base + u * 64 + lane;
+            a[u] = i < n ? lo[i] : 0ull;
+            b[u] = (i < n && hi) ? hi[i] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < OS_HIST_U; u++) {
+            const int64_t i = base + u * 64 + lane;
+            const bool valid = i < n;
+            const uint64_t vmask = __ballot(valid);
+            if (vmask == 0ull) continue;  // wave-uniform
+            if (valid && hi && a[u] != 0ull && !(b[u] == ~0ull && a[u] == (uint64_t)i)) viol = 1;
+            for (int p = 0; p < np; p++) {
+                const uint64_t k = p < 8 ? a[u] : b[u];
+                const uint32_t d = os_digit(k, 8 * (p & 7));
+                const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane(d);
+                const uint64_t same = __ballot(valid && d == d0);
+                if (same == vmask) {
+                    if (lane == 0) atomicAdd(&h[p * OS_RADIX + d0], (uint32_t)__popcll(vmask));
+                } else if (valid) {
+                    atomicAdd(&h[p * OS_RADIX + d], 1u);
+                }
             }
         }
     }
@@ -130,7 +149,7 @@ __global__ void __launch_bounds__(OS_BLOCK) os_plan_kernel(SortPlan *__restrict_
         if (t == 0) act[p] = (exists && n > 1 && !full && !(p < 8 && implied)) ? 1u : 0u;
         __syncthreads();
     }
-    if (t < OS_PASSES) tile_ctr[t] = 0u;
+    if (t < OS_PASSES * OS_GROUPS) tile_ctr[t] = 0u;
     if (t == 0) {
         uint32_t carry = 0;
         if (has_hi)
@@ -166,16 +185,16 @@ __global__ void __launch_bounds__(OS_BLOCK) os_plan_kernel(SortPlan *__restrict_
 
 // One pass. CARRY: both key words move (the other word is staged too).
 template <bool CARRY, int ITEMS>
-__global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict__ plan, int p, int64_t n,
+__global__ void __launch_bounds__(OS_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) os_pass_kernel(SortPlan *__restrict__ plan, int p, int64_t n,
                                                             OsBufs B) {
     constexpr int TILE = OS_BLOCK * ITEMS;
     constexpr int WAVE_KEYS = 64 * ITEMS;
-    __shared__ uint64_t s_d[TILE];
-    __shared__ uint64_t s_o[CARRY ? TILE : 1];
-    __shared__ uint32_t s_v[TILE];
+    // The tile in digit order, one array at a time (keys, the other key word,
+    // values) through one buffer: 32 KB at 4096 keys, so four blocks fit a CU.
+    __shared__ uint64_t s_buf[TILE];
     __shared__ uint32_t s_cnt[4][OS_RADIX];  // per-wave digit counts, then per-wave offsets
-    __shared__ uint32_t s_dex[OS_RADIX];     // tile-local exclusive digit offsets
-    __shared__ uint64_t s_gb[OS_RADIX];      // global position of LDS slot 0 of digit d
+    // global position of LDS slot 0 of digit d, mod 2^32 (positions are < n <= 2^32 - 1)
+    __shared__ uint32_t s_gb[OS_RADIX];
     __shared__ uint64_t s_scan[4];
     __shared__ int64_t s_tile;
 
@@ -198,15 +217,26 @@ __global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict_
     const int t = threadIdx.x, w = t >> 6, lane = t & 63;
     const uint64_t lt = lanemask_lt(lane);
     const int64_t ntiles = (n + TILE - 1) / TILE;
+    // block group and its chunk of consecutive tiles (every group has a block:
+    // groups = min(OS_GROUPS, grid)); a group's k-th tile is increasing in k
+    const int groups = (int)gridDim.x < OS_GROUPS ? (int)gridDim.x : OS_GROUPS;
+    const int grp = (int)blockIdx.x % groups;
+    const int64_t chunk = (int64_t)gridDim.x / groups;
+    uint32_t *ctr = B.tile_ctr + p * OS_GROUPS + grp;
 
     while (true) {
-        if (t == 0) s_tile = (int64_t)atomicAdd(&B.tile_ctr[p], 1u);
+        if (t == 0) {
+            const int64_t k = (int64_t)atomicAdd(ctr, 1u);
+            s_tile = (k / chunk) * chunk * groups + (int64_t)grp * chunk + k % chunk;
+        }
         for (int i = t; i < 4 * OS_RADIX; i += OS_BLOCK) (&s_cnt[0][0])[i] = 0u;
         __syncthreads();
         const int64_t tile = s_tile;
         if (tile >= ntiles) break;  // block-uniform
         const int64_t tbase = tile * TILE;
         const int64_t wbase = tbase + (int64_t)w * WAVE_KEYS;
+        // keys of this wave's part of the tile: item r of a lane is valid when r * 64 + lane < nv
+        const int nv = (int)(n - wbase < 0 ? 0 : (n - wbase < WAVE_KEYS ? n - wbase : WAVE_KEYS));
 
         // rank: rounds of 64 consecutive keys per wave, lanes in order
         uint64_t key[ITEMS];
@@ -214,16 +244,13 @@ __global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict_
         uint32_t val[ITEMS], rnk[ITEMS];
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
-            const int64_t i = wbase + (int64_t)r * 64 + lane;
-            const bool valid = i < n;
-            key[r] = valid ? Din[i] : 0ull;
-            if (CARRY) oth[CARRY ? r : 0] = valid ? Oin[i] : 0ull;
-            val[r] = valid ? (first ? (uint32_t)i : Vin[i]) : 0u;
+            const bool valid = r * 64 + lane < nv;
+            key[r] = valid ? Din[wbase + r * 64 + lane] : 0ull;
+            if (CARRY) oth[CARRY ? r : 0] = valid ? Oin[wbase + r * 64 + lane] : 0ull;
         }
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
-            const int64_t i = wbase + (int64_t)r * 64 + lane;
-            const bool valid = i < n;
+            const bool valid = r * 64 + lane < nv;
             const uint32_t d = os_digit(key[r], shift);
             uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -250,22 +277,22 @@ __global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict_
         s_cnt[1][t] = dex + c0;
         s_cnt[2][t] = dex + c0 + c1;
         s_cnt[3][t] = dex + c0 + c1 + c2;
-        s_dex[t] = dex;
-        // publish this tile's count of digit t, look back for the preceding tiles'
+        // publish this tile's count of digit t
         uint64_t *my = st_cur + tile * OS_RADIX + t;
         if (tile == 0) {
             __hip_atomic_store(my, OS_FLAG_INCL | (uint64_t)tcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(my, OS_FLAG_AGG | (uint64_t)tcnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        // look back for the preceding tiles' counts of digit t
         uint64_t prefix = 0;
         if (tile > 0) {
             int64_t q = tile - 1;
             uint32_t spins = 0;
             while (true) {
-                const uint64_t s = __hip_atomic_load(st_cur + q * OS_RADIX + t, __ATOMIC_RELAXED,
+                const uint64_t e = __hip_atomic_load(st_cur + q * OS_RADIX + t, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_AGENT);
-                const uint64_t f = s & ~OS_COUNT_MASK;
+                const uint64_t f = e & ~OS_COUNT_MASK;
                 if (f == 0ull) {
                     if (++spins > OS_SPIN_LIMIT) {
                         atomicOr(&plan->error, 1u);
@@ -274,7 +301,7 @@ __global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict_
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += s & OS_COUNT_MASK;
+                prefix += e & OS_COUNT_MASK;
                 if (f == OS_FLAG_INCL || q == 0) break;
                 q--;
             }
@@ -282,33 +309,64 @@ __global__ void __launch_bounds__(OS_BLOCK) os_pass_kernel(SortPlan *__restrict_
         }
         // the next pass's status entries of this tile start cleared
         st_next[tile * OS_RADIX + t] = 0ull;
-        s_gb[t] = gdo[t] + prefix - (uint64_t)dex;
+        s_gb[t] = (uint32_t)(gdo[t] + prefix - (uint64_t)dex);
         __syncthreads();
-        // stage the tile in digit order
+        // stage the keys in digit order (rnk becomes the LDS slot)
 #pragma unroll
         for (int r = 0; r < ITEMS; r++) {
-            const int64_t i = wbase + (int64_t)r * 64 + lane;
-            if (i < n) {
-                const uint32_t d = os_digit(key[r], shift);
-                const uint32_t pos = s_cnt[w][d] + rnk[r];
-                s_d[pos] = key[r];
-                if (CARRY) s_o[pos] = oth[CARRY ? r : 0];
-                s_v[pos] = val[r];
+            if (r * 64 + lane < nv) {
+                rnk[r] += s_cnt[w][os_digit(key[r], shift)];
+                s_buf[rnk[r]] = key[r];
             }
         }
         __syncthreads();
-        // write out: consecutive LDS slots of a digit go to consecutive positions
-        const int tn = (int)((n - tbase) < TILE ? (n - tbase) : TILE);
-        for (int j = t; j < tn; j += OS_BLOCK) {
-            const uint64_t k = s_d[j];
-            const uint64_t gp = s_gb[os_digit(k, shift)] + (uint64_t)j;
-            if (gp >= (uint64_t)n) continue;  // only after a look-back timeout (plan->error)
-            if (write_keys) {
-                Dout[gp] = k;
-                if (CARRY) Oout[gp] = s_o[CARRY ? j : 0];
-            }
-            Vout[gp] = s_v[j];
+        // the values load while the keys are written (fewer registers held
+        // through the ranking and the look-back)
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const int li = r * 64 + lane;
+            val[r] = li < nv ? (first ? (uint32_t)(wbase + li) : Vin[wbase + li]) : 0u;
         }
+        // write out: consecutive LDS slots of a digit go to consecutive positions;
+        // slot j = t + 256 r of this thread goes to gpos[r] (OS_NO_POS: nothing to
+        // write, which also holds after a look-back timeout, plan->error; n <= 2^32 - 1
+        // keeps positions below it)
+        const int tn = (int)((n - tbase) < TILE ? (n - tbase) : TILE);
+        uint32_t gpos[ITEMS];
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++) {
+            const int j = t + r * OS_BLOCK;
+            gpos[r] = OS_NO_POS;
+            if (j < tn) {
+                const uint64_t k = s_buf[j];
+                const uint32_t gp = s_gb[os_digit(k, shift)] + (uint32_t)j;
+                if ((uint64_t)gp < (uint64_t)n) {
+                    gpos[r] = gp;
+                    if (write_keys) Dout[gp] = k;
+                }
+            }
+            // bound the loads in flight (registers), four slots at a time
+            if ((r & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+        __syncthreads();
+        if (CARRY) {
+#pragma unroll
+            for (int r = 0; r < ITEMS; r++)
+                if (r * 64 + lane < nv) s_buf[rnk[r]] = oth[CARRY ? r : 0];
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < ITEMS; r++)
+                if (write_keys && gpos[r] != OS_NO_POS) Oout[gpos[r]] = s_buf[t + r * OS_BLOCK];
+            __syncthreads();
+        }
+        uint32_t *s_val = reinterpret_cast<uint32_t *>(s_buf);
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++)
+            if (r * 64 + lane < nv) s_val[rnk[r]] = val[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < ITEMS; r++)
+            if (gpos[r] != OS_NO_POS) Vout[gpos[r]] = s_val[t + r * OS_BLOCK];
         __syncthreads();
     }
 }

@@ -8,12 +8,13 @@
 //   perm       = stable sort by coord
 // Design: both orders come from the one-sweep radix sort (ma_onesweep.hpp), a
 // fixed chain of kernels with no host round trip. Duplicates are marked in the
-// (dhi, dlo) order: a tile's head flags are scanned into group ids; each wave
-// then takes a range of consecutive sorted positions, gathers each record's
-// (score << 32 | ~pair) once, finds each group's best in LDS and writes is_dup
-// (in original order) for every group that lies inside the range; only groups
-// reaching past a range meet in HBM (one 64-bit atomicMax per range) and are
-// marked by a last sweep.
+// (dhi, dlo) order: a tile's head flags are scanned into group ids (the same
+// sweep clears is_dup); each wave then takes a range of consecutive sorted
+// positions, gathers (score << 32 | ~pair) only for records of groups of two or
+// more, finds each group's best in LDS and writes is_dup = 1 (in original
+// order) for the duplicates of every group that lies inside the range; only
+// groups reaching past a range meet in HBM (one 64-bit atomicMax per range)
+// and are marked by a last pass over the ranges they touch.
 #include <stddef.h>
 
 #include "ma_onesweep.hpp"
@@ -54,6 +55,10 @@ __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
 // ---- markdup in the (dhi, dlo) order
 constexpr int MD_ITEMS = 16;
 constexpr int MD_TILE = 256 * MD_ITEMS;
+// A wave of md_best_mark_kernel takes MD_RANGE consecutive sorted positions
+// (lane l: MD_W consecutive ones).
+constexpr int MD_W = 8;
+constexpr int MD_RANGE = 64 * MD_W;
 
 // The sorted dup keys and permutation as the dup sort's plan left them.
 struct MdBufs {
@@ -66,9 +71,8 @@ struct MdView {
     const uint64_t *hi, *lo;  // lo null when it did not move (implied by hi, or constant)
     const uint32_t *idx;      // the (dhi, dlo) order: original record of each sorted position
     uint32_t *gid;            // group of each sorted position (the free permutation buffer)
-    uint64_t *vorig;          // (score << 32 | ~pair) in original order (the final lo buffer,
-                              // free once the group ids are made)
-    uint64_t *best;           // per group: the largest of those (the other lo buffer)
+    uint64_t *best;           // per group reaching past a range: the largest (score << 32 | ~pair)
+                              // of its records (the free key buffer)
 };
 
 __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B) {
@@ -78,7 +82,6 @@ __device__ __forceinline__ MdView md_view(const SortPlan *plan, const MdBufs &B)
     m.lo = plan->carry ? B.lo[f] : nullptr;
     m.idx = B.out_v ? B.out_v : B.v[f];
     m.gid = B.v[f ^ 1u];
-    m.vorig = B.lo[f];
     m.best = B.lo[f ^ 1u];
     return m;
 }
@@ -106,9 +109,12 @@ __global__ void __launch_bounds__(256) md_tile_heads_kernel(const SortPlan *__re
 }
 
 // gid[i] = the group of sorted position i (heads scanned within the tile on
-// top of the tile's exclusive head count).
+// top of the tile's exclusive head count). The same sweep clears is_dup (only
+// duplicates are written afterwards) and best[g] of every group at a range's
+// first or last position (the only groups that can reach past a range).
 __global__ void __launch_bounds__(256) md_gid_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
-                                                      const uint64_t *__restrict__ partial) {
+                                                      const uint64_t *__restrict__ partial,
+                                                      uint32_t *__restrict__ is_dup) {
     __shared__ uint64_t sh[4];
     const MdView m = md_view(plan, B);
     const int64_t base = (int64_t)blockIdx.x * MD_TILE;
@@ -121,35 +127,29 @@ __global__ void __launch_bounds__(256) md_gid_kernel(const SortPlan *__restrict_
         const uint32_t h = valid ? md_head(m, i) : 0u;
         uint64_t tot;
         const uint64_t ex = block_excl_scan_256(h, sh, tot);
-        if (valid) m.gid[i] = (uint32_t)(run + ex + h - 1);
+        if (valid) {
+            const uint32_t g = (uint32_t)(run + ex + h - 1);
+            m.gid[i] = g;
+            is_dup[i] = 0u;
+            const int64_t rr = i % MD_RANGE;
+            if (rr == 0 || rr == MD_RANGE - 1) m.best[g] = 0ull;
+        }
         run += tot;
     }
 }
 
-// Per record, in original order: (score << 32 | ~pair), so the best of a
-// group is the largest (highest score, then lowest pair id); best cleared.
-__global__ void md_pack_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
-                               const uint32_t *__restrict__ score, const uint32_t *__restrict__ pair) {
-    const MdView m = md_view(plan, B);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        m.vorig[i] = ((uint64_t)score[i] << 32) | (uint64_t)(0xFFFFFFFFu - pair[i]);
-        m.best[i] = 0ull;
-    }
-}
-
 // Best + mark, fused. A wave takes ranges of MD_RANGE consecutive sorted
-// positions (lane l: MD_W consecutive ones), gathers each record's value
-// (score << 32 | ~pair) and folds it into its group's slot in LDS: the group
-// ids of a range are consecutive, so group g has slot g - (the range's first
-// id), one of at most MD_RANGE. A group whose positions all lie in the range
-// is then resolved at once: a mapped record is a duplicate unless its pair is
-// the group's best. The range's first and last group, when they reach past
-// the range, fold their part into best[g] in HBM instead (one atomic each)
-// and are marked by md_mark_edges_kernel, so a group of any size costs at
-// most one atomic per range it spans and a singleton none.
-constexpr int MD_W = 8;
-constexpr int MD_RANGE = 64 * MD_W;
-
+// positions. A record alone in its group, or unmapped (dhi == ~0), is never a
+// duplicate and costs nothing past the sorted arrays' sweep. A record of a
+// larger group gathers its (score << 32 | ~pair) once and folds it into its
+// group's slot in LDS: the group ids of a range are consecutive, so group g
+// has slot g - (the range's first id), one of at most MD_RANGE. A group whose
+// positions all lie in the range is resolved at once: a record is a duplicate
+// unless its pair is the group's best, and only duplicates are written. The
+// range's first and last group, when they reach past the range, fold their
+// part into best[g] in HBM instead (one atomic each) and are marked by
+// md_mark_edges_kernel, so a group of any size costs at most one atomic per
+// range it spans.
 __device__ __forceinline__ bool md_first_spills(const MdView &m, int64_t b) {
     return b > 0 && m.gid[b - 1] == m.gid[b];
 }
@@ -158,7 +158,11 @@ __device__ __forceinline__ bool md_last_spills(const MdView &m, int64_t e, int64
     return e < n && m.gid[e] == m.gid[e - 1];
 }
 
+constexpr uint32_t MD_NONE = 0xFFFFFFFFu;  // no group (ids are < n <= 2^32 - 1)
+
 __global__ void __launch_bounds__(256) md_best_mark_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                                           const uint32_t *__restrict__ score,
+                                                           const uint32_t *__restrict__ pair,
                                                            uint32_t *__restrict__ is_dup) {
     __shared__ unsigned long long s_best[4][MD_RANGE];
     const MdView m = md_view(plan, B);
@@ -167,21 +171,16 @@ __global__ void __launch_bounds__(256) md_best_mark_kernel(const SortPlan *__res
     for (int64_t b = ((int64_t)blockIdx.x * 4 + w) * MD_RANGE; b < n; b += (int64_t)gridDim.x * 4 * MD_RANGE) {
         const int64_t e = b + MD_RANGE < n ? b + MD_RANGE : n;
         const uint32_t g0 = m.gid[b];
-        const uint32_t g_first_spill = md_first_spills(m, b) ? g0 : 0xFFFFFFFFu;
-        const uint32_t g_last_spill = md_last_spills(m, e, n) ? m.gid[e - 1] : 0xFFFFFFFFu;
-#pragma unroll
-        for (int j = 0; j < MD_W; j++) sb[lane * MD_W + j] = 0ull;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const uint32_t g_first_spill = md_first_spills(m, b) ? g0 : MD_NONE;
+        const uint32_t g_last_spill = md_last_spills(m, e, n) ? m.gid[e - 1] : MD_NONE;
+        const int64_t i0 = b + (int64_t)lane * MD_W;
         uint32_t o[MD_W], g[MD_W];
-        uint64_t x[MD_W];
         bool mapped[MD_W];
 #pragma unroll
         for (int j = 0; j < MD_W; j++) {
-            const int64_t i = b + lane * MD_W + j;
-            o[j] = 0xFFFFFFFFu;
-            g[j] = g0;
+            const int64_t i = i0 + j;
+            o[j] = MD_NONE;
+            g[j] = MD_NONE;
             mapped[j] = false;
             if (i < e) {
                 o[j] = m.idx[i];
@@ -189,26 +188,50 @@ __global__ void __launch_bounds__(256) md_best_mark_kernel(const SortPlan *__res
                 mapped[j] = m.hi[i] != ~0ull;
             }
         }
+        // the neighbours of the lane's first and last position (positions past
+        // e on this lane only occur at the end of the sorted order)
+        const uint32_t g_prev = (i0 < e && i0 > 0) ? m.gid[i0 - 1] : MD_NONE;
+        const uint32_t g_next = (i0 + MD_W - 1 < e && i0 + MD_W < n) ? m.gid[i0 + MD_W] : MD_NONE;
+        bool need[MD_W];
 #pragma unroll
-        for (int j = 0; j < MD_W; j++)  // o >= n only after a radix pass's timeout (plan->error)
-            x[j] = (int64_t)o[j] < n ? m.vorig[o[j]] : 0ull;
+        for (int j = 0; j < MD_W; j++) {
+            const uint32_t gp = j > 0 ? g[j - 1] : g_prev, gn = j < MD_W - 1 ? g[j + 1] : g_next;
+            // o >= n only after a radix pass's timeout (plan->error)
+            need[j] = mapped[j] && (int64_t)o[j] < n && (gp == g[j] || gn == g[j]);
+        }
+        const uint64_t any = __ballot(need[0] || need[1] || need[2] || need[3] || need[4] || need[5] ||
+                                      need[6] || need[7]);
+        if (any == 0ull) continue;  // wave-uniform: no group of two or more in the range
 #pragma unroll
-        for (int j = 0; j < MD_W; j++)
-            if (mapped[j] && (int64_t)o[j] < n) atomicMax(&sb[g[j] - g0], (unsigned long long)x[j]);
+        for (int j = 0; j < MD_W; j++) sb[lane * MD_W + j] = 0ull;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // the spilling groups' parts go to HBM (their first lane in the range)
+        uint32_t xs[MD_W], xp[MD_W];
 #pragma unroll
         for (int j = 0; j < MD_W; j++) {
-            const int64_t i = b + lane * MD_W + j;
-            if (i >= e) continue;
+            xs[j] = need[j] ? score[o[j]] : 0u;
+            xp[j] = need[j] ? 0xFFFFFFFFu - pair[o[j]] : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < MD_W; j++)
+            if (need[j]) atomicMax(&sb[g[j] - g0], ((unsigned long long)xs[j] << 32) | xp[j]);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // the spilling groups' parts go to HBM (their first position in the range)
+#pragma unroll
+        for (int j = 0; j < MD_W; j++) {
+            if (!need[j]) continue;
             const bool spill = g[j] == g_first_spill || g[j] == g_last_spill;
-            const bool first_here = i == b || m.gid[i - 1] != g[j];
-            if (spill && first_here && sb[g[j] - g0] != 0ull)
-                atomicMax((unsigned long long *)&m.best[g[j]], sb[g[j] - g0]);
-            if (!spill && (int64_t)o[j] < n)
-                is_dup[o[j]] = mapped[j] && (uint32_t)x[j] != (uint32_t)sb[g[j] - g0] ? 1u : 0u;
+            const unsigned long long best = sb[g[j] - g0];
+            if (spill) {
+                const int64_t i = i0 + j;
+                const uint32_t gp = j > 0 ? g[j - 1] : g_prev;
+                if (i == b || gp != g[j]) atomicMax((unsigned long long *)&m.best[g[j]], best);
+            } else if (xp[j] != (uint32_t)best) {
+                is_dup[o[j]] = 1u;
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -216,20 +239,28 @@ __global__ void __launch_bounds__(256) md_best_mark_kernel(const SortPlan *__res
     }
 }
 
-// The records of the groups that reach past a range: marked against best[g].
-__global__ void md_mark_edges_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
-                                     uint32_t *__restrict__ is_dup) {
+// The records of the groups that reach past a range, marked against best[g]:
+// a wave per range, only for ranges whose first or last group spills.
+__global__ void __launch_bounds__(256) md_mark_edges_kernel(const SortPlan *__restrict__ plan, MdBufs B, int64_t n,
+                                                            const uint32_t *__restrict__ pair,
+                                                            uint32_t *__restrict__ is_dup) {
     const MdView m = md_view(plan, B);
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = i / MD_RANGE * MD_RANGE;
+    const int lane = threadIdx.x & 63;
+    const int64_t nr = (n + MD_RANGE - 1) / MD_RANGE;
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x >> 6);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < nr; r += nw) {
+        const int64_t b = r * MD_RANGE;
         const int64_t e = b + MD_RANGE < n ? b + MD_RANGE : n;
-        const uint32_t gi = m.gid[i];
-        const bool spill = (gi == m.gid[b] && md_first_spills(m, b)) || (gi == m.gid[e - 1] && md_last_spills(m, e, n));
-        if (!spill) continue;
-        const uint32_t o = m.idx[i];
-        if ((int64_t)o >= n) continue;
-        const bool dup = m.hi[i] != ~0ull && (uint32_t)m.vorig[o] != (uint32_t)m.best[gi];
-        is_dup[o] = dup ? 1u : 0u;
+        const uint32_t fs = md_first_spills(m, b) ? m.gid[b] : MD_NONE;
+        const uint32_t ls = md_last_spills(m, e, n) ? m.gid[e - 1] : MD_NONE;
+        if (fs == MD_NONE && ls == MD_NONE) continue;  // wave-uniform
+        for (int64_t i = b + lane; i < e; i += 64) {
+            const uint32_t gi = m.gid[i];
+            if (gi != fs && gi != ls) continue;
+            const uint32_t o = m.idx[i];
+            if ((int64_t)o >= n || m.hi[i] == ~0ull) continue;
+            if (0xFFFFFFFFu - pair[o] != (uint32_t)m.best[gi]) is_dup[o] = 1u;
+        }
     }
 }
 
@@ -396,7 +427,7 @@ MdLayout md_layout(int64_t n) {
     L.hist = take(OS_PASSES * OS_RADIX * 8);
     L.plan_dup = take(sizeof(SortPlan));
     L.plan_coord = take(sizeof(SortPlan));
-    L.tile_ctr = take(OS_PASSES * 4);
+    L.tile_ctr = take(OS_PASSES * OS_GROUPS * 4);
     L.partial = take((size_t)((n + MD_TILE - 1) / MD_TILE + 1) * 8);
     L.total = off;
     return L;
@@ -503,16 +534,13 @@ int ma_sort_markdup_dev(void *stream, int64_t n, const uint64_t *d_coord, const 
         MA_LAUNCHED();
         scan_top_kernel<<<1, RS_BLOCK, 0, st>>>(partial, nt, nullptr);
         MA_LAUNCHED();
-        md_gid_kernel<<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial);
-        MA_LAUNCHED();
-        const int gw = (int)((n + 255) / 256 < (1 << 20) ? (n + 255) / 256 : (1 << 20));
-        md_pack_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_score, d_pair);
+        md_gid_kernel<<<(unsigned)nt, 256, 0, st>>>(S.plan, B, n, partial, d_is_dup);
         MA_LAUNCHED();
         const int64_t nr = (n + MD_RANGE - 1) / MD_RANGE;
         const int gr = (int)((nr + 3) / 4 < (1 << 20) ? (nr + 3) / 4 : (1 << 20));
-        md_best_mark_kernel<<<gr, 256, 0, st>>>(S.plan, B, n, d_is_dup);
+        md_best_mark_kernel<<<gr, 256, 0, st>>>(S.plan, B, n, d_score, d_pair, d_is_dup);
         MA_LAUNCHED();
-        md_mark_edges_kernel<<<gw, 256, 0, st>>>(S.plan, B, n, d_is_dup);
+        md_mark_edges_kernel<<<gr, 256, 0, st>>>(S.plan, B, n, d_pair, d_is_dup);
         MA_LAUNCHED();
     } else {
         fill_u32_kernel<<<g, 256, 0, st>>>(d_is_dup, n, 0u);
