@@ -950,7 +950,7 @@ class GenomeScale:
 # reference, 10M x 150 bp reads): rocprofv3 --pmc FETCH_SIZE x 2.00 +
 # WRITE_SIZE x 1.00, the factors read off copy_u64_kernel (a copy of known
 # size) in the same runs.
-MAP_TRAFFIC_PROFILED_GS = {"reads": 10_000_000, "read_len": 150, "bytes": 362.3e9, "kernel_ms": 128.1,
+MAP_TRAFFIC_PROFILED_GS = {"reads": 10_000_000, "read_len": 150, "bytes": 281.7e9, "kernel_ms": 90.4,
                            "source": "profiles/map_reads_kernel_3100Mbp_10Mx150.md"}
 
 
