@@ -1199,6 +1199,7 @@ def main() -> int:
     ap.add_argument("--launch", choices=("persistent", "batch"), default="persistent",
                     help="map-kernel grid: persistent (8 blocks/CU + work counter) or sized to the batch")
     ap.add_argument("--no-config2", action="store_true", help="skip the 100 Mbp section and what builds on it")
+    ap.add_argument("--config2-only", action="store_true", help="only the 100 Mbp section and what builds on it")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 50-250 bp launch (and serving on it)")
     ap.add_argument("--gs-genome", type=int, default=3_100_000_000,
                     help="reference size of the headline, uneven and served workloads")
@@ -1240,6 +1241,12 @@ def main() -> int:
     if args.sort_only:
         print(json.dumps({"sort_stress": sort_section(args.sort_records, args.sort_cpu_sample, not args.no_cpu)}),
               flush=True)
+        return 0
+
+    if args.config2_only:
+        a2 = argparse.Namespace(**vars(args))
+        a2.reads = args.c2_reads
+        print(json.dumps({"config2_100mbp": config2_section(a2)}), flush=True)
         return 0
 
     if args.serve_only:
