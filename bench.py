@@ -433,22 +433,21 @@ def pe_section(sr, reads, tid, blob, name_off, qual, with_cpu: bool) -> dict:
     return out
 
 
-def indel_section(sr, ix, dual, reads, with_cpu: bool, parity_reads: int) -> dict:
+def indel_run(sr, steps: int = 3):
     """The reference's default extension (max_indel 4, the single-indel branch
     on) over the same step: reads/s of map + keys + sort + markdup, the map
-    kernel's time, indel hits, and GPU == oracle (all ten record fields) on
-    the first reads."""
+    kernel's time and indel hits; returns (section dict, merged records). The
+    gapless records and order are made again afterwards for what follows."""
     import mialign
-    from oracle import oracle as orc
 
     gapless = sr.prm
     sr.prm = mialign.default_params(k=21, max_indel=4)
     try:
-        value, ms, map_ms = sr.run(3, 1, "indel cap 4")
+        value, ms, map_ms = sr.run(steps, 1, "indel cap 4")
         merged = sr.recs.download()
     finally:
         sr.prm = gapless
-        sr.step(False)  # the gapless records and order again, for the sections after this one
+        sr.step(False)
         sr.stream.sync()
     n = sr.n
     out = {"workload": f"the step's {n} reads with max_indel 4 (the reference default)", "max_indel": 4,
@@ -456,18 +455,31 @@ def indel_section(sr, ix, dual, reads, with_cpu: bool, parity_reads: int) -> dic
            "indel_hits": int((merged["iop"] != 0).sum()), "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4)}
     log(f"indel cap 4: {ms:.1f} ms/step, map {map_ms:.1f} ms, {value / 1e6:.1f} M reads/s, "
         f"{out['indel_hits']} indel hits")
-    if with_cpu:
-        ns = min(parity_reads, n)
-        km, of, po = ix.export_dense()
-        oix = orc.OracleIndex(km, of, po, dual.seq, dual.contig_off)
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        want, _, _, _ = orc.map_batch(oix, orc.convert_reads(reads.bases[:ns], 1), reads.lens[:ns],
-                                      orc.params(max_indel=4), nthreads=threads)
-        ok = all(np.array_equal(merged[f][:ns], want[f]) for f in ALL_FIELDS)
-        out["parity"] = {"reads": ns, "records_bit_exact": bool(ok), "oracle_indel_hits": int((want["iop"] != 0).sum())}
-        log(f"indel cap 4: first {ns} reads equal the oracle's: {ok}")
-    return out
+    return out, merged
 
+
+def indel_parity(out: dict, merged: dict, oix, reads, ns: int) -> None:
+    """GPU == oracle (all ten record fields) at max_indel 4 on the first ns reads."""
+    from oracle import oracle as orc
+
+    ns = min(ns, reads.lens.size)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    want, _, _, _ = orc.map_batch(oix, orc.convert_reads(reads.bases[:ns], 1), reads.lens[:ns],
+                                  orc.params(max_indel=4), nthreads=threads)
+    ok = all(np.array_equal(merged[f][:ns], want[f]) for f in ALL_FIELDS)
+    out["parity"] = {"reads": ns, "records_bit_exact": bool(ok), "oracle_indel_hits": int((want["iop"] != 0).sum())}
+    log(f"indel cap 4: first {ns} reads equal the oracle's: {ok}")
+
+
+def indel_section(sr, ix, dual, reads, with_cpu: bool, parity_reads: int) -> dict:
+    """indel_run on the step, then its parity sample against the oracle."""
+    from oracle import oracle as orc
+
+    out, merged = indel_run(sr)
+    if with_cpu:
+        km, of, po = ix.export_dense()
+        indel_parity(out, merged, orc.OracleIndex(km, of, po, dual.seq, dual.contig_off), reads, parity_reads)
+    return out
 
 
 def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
@@ -1003,8 +1015,11 @@ def headline_section(args, gs: GenomeScale) -> dict:
                        "outputs_equal_default": bool(same)})
         sr.step(False)  # the tuned step's outputs again
     merged = sr.recs.download()
+    indel = indel_mer = None
+    if not args.no_indel and not args.headline_only:
+        indel, indel_mer = indel_run(sr, max(2, args.steps // 4))
     out = {"value": v, "ms_per_step": ms, "map_ms": map_ms, "reads": n, "read_len": L, "tuning": tuning,
-           "graph": graph,
+           "graph": graph, "indel_mode": indel,
            "mapped_frac": round(float((merged["cid"] >= 0).mean()), 4),
            "duplicates_marked": int(sr.outs["is_dup"].download().sum()),
            "cpu_baseline": None, "roofline": None, "parity_sample": None, "sort_parity": None}
@@ -1035,6 +1050,8 @@ def headline_section(args, gs: GenomeScale) -> dict:
                                                               "windows": round(cnt.windows / ns, 2),
                                                               "rc_pass_frac": round(cnt.reads_rc / ns, 3)})
         log(f"headline: sample parity {ok}, {bpr:.0f} B/read algorithmic, x{v / cpu['value']:.0f} the host")
+        if indel is not None:
+            indel_parity(indel, indel_mer, gs.oracle(), reads, args.indel_parity_sample)
     return out
 
 
@@ -1321,6 +1338,7 @@ def main() -> int:
         "sort_parity": head["sort_parity"],
         "tuning": head["tuning"],
         "graph": head["graph"],
+        "indel_mode": head["indel_mode"],
         "uneven_50_250bp": uneven,
         "serving": serving,
         "sort_stress": sort_stress,
