@@ -10,12 +10,13 @@
 //
 // Execution model (gfx950, wave64):
 //   * a wave owns one read; its state (read words, seed order, the 256-entry
-//     vote table, KEEP picks, 48-entry exact tally) lives in LDS (~4.7 KB);
+//     vote table, KEEP picks) lives in LDS (~8.5 KB);
 //   * reads and reference are 2-bit packed (32 bases per u64, MSB-first) with
 //     parallel N-flag words, so one lane scores a whole read against a
 //     reference window with ~2W XOR/popcount ops (W = read words);
 //   * lanes run in parallel over seeds (encode + one 64-byte hash-bucket probe
-//     each), over the postings of one seed (vote / rescue / exact tally) and
+//     each), over the postings of one seed (vote / rescue / the exact tally's
+//     first high-occ seed, which decides the tally in closed form) and
 //     over (pick, shift) extend candidates; ballots and prefix counts keep the
 //     reference's first-seen table order, and first-minimum reductions keep its
 //     sequential best-hit choice (picks are in non-increasing vote order, so
@@ -39,9 +40,8 @@ constexpr int TOP = 256;  // vote table (TOP, linear_gpu_locate.mojo:812)
 constexpr int KEEP = 16;  // extend picks (KEEP, :813)
 constexpr int VHASH_BITS = 9;  // vote hash: 512 slots for at most TOP keys
 constexpr int VHASH = 1 << VHASH_BITS;
-constexpr int HX = 48;    // exact tally (HX, :1249)
-constexpr int HXHASH_BITS = 7;  // tally hash: 128 slots for HX keys
-constexpr int HXHASH = 1 << HXHASH_BITS;
+constexpr int HX = 48;    // exact tally entries (HX, :1249); the tally's closed form needs HX >= 2
+static_assert(HX >= 2, "the closed-form tally tells one entry from several");
 constexpr int MAXS = 64;  // seeds per read (one per lane)
 constexpr int WPB = 4;    // waves per block
 constexpr int BLOCK = WPB * 64;
