@@ -11,11 +11,12 @@
 // words, name / row / flags) to its output position inv[i] (inv = the
 // permutation's inverse); the lengths in output order are scanned into byte
 // offsets (the total lands in rec_off[n]); in bam_pack_kernel each wave takes
-// 64 consecutive layouts into LDS and writes the records in turn, its lanes
-// each producing a 4-byte-aligned word of the output stream (byte stores only
-// where a word is shared with a neighbouring record) from the record's name,
-// bases and qualities staged in LDS (the next record's loads in flight), so the
-// stream is written once, in order, with no host gather and no staging copy.
+// 64 consecutive layouts into LDS and writes the records in turn: a record is
+// assembled in LDS at the output's alignment, section by section (a byte per
+// lane: header, name, CIGAR, bases from 4-bit codes, qualities, tags), then
+// leaves as aligned 4-byte words (byte stores only where a word is shared with
+// a neighbouring record), the next record's name, bases and qualities loading
+// meanwhile, so the stream is written once, in order, with no host gather.
 #include <zlib.h>
 
 #include <algorithm>
@@ -218,6 +219,15 @@ __device__ __forceinline__ RecLay lay_out(const RecF &f, int64_t name_len, int64
 
 __device__ __forceinline__ uint32_t word_byte(uint32_t w, int k) { return (w >> (8 * k)) & 0xFFu; }
 
+// Byte t of a record's tags: RG:Z<id>\0 then NM:C<nm>.
+__device__ __forceinline__ uint32_t tag_byte(int t, const uint8_t *__restrict__ rg, int rg_n, uint32_t nm8) {
+    if (t < 3) return t == 0 ? 'R' : (t == 1 ? 'G' : 'Z');
+    if (t < 3 + rg_n) return rg[t - 3];
+    if (t == 3 + rg_n) return 0u;
+    const int u = t - 4 - rg_n;
+    return u == 0 ? 'N' : (u == 1 ? 'M' : (u == 2 ? 'C' : nm8));
+}
+
 // Byte b of a record (0 <= b < L.total); `words` = the record's 9 header words
 // and 3 CIGAR words (LDS: a lane-indexed array would live in scratch).
 __device__ __forceinline__ uint32_t rec_byte(const RecLay &L, const uint32_t *words, int b,
@@ -252,12 +262,7 @@ __device__ __forceinline__ uint32_t rec_byte(const RecLay &L, const uint32_t *wo
         const uint32_t q = qual[L.rev ? L.qlen - 1 - t : t];
         return q >= 33 ? q - 33 : 255u;
     }
-    const int t = b - L.o_tags;
-    if (t < 3) return t == 0 ? 'R' : (t == 1 ? 'G' : 'Z');
-    if (t < 3 + rg_n) return rg[t - 3];
-    if (t == 3 + rg_n) return 0u;
-    const int u = t - 4 - rg_n;
-    return u == 0 ? 'N' : (u == 1 ? 'M' : (u == 2 ? 'C' : (uint32_t)L.nm8));
+    return tag_byte(b - L.o_tags, rg, rg_n, (uint32_t)L.nm8);
 }
 
 // A wave-uniform 64-bit value into scalar registers (both halves through uint32_t).
@@ -348,32 +353,53 @@ __device__ __forceinline__ void write_record(const RecLay &L, const uint32_t *wl
     }
 }
 
+// LDS writes of the wave visible to its own later LDS reads.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A record's image: its bytes at the output's alignment (image byte a + i is
+// record byte i, a = the record's output offset mod 4).
+constexpr int IMG_BYTES = 4 + 36 + (STAGE + 1) + 12 + (STAGE + 1) / 2 + STAGE + 3 + BAM_RG_MAX + 1 + 4;
+constexpr int IMG_WORDS = (IMG_BYTES + 7) / 8;
+
 __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, ReadSrc rs, const RecLayout *__restrict__ lay,
                                                        const uint8_t *__restrict__ rg, int rg_n,
                                                        const uint64_t *__restrict__ rec_off,
                                                        uint8_t *__restrict__ out, int64_t out_cap) {
     // Each wave takes 64 consecutive output records: their layouts (one 64-byte
     // line each, read in sequence) go to LDS, then the wave writes the 64
-    // records one after another.
+    // records one after another. A record is assembled in LDS section by
+    // section (header, name, CIGAR, bases from 4-bit codes, qualities, tags; a
+    // byte per lane), then leaves as aligned 4-byte words, byte stores only for
+    // the words it shares with its neighbours. Its name, bases and qualities
+    // load one record ahead, into registers, while the record before it is
+    // written.
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * BAM_WPB + (threadIdx.x >> 6));
+    const int wv = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * BAM_WPB + wv);
     const int64_t n_waves = (int64_t)gridDim.x * BAM_WPB;
     uint32_t *const out32 = reinterpret_cast<uint32_t *>(out);
     __shared__ RecLayout s_rec[BAM_WPB][64];
     __shared__ uint64_t s_base[BAM_WPB][64];
-    __shared__ uint8_t s_stage[BAM_WPB][3 * STAGE];
-    RecLayout *const slots = s_rec[threadIdx.x >> 6];
-    uint64_t *const bases = s_base[threadIdx.x >> 6];
+    __shared__ uint64_t s_img[BAM_WPB][IMG_WORDS];
+    __shared__ uint8_t s_code[BAM_WPB][STAGE + 8];
+    RecLayout *const slots = s_rec[wv];
+    uint64_t *const bases = s_base[wv];
+    uint8_t *const img = reinterpret_cast<uint8_t *>(s_img[wv]);
+    uint8_t *const code = s_code[wv];
     for (int64_t j0 = wave * 64; j0 < n; j0 += n_waves * 64) {
         const int64_t j = j0 + lane;
         if (j < n) {
             slots[lane] = lay[j];
             bases[lane] = rec_off[j];
         }
-        __builtin_amdgcn_wave_barrier();
+        wsync();
         const int cnt = (int)(n - j0 < 64 ? n - j0 : 64);
-        // record r's name, bases and qualities: 4 bytes of each per lane, loaded
-        // one record ahead so the loads of r + 1 overlap the writing of r
+        // record r's name, bases and qualities: bytes lane + 64 k of each
         uint32_t pn[STAGE / 64], ps[STAGE / 64], pq[STAGE / 64];
         auto fetch = [&](int r) {
             const RecLayout &S = slots[r];
@@ -391,19 +417,8 @@ __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, ReadSrc rs, co
                 pq[k] = (qu && x < ql) ? qu[x] : 0u;
             }
         };
-        uint8_t *const st_name = s_stage[threadIdx.x >> 6];
-        uint8_t *const st_seq = st_name + STAGE;
-        uint8_t *const st_qual = st_seq + STAGE;
         if (cnt > 0) fetch(0);
         for (int r = 0; r < cnt; r++) {
-#pragma unroll
-            for (int k = 0; k < STAGE / 64; k++) {
-                st_name[lane + 64 * k] = (uint8_t)pn[k];
-                st_seq[lane + 64 * k] = (uint8_t)ps[k];
-                st_qual[lane + 64 * k] = (uint8_t)pq[k];
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (r + 1 < cnt) fetch(r + 1);
             const RecLayout &S = slots[r];
             const uint32_t fl = __builtin_amdgcn_readfirstlane(S.flags);
             RecLay L;
@@ -419,18 +434,61 @@ __global__ __launch_bounds__(256) void bam_pack_kernel(int64_t n, ReadSrc rs, co
             L.total = (int32_t)__builtin_amdgcn_readfirstlane(S.w[0]) + 4;
             const int64_t base = uni64(bases[r]);
             const int64_t row = (int64_t)__builtin_amdgcn_readfirstlane(S.row);
-            // staged copies unless the record is longer than the stage (then from HBM);
-            // two calls so that the staged one reads LDS with LDS instructions
-            if (base + L.total <= out_cap) {  // else the caller sees rec_off[n] > out_cap
-                if (L.name_len <= STAGE && L.qlen <= STAGE)
-                    write_record(L, S.w, base, st_name, st_seq, st_qual, rg, rg_n, lane, out, out32);
-                else
+            const bool fits = L.name_len <= STAGE && L.qlen <= STAGE;
+            const bool room = base + L.total <= out_cap;  // else the caller sees rec_off[n] > out_cap
+            if (fits && room) {
+                const int a = (int)(base & 3);
+                uint8_t *const im = img + a;
+                if (lane < 36) im[lane] = (uint8_t)word_byte(S.w[lane >> 2], lane & 3);
+                const int nc4 = L.o_seq - L.o_cig;
+                if (lane < nc4) im[L.o_cig + lane] = (uint8_t)word_byte(S.w[9 + (lane >> 2)], lane & 3);
+                if (lane == 0) im[36 + L.name_len] = 0;
+#pragma unroll
+                for (int k = 0; k < STAGE / 64; k++) {
+                    const int x = lane + 64 * k;
+                    if (x < L.name_len) im[36 + x] = (uint8_t)pn[k];
+                    if (x < L.qlen) {
+                        // output base / quality index of source byte x
+                        const int pos = L.rev ? L.qlen - 1 - x : x;
+                        const uint32_t c = nt16(ps[k]);
+                        code[pos] = (uint8_t)(L.rev ? nt16_comp(c) : c);
+                        const uint32_t q = pq[k];
+                        im[L.o_qual + pos] = (uint8_t)(L.has_qual ? (q >= 33 ? q - 33 : 255u) : 255u);
+                    }
+                }
+                if (lane == 0) code[L.qlen] = 0;  // the low half of an odd length's last byte
+                const int n_tag = 3 + rg_n + 1 + 4;
+                for (int t = lane; t < n_tag; t += 64) im[L.o_tags + t] = (uint8_t)tag_byte(t, rg, rg_n, (uint32_t)L.nm8);
+                wsync();
+                if (r + 1 < cnt) fetch(r + 1);  // in flight while this record leaves
+                const int n_sb = (L.qlen + 1) / 2;
+                for (int t = lane; t < n_sb; t += 64)
+                    im[L.o_seq + t] = (uint8_t)((code[2 * t] << 4) | code[2 * t + 1]);
+                wsync();
+                // out: word d holds record bytes 4d - base .. + 3, image bytes 4 (d - d0) ..
+                const int64_t d0 = base >> 2, d1 = (base + L.total - 1) >> 2;
+                for (int64_t d = d0 + lane; d <= d1; d += 64) {
+                    const int i0 = (int)(4 * d - base);
+                    const int m = (int)(4 * (d - d0));
+                    if (i0 >= 0 && i0 + 4 <= L.total) {
+                        out32[d] = *reinterpret_cast<const uint32_t *>(img + m);
+                    } else {
+                        for (int k = 0; k < 4; k++) {
+                            const int b = i0 + k;
+                            if (b >= 0 && b < L.total) out[base + b] = img[m + k];
+                        }
+                    }
+                }
+            } else {
+                if (r + 1 < cnt) fetch(r + 1);
+                // longer than the stage: every byte from HBM
+                if (room)
                     write_record(L, S.w, base, rs.names + uni64(S.name), rs.seq + row * rs.stride,
                                  rs.qual ? rs.qual + row * rs.stride : nullptr, rg, rg_n, lane, out, out32);
             }
-            __builtin_amdgcn_wave_barrier();  // the stage is rewritten for the next record
+            wsync();  // the image and codes are rewritten for the next record
         }
-        __builtin_amdgcn_wave_barrier();  // the slots are reused by the next batch
+        wsync();  // the slots are reused by the next batch
     }
 }
 

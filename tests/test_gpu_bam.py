@@ -69,6 +69,22 @@ def test_bam_pack_se_matches_oracle(seed, n):
         assert np.array_equal(off, np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64))
 
 
+def test_bam_pack_reads_longer_than_the_stage():
+    """Reads over 256 bases (the per-record LDS stage) take the byte-by-byte
+    path from HBM; mixed with staged ones, the stream still equals the oracle's."""
+    say("test_bam_pack_reads_longer_than_the_stage")
+    n = 600
+    rec, tid, names, blob, name_off, seqs, lens, quals, qual_lens = random_se_batch(21, n, max_len=400)
+    rng = np.random.default_rng(21)
+    perm = rng.permutation(n).astype(np.uint32)
+    dup = (rng.random(n) < 0.3).astype(np.uint32)
+    assert (lens > 256).sum() > 100 and (lens <= 256).sum() > 100
+    want = orc.bam_pack_se(rec, tid, blob, name_off, seqs, lens, quals, b"rg1", perm=perm, is_dup=dup,
+                           apply_dup=True, qual_lens=qual_lens)
+    got, _ = gpu_pack_se(rec, tid, blob, name_off, seqs, lens, quals, b"rg1", perm, dup, True, qual_lens)
+    assert got == want
+
+
 def test_bam_pack_fields_matches_oracle():
     say("test_bam_pack_fields_matches_oracle")
     n = 3000
