@@ -8,22 +8,25 @@
 // decode to the same stream:
 //
 //   * one 1024-thread workgroup per block (persistent grid), the block staged
-//     in LDS (64 bytes per thread);
-//   * LZ77: a 4-byte hash table in LDS filled chunk by chunk (1024 positions
-//     per step, in order), so each position gets the latest earlier-chunk
-//     position with its hash as a match candidate, plus distance 1 for runs;
-//     each thread parses its 64-byte segment greedily (matches of 4..64 bytes
-//     stay inside the segment, so the parses are independent);
+//     in LDS;
+//   * LZ77 by wave: each of the 16 waves parses its own 4096-byte sub-block
+//     with its own 1024-entry hash table in LDS, 64 positions (a segment) per
+//     round: a position's candidate is the latest position with its 4-byte
+//     hash from an earlier round (the table is seeded with the previous
+//     sub-block's last 512 positions), checked 4 bytes at a time, and the
+//     distance-1 run; matches of 4..64 bytes stay inside the segment, which is
+//     parsed greedily in scalar code over the lanes' match masks; one token
+//     word per position goes to a scratch, and each wave counts its symbols
+//     (two 16-bit counters per LDS word);
 //   * one dynamic-Huffman block (RFC 1951 3.2.7) per member: literal/length
 //     and distance code lengths from two-queue Huffman builds (two threads at
 //     once) limited to 15 bits (7 for the code-length code) by Kraft-sum
 //     repair, canonical codes; a stored block when that is not smaller;
-//   * per-thread bit offsets from a block scan; each thread writes its
-//     segment's codes straight into the block's output slot (plain stores for
-//     the words it owns, atomic OR for the two it may share, which alone are
-//     cleared first); tokens live in a scratch laid out so a wave's accesses
-//     are 64 consecutive words;
-//   * CRC32: per-thread CRCs of the 64-byte segments joined pairwise by the
+//   * each wave's body bits follow from its symbol counts and the code
+//     lengths, so the sub-blocks' bit offsets need no second pass; the body is
+//     assembled in LDS (where the hash tables were) by OR of every token's
+//     bits at its offset, then copied to the block's output slot;
+//   * CRC32: per-thread CRCs of 64-byte segments (slicing by 4) joined by the
 //     GF(2) shift crc(A||B) = crc(A) * x^(8|B|) + crc(B) mod P;
 //   * a second kernel writes every member (18-byte header, body, trailer) at
 //     its offset from a scan of the member sizes, and the EOF member.
@@ -34,15 +37,15 @@ using namespace ma;
 namespace {
 
 constexpr int DF_T = 1024;            // threads per workgroup
-constexpr int DF_SEG = 64;            // input bytes per thread
+constexpr int DF_SEG = 64;            // input bytes per thread's CRC segment
 constexpr int DF_IN = 65280;          // bytes per BGZF block
-constexpr int DF_OUT_WORDS = 16384;   // 64 KiB output slot per block
+constexpr int DF_OUT_WORDS = 16384;   // 64 KiB of deflate body per block at most
 constexpr int64_t DF_SLOT = 65536;    // staging bytes per block
 constexpr int NLIT = 286;             // literal/length alphabet
 constexpr int NDC = 30;               // distance alphabet
 constexpr int NCL = 19;               // code-length alphabet
 constexpr int NSEQ = NLIT + NDC;      // code-length sequence entries at most
-constexpr int HASH_BITS = 13;         // match-candidate table: 8192 entries
+static_assert(NSEQ <= DF_T, "one code-length symbol per thread");
 constexpr uint32_t CRC_POLY = 0xEDB88320u;  // reflected CRC-32 polynomial
 constexpr uint32_t STORED = 0xFFFFFFFFu;
 
@@ -204,31 +207,41 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t *w
     return pre + x - v;
 }
 
+constexpr int DF_W = DF_T / 64;      // waves per workgroup
+constexpr int SUB = 4096;            // bytes of the block parsed by one wave (64 segments)
+constexpr int PRE = 512;             // positions before a sub-block seeded into its wave's table
+constexpr int TAB_BITS = 10;         // per-wave match-candidate table: 1024 entries
+constexpr int NSYM = NLIT + NDC;     // literal/length and distance symbols, counted together
+constexpr int HW = NSYM / 2;         // per-wave counters: two 16-bit counts per word
+
 struct DfShared {
-    uint8_t data[DF_IN];
-    uint32_t head[1 << HASH_BITS];  // per 4-byte hash: 1 + the latest position of earlier chunks
-    uint32_t crc_tab[256];
-    uint32_t xpow[40];         // x^(2^k) mod P (reflected), k = 0..39
-    uint32_t shift64[DF_T];    // x^(8 * 64 k) mod P: the CRC shift over k whole segments
-    uint32_t crcw[DF_T / 64];  // per-wave XOR of the shifted segment CRCs
-    uint32_t sf[NLIT];         // frequencies in ranked order
+    uint32_t data[1 + DF_IN / 4 + 24];  // a guard word, the block, padding: every window read stays inside
+    union {
+        uint32_t tab[DF_W][1 << TAB_BITS];  // per wave: 1 + the latest earlier-round position per hash
+        uint32_t out[DF_OUT_WORDS];         // then the deflate body, assembled by OR
+    } u;
+    uint32_t hist[DF_W][HW];        // per wave: symbol counts of its sub-block (16-bit halves)
+    uint32_t crc4[4][256];          // slicing-by-4 CRC tables
+    uint32_t xpow[40];              // x^(2^k) mod P (reflected), k = 0..39
+    uint32_t crcw[DF_W];            // per-wave XOR of the shifted segment CRCs
+    uint32_t wextra[DF_W];          // per-wave extra bits of lengths and distances
+    uint32_t wbits[DF_W];           // per-wave body bits
+    uint32_t sf[NLIT];              // frequencies in ranked order
     uint32_t dsf[NDC];
     uint32_t clblc[32];
-    uint32_t part[DF_T];       // per-thread CRCs, then bit counts
     uint32_t wsum[DF_T / 64];
-    uint8_t ntok[DF_T];        // tokens of each thread's segment
     uint32_t freq[NLIT];
     uint32_t dfreq[NDC];
     uint8_t lit_len[NLIT];
-    uint16_t lit_code[NLIT];   // bit-reversed canonical codes
+    uint16_t lit_code[NLIT];        // bit-reversed canonical codes
     uint8_t d_len[NDC];
     uint16_t d_code[NDC];
-    uint16_t sorted[NLIT];     // used symbols by ascending (freq, symbol)
+    uint16_t sorted[NLIT];          // used symbols by ascending (freq, symbol)
     uint16_t dsorted[NDC];
-    uint32_t nf[NLIT];         // Huffman build scratch (literal/length and code-length trees)
+    uint32_t nf[NLIT];              // Huffman build scratch (literal/length and code-length trees)
     uint16_t par[2 * NLIT];
     uint8_t dep[2 * NLIT];
-    uint32_t dnf[NDC];         // Huffman build scratch (distance tree, built concurrently)
+    uint32_t dnf[NDC];              // Huffman build scratch (distance tree, built concurrently)
     uint16_t dpar[2 * NDC];
     uint8_t ddep[2 * NDC];
     uint32_t dblc[32];
@@ -239,9 +252,9 @@ struct DfShared {
     uint16_t cl_code[NCL];
     uint16_t cl_sorted[NCL];
     uint32_t blc[32];
-    uint32_t misc[8];          // 0 used lit/len symbols, 1 CL entries, 2 last used lit/len symbol + 1,
-                               // 3 hclen, 5 used distance symbols, 6 last used distance symbol + 1,
-                               // 7 used CL symbols
+    uint32_t misc[8];               // 0 used lit/len symbols, 1 CL entries, 2 last used lit/len symbol + 1,
+                                    // 3 hclen, 5 used distance symbols, 6 last used distance symbol + 1,
+                                    // 7 used CL symbols
 };
 
 // Length symbol (257..285) and extra bits of a match length 3..258 (RFC 1951 3.2.5).
@@ -277,15 +290,79 @@ __device__ __forceinline__ void dist_code(uint32_t D, uint32_t &sym, uint32_t &e
     extra = d & ((1u << e) - 1);
 }
 
-__device__ __forceinline__ uint32_t hash4(const uint8_t *d, int p) {
-    const uint32_t x = (uint32_t)d[p] | (uint32_t)d[p + 1] << 8 | (uint32_t)d[p + 2] << 16 | (uint32_t)d[p + 3] << 24;
-    return (x * 2654435761u) >> (32 - HASH_BITS);
+// The 4 bytes at byte offset x of a word array (two aligned reads).
+__device__ __forceinline__ uint32_t load4(const uint32_t *dw, int x) {
+    const int a = x >> 2;
+    return __builtin_amdgcn_alignbyte(dw[a + 1], dw[a], (uint32_t)(x & 3));
 }
 
-__device__ __forceinline__ int match_len(const uint8_t *d, int q, int p, int maxl) {
-    int i = 0;
-    while (i < maxl && d[q + i] == d[p + i]) i++;
-    return i;
+__device__ __forceinline__ uint32_t hash_at(const uint32_t *dw, int p) {
+    return (load4(dw, p) * 2654435761u) >> (32 - TAB_BITS);
+}
+
+// Length (at most 32) of the common prefix of two byte strings given as
+// eight 4-byte windows each.
+__device__ __forceinline__ int prefix32(const uint32_t (&a)[8], const uint32_t (&b)[8]) {
+    int l = 32;
+#pragma unroll
+    for (int j = 7; j >= 0; j--) {
+        const uint32_t x = a[j] ^ b[j];
+        if (x) l = 4 * j + (int)(__builtin_ctz(x) >> 3);
+    }
+    return l;
+}
+
+// The eight 4-byte windows of the 32 bytes at byte offset x (nine aligned reads
+// issued together).
+__device__ __forceinline__ void windows32(const uint32_t *dw, int x, uint32_t (&win)[8]) {
+    const int a = x >> 2;
+    const uint32_t s = (uint32_t)(x & 3);
+    uint32_t w[9];
+#pragma unroll
+    for (int j = 0; j < 9; j++) w[j] = dw[a + j];
+#pragma unroll
+    for (int j = 0; j < 8; j++) win[j] = __builtin_amdgcn_alignbyte(w[j + 1], w[j], s);
+}
+
+// Common prefix of the bytes at q and p (q < p) from byte 32 on, at most 32 more.
+__device__ __forceinline__ int match_more(const uint32_t *dw, int q, int p) {
+    uint32_t a[8], b[8];
+    windows32(dw, q + 32, a);
+    windows32(dw, p + 32, b);
+    return prefix32(a, b);
+}
+
+// ORs the n low bits of v into the bit stream at bit offset o (LSB first).
+__device__ __forceinline__ void or_bits(uint32_t *out, uint32_t o, uint64_t v, int n) {
+    if (n == 0) return;
+    const uint32_t wi = o >> 5, s = o & 31;
+    const uint64_t lo = v << s;
+    const uint32_t hi = s ? (uint32_t)(v >> (64 - s)) : 0u;
+    if ((uint32_t)lo) atomicOr(&out[wi], (uint32_t)lo);
+    if ((uint32_t)(lo >> 32)) atomicOr(&out[wi + 1], (uint32_t)(lo >> 32));
+    if (hi) atomicOr(&out[wi + 2], hi);
+}
+
+// Bits [a, b) of a 64-bit mask, 0 <= a <= b <= 64, a < 64.
+__device__ __forceinline__ uint64_t span64(int a, int b) {
+    const uint64_t hi = b >= 64 ? ~0ull : (1ull << b) - 1;
+    return hi & ~((1ull << a) - 1);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const int lane = (int)(threadIdx.x & 63);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
 }
 
 // Ranks the used symbols of freq[0..nsym) by (freq, symbol) into sorted[];
@@ -422,9 +499,16 @@ __device__ int rle_emit(int v, int run, uint8_t *sym, uint8_t *extra) {
     return ns;
 }
 
-// Tokens: a literal byte b is b; a match is 1 << 31 | length << 16 | distance.
+// Tokens, one word per block position: a literal byte b is TOK_LIT | b, a
+// match starting there TOK_MATCH | length << 16 | distance, a position inside
+// a match 0.
 constexpr uint32_t TOK_MATCH = 1u << 31;
+constexpr uint32_t TOK_LIT = 1u << 30;
 constexpr int MIN_MATCH = 4;
+
+__device__ __forceinline__ void count_sym(uint32_t *hist, uint32_t u) {
+    atomicAdd(&hist[u >> 1], 1u << (16 * (u & 1)));
+}
 
 __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__restrict__ src, int64_t n,
                                                             int64_t nb, int level,
@@ -432,18 +516,17 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                                                             uint32_t *__restrict__ body_len,
                                                             uint32_t *__restrict__ crc_out,
                                                             uint32_t *__restrict__ member_bytes,
-                                                            uint16_t *__restrict__ cand_all,
                                                             uint32_t *__restrict__ tok_all) {
     __shared__ DfShared S;
-    const int t = threadIdx.x;
-    const bool huffman = level > 0, lz = level > 0;
-    uint16_t *const cand = cand_all + (int64_t)blockIdx.x * DF_IN;        // this workgroup's scratch
-    // token k of thread t at tok[k * DF_T]: a wave's k-th tokens are 64 consecutive words
-    uint32_t *const tok = tok_all + (int64_t)blockIdx.x * DF_T * DF_SEG + t;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // uniform: sub-block bounds stay scalar
+    uint32_t *const dw = S.data + 1;  // the block's words (dw[-1] is the guard)
+    const uint8_t *const db = reinterpret_cast<const uint8_t *>(dw);
+    uint32_t *const tok = tok_all + (int64_t)blockIdx.x * DF_IN;  // this workgroup's tokens
     if (t < 256) {
         uint32_t c = (uint32_t)t;
         for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
-        S.crc_tab[t] = c;
+        S.crc4[0][t] = c;
     }
     if (t == 0) {
         uint32_t x = 0x40000000u;  // x^1
@@ -453,113 +536,189 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
         }
     }
     __syncthreads();
-    S.shift64[t] = x8n(S.xpow, (uint32_t)(DF_SEG * t));
+    if (t < 256) {  // slicing by 4 (Kounavis and Berry): table k advances a byte k more places
+        uint32_t c = S.crc4[0][t];
+        for (int k = 1; k < 4; k++) {
+            c = (c >> 8) ^ S.crc4[0][c & 255];
+            S.crc4[k][t] = c;
+        }
+    }
+    // the CRC shift over the bytes after this thread's segment in a full block
+    const uint32_t sh_full = t < DF_IN / DF_SEG ? x8n(S.xpow, (uint32_t)(DF_IN - DF_SEG * (t + 1))) : 0u;
     __syncthreads();
     for (int64_t b = blockIdx.x; b < nb; b += gridDim.x) {
         const int64_t base = b * DF_IN;
         const int bl = (int)(n - base < DF_IN ? n - base : DF_IN);
         uint32_t *const slot = slots + b * (DF_SLOT / 4);
-        // 1. stage the block, clear the tables and the block's output slot
-        for (int i = t; i < (1 << HASH_BITS); i += DF_T) S.head[i] = 0;
-        for (int i = t; i < NLIT; i += DF_T) {
-            S.freq[i] = 0;
-            S.lit_len[i] = 0;
-        }
-        if (t < NDC) {
-            S.dfreq[t] = 0;
-            S.d_len[t] = 0;
-        }
-        if (t < NCL) {
-            S.cl_freq[t] = 0;
-            S.cl_len[t] = 0;
-        }
-        if (t < 8) S.misc[t] = 0;
-        const uint8_t *blk = src + base;
-        for (int i = 4 * t; i < bl; i += 4 * DF_T) {
-            if (i + 4 <= bl) {
-                *reinterpret_cast<uint32_t *>(&S.data[i]) = *reinterpret_cast<const uint32_t *>(blk + i);
-            } else {
-                for (int k = i; k < bl; k++) S.data[k] = blk[k];
+        // 1. stage the block: four words' loads issued before their stores
+        {
+            const uint8_t *blk = src + base;
+            for (int i0 = 4 * t; i0 < bl; i0 += 4 * 4 * DF_T) {
+                uint32_t v[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int i = i0 + 4 * DF_T * k;
+                    v[k] = i + 4 <= bl ? *reinterpret_cast<const uint32_t *>(blk + i) : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int i = i0 + 4 * DF_T * k;
+                    if (i < bl) {
+                        uint32_t x = v[k];
+                        if (i + 4 > bl)
+                            for (int j = 0; j < bl - i; j++) x |= (uint32_t)blk[i + j] << (8 * j);
+                        dw[i >> 2] = x;
+                    }
+                }
             }
         }
         __syncthreads();
-        // 2. per-segment CRC
-        const int s0 = t * DF_SEG, s1 = min(bl, s0 + DF_SEG);
-        uint32_t c = 0xFFFFFFFFu;
-        for (int i = s0; i < s1; i++) c = S.crc_tab[(c ^ S.data[i]) & 255] ^ (c >> 8);
+        // 2. CRC of this thread's 64-byte segment (slicing by 4), shifted to its place
         {
-            // crc(S_0 .. S_k) = XOR over segments of crc(S_t) * x^(8 * bytes after S_t) mod P
-            uint32_t v = 0;
+            const int s0 = t * DF_SEG, s1 = min(bl, s0 + DF_SEG);
+            uint32_t c = 0xFFFFFFFFu, v = 0;
             if (s0 < bl) {
-                const int after = bl - s1;
-                v = multmodp(after % DF_SEG == 0 ? S.shift64[after / DF_SEG] : x8n(S.xpow, (uint32_t)after), ~c);
+                int i = s0;
+                for (; i + 4 <= s1; i += 4) {
+                    c ^= dw[i >> 2];
+                    c = S.crc4[3][c & 255] ^ S.crc4[2][(c >> 8) & 255] ^ S.crc4[1][(c >> 16) & 255] ^
+                        S.crc4[0][c >> 24];
+                }
+                for (; i < s1; i++) c = S.crc4[0][(c ^ db[i]) & 255] ^ (c >> 8);
+                // crc(S_0 .. S_k) = XOR over segments of crc(S_t) * x^(8 * bytes after S_t) mod P
+                v = multmodp(bl == DF_IN ? sh_full : x8n(S.xpow, (uint32_t)(bl - s1)), ~c);
             }
 #pragma unroll
             for (int o = 32; o > 0; o >>= 1) v ^= __shfl_xor(v, o, 64);
-            if ((t & 63) == 0) S.crcw[t >> 6] = v;
+            if (lane == 0) S.crcw[w] = v;
         }
-        if (t == 0) S.freq[256] = 1;  // end of block
-        // 3. match candidates: chunks of DF_T consecutive positions in order;
-        //    a position sees the latest earlier-chunk position with its hash
-        if (lz) {
-            for (int c0 = 0; c0 < bl; c0 += DF_T) {
-                const int p = c0 + t;
-                const bool has = p + MIN_MATCH <= bl;
-                const uint32_t h = has ? hash4(S.data, p) : 0;
-                if (p < bl) cand[p] = has ? (uint16_t)S.head[h] : (uint16_t)0;
-                __syncthreads();
-                if (has) atomicMax(&S.head[h], (uint32_t)(p + 1));
-                __syncthreads();
+        // 3. LZ77, one wave per 4096-byte sub-block, 64 positions (a segment) per round:
+        //    a position's candidate is the latest position of an earlier round (or of the
+        //    previous sub-block's last PRE bytes) with its 4-byte hash; the longer of its
+        //    match and the distance-1 run (4..64 bytes, inside the segment) is its match;
+        //    the segment is parsed greedily from those, in scalar code over lane masks
+        const int sb0 = w * SUB, sb1 = min(sb0 + SUB, bl);
+        if (level > 0) {
+            uint32_t *const tab = S.u.tab[w];
+            uint32_t *const hist = S.hist[w];
+            for (int k = lane; k < (1 << TAB_BITS); k += 64) tab[k] = 0;
+            for (int k = lane; k < HW; k += 64) hist[k] = 0;
+            uint32_t extra = 0;
+            if (sb0 < sb1) {
+                for (int p = (sb0 >= PRE ? sb0 - PRE : 0) + lane; p < sb0; p += 64)
+                    if (p + 4 <= bl) atomicMax(&tab[hash_at(dw, p)], (uint32_t)(p + 1));
+                for (int r0 = sb0; r0 < sb1; r0 += 64) {
+                    const int P = r0 + lane;
+                    const int E = min(r0 + 64, sb1);
+                    const bool valid = P < E;
+                    const bool ins = valid && P + 4 <= bl;
+                    // the 36 bytes from P - 4 (aligned down), read once (the guard word and the
+                    // padding keep every lane's reads inside): P's windows, the hash, byte P - 1
+                    const int pa = P >> 2;
+                    const uint32_t ps = (uint32_t)(P & 3);
+                    uint32_t pw[10];
+#pragma unroll
+                    for (int j = 0; j < 10; j++) pw[j] = dw[pa - 1 + j];
+                    uint32_t pwin[8];
+#pragma unroll
+                    for (int j = 0; j < 8; j++) pwin[j] = __builtin_amdgcn_alignbyte(pw[j + 2], pw[j + 1], ps);
+                    uint32_t h = 0, c = 0;
+                    if (ins) {
+                        h = (pwin[0] * 2654435761u) >> (32 - TAB_BITS);
+                        c = tab[h];  // the wave's reads complete before its updates below
+                    }
+                    if (ins) atomicMax(&tab[h], (uint32_t)(P + 1));
+                    int best = 0, bd = 0;
+                    const int maxl = E - P;
+                    if (valid && maxl >= MIN_MATCH) {
+                        if (c) {
+                            const int q = (int)c - 1;
+                            uint32_t qwin[8];
+                            windows32(dw, q, qwin);
+                            int l = prefix32(qwin, pwin);
+                            if (l == 32 && maxl > 32) l += match_more(dw, q, P);
+                            l = min(l, maxl);
+                            best = l;
+                            bd = P - q;
+                        }
+                        if (P >= 1 && best < maxl) {
+                            // the run: P's bytes equal to byte P - 1 (distance 1)
+                            const uint32_t prev = ps ? (pw[1] >> (8 * (ps - 1))) & 255u : pw[0] >> 24;
+                            uint32_t rwin[8];
+#pragma unroll
+                            for (int j = 0; j < 8; j++) rwin[j] = prev * 0x01010101u;
+                            int l = prefix32(rwin, pwin);
+                            if (l == 32 && maxl > 32) l += match_more(dw, P - 1, P);
+                            l = min(l, maxl);
+                            if (l > best) {
+                                best = l;
+                                bd = 1;
+                            }
+                        }
+                    }
+                    const uint64_t M = __ballot(best >= MIN_MATCH);
+                    const int nv = E - r0;
+                    uint64_t lit = 0, st = 0;
+                    int p = 0;
+                    while (p < nv) {
+                        const uint64_t rest = M >> p << p;
+                        if (!rest) {
+                            lit |= span64(p, nv);
+                            break;
+                        }
+                        const int q = __builtin_ctzll(rest);
+                        lit |= span64(p, q);
+                        st |= 1ull << q;
+                        p = q + __builtin_amdgcn_readlane(best, q);
+                    }
+                    uint32_t tk = 0;
+                    if ((lit >> lane) & 1) {
+                        const uint32_t by = db[P];
+                        tk = TOK_LIT | by;
+                        count_sym(hist, by);
+                    } else if ((st >> lane) & 1) {
+                        uint32_t ls, le, lx, ds, de, dx;
+                        len_code((uint32_t)best, ls, le, lx);
+                        dist_code((uint32_t)bd, ds, de, dx);
+                        tk = TOK_MATCH | (uint32_t)best << 16 | (uint32_t)bd;
+                        count_sym(hist, ls);
+                        count_sym(hist, NLIT + ds);
+                        extra += le + de;
+                    }
+                    if (valid) tok[P] = tk;
+                }
             }
-        } else {
-            __syncthreads();
+            extra = wave_sum(extra);
+            if (lane == 0) S.wextra[w] = extra;
         }
-        // 4. CRC of the block (the barriers above ordered the per-wave parts)
-        uint32_t crc = 0;
-        for (int w = 0; w < DF_T / 64; w++) crc ^= S.crcw[w];
+        __syncthreads();
+        const uint32_t crc = [&] {
+            uint32_t x = 0;
+            for (int v = 0; v < DF_W; v++) x ^= S.crcw[v];
+            return x;
+        }();
         uint32_t nbytes = 0xFFFFFFFFu;
-        if (huffman) {
-            // 5. greedy parse of the segment (matches stay inside it), symbol counts
-            int k = 0;
-            for (int p = s0; p < s1;) {
-                int best = 0, bd = 0;
-                const int maxl = min(258, s1 - p);
-                if (lz && maxl >= MIN_MATCH) {
-                    const int cq = cand[p];
-                    if (cq && p - (cq - 1) <= 32768) {
-                        const int l = match_len(S.data, cq - 1, p, maxl);
-                        if (l > best) {
-                            best = l;
-                            bd = p - (cq - 1);
-                        }
-                    }
-                    if (p >= 1 && best < maxl) {  // runs: distance 1
-                        const int l = match_len(S.data, p - 1, p, maxl);
-                        if (l > best) {
-                            best = l;
-                            bd = 1;
-                        }
-                    }
-                }
-                uint32_t tk;
-                if (best >= MIN_MATCH) {
-                    uint32_t ls, le, lx, ds, de, dx;
-                    len_code((uint32_t)best, ls, le, lx);
-                    dist_code((uint32_t)bd, ds, de, dx);
-                    atomicAdd(&S.freq[ls], 1u);
-                    atomicAdd(&S.dfreq[ds], 1u);
-                    tk = TOK_MATCH | (uint32_t)best << 16 | (uint32_t)bd;
-                    p += best;
+        if (level > 0) {
+            // 4. block symbol counts; the output words cleared (the tables are done)
+            if (t < NSYM) {
+                uint32_t f = 0;
+                for (int v = 0; v < DF_W; v++) f += (S.hist[v][t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
+                if (t < NLIT) {
+                    S.freq[t] = f + (t == 256 ? 1u : 0u);  // end of block
+                    S.lit_len[t] = 0;
                 } else {
-                    tk = S.data[p];
-                    atomicAdd(&S.freq[tk], 1u);
-                    p += 1;
+                    S.dfreq[t - NLIT] = f;
+                    S.d_len[t - NLIT] = 0;
                 }
-                tok[(k++) * DF_T] = tk;
             }
-            S.ntok[t] = (uint8_t)k;
+            for (int k = t; k < DF_OUT_WORDS; k += DF_T) S.u.out[k] = 0;
+            if (t < NCL) {
+                S.cl_freq[t] = 0;
+                S.cl_len[t] = 0;
+            }
+            if (t < 8) S.misc[t] = 0;
             __syncthreads();
-            // 6. the two codes: symbols ranked in parallel, the two-queue merges by
+            // 5. the two codes: symbols ranked in parallel, the two-queue merges by
             //    threads 0 and 64 at once, then depths, lengths and codes in parallel
             rank_symbols(S.freq, NLIT, S.sorted, S.sf, &S.misc[0], 0);
             rank_symbols(S.dfreq, NDC, S.dsorted, S.dsf, &S.misc[5], 512);
@@ -604,6 +763,15 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 S.d_code[t - 512] = canon_code(S.d_len, S.dblc, t - 512);
                 if (S.d_len[t - 512]) atomicMax(&S.misc[6], (uint32_t)(t - 511));
             }
+            {  // this wave's body bits: its symbol counts times their code lengths, plus extra bits
+                uint32_t bits = 0;
+                for (int u = lane; u < NSYM; u += 64) {
+                    const uint32_t cnt = (S.hist[w][u >> 1] >> (16 * (u & 1))) & 0xFFFFu;
+                    bits += cnt * (u < NLIT ? S.lit_len[u] : S.d_len[u - NLIT]);
+                }
+                bits = wave_sum(bits);
+                if (lane == 0) S.wbits[w] = bits + S.wextra[w];
+            }
             __syncthreads();
             // code-length sequence (literal/length then distance lengths): maximal runs
             // found in parallel, each coded at its scanned offset
@@ -637,89 +805,99 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 S.misc[3] = (uint32_t)hclen;
             }
             __syncthreads();
-            uint32_t hpart = 0;
-            for (uint32_t q = (uint32_t)t; q < ns; q += DF_T) {
-                const int sym = S.cl_sym[q];
-                hpart += S.cl_len[sym] + (sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0);
+            // the code-length symbols' bits (one symbol per thread) and their offsets
+            uint32_t hpart = 0, hv = 0;
+            if ((uint32_t)t < ns) {
+                const int sym = S.cl_sym[t];
+                const uint32_t eb = sym == 16 ? 2 : sym == 17 ? 3 : sym == 18 ? 7 : 0;
+                hv = S.cl_code[sym] | (uint32_t)S.cl_extra[t] << S.cl_len[sym];
+                hpart = S.cl_len[sym] + eb;
             }
             uint32_t hcl;
-            block_excl_scan_1024(hpart, S.wsum, hcl);
-            const uint32_t hb_total = 3 + 5 + 5 + 4 + 3 * S.misc[3] + hcl;
-            // 7. bit offsets of the segments
-            const int ntok = S.ntok[t];
-            uint32_t bits = 0;
-            for (int q = 0; q < ntok; q++) {
-                const uint32_t tk = tok[q * DF_T];
-                if (tk & TOK_MATCH) {
-                    uint32_t ls, le, lx, ds, de, dx;
-                    len_code((tk >> 16) & 0x1FF, ls, le, lx);
-                    dist_code(tk & 0xFFFF ? tk & 0xFFFF : 65536, ds, de, dx);
-                    bits += S.lit_len[ls] + le + S.d_len[ds] + de;
-                } else {
-                    bits += S.lit_len[tk];
-                }
+            const uint32_t hoff = block_excl_scan_1024(hpart, S.wsum, hcl);
+            const uint32_t hclen = S.misc[3];
+            const uint32_t hb = 3 + 5 + 5 + 4 + 3 * hclen + hcl;
+            // 6. bit offsets of the sub-blocks
+            uint32_t dbits = 0, wpre = 0;
+            for (int v = 0; v < DF_W; v++) {
+                const uint32_t x = S.wbits[v];
+                if (v < w) wpre += x;
+                dbits += x;
             }
-            uint32_t dbits;
-            const uint32_t off = block_excl_scan_1024(bits, S.wsum, dbits);
-            const uint32_t hb = hb_total;
             const uint32_t total_bits = hb + dbits + S.lit_len[256];
             nbytes = (total_bits + 7) >> 3;
             if (nbytes < (uint32_t)bl + 5) {
-                // 8. header (thread 0), then every segment's codes, EOB after the last one.
-                //    Only the first and last word of each writer's bit range can be shared
-                //    (ORed by two writers): those are cleared first; the others are stored whole.
-                const uint32_t a = hb + off, e = a + bits + (s1 == bl ? S.lit_len[256] : 0u);
-                if (s0 < bl) {
-                    slot[a >> 5] = 0;
-                    slot[(e - 1) >> 5] = 0;
-                }
+                // 7. the header and end of block, then every token, each ORed in at its
+                //    scanned bit offset: BFINAL = 1, BTYPE = 2 (dynamic), HLIT, HDIST, HCLEN,
+                //    the code-length code's lengths in transmission order, the code lengths
                 if (t == 0) {
-                    slot[0] = 0;
-                    slot[(hb - 1) >> 5] = 0;
+                    or_bits(S.u.out, 0, 1u | 2u << 1 | (uint32_t)(hlit - 257) << 3 | (uint32_t)(hdist - 1) << 8 |
+                                            (hclen - 4) << 13, 17);
+                    or_bits(S.u.out, hb + dbits, S.lit_code[256], S.lit_len[256]);
+                }
+                if ((uint32_t)t < hclen) or_bits(S.u.out, 17 + 3 * t, S.cl_len[cl_order(t)], 3);
+                or_bits(S.u.out, 17 + 3 * hclen + hoff, hv, (int)hpart);
+                // each lane takes 8 consecutive positions per 512: one scan per 512 positions
+                uint32_t off = hb + wpre;
+                for (int r0 = sb0; r0 < sb1; r0 += 8 * 64) {
+                    const int P0 = r0 + 8 * lane;
+                    uint32_t tk[8];
+                    if (P0 < sb1) {
+                        const uint4 a = *reinterpret_cast<const uint4 *>(tok + P0);
+                        const uint4 c = *reinterpret_cast<const uint4 *>(tok + P0 + 4);
+                        tk[0] = a.x, tk[1] = a.y, tk[2] = a.z, tk[3] = a.w;
+                        tk[4] = c.x, tk[5] = c.y, tk[6] = c.z, tk[7] = c.w;
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) tk[k] = 0;
+                    }
+                    // per token: c1 = literal or length code + extra bits (at most 20) | their
+                    // count << 20 | the distance part's count << 25; c2 = distance code + extra
+                    uint32_t c1[8], c2[8];
+                    uint32_t bits = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        const uint32_t x = P0 + k < sb1 ? tk[k] : 0u;
+                        uint32_t v1 = 0, v2 = 0, b1 = 0, b2 = 0;
+                        if (x & TOK_MATCH) {
+                            uint32_t ls, le, lx, ds, de, dx;
+                            len_code((x >> 16) & 0x7FFF, ls, le, lx);
+                            dist_code(x & 0xFFFF, ds, de, dx);
+                            b1 = S.lit_len[ls];
+                            v1 = S.lit_code[ls] | lx << b1;
+                            b1 += le;
+                            b2 = S.d_len[ds];
+                            v2 = S.d_code[ds] | dx << b2;
+                            b2 += de;
+                        } else if (x & TOK_LIT) {
+                            b1 = S.lit_len[x & 255];
+                            v1 = S.lit_code[x & 255];
+                        }
+                        c1[k] = v1 | b1 << 20 | b2 << 25;
+                        c2[k] = v2;
+                        bits += b1 + b2;
+                    }
+                    const uint32_t incl = wave_incl_scan(bits);
+                    if (bits) {
+                        BitWriter bw(S.u.out, off + incl - bits);
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const int b1 = (int)((c1[k] >> 20) & 31), b2 = (int)(c1[k] >> 25);
+                            if (b1) bw.put(c1[k] & 0xFFFFFu, b1);
+                            if (b2) bw.put(c2[k], b2);
+                        }
+                        bw.finish();
+                    }
+                    off += __builtin_amdgcn_readlane(incl, 63);
                 }
                 __syncthreads();
-                if (t == 0) {
-                    BitWriter w(slot, 0);
-                    w.put(1, 1);  // BFINAL
-                    w.put(2, 2);  // BTYPE = dynamic Huffman
-                    w.put((uint32_t)hlit - 257, 5);
-                    w.put((uint32_t)hdist - 1, 5);
-                    w.put(S.misc[3] - 4, 4);
-                    for (uint32_t q = 0; q < S.misc[3]; q++) w.put(S.cl_len[cl_order((int)q)], 3);
-                    for (uint32_t q = 0; q < S.misc[1]; q++) {
-                        const int sym = S.cl_sym[q];
-                        w.put(S.cl_code[sym], S.cl_len[sym]);
-                        if (sym == 16) w.put(S.cl_extra[q], 2);
-                        if (sym == 17) w.put(S.cl_extra[q], 3);
-                        if (sym == 18) w.put(S.cl_extra[q], 7);
-                    }
-                    w.finish();
-                }
-                if (s0 < bl) {
-                    BitWriter w(slot, hb + off);
-                    for (int q = 0; q < ntok; q++) {
-                        const uint32_t tk = tok[q * DF_T];
-                        if (tk & TOK_MATCH) {
-                            uint32_t ls, le, lx, ds, de, dx;
-                            len_code((tk >> 16) & 0x1FF, ls, le, lx);
-                            dist_code(tk & 0xFFFF ? tk & 0xFFFF : 65536, ds, de, dx);
-                            w.put(S.lit_code[ls], S.lit_len[ls]);
-                            if (le) w.put(lx, (int)le);
-                            w.put(S.d_code[ds], S.d_len[ds]);
-                            if (de) w.put(dx, (int)de);
-                        } else {
-                            w.put(S.lit_code[tk], S.lit_len[tk]);
-                        }
-                    }
-                    if (s1 == bl) w.put(S.lit_code[256], S.lit_len[256]);
-                    w.finish();
-                }
+                const int nw = (int)((total_bits + 31) >> 5);
+                for (int k = t; k < nw; k += DF_T) slot[k] = S.u.out[k];
             } else {
                 nbytes = 0xFFFFFFFFu;
             }
         }
-        __syncthreads();
-        // 9. sizes and CRC (the body is already in the block's slot)
+        // 8. sizes and CRC (the body is in the block's slot)
         if (t == 0) {
             body_len[b] = nbytes == 0xFFFFFFFFu ? STORED : nbytes;
             crc_out[b] = crc;
@@ -806,7 +984,7 @@ __global__ void set_u32_kernel(uint32_t *p, uint32_t v) {
 constexpr int DF_MAX_GRID = 256;  // resident workgroups at most (one per CU)
 
 struct DfScratch {
-    size_t slots, body, crc, bytes, off, partial, cand, tok, total;
+    size_t slots, body, crc, bytes, off, partial, tok, total;
 };
 
 DfScratch df_layout(int64_t n) {
@@ -825,8 +1003,7 @@ DfScratch df_layout(int64_t n) {
     s.bytes = take((size_t)nm * 4);
     s.off = take((size_t)(nm + 1) * 8);
     s.partial = take(scan_scratch_bytes(nm));
-    s.cand = take((size_t)g * DF_IN * 2);
-    s.tok = take((size_t)g * DF_T * DF_SEG * 4);
+    s.tok = take((size_t)g * DF_IN * 4);
     s.total = o;
     return s;
 }
@@ -859,8 +1036,7 @@ int ma_bgzf_compress_dev(void *stream, const uint8_t *d_src, int64_t n, int32_t 
         int64_t grid = nb < n_cu ? nb : n_cu;
         if (grid > DF_MAX_GRID) grid = DF_MAX_GRID;
         bgzf_deflate_kernel<<<(unsigned)grid, DF_T, 0, st>>>(d_src, n, nb, level, (uint32_t *)(s + L.slots),
-                                                              body, crc, bytes, (uint16_t *)(s + L.cand),
-                                                              (uint32_t *)(s + L.tok));
+                                                              body, crc, bytes, (uint32_t *)(s + L.tok));
         MA_LAUNCHED();
     }
     if (add_eof) {
