@@ -19,9 +19,11 @@
 //     word per position goes to a scratch, and each wave counts its symbols
 //     (two 16-bit counters per LDS word);
 //   * one dynamic-Huffman block (RFC 1951 3.2.7) per member: literal/length
-//     and distance code lengths from two-queue Huffman builds (two threads at
-//     once) limited to 15 bits (7 for the code-length code) by Kraft-sum
-//     repair, canonical codes; a stored block when that is not smaller;
+//     and distance code lengths optimal under the 15-bit limit by a parallel
+//     package-merge (both codes at once, one barrier per level); the
+//     code-length code by a two-queue Huffman build limited to 7 bits by
+//     Kraft-sum repair; canonical codes; a stored block when that is not
+//     smaller;
 //   * each wave's body bits follow from its symbol counts and the code
 //     lengths, so the sub-blocks' bit offsets need no second pass; the body is
 //     assembled in LDS (where the hash tables were) by OR of every token's
@@ -213,11 +215,20 @@ constexpr int PRE = 512;             // positions before a sub-block seeded into
 constexpr int TAB_BITS = 10;         // per-wave match-candidate table: 1024 entries
 constexpr int NSYM = NLIT + NDC;     // literal/length and distance symbols, counted together
 constexpr int HW = NSYM / 2;         // per-wave counters: two 16-bit counts per word
+constexpr int PM_L = 15;             // code length limit of the literal/length and distance codes
+constexpr int PM_D = 640;            // first thread of the distance code's package-merge
 
 struct DfShared {
     uint32_t data[1 + DF_IN / 4 + 24];  // a guard word, the block, padding: every window read stays inside
-    union {
+    union alignas(16) {
         uint32_t tab[DF_W][1 << TAB_BITS];  // per wave: 1 + the latest earlier-round position per hash
+        struct {                            // then the package-merge lists of both codes
+            uint32_t a[2][2 * NLIT];        // weights of a list in merged order (two levels)
+            uint32_t da[2][2 * NDC];
+            uint16_t pos[PM_L][NLIT];       // per level: each leaf's place in the merged list
+            uint16_t dpos[PM_L][NDC];
+            uint32_t x[2][PM_L + 1];        // per level: leaves among the selected items
+        } pm;
         uint32_t out[DF_OUT_WORDS];         // then the deflate body, assembled by OR
     } u;
     uint32_t hist[DF_W][HW];        // per wave: symbol counts of its sub-block (16-bit halves)
@@ -238,12 +249,9 @@ struct DfShared {
     uint16_t d_code[NDC];
     uint16_t sorted[NLIT];          // used symbols by ascending (freq, symbol)
     uint16_t dsorted[NDC];
-    uint32_t nf[NLIT];              // Huffman build scratch (literal/length and code-length trees)
+    uint32_t nf[NLIT];              // Huffman build scratch (the code-length code)
     uint16_t par[2 * NLIT];
     uint8_t dep[2 * NLIT];
-    uint32_t dnf[NDC];              // Huffman build scratch (distance tree, built concurrently)
-    uint16_t dpar[2 * NDC];
-    uint8_t ddep[2 * NDC];
     uint32_t dblc[32];
     uint8_t cl_sym[NSEQ];
     uint8_t cl_extra[NSEQ];
@@ -385,74 +393,6 @@ __device__ __forceinline__ void rank_symbols(const uint32_t *freq, int nsym, uin
     }
 }
 
-// Two-queue Huffman over m >= 2 leaves with ascending weights sf[0..m):
-// internal node k (id m + k) gets weight nf[k]; par[] = parent id of every
-// node (leaves first). One thread.
-__device__ void huff_tree(const uint32_t *sf, int m, uint32_t *nf, uint16_t *par) {
-    int i = 0, j = 0;
-    uint32_t li = sf[0];
-    for (int k = 0; k < m - 1; k++) {
-        uint32_t f = 0;
-        for (int pick = 0; pick < 2; pick++) {
-            if (i < m && (j >= k || li <= nf[j])) {
-                par[i] = (uint16_t)(m + k);
-                f += li;
-                i++;
-                li = i < m ? sf[i] : 0;
-            } else {
-                par[m + j] = (uint16_t)(m + k);
-                f += nf[j];
-                j++;
-            }
-        }
-        nf[k] = f;
-    }
-}
-
-// Depth of leaf x: steps up to the root (id root).
-__device__ __forceinline__ int leaf_depth(const uint16_t *par, int x, int root) {
-    int d = 0;
-    while (x != root && d < 2 * NLIT) {
-        x = par[x];
-        d++;
-    }
-    return d;
-}
-
-// Per-length counts blc[1..maxlen] (deeper leaves already counted at maxlen)
-// repaired so that the Kraft sum is exactly 1. One thread.
-__device__ void kraft_repair(uint32_t *blc, int maxlen) {
-    uint32_t kraft = 0;
-    for (int l = 1; l <= maxlen; l++) kraft += blc[l] << (maxlen - l);
-    const uint32_t target = 1u << maxlen;
-    while (kraft > target) {  // lengthen the deepest code that can still grow
-        int l = maxlen - 1;
-        while (l > 1 && blc[l] == 0) l--;
-        if (blc[l] == 0) break;
-        blc[l]--;
-        blc[l + 1]++;
-        kraft -= 1u << (maxlen - l - 1);
-    }
-    while (kraft < target) {  // shorten the deepest code: its 2^(maxlen-l) divides the deficit
-        int l = maxlen;
-        while (l > 2 && blc[l] == 0) l--;
-        if (blc[l] == 0) break;
-        blc[l]--;
-        blc[l - 1]++;
-        kraft += 1u << (maxlen - l);
-    }
-}
-
-// Length of the r-th least frequent symbol: lengths handed out longest first.
-__device__ __forceinline__ uint8_t rank_length(const uint32_t *blc, int maxlen, int r) {
-    uint32_t acc = 0;
-    for (int l = maxlen; l >= 1; l--) {
-        acc += blc[l];
-        if ((uint32_t)r < acc) return (uint8_t)l;
-    }
-    return 1;
-}
-
 // Canonical code of symbol s (bit-reversed): the first code of its length
 // plus the number of smaller symbols of that length.
 __device__ __forceinline__ uint16_t canon_code(const uint8_t *len, const uint32_t *blc, int s) {
@@ -497,6 +437,70 @@ __device__ int rle_emit(int v, int run, uint8_t *sym, uint8_t *extra) {
     }
     for (; run > 0; run--) put(v, 0);
     return ns;
+}
+
+// First index in [0, n) of an ascending array whose value is >= v (strict: > v).
+template <bool STRICT, typename F>
+__device__ __forceinline__ int search_first(int n, uint32_t v, F at) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint32_t x = at(mid);
+        if (STRICT ? x <= v : x < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// One level of package-merge for a code of n leaves with ascending weights w[]:
+// list A (size s, ascending) becomes the next level's list B = merge(leaves,
+// packages of consecutive pairs of A), a leaf before a package of equal weight;
+// pos[] gets each leaf's place in B. Thread e < n + s / 2 places element e.
+__device__ __forceinline__ void pm_level(int e, int n, int s, const uint32_t *w, const uint32_t *A, uint32_t *B,
+                                         uint16_t *pos) {
+    const int np = s >> 1;
+    if (e >= n + np) return;
+    const uint2 *P = reinterpret_cast<const uint2 *>(A);
+    if (e < n) {
+        const uint32_t v = w[e];
+        const int at = e + search_first<false>(np, v, [&](int k) { const uint2 q = P[k]; return q.x + q.y; });
+        B[at] = v;
+        pos[e] = (uint16_t)at;
+    } else {
+        const int k = e - n;
+        const uint2 q = P[k];
+        const uint32_t v = q.x + q.y;
+        B[k + search_first<true>(n, v, [&](int i) { return w[i]; })] = v;
+    }
+}
+
+// Leaves among the first c items of each level's list, level 1 first (a wave):
+// c_1 = 2n - 2 items, and the packages among a level's first c_l items are the
+// first c_(l+1) = 2 (c_l - x_l) items of the level below.
+__device__ __forceinline__ void pm_select(int n, const uint16_t (*pos)[NLIT], uint32_t *x, int lane) {
+    int c = 2 * n - 2;
+#pragma unroll 1
+    for (int l = 0; l < PM_L; l++) {
+        uint32_t cnt = 0;
+        for (int i = lane; i < n; i += 64) cnt += pos[l][i] < (uint32_t)c ? 1u : 0u;
+        const int xl = (int)wave_sum(cnt);
+        if (lane == 0) x[l] = (uint32_t)xl;
+        c = 2 * (c - xl);
+    }
+}
+
+__device__ __forceinline__ void pm_select_d(int n, const uint16_t (*pos)[NDC], uint32_t *x, int lane) {
+    int c = 2 * n - 2;
+#pragma unroll 1
+    for (int l = 0; l < PM_L; l++) {
+        uint32_t cnt = 0;
+        for (int i = lane; i < n; i += 64) cnt += pos[l][i] < (uint32_t)c ? 1u : 0u;
+        const int xl = (int)wave_sum(cnt);
+        if (lane == 0) x[l] = (uint32_t)xl;
+        c = 2 * (c - xl);
+    }
 }
 
 // Tokens, one word per block position: a literal byte b is TOK_LIT | b, a
@@ -711,15 +715,13 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                     S.d_len[t - NLIT] = 0;
                 }
             }
-            for (int k = t; k < DF_OUT_WORDS; k += DF_T) S.u.out[k] = 0;
             if (t < NCL) {
                 S.cl_freq[t] = 0;
                 S.cl_len[t] = 0;
             }
             if (t < 8) S.misc[t] = 0;
             __syncthreads();
-            // 5. the two codes: symbols ranked in parallel, the two-queue merges by
-            //    threads 0 and 64 at once, then depths, lengths and codes in parallel
+            // 5. the symbols of both codes ranked by (count, symbol) in parallel
             rank_symbols(S.freq, NLIT, S.sorted, S.sf, &S.misc[0], 0);
             rank_symbols(S.dfreq, NDC, S.dsorted, S.dsf, &S.misc[5], 512);
             if (t < 32) {
@@ -728,32 +730,60 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 S.clblc[t] = 0;
             }
             __syncthreads();
+            // 6. code lengths limited to 15 bits, optimal for the counts: package-merge
+            //    (Larmore and Hirschberg) over the ranked weights, both codes at once
+            //    (threads from 0 and from PM_D), one barrier per level
             const int m = (int)S.misc[0], md = (int)S.misc[5];  // m >= 2: a byte and end of block
-            if (t == 0) huff_tree(S.sf, m, S.nf, S.par);
-            if (t == 64 && md >= 2) huff_tree(S.dsf, md, S.dnf, S.dpar);
-            __syncthreads();
-            if (t < m) atomicAdd(&S.blc[min(leaf_depth(S.par, t, 2 * m - 2), 15)], 1u);
-            if (t >= 512 && t - 512 < md && md >= 2)
-                atomicAdd(&S.dblc[min(leaf_depth(S.dpar, t - 512, 2 * md - 2), 15)], 1u);
-            __syncthreads();
-            if (t == 0) kraft_repair(S.blc, 15);
-            if (t == 64) {
-                if (md >= 2) {
-                    kraft_repair(S.dblc, 15);
-                } else {  // no match: two 1-bit codes; one distance symbol: it and a spare
-                    if (md == 0) {
-                        S.d_len[0] = S.d_len[1] = 1;
-                    } else {
-                        const int s = S.dsorted[0];
-                        S.d_len[s] = 1;
-                        S.d_len[s == 0 ? 1 : 0] = 1;
-                    }
-                    S.dblc[1] = 2;
-                }
+            const bool dtree = md >= 2;
+            if (t < m) {
+                S.u.pm.a[1][t] = S.sf[t];
+                S.u.pm.pos[PM_L - 1][t] = (uint16_t)t;
+            }
+            if (dtree && t >= PM_D && t - PM_D < md) {
+                S.u.pm.da[1][t - PM_D] = S.dsf[t - PM_D];
+                S.u.pm.dpos[PM_L - 1][t - PM_D] = (uint16_t)(t - PM_D);
             }
             __syncthreads();
-            if (t < m) S.lit_len[S.sorted[t]] = rank_length(S.blc, 15, t);
-            if (t >= 512 && t - 512 < md && md >= 2) S.d_len[S.dsorted[t - 512]] = rank_length(S.dblc, 15, t - 512);
+            {
+                int s = m, sd = md, cur = 1;
+#pragma unroll 1
+                for (int l = PM_L - 2; l >= 0; l--) {  // list of level l + 1 from that of level l + 2
+                    pm_level(t, m, s, S.sf, S.u.pm.a[cur], S.u.pm.a[cur ^ 1], S.u.pm.pos[l]);
+                    if (dtree && t >= PM_D)
+                        pm_level(t - PM_D, md, sd, S.dsf, S.u.pm.da[cur], S.u.pm.da[cur ^ 1], S.u.pm.dpos[l]);
+                    __syncthreads();
+                    s = m + (s >> 1);
+                    sd = md + (sd >> 1);
+                    cur ^= 1;
+                }
+            }
+            if (w == 0) pm_select(m, S.u.pm.pos, S.u.pm.x[0], lane);
+            if (w == 1 && dtree) pm_select_d(md, S.u.pm.dpos, S.u.pm.x[1], lane);
+            if (t == 128 && !dtree) {  // no match: two 1-bit codes; one distance symbol: it and a spare
+                if (md == 0) {
+                    S.d_len[0] = S.d_len[1] = 1;
+                } else {
+                    const int s = S.dsorted[0];
+                    S.d_len[s] = 1;
+                    S.d_len[s == 0 ? 1 : 0] = 1;
+                }
+                S.dblc[1] = 2;
+            }
+            __syncthreads();
+            if (t < m) {  // a leaf's length: the levels whose selection holds it
+                int len = 0;
+#pragma unroll 1
+                for (int l = 0; l < PM_L; l++) len += t < (int)S.u.pm.x[0][l] ? 1 : 0;
+                S.lit_len[S.sorted[t]] = (uint8_t)len;
+                atomicAdd(&S.blc[len], 1u);
+            }
+            if (dtree && t >= PM_D && t - PM_D < md) {
+                int len = 0;
+#pragma unroll 1
+                for (int l = 0; l < PM_L; l++) len += t - PM_D < (int)S.u.pm.x[1][l] ? 1 : 0;
+                S.d_len[S.dsorted[t - PM_D]] = (uint8_t)len;
+                atomicAdd(&S.dblc[len], 1u);
+            }
             __syncthreads();
             if (t < NLIT) {
                 S.lit_code[t] = canon_code(S.lit_len, S.blc, t);
@@ -763,6 +793,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 S.d_code[t - 512] = canon_code(S.d_len, S.dblc, t - 512);
                 if (S.d_len[t - 512]) atomicMax(&S.misc[6], (uint32_t)(t - 511));
             }
+            for (int k = t; k < DF_OUT_WORDS; k += DF_T) S.u.out[k] = 0;  // the lists are done
             {  // this wave's body bits: its symbol counts times their code lengths, plus extra bits
                 uint32_t bits = 0;
                 for (int u = lane; u < NSYM; u += 64) {
