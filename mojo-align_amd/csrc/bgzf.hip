@@ -243,11 +243,13 @@ struct DfShared {
     uint32_t wsum[DF_T / 64];
     uint32_t freq[NLIT];
     uint32_t dfreq[NDC];
-    uint8_t lit_len[NLIT];
-    uint16_t lit_code[NLIT];        // bit-reversed canonical codes
-    uint8_t d_len[NDC];
-    uint16_t d_code[NDC];
+    alignas(4) uint8_t lit_len[NLIT];
+    uint32_t lpk[NLIT];             // bit-reversed canonical code | its length << 16
+    uint32_t mpk[65];               // per match length: its length code + extra bits | their count << 24
+    alignas(4) uint8_t d_len[NDC];  // (read four at a time)
+    uint32_t dpk[NDC];              // as lpk, for the distance code
     uint16_t sorted[NLIT];          // used symbols by ascending (freq, symbol)
+    uint32_t rnk[NLIT];             // a literal/length symbol's rank
     uint16_t dsorted[NDC];
     uint32_t nf[NLIT];              // Huffman build scratch (the code-length code)
     uint16_t par[2 * NLIT];
@@ -363,13 +365,17 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
     return v;
 }
 
+// Inclusive wave scan by DPP: shifts of 1, 2, 4, 8 lanes inside each row of 16,
+// then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then the
+// first two rows' total into rows 2 and 3 (row_bcast:31); lanes whose source is
+// outside their row, and rows outside the row mask, add 0.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
-    const int lane = (int)(threadIdx.x & 63);
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(v, o, 64);
-        if (lane >= o) v += y;
-    }
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);
     return v;
 }
 
@@ -400,7 +406,15 @@ __device__ __forceinline__ uint16_t canon_code(const uint8_t *len, const uint32_
     if (l == 0) return 0;
     uint32_t code = 0;
     for (int k = 1; k <= l; k++) code = (code + (k > 1 ? blc[k - 1] : 0)) << 1;
-    for (int u = 0; u < s; u++) code += len[u] == l;
+    // symbols before s with length l, four length bytes per step (zero-byte count of len ^ l)
+    const uint32_t *lw = reinterpret_cast<const uint32_t *>(len);
+    const uint32_t pat = (uint32_t)l * 0x01010101u;
+    int u = 0;
+    for (; u + 4 <= s; u += 4) {
+        const uint32_t x = lw[u >> 2] ^ pat;
+        code += (uint32_t)__popc(~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu));
+    }
+    for (; u < s; u++) code += len[u] == l;
     return (uint16_t)bitrev(code, l);
 }
 
@@ -483,9 +497,9 @@ __device__ __forceinline__ void pm_select(int n, const uint16_t (*pos)[NLIT], ui
     int c = 2 * n - 2;
 #pragma unroll 1
     for (int l = 0; l < PM_L; l++) {
-        uint32_t cnt = 0;
-        for (int i = lane; i < n; i += 64) cnt += pos[l][i] < (uint32_t)c ? 1u : 0u;
-        const int xl = (int)wave_sum(cnt);
+        int xl = 0;
+        for (int i0 = 0; i0 < n; i0 += 64)
+            xl += __popcll(__ballot(i0 + lane < n && pos[l][i0 + lane] < (uint32_t)c));
         if (lane == 0) x[l] = (uint32_t)xl;
         c = 2 * (c - xl);
     }
@@ -495,9 +509,9 @@ __device__ __forceinline__ void pm_select_d(int n, const uint16_t (*pos)[NDC], u
     int c = 2 * n - 2;
 #pragma unroll 1
     for (int l = 0; l < PM_L; l++) {
-        uint32_t cnt = 0;
-        for (int i = lane; i < n; i += 64) cnt += pos[l][i] < (uint32_t)c ? 1u : 0u;
-        const int xl = (int)wave_sum(cnt);
+        int xl = 0;
+        for (int i0 = 0; i0 < n; i0 += 64)
+            xl += __popcll(__ballot(i0 + lane < n && pos[l][i0 + lane] < (uint32_t)c));
         if (lane == 0) x[l] = (uint32_t)xl;
         c = 2 * (c - xl);
     }
@@ -626,6 +640,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                     uint32_t pwin[8];
 #pragma unroll
                     for (int j = 0; j < 8; j++) pwin[j] = __builtin_amdgcn_alignbyte(pw[j + 2], pw[j + 1], ps);
+                    // bit k: byte r0 + k equals the byte before it (a distance-1 run goes on)
+                    const uint32_t prev = ps ? (pw[1] >> (8 * (ps - 1))) & 255u : pw[0] >> 24;
+                    const uint64_t eqp = __ballot(valid && (pwin[0] & 255u) == prev);
                     uint32_t h = 0, c = 0;
                     if (ins) {
                         h = (pwin[0] * 2654435761u) >> (32 - TAB_BITS);
@@ -645,15 +662,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                             best = l;
                             bd = P - q;
                         }
-                        if (P >= 1 && best < maxl) {
-                            // the run: P's bytes equal to byte P - 1 (distance 1)
-                            const uint32_t prev = ps ? (pw[1] >> (8 * (ps - 1))) & 255u : pw[0] >> 24;
-                            uint32_t rwin[8];
-#pragma unroll
-                            for (int j = 0; j < 8; j++) rwin[j] = prev * 0x01010101u;
-                            int l = prefix32(rwin, pwin);
-                            if (l == 32 && maxl > 32) l += match_more(dw, P - 1, P);
-                            l = min(l, maxl);
+                        if (P >= 1 && best < maxl) {  // the run: the set bits of eqp from P on
+                            const uint64_t stop = ~eqp >> lane;
+                            const int l = min(stop ? (int)__builtin_ctzll(stop) : 64 - lane, maxl);
                             if (l > best) {
                                 best = l;
                                 bd = 1;
@@ -710,6 +721,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 if (t < NLIT) {
                     S.freq[t] = f + (t == 256 ? 1u : 0u);  // end of block
                     S.lit_len[t] = 0;
+                    S.rnk[t] = 0;
                 } else {
                     S.dfreq[t - NLIT] = f;
                     S.d_len[t - NLIT] = 0;
@@ -722,8 +734,27 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             if (t < 8) S.misc[t] = 0;
             __syncthreads();
             // 5. the symbols of both codes ranked by (count, symbol) in parallel
-            rank_symbols(S.freq, NLIT, S.sorted, S.sf, &S.misc[0], 0);
-            rank_symbols(S.dfreq, NDC, S.dsorted, S.dsf, &S.misc[5], 512);
+            //    (a literal/length symbol's rank summed over three thirds of the alphabet)
+            if (t < 3 * NLIT) {
+                const int part = t / NLIT, sy = t - part * NLIT;
+                const uint32_t f = S.freq[sy];
+                if (f) {
+                    uint32_t r = 0;
+                    for (int u = part * 96; u < min(NLIT, part * 96 + 96); u++) {
+                        const uint32_t g = S.freq[u];
+                        r += g && (g < f || (g == f && u < sy));
+                    }
+                    atomicAdd(&S.rnk[sy], r);
+                }
+            }
+            rank_symbols(S.dfreq, NDC, S.dsorted, S.dsf, &S.misc[5], 896);
+            __syncthreads();
+            if (t < NLIT && S.freq[t]) {
+                const uint32_t r = S.rnk[t];
+                S.sorted[r] = (uint16_t)t;
+                S.sf[r] = S.freq[t];
+                atomicAdd(&S.misc[0], 1u);
+            }
             if (t < 32) {
                 S.blc[t] = 0;
                 S.dblc[t] = 0;
@@ -786,11 +817,11 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
             __syncthreads();
             if (t < NLIT) {
-                S.lit_code[t] = canon_code(S.lit_len, S.blc, t);
+                S.lpk[t] = canon_code(S.lit_len, S.blc, t) | (uint32_t)S.lit_len[t] << 16;
                 if (S.lit_len[t]) atomicMax(&S.misc[2], (uint32_t)(t + 1));
             }
             if (t >= 512 && t - 512 < NDC) {
-                S.d_code[t - 512] = canon_code(S.d_len, S.dblc, t - 512);
+                S.dpk[t - 512] = canon_code(S.d_len, S.dblc, t - 512) | (uint32_t)S.d_len[t - 512] << 16;
                 if (S.d_len[t - 512]) atomicMax(&S.misc[6], (uint32_t)(t - 511));
             }
             for (int k = t; k < DF_OUT_WORDS; k += DF_T) S.u.out[k] = 0;  // the lists are done
@@ -804,6 +835,13 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 if (lane == 0) S.wbits[w] = bits + S.wextra[w];
             }
             __syncthreads();
+            if (t >= DF_T - 64 && t - (DF_T - 64) + MIN_MATCH <= 64) {  // the match lengths' codes
+                const uint32_t L = (uint32_t)(t - (DF_T - 64) + MIN_MATCH);
+                uint32_t ls, le, lx;
+                len_code(L, ls, le, lx);
+                const uint32_t lk = S.lpk[ls], ll = lk >> 16;
+                S.mpk[L] = ((lk & 0xFFFFu) | lx << ll) | (ll + le) << 24;
+            }
             // code-length sequence (literal/length then distance lengths): maximal runs
             // found in parallel, each coded at its scanned offset
             const int hlit = max(257, (int)S.misc[2]), hdist = max(1, (int)S.misc[6]);
@@ -864,7 +902,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 if (t == 0) {
                     or_bits(S.u.out, 0, 1u | 2u << 1 | (uint32_t)(hlit - 257) << 3 | (uint32_t)(hdist - 1) << 8 |
                                             (hclen - 4) << 13, 17);
-                    or_bits(S.u.out, hb + dbits, S.lit_code[256], S.lit_len[256]);
+                    or_bits(S.u.out, hb + dbits, S.lpk[256] & 0xFFFFu, (int)(S.lpk[256] >> 16));
                 }
                 if ((uint32_t)t < hclen) or_bits(S.u.out, 17 + 3 * t, S.cl_len[cl_order(t)], 3);
                 or_bits(S.u.out, 17 + 3 * hclen + hoff, hv, (int)hpart);
@@ -891,18 +929,18 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                         const uint32_t x = P0 + k < sb1 ? tk[k] : 0u;
                         uint32_t v1 = 0, v2 = 0, b1 = 0, b2 = 0;
                         if (x & TOK_MATCH) {
-                            uint32_t ls, le, lx, ds, de, dx;
-                            len_code((x >> 16) & 0x7FFF, ls, le, lx);
+                            const uint32_t mk = S.mpk[(x >> 16) & 0x7FFF];
+                            uint32_t ds, de, dx;
                             dist_code(x & 0xFFFF, ds, de, dx);
-                            b1 = S.lit_len[ls];
-                            v1 = S.lit_code[ls] | lx << b1;
-                            b1 += le;
-                            b2 = S.d_len[ds];
-                            v2 = S.d_code[ds] | dx << b2;
-                            b2 += de;
+                            const uint32_t dk = S.dpk[ds];
+                            v1 = mk & 0xFFFFFFu;
+                            b1 = mk >> 24;
+                            v2 = (dk & 0xFFFFu) | dx << (dk >> 16);
+                            b2 = (dk >> 16) + de;
                         } else if (x & TOK_LIT) {
-                            b1 = S.lit_len[x & 255];
-                            v1 = S.lit_code[x & 255];
+                            const uint32_t lk = S.lpk[x & 255];
+                            v1 = lk & 0xFFFFu;
+                            b1 = lk >> 16;
                         }
                         c1[k] = v1 | b1 << 20 | b2 << 25;
                         c2[k] = v2;
