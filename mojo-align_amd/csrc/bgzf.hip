@@ -225,8 +225,8 @@ struct DfShared {
         struct {                            // then the package-merge lists of both codes
             uint32_t a[2][2 * NLIT];        // weights of a list in merged order (two levels)
             uint32_t da[2][2 * NDC];
-            uint16_t pos[PM_L][NLIT];       // per level: each leaf's place in the merged list
-            uint16_t dpos[PM_L][NDC];
+            uint16_t lc[PM_L][2 * NLIT + 1];  // per level and place p: leaves among the first p items
+            uint16_t dlc[PM_L][2 * NDC + 1];
             uint32_t x[2][PM_L + 1];        // per level: leaves among the selected items
         } pm;
         uint32_t out[DF_OUT_WORDS];         // then the deflate body, assembled by OR
@@ -250,6 +250,8 @@ struct DfShared {
     uint32_t dpk[NDC];              // as lpk, for the distance code
     uint16_t sorted[NLIT];          // used symbols by ascending (freq, symbol)
     uint32_t rnk[NLIT];             // a literal/length symbol's rank
+    uint32_t key[NLIT];             // count << 9 | symbol, all ones for an unused symbol
+    uint64_t rstart[(NSEQ + 63) / 64];  // code-length sequence: where runs of equal lengths start
     uint16_t dsorted[NDC];
     uint32_t nf[NLIT];              // Huffman build scratch (the code-length code)
     uint16_t par[2 * NLIT];
@@ -471,48 +473,38 @@ __device__ __forceinline__ int search_first(int n, uint32_t v, F at) {
 // One level of package-merge for a code of n leaves with ascending weights w[]:
 // list A (size s, ascending) becomes the next level's list B = merge(leaves,
 // packages of consecutive pairs of A), a leaf before a package of equal weight;
-// pos[] gets each leaf's place in B. Thread e < n + s / 2 places element e.
+// lc[p] gets the number of leaves among B's first p items (p <= |B|). Thread
+// e < n + s / 2 places element e.
 __device__ __forceinline__ void pm_level(int e, int n, int s, const uint32_t *w, const uint32_t *A, uint32_t *B,
-                                         uint16_t *pos) {
+                                         uint16_t *lc) {
     const int np = s >> 1;
     if (e >= n + np) return;
     const uint2 *P = reinterpret_cast<const uint2 *>(A);
+    if (e == 0) lc[n + np] = (uint16_t)n;
     if (e < n) {
         const uint32_t v = w[e];
         const int at = e + search_first<false>(np, v, [&](int k) { const uint2 q = P[k]; return q.x + q.y; });
         B[at] = v;
-        pos[e] = (uint16_t)at;
+        lc[at] = (uint16_t)e;
     } else {
         const int k = e - n;
         const uint2 q = P[k];
         const uint32_t v = q.x + q.y;
-        B[k + search_first<true>(n, v, [&](int i) { return w[i]; })] = v;
+        const int c = search_first<true>(n, v, [&](int i) { return w[i]; });
+        B[k + c] = v;
+        lc[k + c] = (uint16_t)c;
     }
 }
 
-// Leaves among the first c items of each level's list, level 1 first (a wave):
+// Leaves x_l among the first c_l items of each level's list, level 1 first:
 // c_1 = 2n - 2 items, and the packages among a level's first c_l items are the
-// first c_(l+1) = 2 (c_l - x_l) items of the level below.
-__device__ __forceinline__ void pm_select(int n, const uint16_t (*pos)[NLIT], uint32_t *x, int lane) {
+// first c_(l+1) = 2 (c_l - x_l) items of the level below. One thread.
+template <int W>
+__device__ __forceinline__ void pm_select(int n, const uint16_t (*lc)[W], uint32_t *x) {
     int c = 2 * n - 2;
-#pragma unroll 1
     for (int l = 0; l < PM_L; l++) {
-        int xl = 0;
-        for (int i0 = 0; i0 < n; i0 += 64)
-            xl += __popcll(__ballot(i0 + lane < n && pos[l][i0 + lane] < (uint32_t)c));
-        if (lane == 0) x[l] = (uint32_t)xl;
-        c = 2 * (c - xl);
-    }
-}
-
-__device__ __forceinline__ void pm_select_d(int n, const uint16_t (*pos)[NDC], uint32_t *x, int lane) {
-    int c = 2 * n - 2;
-#pragma unroll 1
-    for (int l = 0; l < PM_L; l++) {
-        int xl = 0;
-        for (int i0 = 0; i0 < n; i0 += 64)
-            xl += __popcll(__ballot(i0 + lane < n && pos[l][i0 + lane] < (uint32_t)c));
-        if (lane == 0) x[l] = (uint32_t)xl;
+        const int xl = lc[l][c];
+        x[l] = (uint32_t)xl;
         c = 2 * (c - xl);
     }
 }
@@ -719,7 +711,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 uint32_t f = 0;
                 for (int v = 0; v < DF_W; v++) f += (S.hist[v][t >> 1] >> (16 * (t & 1))) & 0xFFFFu;
                 if (t < NLIT) {
-                    S.freq[t] = f + (t == 256 ? 1u : 0u);  // end of block
+                    f += t == 256 ? 1u : 0u;  // end of block
+                    S.freq[t] = f;
+                    S.key[t] = f ? f << 9 | (uint32_t)t : 0xFFFFFFFFu;
                     S.lit_len[t] = 0;
                     S.rnk[t] = 0;
                 } else {
@@ -737,13 +731,12 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             //    (a literal/length symbol's rank summed over three thirds of the alphabet)
             if (t < 3 * NLIT) {
                 const int part = t / NLIT, sy = t - part * NLIT;
-                const uint32_t f = S.freq[sy];
-                if (f) {
+                const uint32_t ks = S.key[sy];
+                if (ks != 0xFFFFFFFFu) {
                     uint32_t r = 0;
-                    for (int u = part * 96; u < min(NLIT, part * 96 + 96); u++) {
-                        const uint32_t g = S.freq[u];
-                        r += g && (g < f || (g == f && u < sy));
-                    }
+                    const int u1 = min(NLIT, part * 96 + 96);
+#pragma unroll 8
+                    for (int u = part * 96; u < u1; u++) r += S.key[u] < ks ? 1u : 0u;
                     atomicAdd(&S.rnk[sy], r);
                 }
             }
@@ -766,30 +759,30 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             //    (threads from 0 and from PM_D), one barrier per level
             const int m = (int)S.misc[0], md = (int)S.misc[5];  // m >= 2: a byte and end of block
             const bool dtree = md >= 2;
-            if (t < m) {
-                S.u.pm.a[1][t] = S.sf[t];
-                S.u.pm.pos[PM_L - 1][t] = (uint16_t)t;
+            if (t <= m) {  // the deepest level: the leaves alone
+                if (t < m) S.u.pm.a[1][t] = S.sf[t];
+                S.u.pm.lc[PM_L - 1][t] = (uint16_t)t;
             }
-            if (dtree && t >= PM_D && t - PM_D < md) {
-                S.u.pm.da[1][t - PM_D] = S.dsf[t - PM_D];
-                S.u.pm.dpos[PM_L - 1][t - PM_D] = (uint16_t)(t - PM_D);
+            if (dtree && t >= PM_D && t - PM_D <= md) {
+                if (t - PM_D < md) S.u.pm.da[1][t - PM_D] = S.dsf[t - PM_D];
+                S.u.pm.dlc[PM_L - 1][t - PM_D] = (uint16_t)(t - PM_D);
             }
             __syncthreads();
             {
                 int s = m, sd = md, cur = 1;
 #pragma unroll 1
                 for (int l = PM_L - 2; l >= 0; l--) {  // list of level l + 1 from that of level l + 2
-                    pm_level(t, m, s, S.sf, S.u.pm.a[cur], S.u.pm.a[cur ^ 1], S.u.pm.pos[l]);
+                    pm_level(t, m, s, S.sf, S.u.pm.a[cur], S.u.pm.a[cur ^ 1], S.u.pm.lc[l]);
                     if (dtree && t >= PM_D)
-                        pm_level(t - PM_D, md, sd, S.dsf, S.u.pm.da[cur], S.u.pm.da[cur ^ 1], S.u.pm.dpos[l]);
+                        pm_level(t - PM_D, md, sd, S.dsf, S.u.pm.da[cur], S.u.pm.da[cur ^ 1], S.u.pm.dlc[l]);
                     __syncthreads();
                     s = m + (s >> 1);
                     sd = md + (sd >> 1);
                     cur ^= 1;
                 }
             }
-            if (w == 0) pm_select(m, S.u.pm.pos, S.u.pm.x[0], lane);
-            if (w == 1 && dtree) pm_select_d(md, S.u.pm.dpos, S.u.pm.x[1], lane);
+            if (t == 0) pm_select(m, S.u.pm.lc, S.u.pm.x[0]);
+            if (t == 64 && dtree) pm_select(md, S.u.pm.dlc, S.u.pm.x[1]);
             if (t == 128 && !dtree) {  // no match: two 1-bit codes; one distance symbol: it and a spare
                 if (md == 0) {
                     S.d_len[0] = S.d_len[1] = 1;
@@ -848,14 +841,24 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             const int total = hlit + hdist;
             int rv = 0, run = 0;
             uint32_t cnt = 0;
+            bool start = false;
             if (t < total) {
                 rv = t < hlit ? S.lit_len[t] : S.d_len[t - hlit];
                 const int pv = t == 0 ? -1 : (t - 1 < hlit ? S.lit_len[t - 1] : S.d_len[t - 1 - hlit]);
-                if (rv != pv) {
-                    run = 1;
-                    while (t + run < total && (t + run < hlit ? S.lit_len[t + run] : S.d_len[t + run - hlit]) == rv) run++;
-                    cnt = (uint32_t)rle_emit(rv, run, nullptr, nullptr);
-                }
+                start = rv != pv;
+            }
+            {
+                const uint64_t sm = __ballot(start);
+                if (w < (NSEQ + 63) / 64 && lane == 0) S.rstart[w] = sm;
+            }
+            __syncthreads();
+            if (start) {  // a run ends where the next one starts
+                int wi = t >> 6;
+                uint64_t rest = S.rstart[wi] & ~((2ull << (t & 63)) - 1);
+                while (!rest && ++wi < (NSEQ + 63) / 64) rest = S.rstart[wi];
+                const int next = rest ? 64 * wi + (int)__builtin_ctzll(rest) : total;
+                run = min(next, total) - t;
+                cnt = (uint32_t)rle_emit(rv, run, nullptr, nullptr);
             }
             uint32_t ns;
             const uint32_t co = block_excl_scan_1024(cnt, S.wsum, ns);
