@@ -83,83 +83,6 @@ __device__ __forceinline__ uint32_t x8n(const uint32_t *xpow, uint32_t len) {
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t c, int len) { return __brev(c) >> (32 - len); }
 
-// Code lengths (<= maxlen) for the m used symbols sorted[0..m) (ascending
-// (freq, symbol)); lengths of unused symbols stay 0. m >= 1. A single used
-// symbol gets length 1 and a second symbol `spare` length 1 too, so every
-// code is complete (inflate rejects incomplete code-length codes).
-__device__ void huff_lengths(const uint32_t *freq, const uint16_t *sorted, int m, int maxlen, int spare,
-                             uint8_t *len, uint32_t *nf, uint16_t *par, uint8_t *dep, uint32_t *blc) {
-    if (m == 1) {
-        len[sorted[0]] = 1;
-        len[sorted[0] == spare ? spare + 1 : spare] = 1;
-        return;
-    }
-    // two-queue Huffman: leaves 0..m-1 in sorted order, internal node k has id m + k
-    int i = 0, j = 0;
-    for (int k = 0; k < m - 1; k++) {
-        uint32_t f = 0;
-        for (int pick = 0; pick < 2; pick++) {
-            const bool leaf = i < m && (j >= k || freq[sorted[i]] <= nf[j]);
-            if (leaf) {
-                par[i] = (uint16_t)(m + k);
-                f += freq[sorted[i]];
-                i++;
-            } else {
-                par[m + j] = (uint16_t)(m + k);
-                f += nf[j];
-                j++;
-            }
-        }
-        nf[k] = f;
-    }
-    dep[2 * m - 2] = 0;
-    for (int node = 2 * m - 3; node >= 0; node--) {
-        const int d = dep[par[node]] + 1;
-        dep[node] = (uint8_t)(d > 255 ? 255 : d);
-    }
-    for (int l = 0; l < 32; l++) blc[l] = 0;
-    for (int s = 0; s < m; s++) blc[dep[s] > maxlen ? maxlen : dep[s]]++;
-    // Kraft repair after clamping: sum 2^(maxlen - l) over leaves must equal 2^maxlen
-    uint32_t kraft = 0;
-    for (int l = 1; l <= maxlen; l++) kraft += blc[l] << (maxlen - l);
-    const uint32_t target = 1u << maxlen;
-    // (the inner searches stop at length 1: every step below is bounded)
-    while (kraft > target) {  // lengthen the deepest code that can still grow
-        int l = maxlen - 1;
-        while (l > 1 && blc[l] == 0) l--;
-        if (blc[l] == 0) break;
-        blc[l]--;
-        blc[l + 1]++;
-        kraft -= 1u << (maxlen - l - 1);
-    }
-    while (kraft < target) {  // shorten the deepest code: its 2^(maxlen-l) divides the deficit
-        int l = maxlen;
-        while (l > 2 && blc[l] == 0) l--;
-        if (blc[l] == 0) break;
-        blc[l]--;
-        blc[l - 1]++;
-        kraft += 1u << (maxlen - l);
-    }
-    int s = 0;  // least frequent symbols take the longest codes
-    for (int l = maxlen; l >= 1; l--)
-        for (uint32_t c = 0; c < blc[l]; c++) len[sorted[s++]] = (uint8_t)l;
-}
-
-// Canonical codes (bit-reversed for LSB-first output) from lengths; blc has
-// 32 entries: counts in 0..15, next codes in 16..31 (shared memory).
-__device__ void canonical(const uint8_t *len, int nsym, int maxlen, uint16_t *code, uint32_t *blc) {
-    for (int l = 0; l < 32; l++) blc[l] = 0;
-    for (int s = 0; s < nsym; s++) blc[len[s]]++;
-    blc[0] = 0;
-    uint32_t c = 0;
-    for (int l = 1; l <= maxlen; l++) {
-        c = (c + blc[l - 1]) << 1;
-        blc[16 + l] = c;
-    }
-    for (int s = 0; s < nsym; s++)
-        if (len[s]) code[s] = (uint16_t)bitrev(blc[16 + len[s]]++, len[s]);
-}
-
 // LSB-first bit writer into the shared buffer: words strictly inside the
 // writer's range are stored, the first and last (possibly shared) are ORed.
 struct BitWriter {
@@ -217,6 +140,7 @@ constexpr int NSYM = NLIT + NDC;     // literal/length and distance symbols, cou
 constexpr int HW = NSYM / 2;         // per-wave counters: two 16-bit counts per word
 constexpr int PM_L = 15;             // code length limit of the literal/length and distance codes
 constexpr int PM_D = 640;            // first thread of the distance code's package-merge
+constexpr int CL_L = 7;              // code length limit of the code-length code
 
 struct DfShared {
     uint32_t data[1 + DF_IN / 4 + 24];  // a guard word, the block, padding: every window read stays inside
@@ -253,20 +177,20 @@ struct DfShared {
     uint32_t key[NLIT];             // count << 9 | symbol, all ones for an unused symbol
     uint64_t rstart[(NSEQ + 63) / 64];  // code-length sequence: where runs of equal lengths start
     uint16_t dsorted[NDC];
-    uint32_t nf[NLIT];              // Huffman build scratch (the code-length code)
-    uint16_t par[2 * NLIT];
-    uint8_t dep[2 * NLIT];
     uint32_t dblc[32];
     uint8_t cl_sym[NSEQ];
     uint8_t cl_extra[NSEQ];
     uint32_t cl_freq[NCL];
-    uint8_t cl_len[NCL];
+    alignas(4) uint8_t cl_len[NCL];
+    alignas(8) uint32_t cla[2][2 * NCL];    // the code-length code's package-merge lists
+    uint16_t cllc[CL_L][2 * NCL + 1];
+    uint32_t clx[CL_L];
     uint16_t cl_code[NCL];
     uint16_t cl_sorted[NCL];
     uint32_t blc[32];
-    uint32_t misc[8];               // 0 used lit/len symbols, 1 CL entries, 2 last used lit/len symbol + 1,
-                                    // 3 hclen, 5 used distance symbols, 6 last used distance symbol + 1,
-                                    // 7 used CL symbols
+    uint32_t misc[8];               // 0 used lit/len symbols, 2 last used lit/len symbol + 1,
+                                    // 3 last sent code-length code length + 1, 5 used distance
+                                    // symbols, 6 last used distance symbol + 1, 7 used CL symbols
 };
 
 // Length symbol (257..285) and extra bits of a match length 3..258 (RFC 1951 3.2.5).
@@ -365,6 +289,14 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// Orders a wave's LDS writes before its later reads by other lanes.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Inclusive wave scan by DPP: shifts of 1, 2, 4, 8 lanes inside each row of 16,
@@ -868,13 +800,49 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             __syncthreads();
             rank_symbols(S.cl_freq, NCL, S.cl_sorted, S.sf, &S.misc[7], 0);
             __syncthreads();
-            if (t == 0) {
-                huff_lengths(S.cl_freq, S.cl_sorted, (int)S.misc[7], 7, 0, S.cl_len, S.nf, S.par, S.dep, S.clblc);
-                canonical(S.cl_len, NCL, 7, S.cl_code, S.clblc);
-                int hclen = NCL;
-                while (hclen > 4 && S.cl_len[cl_order(hclen - 1)] == 0) hclen--;
-                S.misc[1] = ns;
-                S.misc[3] = (uint32_t)hclen;
+            if (w == 0) {  // the code-length code: package-merge limited to 7 bits, one wave
+                const int nc = (int)S.misc[7];
+                if (nc == 1) {  // one symbol: it and a spare, both 1 bit
+                    if (lane == 0) {
+                        const int sy = S.cl_sorted[0];
+                        S.cl_len[sy] = 1;
+                        S.cl_len[sy == 0 ? 1 : 0] = 1;
+                        S.clblc[1] = 2;
+                    }
+                } else {
+                    if (lane <= nc) {
+                        if (lane < nc) S.cla[1][lane] = S.sf[lane];
+                        S.cllc[CL_L - 1][lane] = (uint16_t)lane;
+                    }
+                    wsync();
+                    int sz = nc, cur = 1;
+                    for (int l = CL_L - 2; l >= 0; l--) {
+                        pm_level(lane, nc, sz, S.sf, S.cla[cur], S.cla[cur ^ 1], S.cllc[l]);
+                        wsync();
+                        sz = nc + (sz >> 1);
+                        cur ^= 1;
+                    }
+                    if (lane == 0) {
+                        int c = 2 * nc - 2;
+                        for (int l = 0; l < CL_L; l++) {
+                            const int xl = S.cllc[l][c];
+                            S.clx[l] = (uint32_t)xl;
+                            c = 2 * (c - xl);
+                        }
+                    }
+                    wsync();
+                    if (lane < nc) {
+                        int len = 0;
+                        for (int l = 0; l < CL_L; l++) len += lane < (int)S.clx[l] ? 1 : 0;
+                        S.cl_len[S.cl_sorted[lane]] = (uint8_t)len;
+                        atomicAdd(&S.clblc[len], 1u);
+                    }
+                }
+            }
+            __syncthreads();
+            if (t < NCL) {
+                S.cl_code[t] = canon_code(S.cl_len, S.clblc, t);
+                if (S.cl_len[cl_order(t)]) atomicMax(&S.misc[3], (uint32_t)(t + 1));
             }
             __syncthreads();
             // the code-length symbols' bits (one symbol per thread) and their offsets
@@ -887,7 +855,7 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
             }
             uint32_t hcl;
             const uint32_t hoff = block_excl_scan_1024(hpart, S.wsum, hcl);
-            const uint32_t hclen = S.misc[3];
+            const uint32_t hclen = max(4u, S.misc[3]);
             const uint32_t hb = 3 + 5 + 5 + 4 + 3 * hclen + hcl;
             // 6. bit offsets of the sub-blocks
             uint32_t dbits = 0, wpre = 0;
