@@ -13,21 +13,25 @@
 //     with its own 1024-entry hash table in LDS, 64 positions (a segment) per
 //     round: a position's candidate is the latest position with its 4-byte
 //     hash from an earlier round (the table is seeded with the previous
-//     sub-block's last 512 positions), checked 4 bytes at a time, and the
-//     distance-1 run; matches of 4..64 bytes stay inside the segment, which is
-//     parsed greedily in scalar code over the lanes' match masks; one token
-//     word per position goes to a scratch, and each wave counts its symbols
-//     (two 16-bit counters per LDS word);
-//   * one dynamic-Huffman block (RFC 1951 3.2.7) per member: literal/length
-//     and distance code lengths optimal under the 15-bit limit by a parallel
-//     package-merge (both codes at once, one barrier per level); the
-//     code-length code by a two-queue Huffman build limited to 7 bits by
-//     Kraft-sum repair; canonical codes; a stored block when that is not
-//     smaller;
+//     sub-block's last 512 positions), compared 4 bytes at a time from windows
+//     read once per round; the distance-1 run's length comes from a ballot of
+//     "byte equals the one before"; matches of 4..64 bytes stay inside the
+//     segment, which is parsed greedily in scalar code over the lanes' match
+//     masks; one token word per position goes to a scratch, and each wave
+//     counts its symbols (two 16-bit counters per LDS word);
+//   * one dynamic-Huffman block (RFC 1951 3.2.7) per member, every code's
+//     lengths optimal under its limit by package-merge (Larmore and
+//     Hirschberg): literal/length and distance codes (15 bits) at once across
+//     the workgroup, one barrier per level, over symbols ranked by packed
+//     (count, symbol) keys; the code-length code (7 bits) in one wave; the
+//     code-length sequence's runs from a ballot of run starts; canonical codes
+//     in parallel; a stored block when that is not smaller;
 //   * each wave's body bits follow from its symbol counts and the code
 //     lengths, so the sub-blocks' bit offsets need no second pass; the body is
-//     assembled in LDS (where the hash tables were) by OR of every token's
-//     bits at its offset, then copied to the block's output slot;
+//     assembled in LDS (where the hash tables were) by OR: the header's fields
+//     and code-length symbols at offsets from a block scan, every token's bits
+//     at offsets from a DPP wave scan (each lane writes 8 consecutive tokens);
+//     then it is copied to the block's output slot;
 //   * CRC32: per-thread CRCs of 64-byte segments (slicing by 4) joined by the
 //     GF(2) shift crc(A||B) = crc(A) * x^(8|B|) + crc(B) mod P;
 //   * a second kernel writes every member (18-byte header, body, trailer) at
