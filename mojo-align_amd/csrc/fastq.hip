@@ -14,9 +14,11 @@
 // thread): one sweep counts the tile's newlines, a scan gives every tile its
 // first line number, a second sweep writes each newline's position at its line
 // number; one thread then settles the record count, the bytes consumed and the
-// status; a wave per record measures its four lines and name; a scan of the
-// name lengths places the names; a wave per record copies bases, converted
-// bases, qualities and the name.
+// status; a wave per record measures its four lines and name and writes its
+// base, converted-base and quality rows (4-byte words gathered from the text,
+// converted four bytes at a time, stored whole except where a word is shared
+// with the neighbouring row); a scan of the name lengths places the names,
+// which a last pass copies.
 #include "ma_sort.hpp"
 
 using namespace ma;
@@ -155,37 +157,140 @@ __global__ void fq_settle_kernel(const uint8_t *__restrict__ buf, int64_t n, int
     info[7] = ~0ull;
 }
 
-// A wave per record: lengths and name span (the first ' ' or '/' found by a
-// ballot over 64 header bytes at a time); the first read longer than stride.
-__global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
-                                                         int64_t max_records, int64_t stride,
-                                                         uint64_t *__restrict__ info, uint32_t *__restrict__ lens,
-                                                         uint32_t *__restrict__ qual_lens,
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {  // 0x80 in each zero byte of x
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+}
+
+// The 4 text bytes at p (p + 4 <= n): the aligned word holding p and, when p
+// is not aligned, the next one (which holds byte p + 3, so both are inside).
+__device__ __forceinline__ uint32_t text4(const uint8_t *buf, int64_t p) {
+    const uintptr_t a = (uintptr_t)(buf + p);
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+    const uint32_t s = (uint32_t)(a & 3);
+    return s ? __builtin_amdgcn_alignbyte(w[1], w[0], s) : w[0];
+}
+
+// Row bytes x0 .. x0 + 3 of a line [ls, ls + len) padded with `pad`: text
+// bytes where x < len, pad bytes after.
+__device__ __forceinline__ uint32_t row4(const uint8_t *buf, int64_t n, int64_t ls, int64_t len, int64_t x0,
+                                         uint32_t pad) {
+    const int64_t v = len - x0;  // text bytes among the four
+    if (v <= 0) return pad;
+    uint32_t w;
+    if (ls + x0 >= 0 && ls + x0 + 4 <= n) {
+        w = text4(buf, ls + x0);
+    } else {
+        w = 0;
+        for (int j = 0; j < 4; j++) {
+            const int64_t q = ls + x0 + j;
+            if (q >= 0 && q < n) w |= (uint32_t)buf[q] << (8 * j);
+        }
+    }
+    if (v >= 4) return w;
+    const uint32_t keep = (1u << (8 * v)) - 1;
+    return (w & keep) | (pad & ~keep);
+}
+
+// C/c -> T (mode 1) or G/g -> A (mode 2) in four bytes at once.
+__device__ __forceinline__ uint32_t cvt4(uint32_t w, int mode) {
+    if (mode == 0) return w;
+    const uint32_t up = mode == 1 ? 0x43434343u : 0x47474747u, to = mode == 1 ? 0x54545454u : 0x41414141u;
+    const uint32_t hit = zero_bytes(w ^ up) | zero_bytes(w ^ (up | 0x20202020u));
+    const uint32_t m = (hit >> 7) * 0xFFu;
+    return (w & ~m) | (to & m);
+}
+
+// Orders a wave's LDS writes before its later reads by other lanes.
+__device__ __forceinline__ void wsync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int FQ_CHUNK = 64;  // records per wave step (one per lane while measuring)
+
+struct FqLds {
+    int64_t src[4][FQ_CHUNK];  // per wave: the chunk's line starts (bases, then qualities)
+    int32_t len[4][FQ_CHUNK];
+    int64_t off[4][FQ_CHUNK + 1];  // name offsets
+};
+
+// The chunk's rows of one kind, `stride` bytes each from row0 on (any
+// alignment): the lanes take (record, word) slots, a record's slots covering
+// the aligned words of its row; a word a row shares with a neighbouring row is
+// written by bytes, every other word whole.
+__device__ __forceinline__ void put_rows(uint8_t *row0, int64_t stride, int nrec, const uint8_t *buf, int64_t n,
+                                         const int64_t *src, const int32_t *len, uint32_t pad, int mode, int lane) {
+    const int W = (int)((stride + 6) >> 2) + 1;  // aligned words a row can touch, at most
+    int j = lane / W, k = lane - j * W;
+#pragma unroll 2
+    for (int slot = lane; slot < nrec * W; slot += 64) {
+        uint8_t *dst = row0 + (int64_t)j * stride;
+        const uintptr_t a0 = (uintptr_t)dst;
+        const uintptr_t wa = (a0 & ~(uintptr_t)3) + 4 * (uintptr_t)k;
+        const int64_t x0 = (int64_t)(wa - a0);  // row offset of the word's first byte
+        if (x0 < stride) {
+            const uint32_t v = cvt4(row4(buf, n, src[j], len[j], x0, pad), mode);
+            if (x0 >= 0 && x0 + 4 <= stride) {
+                *reinterpret_cast<uint32_t *>(wa) = v;
+            } else {
+                for (int q = 0; q < 4; q++)
+                    if (x0 + q >= 0 && x0 + q < stride) dst[x0 + q] = (uint8_t)(v >> (8 * q));
+            }
+        }
+        k += 64;
+        while (k >= W) {
+            k -= W;
+            j++;
+        }
+    }
+}
+
+// Chunks of 64 records per wave. A lane per record: line spans, lengths and
+// the name span (the first ' ' or '/' of the header, four bytes at a time); the
+// first read longer than stride. Then the wave writes the chunk's base,
+// converted-base and quality rows.
+__global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restrict__ buf, int64_t n,
+                                                         const uint64_t *__restrict__ nl, int64_t max_records,
+                                                         int64_t stride, int mode, uint64_t *__restrict__ info,
+                                                         uint32_t *__restrict__ lens, uint32_t *__restrict__ qual_lens,
                                                          uint32_t *__restrict__ name_len,
-                                                         uint64_t *__restrict__ name_src) {
+                                                         uint64_t *__restrict__ name_src, uint8_t *__restrict__ rows,
+                                                         uint8_t *__restrict__ conv_rows,
+                                                         uint8_t *__restrict__ qual_rows) {
+    __shared__ FqLds S;
     const int64_t recs = (int64_t)info[0];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
     const int64_t n_waves = (int64_t)gridDim.x * 4;
     // records past the batch: name length 0 (the offset scan reads them)
     for (int64_t r = recs + wave * 64 + lane; r < max_records; r += n_waves * 64) name_len[r] = 0;
-    for (int64_t r = wave; r < recs; r += n_waves) {
-        int64_t hs, he, ss, se, qs, qe;
-        line_span(buf, nl, 4 * r, hs, he);
-        line_span(buf, nl, 4 * r + 1, ss, se);
-        line_span(buf, nl, 4 * r + 3, qs, qe);
-        const int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs;
-        int64_t e0 = he;
-        for (int64_t c = s0; c < he; c += 64) {
-            const int64_t p = c + lane;
-            const bool stop = p < he && (buf[p] == ' ' || buf[p] == '/');
-            const uint64_t m = __ballot(stop);
-            if (m) {
-                e0 = c + __builtin_ctzll(m);
-                break;
+    for (int64_t c0 = wave * FQ_CHUNK; c0 < recs; c0 += n_waves * FQ_CHUNK) {
+        const int nrec = (int)(recs - c0 < FQ_CHUNK ? recs - c0 : FQ_CHUNK);
+        const int64_t r = c0 + lane;
+        int64_t ss = 0, se = 0, qs = 0, qe = 0;
+        if (lane < nrec) {
+            int64_t hs, he;
+            line_span(buf, nl, 4 * r, hs, he);
+            line_span(buf, nl, 4 * r + 1, ss, se);
+            line_span(buf, nl, 4 * r + 3, qs, qe);
+            const int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs;
+            int64_t e0 = he;
+            for (int64_t q = s0; q < he; q += 4) {
+                uint32_t w = 0;
+                if (q + 4 <= n) {
+                    w = text4(buf, q);
+                } else {
+                    for (int j = 0; j < 4 && q + j < n; j++) w |= (uint32_t)buf[q + j] << (8 * j);
+                }
+                uint32_t hit = zero_bytes(w ^ 0x20202020u) | zero_bytes(w ^ 0x2F2F2F2Fu);
+                if (he - q < 4) hit &= (1u << (8 * (he - q))) - 1;
+                if (hit) {
+                    e0 = q + (__builtin_ctz(hit) >> 3);
+                    break;
+                }
             }
-        }
-        if (lane == 0) {
             const int64_t slen = se - ss;
             lens[r] = (uint32_t)slen;
             qual_lens[r] = (uint32_t)(qe - qs);
@@ -193,44 +298,55 @@ __global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restri
             name_src[r] = (uint64_t)s0;
             if (slen > stride) atomicMin((unsigned long long *)&info[7], (unsigned long long)r);
         }
+        S.src[wv][lane] = ss;
+        S.len[wv][lane] = (int32_t)(se - ss);
+        wsync();
+        put_rows(rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, 0, lane);
+        if (conv_rows)
+            put_rows(conv_rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, mode, lane);
+        wsync();
+        S.src[wv][lane] = qs;
+        S.len[wv][lane] = (int32_t)(qe - qs);
+        wsync();
+        if (qual_rows) put_rows(qual_rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0u, 0, lane);
+        wsync();
     }
 }
 
-__device__ __forceinline__ uint8_t cvt(uint8_t c, int mode) {
-    if (mode == 1) return (c == 'C' || c == 'c') ? 'T' : c;
-    if (mode == 2) return (c == 'G' || c == 'g') ? 'A' : c;
-    return c;
-}
-
-// A wave per record: bases, converted bases and qualities into their rows,
-// the name at its offset.
-__global__ __launch_bounds__(256) void fq_copy_kernel(const uint8_t *__restrict__ buf, const uint64_t *__restrict__ nl,
-                                                      int64_t stride, int mode, const uint64_t *__restrict__ info,
-                                                      const uint32_t *__restrict__ name_len,
-                                                      const uint64_t *__restrict__ name_src,
-                                                      const int64_t *__restrict__ name_off, int64_t names_cap,
-                                                      uint8_t *__restrict__ rows, uint8_t *__restrict__ conv_rows,
-                                                      uint8_t *__restrict__ qual_rows, uint8_t *__restrict__ names) {
+// Chunks of 64 records per wave: the chunk's names are one run of the output
+// (the offsets are a scan), taken a byte per lane; a byte's record is found by
+// a binary search over the chunk's offsets.
+__global__ __launch_bounds__(256) void fq_names_kernel(const uint8_t *__restrict__ buf,
+                                                       const uint64_t *__restrict__ info,
+                                                       const uint64_t *__restrict__ name_src,
+                                                       const int64_t *__restrict__ name_off, int64_t names_cap,
+                                                       uint8_t *__restrict__ names) {
+    __shared__ FqLds S;
     const int64_t recs = (int64_t)info[0];
-    const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    for (int64_t r = wave; r < recs; r += (int64_t)gridDim.x * 4) {
-        int64_t ss, se, qs, qe;
-        line_span(buf, nl, 4 * r + 1, ss, se);
-        line_span(buf, nl, 4 * r + 3, qs, qe);
-        const int64_t slen = se - ss, qlen = qe - qs;
-        uint8_t *row = rows + r * stride;
-        uint8_t *crow = conv_rows ? conv_rows + r * stride : nullptr;
-        uint8_t *qrow = qual_rows ? qual_rows + r * stride : nullptr;
-        for (int64_t x = lane; x < stride; x += 64) {
-            const uint8_t b = x < slen ? buf[ss + x] : (uint8_t)'N';
-            row[x] = b;
-            if (crow) crow[x] = x < slen ? cvt(b, mode) : (uint8_t)'N';
-            if (qrow) qrow[x] = x < qlen ? buf[qs + x] : (uint8_t)0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t wave = (int64_t)__builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    for (int64_t c0 = wave * FQ_CHUNK; c0 < recs; c0 += (int64_t)gridDim.x * 4 * FQ_CHUNK) {
+        const int nrec = (int)(recs - c0 < FQ_CHUNK ? recs - c0 : FQ_CHUNK);
+        if (lane < nrec) {
+            S.src[wv][lane] = (int64_t)name_src[c0 + lane];
+            S.off[wv][lane] = name_off[c0 + lane];
         }
-        const int64_t no = name_off[r], nlen = name_len[r], src = (int64_t)name_src[r];
-        if (no + nlen <= names_cap)
-            for (int64_t x = lane; x < nlen; x += 64) names[no + x] = buf[src + x];
+        if (lane == 0) S.off[wv][nrec] = name_off[c0 + nrec];
+        wsync();
+        const int64_t o0 = S.off[wv][0], o1 = S.off[wv][nrec];
+        const int64_t end = o1 < names_cap ? o1 : names_cap;  // names past the capacity are not written
+        for (int64_t x = o0 + lane; x < end; x += 64) {
+            int lo = 0, hi = nrec - 1;  // the record whose name holds output byte x
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (S.off[wv][mid] <= x)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            if (S.off[wv][lo + 1] <= names_cap) names[x] = buf[S.src[wv][lo] + (x - S.off[wv][lo])];
+        }
+        wsync();
     }
 }
 
@@ -354,9 +470,10 @@ int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32
     uint32_t *name_len = (uint32_t *)(s + L.name_len);
     uint64_t *name_src = (uint64_t *)(s + L.name_src);
     const unsigned g = (unsigned)grid_for(max_records);
-    const unsigned gm = (unsigned)((max_records + 3) / 4 < 65536 ? (max_records + 3) / 4 : 65536);
-    fq_measure_kernel<<<gm, 256, 0, st>>>(d_buf, nl, max_records, stride, d_info, d_lens, d_qual_lens, name_len,
-                                          name_src);
+    const int64_t chunks = (max_records + FQ_CHUNK - 1) / FQ_CHUNK;  // a wave per chunk of records
+    const unsigned gm = (unsigned)((chunks + 3) / 4 < 65536 ? (chunks + 3) / 4 : 65536);
+    fq_measure_kernel<<<gm, 256, 0, st>>>(d_buf, nbytes, nl, max_records, stride, bs_mode, d_info, d_lens,
+                                          d_qual_lens, name_len, name_src, d_rows, d_conv_rows, d_qual_rows);
     MA_LAUNCHED();
     MA_TRY(exclusive_scan_u32(st, max_records, name_len, (uint64_t *)d_name_off, (uint64_t *)(s + L.name_part),
                               (uint64_t *)d_name_off + max_records));
@@ -364,10 +481,8 @@ int ma_fastq_parse_dev(void *stream, const uint8_t *d_buf, int64_t nbytes, int32
     MA_LAUNCHED();
     fq_maxlen_kernel<<<g < 1024u ? g : 1024u, 256, 0, st>>>(d_lens, d_info);
     MA_LAUNCHED();
-    const int64_t blocks = (max_records + 3) / 4 < 65536 ? (max_records + 3) / 4 : 65536;
-    fq_copy_kernel<<<(unsigned)blocks, 256, 0, st>>>(d_buf, nl, stride, bs_mode, d_info, name_len, name_src,
-                                                     d_name_off, names_cap, d_rows, d_conv_rows, d_qual_rows,
-                                                     d_names);
+    const int64_t blocks = gm;
+    fq_names_kernel<<<(unsigned)blocks, 256, 0, st>>>(d_buf, d_info, name_src, d_name_off, names_cap, d_names);
     MA_LAUNCHED();
     return MA_OK;
 }
