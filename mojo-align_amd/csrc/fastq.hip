@@ -34,6 +34,10 @@ enum : uint32_t { FQ_OK = 0, FQ_NO_SEQ = 1, FQ_NO_PLUS = 2, FQ_NO_QUAL = 3, FQ_T
 // info words (u64): 0 records, 1 bytes consumed, 2 longest read, 3 status,
 // 4 name bytes, 5 newlines, 6 lines, 7 first read longer than stride
 
+__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {  // 0x80 in each zero byte of x
+    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
+}
+
 __device__ __forceinline__ uint32_t count_nl(const uint8_t *buf, int64_t n, int64_t p0) {
     uint32_t c = 0;
     if (p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0) {
@@ -45,8 +49,7 @@ __device__ __forceinline__ uint32_t count_nl(const uint8_t *buf, int64_t n, int6
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const uint32_t x = ws[q] ^ 0x0A0A0A0Au;  // zero bytes where '\n'
-                // exact zero-byte flags (no borrow between bytes): high bit of each zero byte
-                c += __popc(~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu));
+                c += __popc(zero_bytes(x));  // exact: no borrow between bytes
             }
         }
     } else {
@@ -95,13 +98,15 @@ __global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restri
                     const uint4 w = v[k];
                     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-                    for (int q = 0; q < 4; q++)
-#pragma unroll
-                        for (int b = 0; b < 4; b++)
-                            if (((ws[q] >> (8 * b)) & 0xFF) == '\n') {
-                                if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 16 * k + 4 * q + b);
-                                L++;
-                            }
+                    for (int q = 0; q < 4; q++) {
+                        uint32_t hit = zero_bytes(ws[q] ^ 0x0A0A0A0Au);  // 0x80 in each '\n' byte
+                        while (hit) {
+                            const int b = __builtin_ctz(hit) >> 3;
+                            if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 16 * k + 4 * q + b);
+                            L++;
+                            hit &= hit - 1;
+                        }
+                    }
                 }
             } else {
                 for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
@@ -155,10 +160,6 @@ __global__ void fq_settle_kernel(const uint8_t *__restrict__ buf, int64_t n, int
     info[5] = nnl;
     info[6] = lines;
     info[7] = ~0ull;
-}
-
-__device__ __forceinline__ uint32_t zero_bytes(uint32_t x) {  // 0x80 in each zero byte of x
-    return ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x | 0x7F7F7F7Fu);
 }
 
 // The 4 text bytes at p (p + 4 <= n): the aligned word holding p and, when p
@@ -224,7 +225,7 @@ __device__ __forceinline__ void put_rows(uint8_t *row0, int64_t stride, int nrec
                                          const int64_t *src, const int32_t *len, uint32_t pad, int mode, int lane) {
     const int W = (int)((stride + 6) >> 2) + 1;  // aligned words a row can touch, at most
     int j = lane / W, k = lane - j * W;
-#pragma unroll 2
+#pragma unroll 8
     for (int slot = lane; slot < nrec * W; slot += 64) {
         uint8_t *dst = row0 + (int64_t)j * stride;
         const uintptr_t a0 = (uintptr_t)dst;
@@ -271,24 +272,60 @@ __global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restri
         const int64_t r = c0 + lane;
         int64_t ss = 0, se = 0, qs = 0, qe = 0;
         if (lane < nrec) {
-            int64_t hs, he;
-            line_span(buf, nl, 4 * r, hs, he);
-            line_span(buf, nl, 4 * r + 1, ss, se);
-            line_span(buf, nl, 4 * r + 3, qs, qe);
-            const int64_t s0 = (he > hs && buf[hs] == '@') ? hs + 1 : hs;
-            int64_t e0 = he;
-            for (int64_t q = s0; q < he; q += 4) {
-                uint32_t w = 0;
+            // one batch of loads after the newline positions: the bytes before the line
+            // ends ('\r') and the header's first 36 bytes
+            const int64_t L0 = 4 * r;
+            const int64_t hs = L0 == 0 ? 0 : (int64_t)nl[L0 - 1] + 1;
+            const int64_t hn = (int64_t)nl[L0], sn = (int64_t)nl[L0 + 1], pn = (int64_t)nl[L0 + 2];
+            const int64_t qn = (int64_t)nl[L0 + 3];
+            ss = hn + 1;
+            qs = pn + 1;
+            const bool hcr = hn > hs && buf[hn - 1] == '\r';
+            const bool scr = sn > ss && buf[sn - 1] == '\r';
+            const bool qcr = qn > qs && buf[qn - 1] == '\r';
+            uint32_t hw[9];
+#pragma unroll
+            for (int i = 0; i < 9; i++) {
+                const int64_t q = hs + 4 * i;
+                hw[i] = 0;
                 if (q + 4 <= n) {
-                    w = text4(buf, q);
+                    hw[i] = text4(buf, q);
                 } else {
-                    for (int j = 0; j < 4 && q + j < n; j++) w |= (uint32_t)buf[q + j] << (8 * j);
+                    for (int j = 0; j < 4 && q + j < n; j++) hw[i] |= (uint32_t)buf[q + j] << (8 * j);
                 }
+            }
+            const int64_t he = hn - (hcr ? 1 : 0);
+            se = sn - (scr ? 1 : 0);
+            qe = qn - (qcr ? 1 : 0);
+            const int at = (he > hs && (hw[0] & 255u) == '@') ? 1 : 0;  // the name skips a leading '@'
+            const int64_t s0 = hs + at;
+            int64_t e0 = he;
+            bool found = false;
+#pragma unroll
+            for (int i = 7; i >= 0; i--) {  // the first ' ' or '/' among the name's first 32 bytes
+                const uint32_t w = at ? __builtin_amdgcn_alignbyte(hw[i + 1], hw[i], 1u) : hw[i];
+                const int64_t q = s0 + 4 * i;
                 uint32_t hit = zero_bytes(w ^ 0x20202020u) | zero_bytes(w ^ 0x2F2F2F2Fu);
-                if (he - q < 4) hit &= (1u << (8 * (he - q))) - 1;
+                if (he - q < 4) hit = he - q > 0 ? hit & ((1u << (8 * (he - q))) - 1) : 0u;
                 if (hit) {
                     e0 = q + (__builtin_ctz(hit) >> 3);
-                    break;
+                    found = true;
+                }
+            }
+            if (!found) {
+                for (int64_t q = s0 + 32; q < he; q += 4) {  // longer names
+                    uint32_t w = 0;
+                    if (q + 4 <= n) {
+                        w = text4(buf, q);
+                    } else {
+                        for (int j = 0; j < 4 && q + j < n; j++) w |= (uint32_t)buf[q + j] << (8 * j);
+                    }
+                    uint32_t hit = zero_bytes(w ^ 0x20202020u) | zero_bytes(w ^ 0x2F2F2F2Fu);
+                    if (he - q < 4) hit &= (1u << (8 * (he - q))) - 1;
+                    if (hit) {
+                        e0 = q + (__builtin_ctz(hit) >> 3);
+                        break;
+                    }
                 }
             }
             const int64_t slen = se - ss;
