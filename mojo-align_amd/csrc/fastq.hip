@@ -217,27 +217,38 @@ struct FqLds {
     int64_t off[4][FQ_CHUNK + 1];  // name offsets
 };
 
-// The chunk's rows of one kind, `stride` bytes each from row0 on (any
-// alignment): the lanes take (record, word) slots, a record's slots covering
-// the aligned words of its row; a word a row shares with a neighbouring row is
-// written by bytes, every other word whole.
-__device__ __forceinline__ void put_rows(uint8_t *row0, int64_t stride, int nrec, const uint8_t *buf, int64_t n,
-                                         const int64_t *src, const int32_t *len, uint32_t pad, int mode, int lane) {
+// Word v at wa, or the bytes of it that lie in [0, stride) of a row at a0.
+__device__ __forceinline__ void put_word(uintptr_t wa, uintptr_t a0, int64_t x0, int64_t stride, uint32_t v) {
+    if (x0 >= 0 && x0 + 4 <= stride) {
+        *reinterpret_cast<uint32_t *>(wa) = v;
+    } else {
+        uint8_t *dst = reinterpret_cast<uint8_t *>(a0);
+        for (int q = 0; q < 4; q++)
+            if (x0 + q >= 0 && x0 + q < stride) dst[x0 + q] = (uint8_t)(v >> (8 * q));
+    }
+}
+
+// The chunk's rows of one kind, `stride` bytes each from row0 on (converted by
+// pmode), and copies converted by cmode from crow0 on when crow0 is not null
+// (with row0's alignment): the lanes take (record, word) slots, a record's
+// slots covering the aligned words of its row; a word a row shares with a
+// neighbouring row is written by bytes, every other word whole.
+__device__ __forceinline__ void put_rows(uint8_t *row0, uint8_t *crow0, int64_t stride, int nrec, const uint8_t *buf,
+                                         int64_t n, const int64_t *src, const int32_t *len, uint32_t pad, int pmode,
+                                         int cmode, int lane) {
     const int W = (int)((stride + 6) >> 2) + 1;  // aligned words a row can touch, at most
     int j = lane / W, k = lane - j * W;
 #pragma unroll 8
     for (int slot = lane; slot < nrec * W; slot += 64) {
-        uint8_t *dst = row0 + (int64_t)j * stride;
-        const uintptr_t a0 = (uintptr_t)dst;
+        const uintptr_t a0 = (uintptr_t)(row0 + (int64_t)j * stride);
         const uintptr_t wa = (a0 & ~(uintptr_t)3) + 4 * (uintptr_t)k;
         const int64_t x0 = (int64_t)(wa - a0);  // row offset of the word's first byte
         if (x0 < stride) {
-            const uint32_t v = cvt4(row4(buf, n, src[j], len[j], x0, pad), mode);
-            if (x0 >= 0 && x0 + 4 <= stride) {
-                *reinterpret_cast<uint32_t *>(wa) = v;
-            } else {
-                for (int q = 0; q < 4; q++)
-                    if (x0 + q >= 0 && x0 + q < stride) dst[x0 + q] = (uint8_t)(v >> (8 * q));
+            const uint32_t v = row4(buf, n, src[j], len[j], x0, pad);
+            put_word(wa, a0, x0, stride, cvt4(v, pmode));
+            if (crow0) {
+                const uintptr_t d = (uintptr_t)(crow0 - row0);
+                put_word(wa + d, a0 + d, x0, stride, cvt4(v, cmode));
             }
         }
         k += 64;
@@ -338,14 +349,18 @@ __global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restri
         S.src[wv][lane] = ss;
         S.len[wv][lane] = (int32_t)(se - ss);
         wsync();
-        put_rows(rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, 0, lane);
-        if (conv_rows)
-            put_rows(conv_rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, mode, lane);
+        put_rows(rows + c0 * stride, conv_rows && (((uintptr_t)conv_rows - (uintptr_t)rows) & 3) == 0
+                                         ? conv_rows + c0 * stride : nullptr,
+                 stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, 0, mode, lane);
+        if (conv_rows && (((uintptr_t)conv_rows - (uintptr_t)rows) & 3) != 0)
+            put_rows(conv_rows + c0 * stride, nullptr, stride, nrec, buf, n, S.src[wv], S.len[wv], 0x4E4E4E4Eu, mode,
+                     0, lane);
         wsync();
         S.src[wv][lane] = qs;
         S.len[wv][lane] = (int32_t)(qe - qs);
         wsync();
-        if (qual_rows) put_rows(qual_rows + c0 * stride, stride, nrec, buf, n, S.src[wv], S.len[wv], 0u, 0, lane);
+        if (qual_rows)
+            put_rows(qual_rows + c0 * stride, nullptr, stride, nrec, buf, n, S.src[wv], S.len[wv], 0u, 0, 0, lane);
         wsync();
     }
 }

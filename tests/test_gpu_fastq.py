@@ -117,3 +117,44 @@ def test_reader_over_files(tmp_path, gz):
         while rd.read_batch()[0].n:
             pass
     rd.close()
+
+
+def test_parse_unaligned_buffers_match_oracle():
+    """Text and output rows at odd byte offsets (the C ABI takes any pointers):
+    base rows, converted rows and quality rows each with another alignment, so
+    the word writes fall back to bytes at every row edge and the converted rows
+    take their own pass."""
+    from mialign._lib import check, lib, ptr
+
+    say("test_parse_unaligned_buffers_match_oracle")
+    data = random_fastq(31, 3000, crlf=0.2)
+    stride, mode = 151, 1
+    want = orc.fastq_parse(data, stride, mode)
+    n = want["n"]
+    mr = n + 1
+    text = mialign.DeviceArray((len(data) + 32,), np.uint8)
+    host_text = np.frombuffer(data, np.uint8).copy()  # kept alive across the copy
+    check(lib().ma_memcpy_h2d(text.ptr + 3, ptr(host_text), len(data), None), "h2d")
+    size = mr * stride + 16
+    rows, conv, qual = (mialign.DeviceArray((size,), np.uint8) for _ in range(3))
+    offs = {"rows": 1, "conv": 2, "qual": 7}
+    lens = mialign.DeviceArray((mr,), np.uint32)
+    qlens = mialign.DeviceArray((mr,), np.uint32)
+    names = mialign.DeviceArray((len(data),), np.uint8)
+    name_off = mialign.DeviceArray((mr + 1,), np.int64)
+    info = mialign.DeviceArray((8,), np.uint64)
+    sb = int(lib().ma_fastq_scratch_bytes(len(data), mr))
+    scratch = mialign.DeviceArray((sb,), np.uint8)
+    check(lib().ma_fastq_parse_dev(None, text.ptr + 3, len(data), 1, mr, stride, mode, rows.ptr + offs["rows"],
+                                   conv.ptr + offs["conv"], qual.ptr + offs["qual"], lens.ptr, qlens.ptr, names.ptr,
+                                   len(data), name_off.ptr, info.ptr, scratch.ptr, sb), "ma_fastq_parse_dev")
+    got = {}
+    for k, arr in (("rows", rows), ("conv", conv), ("qual", qual)):
+        host = arr.download()
+        got[k] = host[offs[k]:offs[k] + n * stride].reshape(n, stride)
+    assert int(info.download()[0]) == n
+    for k in ("rows", "conv", "qual"):
+        assert np.array_equal(got[k], want[k]), k
+    assert np.array_equal(lens.download()[:n], want["lens"])
+    assert np.array_equal(name_off.download()[:n + 1], want["name_off"])
+    assert names.download()[: int(want["name_off"][-1])].tobytes() == want["names"].tobytes()
