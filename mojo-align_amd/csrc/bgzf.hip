@@ -17,7 +17,7 @@
 //     read once per round; the distance-1 run's length comes from a ballot of
 //     "byte equals the one before"; matches of 4..64 bytes stay inside the
 //     segment, which is parsed greedily in scalar code over the lanes' match
-//     masks; one token word per position goes to a scratch, and each wave
+//     masks; a 2-byte token per position goes to a scratch, and each wave
 //     counts its symbols (two 16-bit counters per LDS word);
 //   * one dynamic-Huffman block (RFC 1951 3.2.7) per member, every code's
 //     lengths optimal under its limit by package-merge (Larmore and
@@ -445,11 +445,13 @@ __device__ __forceinline__ void pm_select(int n, const uint16_t (*lc)[W], uint32
     }
 }
 
-// Tokens, one word per block position: a literal byte b is TOK_LIT | b, a
-// match starting there TOK_MATCH | length << 16 | distance, a position inside
-// a match 0.
-constexpr uint32_t TOK_MATCH = 1u << 31;
-constexpr uint32_t TOK_LIT = 1u << 30;
+// Tokens, two bytes per block position: a literal byte b is TOK_LIT | b, a
+// match starting there TOK_MATCH | distance (at most SUB + PRE), and the
+// position after a match start (always inside the match) holds its length;
+// every other position inside a match is 0.
+constexpr uint32_t TOK_LIT = 1u << 15;
+constexpr uint32_t TOK_MATCH = 1u << 14;
+static_assert(SUB + PRE < (int)TOK_MATCH, "a distance fits below the match flag");
 constexpr int MIN_MATCH = 4;
 
 __device__ __forceinline__ void count_sym(uint32_t *hist, uint32_t u) {
@@ -462,13 +464,13 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                                                             uint32_t *__restrict__ body_len,
                                                             uint32_t *__restrict__ crc_out,
                                                             uint32_t *__restrict__ member_bytes,
-                                                            uint32_t *__restrict__ tok_all) {
+                                                            uint16_t *__restrict__ tok_all) {
     __shared__ DfShared S;
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);  // uniform: sub-block bounds stay scalar
     uint32_t *const dw = S.data + 1;  // the block's words (dw[-1] is the guard)
     const uint8_t *const db = reinterpret_cast<const uint8_t *>(dw);
-    uint32_t *const tok = tok_all + (int64_t)blockIdx.x * DF_IN;  // this workgroup's tokens
+    uint16_t *const tok = tok_all + (int64_t)blockIdx.x * DF_IN;  // this workgroup's tokens
     if (t < 256) {
         uint32_t c = (uint32_t)t;
         for (int k = 0; k < 8; k++) c = (c & 1) ? (c >> 1) ^ CRC_POLY : c >> 1;
@@ -614,7 +616,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                         st |= 1ull << q;
                         p = q + __builtin_amdgcn_readlane(best, q);
                     }
-                    uint32_t tk = 0;
+                    // a covered position right after a match start carries the match's length
+                    const int prev_best = __shfl_up(best, 1, 64);
+                    uint32_t tk = lane >= 1 && ((st >> (lane - 1)) & 1) ? (uint32_t)prev_best : 0u;
                     if ((lit >> lane) & 1) {
                         const uint32_t by = db[P];
                         tk = TOK_LIT | by;
@@ -623,12 +627,12 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                         uint32_t ls, le, lx, ds, de, dx;
                         len_code((uint32_t)best, ls, le, lx);
                         dist_code((uint32_t)bd, ds, de, dx);
-                        tk = TOK_MATCH | (uint32_t)best << 16 | (uint32_t)bd;
+                        tk = TOK_MATCH | (uint32_t)bd;
                         count_sym(hist, ls);
                         count_sym(hist, NLIT + ds);
                         extra += le + de;
                     }
-                    if (valid) tok[P] = tk;
+                    if (valid) tok[P] = (uint16_t)tk;
                 }
             }
             extra = wave_sum(extra);
@@ -885,15 +889,16 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                 uint32_t off = hb + wpre;
                 for (int r0 = sb0; r0 < sb1; r0 += 8 * 64) {
                     const int P0 = r0 + 8 * lane;
-                    uint32_t tk[8];
+                    uint32_t tk[9];  // tk[8]: the next position, the length of a match starting at 7
                     if (P0 < sb1) {
                         const uint4 a = *reinterpret_cast<const uint4 *>(tok + P0);
-                        const uint4 c = *reinterpret_cast<const uint4 *>(tok + P0 + 4);
-                        tk[0] = a.x, tk[1] = a.y, tk[2] = a.z, tk[3] = a.w;
-                        tk[4] = c.x, tk[5] = c.y, tk[6] = c.z, tk[7] = c.w;
+                        const uint32_t wv[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                        for (int k = 0; k < 8; k++) tk[k] = (wv[k >> 1] >> (16 * (k & 1))) & 0xFFFFu;
+                        tk[8] = (tk[7] & TOK_MATCH) ? tok[P0 + 8] : 0u;  // inside the match, so below sb1
                     } else {
 #pragma unroll
-                        for (int k = 0; k < 8; k++) tk[k] = 0;
+                        for (int k = 0; k < 9; k++) tk[k] = 0;
                     }
                     // per token: c1 = literal or length code + extra bits (at most 20) | their
                     // count << 20 | the distance part's count << 25; c2 = distance code + extra
@@ -904,9 +909,9 @@ __global__ __launch_bounds__(DF_T) void bgzf_deflate_kernel(const uint8_t *__res
                         const uint32_t x = P0 + k < sb1 ? tk[k] : 0u;
                         uint32_t v1 = 0, v2 = 0, b1 = 0, b2 = 0;
                         if (x & TOK_MATCH) {
-                            const uint32_t mk = S.mpk[(x >> 16) & 0x7FFF];
+                            const uint32_t mk = S.mpk[tk[k + 1]];
                             uint32_t ds, de, dx;
-                            dist_code(x & 0xFFFF, ds, de, dx);
+                            dist_code(x & 0x3FFF, ds, de, dx);
                             const uint32_t dk = S.dpk[ds];
                             v1 = mk & 0xFFFFFFu;
                             b1 = mk >> 24;
@@ -1047,7 +1052,7 @@ DfScratch df_layout(int64_t n) {
     s.bytes = take((size_t)nm * 4);
     s.off = take((size_t)(nm + 1) * 8);
     s.partial = take(scan_scratch_bytes(nm));
-    s.tok = take((size_t)g * DF_IN * 4);
+    s.tok = take((size_t)g * DF_IN * 2);
     s.total = o;
     return s;
 }
@@ -1080,7 +1085,7 @@ int ma_bgzf_compress_dev(void *stream, const uint8_t *d_src, int64_t n, int32_t 
         int64_t grid = nb < n_cu ? nb : n_cu;
         if (grid > DF_MAX_GRID) grid = DF_MAX_GRID;
         bgzf_deflate_kernel<<<(unsigned)grid, DF_T, 0, st>>>(d_src, n, nb, level, (uint32_t *)(s + L.slots),
-                                                              body, crc, bytes, (uint32_t *)(s + L.tok));
+                                                              body, crc, bytes, (uint16_t *)(s + L.tok));
         MA_LAUNCHED();
     }
     if (add_eof) {
