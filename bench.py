@@ -174,7 +174,7 @@ def serve_section(ix, packed, lens, L, n_req: int, slots: int, launch_reads: int
 # 10M x 150 bp reads): rocprofv3 --pmc FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, the
 # factors read off copy_u64_kernel (a copy of known size) in the same runs.
 MAP_TRAFFIC_PROFILED = {"genome_bp": 100_000_000, "reads": 10_000_000, "read_len": 150,
-                        "bytes": 85.2e9, "kernel_ms": 40.5,
+                        "bytes": 85.9e9, "kernel_ms": 38.9,
                         "source": "profiles/map_reads_kernel_pmc_100Mbp_10Mx150.md"}
 SORT_BYTES_PER_RECORD = 170  # 2 x 13 B x 6 digit passes (45-bit key) + 13 B markdup read + 1 B flag
 
@@ -493,8 +493,8 @@ def _subset_names(blob: np.ndarray, off: np.ndarray, idx: np.ndarray):
 
 # HBM bytes per 10M-record launch (150 bp reads, 100 Mbp workload) from FETCH_SIZE x 2 + WRITE_SIZE,
 # profiles/bam_emit_and_fastq_ingest_10Mx150.md
-BAM_TRAFFIC_PROFILED = {"reads": 10_000_000, "read_len": 150, "pack": 10.93e9, "bgzf_deflate": 43.9e9,
-                        "fastq": 23.5e9, "source": "profiles/bam_emit_and_fastq_ingest_10Mx150.md"}
+BAM_TRAFFIC_PROFILED = {"reads": 10_000_000, "read_len": 150, "pack": 10.94e9, "bgzf_deflate": 19.2e9,
+                        "fastq": 25.5e9, "source": "profiles/bam_emit_and_fastq_ingest_10Mx150.md"}
 
 
 def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int, parity_records: int) -> dict:
@@ -581,7 +581,7 @@ def fastq_section(reads, blob, name_off, qual, with_cpu: bool, cpu_records: int,
         "records_per_s": round(n / (dev_ms / 1000.0), 1),
         "GBps_text": round(nbytes / (dev_ms / 1000.0) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "fq_* (count, lines, measure, copy)",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "kernel": "fq_* (count, lines, measure + rows, names)",
                      "algorithmic_bytes_per_read": round(alg / n, 1),
                      "traffic": _profiled(n, L, "fastq")},
         "pcie_inclusive_records_per_s": round(n / pcie_s, 1),
