@@ -295,14 +295,23 @@ __global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restri
             const bool scr = sn > ss && buf[sn - 1] == '\r';
             const bool qcr = qn > qs && buf[qn - 1] == '\r';
             uint32_t hw[9];
+            {
+                // ten aligned words from the one holding hs (each holds a byte below n, or
+                // is not read), shifted into the nine 4-byte windows from hs
+                const uintptr_t a = (uintptr_t)(buf + hs);
+                const uint32_t *w = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+                const int64_t lead = (int64_t)(a & 3);  // bytes of word 0 before hs
+                uint32_t aw[10];
 #pragma unroll
-            for (int i = 0; i < 9; i++) {
-                const int64_t q = hs + 4 * i;
-                hw[i] = 0;
-                if (q + 4 <= n) {
-                    hw[i] = text4(buf, q);
-                } else {
-                    for (int j = 0; j < 4 && q + j < n; j++) hw[i] |= (uint32_t)buf[q + j] << (8 * j);
+                for (int i = 0; i < 10; i++) aw[i] = hs - lead + 4 * i < n ? w[i] : 0u;
+#pragma unroll
+                for (int i = 0; i < 9; i++) hw[i] = __builtin_amdgcn_alignbyte(aw[i + 1], aw[i], (uint32_t)lead);
+                if (hs + 36 > n) {  // bytes at or past n read as 0 (their words may hold stale bytes)
+#pragma unroll
+                    for (int i = 0; i < 9; i++) {
+                        const int64_t v = n - (hs + 4 * i);
+                        hw[i] = v <= 0 ? 0u : v >= 4 ? hw[i] : hw[i] & ((1u << (8 * v)) - 1);
+                    }
                 }
             }
             const int64_t he = hn - (hcr ? 1 : 0);
