@@ -86,26 +86,37 @@ __global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restri
     __shared__ uint64_t sh[4];
     for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
         const int64_t p0 = tile * FQ_TILE + (int64_t)threadIdx.x * FQ_BYTES;
-        const uint32_t c = count_nl(buf, n, p0);
+        // the thread's 64 bytes: newline flags of each word kept for the writes
+        const bool vec = p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0;
+        uint32_t hit[FQ_BYTES / 4];
+        uint32_t c = 0;
+        if (vec) {
+            const uint4 *v = reinterpret_cast<const uint4 *>(buf + p0);
+#pragma unroll
+            for (int k = 0; k < FQ_BYTES / 16; k++) {
+                const uint4 w = v[k];
+                const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    hit[4 * k + q] = zero_bytes(ws[q] ^ 0x0A0A0A0Au);  // 0x80 in each '\n' byte
+                    c += __popc(hit[4 * k + q]);
+                }
+            }
+        } else {
+            c = count_nl(buf, n, p0);
+        }
         uint64_t tot;
         uint64_t L = tile_base[tile] + block_excl_scan_256(c, sh, tot);
         if (c && L < (uint64_t)cap) {
-            if (p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0) {
-                // the same 64 bytes again (cached), scanned in registers
-                const uint4 *v = reinterpret_cast<const uint4 *>(buf + p0);
+            if (vec) {
 #pragma unroll
-                for (int k = 0; k < FQ_BYTES / 16; k++) {
-                    const uint4 w = v[k];
-                    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        uint32_t hit = zero_bytes(ws[q] ^ 0x0A0A0A0Au);  // 0x80 in each '\n' byte
-                        while (hit) {
-                            const int b = __builtin_ctz(hit) >> 3;
-                            if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 16 * k + 4 * q + b);
-                            L++;
-                            hit &= hit - 1;
-                        }
+                for (int i = 0; i < FQ_BYTES / 4; i++) {
+                    uint32_t h = hit[i];
+                    while (h) {
+                        const int b = __builtin_ctz(h) >> 3;
+                        if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 4 * i + b);
+                        L++;
+                        h &= h - 1;
                     }
                 }
             } else {
