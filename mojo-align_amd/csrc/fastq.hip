@@ -386,8 +386,9 @@ __global__ __launch_bounds__(256) void fq_measure_kernel(const uint8_t *__restri
 }
 
 // Chunks of 64 records per wave: the chunk's names are one run of the output
-// (the offsets are a scan), taken a byte per lane; a byte's record is found by
-// a binary search over the chunk's offsets.
+// (the offsets are a scan), taken a 4-byte output word per lane (whole-word
+// stores inside the run); a word's records come from a binary search over the
+// chunk's offsets.
 __global__ __launch_bounds__(256) void fq_names_kernel(const uint8_t *__restrict__ buf,
                                                        const uint64_t *__restrict__ info,
                                                        const uint64_t *__restrict__ name_src,
@@ -407,16 +408,35 @@ __global__ __launch_bounds__(256) void fq_names_kernel(const uint8_t *__restrict
         wsync();
         const int64_t o0 = S.off[wv][0], o1 = S.off[wv][nrec];
         const int64_t end = o1 < names_cap ? o1 : names_cap;  // names past the capacity are not written
-        for (int64_t x = o0 + lane; x < end; x += 64) {
-            int lo = 0, hi = nrec - 1;  // the record whose name holds output byte x
+        // a lane per aligned output word of [o0, end): its bytes' records from one binary
+        // search (the first byte's) and the offsets after it
+        const int64_t w0 = o0 >> 2;
+        for (int64_t wd = w0 + lane; 4 * wd < end; wd += 64) {
+            const int64_t x0 = 4 * wd;
+            const int64_t xs = x0 > o0 ? x0 : o0;  // the word's first byte inside the run
+            int lo = 0, hi = nrec - 1;  // the record whose name holds byte xs
             while (lo < hi) {
                 const int mid = (lo + hi + 1) >> 1;
-                if (S.off[wv][mid] <= x)
+                if (S.off[wv][mid] <= xs)
                     lo = mid;
                 else
                     hi = mid - 1;
             }
-            if (S.off[wv][lo + 1] <= names_cap) names[x] = buf[S.src[wv][lo] + (x - S.off[wv][lo])];
+            uint32_t v = 0, keep = 0;
+            for (int q = 0; q < 4; q++) {
+                const int64_t x = x0 + q;
+                if (x < xs || x >= end) continue;
+                while (lo + 1 < nrec && S.off[wv][lo + 1] <= x) lo++;
+                if (S.off[wv][lo + 1] > names_cap) continue;  // this name does not fit whole
+                v |= (uint32_t)buf[S.src[wv][lo] + (x - S.off[wv][lo])] << (8 * q);
+                keep |= 0xFFu << (8 * q);
+            }
+            if (keep == 0xFFFFFFFFu && (((uintptr_t)(names + x0)) & 3) == 0) {
+                *reinterpret_cast<uint32_t *>(names + x0) = v;
+            } else {
+                for (int q = 0; q < 4; q++)
+                    if ((keep >> (8 * q)) & 1) names[x0 + q] = (uint8_t)(v >> (8 * q));
+            }
         }
         wsync();
     }
