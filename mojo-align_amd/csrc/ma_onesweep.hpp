@@ -60,7 +60,8 @@ struct SortPlan {
     uint32_t has_hi;
     uint32_t error;             // a look-back wait exceeded OS_SPIN_LIMIT
     uint32_t viol;              // lo is not implied by hi (set by os_hist)
-    uint32_t pad;
+    uint32_t fviol;             // bit p: some key's byte p is 0xFF while byte p - 1 is not, or the
+                                // reverse (set by os_hist; zeroed with viol)
     uint64_t digit_off[OS_PASSES][OS_RADIX];
 };
 
@@ -90,7 +91,7 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int np = hi ? 16 : 8;
-    uint32_t viol = 0;
+    uint32_t viol = 0, fv = 0;
     for (int64_t base = ((int64_t)blockIdx.x * OS_BLOCK + (threadIdx.x & ~63)) * OS_HIST_U; base < n;
          base += (int64_t)gridDim.x * OS_BLOCK * OS_HIST_U) {
         uint64_t a[OS_HIST_U], b[OS_HIST_U];
@@ -107,9 +108,13 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
             const uint64_t vmask = __ballot(valid);
             if (vmask == 0ull) continue;  // wave-uniform
             if (valid && hi && a[u] != 0ull && !(b[u] == ~0ull && a[u] == (uint64_t)i)) viol = 1;
+            bool prev_ff = false;
             for (int p = 0; p < np; p++) {
                 const uint64_t k = p < 8 ? a[u] : b[u];
                 const uint32_t d = os_digit(k, 8 * (p & 7));
+                const bool ff = d == 0xFFu;
+                if (valid && p > 0 && ff != prev_ff) fv |= 1u << p;
+                prev_ff = ff;
                 const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane(d);
                 const uint64_t same = __ballot(valid && d == d0);
                 if (same == vmask) {
@@ -123,6 +128,9 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
     if (__ballot(viol)) {
         if (lane == 0) atomicOr(&plan->viol, 1u);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) fv |= (uint32_t)__shfl_xor((int)fv, o, 64);
+    if (fv && lane == 0) atomicOr(&plan->fviol, fv);
     __syncthreads();
     for (int i = threadIdx.x; i < np * OS_RADIX; i += OS_BLOCK)
         if (h[i]) atomicAdd(&hist[i], (unsigned long long)h[i]);
@@ -139,14 +147,21 @@ __global__ void __launch_bounds__(OS_BLOCK) os_plan_kernel(SortPlan *__restrict_
     __shared__ uint32_t act[OS_PASSES];
     const int t = threadIdx.x;
     const bool implied = has_hi && check_implied && plan->viol == 0;
+    const uint32_t fviol = plan->fviol;
     for (int p = 0; p < OS_PASSES; p++) {
         const bool exists = p < 8 || has_hi;
         const uint64_t c = exists ? hist[p * OS_RADIX + t] : 0ull;
         const int full = __syncthreads_or(exists && c == (uint64_t)n);
+        // a byte that is 0xFF exactly where the byte before it is, and 0 elsewhere, is a
+        // non-decreasing function of that byte: the records are already in its order, and
+        // the stable pass would move nothing (the unmapped sentinels' all-ones high bytes);
+        // not across lo passes skipped as implied, which leave the records unsorted by them
+        const int other = __syncthreads_or(exists && c != 0ull && t != 0 && t != 0xFF);
+        const bool follows = p > 0 && !(p == 8 && implied) && !other && !((fviol >> p) & 1u);
         uint64_t tot;
         const uint64_t ex = block_excl_scan_256(c, sh, tot);
         plan->digit_off[p][t] = ex;
-        if (t == 0) act[p] = (exists && n > 1 && !full && !(p < 8 && implied)) ? 1u : 0u;
+        if (t == 0) act[p] = (exists && n > 1 && !full && !(p < 8 && implied) && !follows) ? 1u : 0u;
         __syncthreads();
     }
     if (t < OS_PASSES * OS_GROUPS) tile_ctr[t] = 0u;
@@ -417,7 +432,7 @@ inline int os_sort(hipStream_t st, int64_t n, const uint64_t *in_lo, const uint6
                    int keep_keys, int check_implied, const OsScratch &s) {
     MA_TRY(fill_u64(st, s.status, 2 * s.status_stride, 0ull));
     MA_TRY(fill_u64(st, (uint64_t *)s.hist, OS_PASSES * OS_RADIX, 0ull));
-    MA_TRY(fill_u64(st, (uint64_t *)&s.plan->viol, 1, 0ull));  // viol and pad
+    MA_TRY(fill_u64(st, (uint64_t *)&s.plan->viol, 1, 0ull));  // viol and fviol
     if (n > 0) {
         os_hist_kernel<<<grid_for(n, OS_BLOCK, 2048), OS_BLOCK, 0, st>>>(n, in_lo, in_hi, s.hist, s.plan);
         MA_LAUNCHED();

@@ -4,7 +4,8 @@ scan (linear_gpu_sort.mojo:156-551), bit for bit: is_dup, perm and dup_perm.
 Covers the sort's own decisions: 128-bit duplicate keys whose low word moves
 with the high word (pairs' ordered ends), single-end keys whose low word is
 implied by the high word (0, or the record index when unmapped) and is not
-sorted, sizes around the radix tiles (2048 / 4096 records), keys that are all
+sorted, high bytes that only follow the unmapped sentinels (no pass of their
+own) and a key breaking that pattern, sizes around the radix tiles (2048 / 4096 records), keys that are all
 equal (no pass runs), one and two records, and the device entry point without
 a dup_perm output, followed by its status check."""
 
@@ -70,6 +71,34 @@ def test_sort_markdup_se_keys_implied_low_word(n):
     dlo[un] = np.arange(n, dtype=np.uint64)[un]
     coord = np.where(un, (np.uint64(0xFFFFFFFF) << np.uint64(32)) | np.arange(n, dtype=np.uint64),
                      rng.integers(0, 1 << 36, n).astype(np.uint64))
+    score = rng.integers(0, 1 << 14, n).astype(np.uint32)
+    pair = np.arange(n, dtype=np.uint32)
+    _both(coord, dhi, dlo, score, pair)
+    _both(coord, dhi, dlo, score, pair, do_markdup=False)
+
+
+@pytest.mark.parametrize("break_one", [False, True])
+def test_sort_markdup_sentinel_bytes_follow(break_one):
+    """High bytes that are 0xFF exactly where the byte below them is, and 0
+    elsewhere (the all-ones high words of unmapped records), need no pass of
+    their own: the records are already in their order. One key breaking the
+    pattern (byte 7 all ones over a byte 6 of 0) makes the pass run again."""
+    say(f"test_sort_markdup_sentinel_bytes_follow break_one={break_one}")
+    rng = np.random.default_rng(7 + int(break_one))
+    n = 200_000
+    idx = np.arange(n, dtype=np.uint64)
+    coord = rng.integers(0, 1 << 32, n).astype(np.uint64) | (rng.integers(0, 200, n).astype(np.uint64) << np.uint64(32))
+    dhi = (rng.integers(1, 200, n).astype(np.uint64) << np.uint64(48)) | (
+        rng.integers(0, 2, n).astype(np.uint64) << np.uint64(47)) | rng.integers(0, 1 << 20, n).astype(np.uint64)
+    dlo = np.zeros(n, np.uint64)
+    un = rng.random(n) < 0.05
+    coord[un] = (np.uint64(0xFFFFFFFF) << np.uint64(32)) | idx[un]
+    dhi[un] = M64
+    dlo[un] = idx[un]
+    if break_one:
+        coord[17] = (np.uint64(0xFF) << np.uint64(56)) | np.uint64(5)
+        dhi[23] = (np.uint64(0xFF) << np.uint64(56)) | (np.uint64(3) << np.uint64(48)) | np.uint64(9)
+        dlo[23] = np.uint64(0)
     score = rng.integers(0, 1 << 14, n).astype(np.uint32)
     pair = np.arange(n, dtype=np.uint32)
     _both(coord, dhi, dlo, score, pair)
