@@ -82,6 +82,14 @@ __device__ __forceinline__ uint32_t os_digit(uint64_t k, int shift) { return (ui
 // takes OS_HIST_U chunks of 64 keys per step, their loads issued together.
 constexpr int OS_HIST_U = 4;
 
+// Bit j set where byte j of x is 0xFF (exact zero-byte flags of ~x, gathered
+// into the low byte: the flag of byte j lands on bit 56 + j of the product).
+__device__ __forceinline__ uint32_t ff_bytes(uint64_t x) {
+    const uint64_t t = ~x;
+    const uint64_t z = ~(((t & 0x7F7F7F7F7F7F7F7Full) + 0x7F7F7F7F7F7F7F7Full) | t | 0x7F7F7F7F7F7F7F7Full);
+    return (uint32_t)(((z >> 7) * 0x0102040810204080ull) >> 56);
+}
+
 __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint64_t *__restrict__ lo,
                                                             const uint64_t *__restrict__ hi,
                                                             unsigned long long *__restrict__ hist,
@@ -108,13 +116,13 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
             const uint64_t vmask = __ballot(valid);
             if (vmask == 0ull) continue;  // wave-uniform
             if (valid && hi && a[u] != 0ull && !(b[u] == ~0ull && a[u] == (uint64_t)i)) viol = 1;
-            bool prev_ff = false;
+            if (valid) {  // bit p of m: byte p (lo bytes, then hi) is all ones; a change from p - 1
+                const uint32_t m = ff_bytes(a[u]) | (hi ? ff_bytes(b[u]) << 8 : 0u);
+                fv |= m ^ (m << 1);
+            }
             for (int p = 0; p < np; p++) {
                 const uint64_t k = p < 8 ? a[u] : b[u];
                 const uint32_t d = os_digit(k, 8 * (p & 7));
-                const bool ff = d == 0xFFu;
-                if (valid && p > 0 && ff != prev_ff) fv |= 1u << p;
-                prev_ff = ff;
                 const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane(d);
                 const uint64_t same = __ballot(valid && d == d0);
                 if (same == vmask) {
@@ -130,6 +138,7 @@ __global__ void __launch_bounds__(OS_BLOCK) os_hist_kernel(int64_t n, const uint
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) fv |= (uint32_t)__shfl_xor((int)fv, o, 64);
+    fv &= 0xFFFEu;  // bit 0 compares byte 0 with nothing
     if (fv && lane == 0) atomicOr(&plan->fviol, fv);
     __syncthreads();
     for (int i = threadIdx.x; i < np * OS_RADIX; i += OS_BLOCK)
