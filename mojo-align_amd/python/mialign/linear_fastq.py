@@ -15,6 +15,7 @@ to 'A' (upper case; the Python reader keeps case), rows are padded with 'N'.
 from __future__ import annotations
 
 import gzip
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Optional
 
@@ -109,49 +110,91 @@ def _open_stream(path: str):
 
 
 class _Side:
-    """One FASTQ file: a host buffer (the unparsed tail carried at its front,
-    the next chunk read in after it; plain pages, since pinning a buffer this
-    size costs more than the copies it would speed up in a one-pass run) and a
-    device staging buffer."""
+    """One FASTQ file. A reader thread reads the file's next chunk into a host
+    buffer while the device parses (and the caller maps) the chunk before it;
+    each chunk is copied into HBM after the unparsed tail of the last parse,
+    which stays on the device (moved to the front of the other device buffer).
+    Host buffers are plain pages: pinning buffers this size costs more than the
+    copies it would speed up in a one-pass run."""
 
-    def __init__(self, path: str, bs: str, chunk: int, device: int):
+    def __init__(self, path: str, bs: str, chunk: int, device: int, stream: Stream):
         if bs not in BS_CODES:
             raise MialignError(f"unknown conversion mode {bs!r} (C2T, G2A or none)")
-        self._alloc = lambda shape, dtype: np.empty(shape, dtype)
         self.fh = _open_stream(path)
         self.mode = BS_CODES[bs]
         self.chunk = int(chunk)
-        self.buf = self._alloc((2 * self.chunk,), np.uint8)
-        self.filled = 0
-        self.eof = False
         self.device = device
-        self.dbuf = DeviceArray((2 * self.chunk + 16,), np.uint8, device)
+        self.stream = stream
+        self.filled = 0  # unparsed bytes at the front of dbuf
+        self.eof = False
+        self.host = [np.empty(self.chunk, np.uint8), np.empty(self.chunk, np.uint8)]
+        self.dbufs = [DeviceArray((2 * self.chunk + 16,), np.uint8, device) for _ in range(2)]
+        self.cur = 0
+        self._slot = 0
+        self._pool = ThreadPoolExecutor(max_workers=1)
+        self._next = self._pool.submit(self._read, 0, self.chunk)
+
+    @property
+    def dbuf(self) -> DeviceArray:
+        return self.dbufs[self.cur]
+
+    def _read(self, slot: int, want: int):
+        """Reader thread: up to `want` bytes of the file into host buffer `slot`;
+        returns (count, the file ended)."""
+        if self.host[slot].size < want:
+            self.host[slot] = np.empty(want, np.uint8)
+        mv = memoryview(self.host[slot])[:want]
+        got = 0
+        while got < want:
+            k = self.fh.readinto(mv[got:])
+            if not k:
+                return got, True
+            got += k
+        return got, False
+
+    def _grow(self, nbytes: int) -> None:
+        """Both device buffers to nbytes, the buffered bytes kept."""
+        dbufs = [DeviceArray((nbytes,), np.uint8, self.device) for _ in range(2)]
+        if self.filled:
+            check(lib().ma_memcpy_d2d(dbufs[0].ptr, self.dbuf.ptr, self.filled, self.stream.handle),
+                  "ma_memcpy_d2d")
+            self.stream.sync()
+        self.dbufs, self.cur = dbufs, 0
 
     def fill(self) -> int:
-        """Read until `chunk` bytes are buffered (or the end); returns the count."""
-        if self.buf.size < self.chunk:
-            nb = self._alloc((2 * self.chunk,), np.uint8)
-            nb[: self.filled] = self.buf[: self.filled]
-            self.buf = nb
-        mv = memoryview(self.buf)
+        """Take read-ahead chunks until `chunk` bytes are buffered in HBM (or the
+        file ends); returns the count."""
         while not self.eof and self.filled < self.chunk:
-            got = self.fh.readinto(mv[self.filled:])
-            if not got:
-                self.eof = True
-                break
+            got, self.eof = self._next.result()
+            slot = self._slot
+            if not self.eof:  # the next chunk is read while this one is copied, parsed and mapped
+                self._slot ^= 1
+                self._next = self._pool.submit(self._read, self._slot, self.chunk)
+            if self.filled + got + 16 > self.dbuf.nbytes:
+                self._grow(2 * (self.filled + got) + 16)
+            if got:
+                check(lib().ma_memcpy_h2d(self.dbuf.ptr + self.filled, self.host[slot].ctypes.data, got,
+                                          self.stream.handle), "ma_memcpy_h2d")
+                self.stream.sync()  # the reader thread refills this host buffer two chunks on
             self.filled += got
-        if self.filled + 16 > self.dbuf.nbytes:
-            self.dbuf = DeviceArray((2 * self.filled + 16,), np.uint8, self.device)
         return self.filled
 
     def consume(self, k: int) -> None:
-        """Drop the first k bytes (the records just parsed)."""
+        """Drop the first k bytes (the records just parsed): the rest moves to
+        the front of the other device buffer."""
         rest = self.filled - k
         if rest > 0:
-            self.buf[:rest] = self.buf[k:self.filled].copy()
+            check(lib().ma_memcpy_d2d(self.dbufs[self.cur ^ 1].ptr, self.dbuf.ptr + k, rest, self.stream.handle),
+                  "ma_memcpy_d2d")
+            self.cur ^= 1
         self.filled = max(rest, 0)
 
     def close(self):
+        try:
+            self._next.result()  # the reader thread is done with the file
+        except Exception:
+            pass
+        self._pool.shutdown(wait=True)
         self.fh.close()
 
 
@@ -167,10 +210,10 @@ class DeviceFastqReader:
         if chunk_bytes <= 0:  # a batch's worth of text at ~1.75 bytes per base per read
             chunk_bytes = max(1 << 20, int(batch_reads) * (2 * self.stride + 96))
         self.batch_reads = int(batch_reads)
-        self.sides = [_Side(path1, bs_r1, chunk_bytes, device)]
-        if path2:
-            self.sides.append(_Side(path2, bs_r2, chunk_bytes, device))
         self.stream = Stream(device)
+        self.sides = [_Side(path1, bs_r1, chunk_bytes, device, self.stream)]
+        if path2:
+            self.sides.append(_Side(path2, bs_r2, chunk_bytes, device, self.stream))
         self.parser = FastqParser(2 * chunk_bytes + 16, self.batch_reads, device)
         names_cap = 2 * chunk_bytes
         self.batches = [DeviceFastqBatch(self.batch_reads, self.stride, names_cap, device, with_conv=with_conv)
@@ -187,9 +230,6 @@ class DeviceFastqReader:
             if nbuf > batch.names_cap:  # names never take more bytes than the buffer
                 batch.names_cap = 2 * nbuf
                 batch.names = DeviceArray((batch.names_cap,), np.uint8, side.device)
-            if nbuf:
-                check(lib().ma_memcpy_h2d(side.dbuf.ptr, side.buf.ctypes.data, nbuf, self.stream.handle),
-                      "ma_memcpy_h2d")
             info = self.parser.parse(self.stream, side.dbuf.ptr, nbuf, side.eof, batch, side.mode, max_records)
             if info.status in FQ_ERRORS:
                 raise MialignError(FQ_ERRORS[info.status])
