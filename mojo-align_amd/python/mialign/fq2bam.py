@@ -15,13 +15,17 @@ ma_sort_keys_pe_dev and ma_bam_pack_pe_dev.
 from __future__ import annotations
 
 import ctypes as C
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from ._lib import BS_MODES, MA_ENC_ASCII, REC_FIELDS, MialignError, Recs, check, default_params, lib
 from .bam_emit import BamPacker, BamWriter, DeviceBgzf, DeviceReads, bam_header, bgzf_compress, tid_table
 from .core import DeviceArray, DeviceRecs, Event, Stream, Workspace, map_batch_dev
 from .linear_fastq import DeviceFastqReader
+from .serve import pinned_empty
 
 
 class _Grow:
@@ -208,12 +212,10 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
         dz = DeviceBgzf(total, device)
         dz.compress(st, packer.d_out.ptr, total, level, True)
         zt = dz.total(st)
-        body = np.empty(max(zt, 1), np.uint8)
-        check(lib().ma_memcpy_d2h(body.ctypes.data, dz.d_dst.ptr, zt, st.handle), "ma_memcpy_d2h")
-        st.sync()
         with open(out_bam, "wb") as fh:
             fh.write(bgzf_compress(head, level, eof=False))
-            fh.write(memoryview(body)[:zt])
+            fh.flush()
+            _write_from_device(fh.fileno(), dz.d_dst.ptr, zt, fh.tell(), st)
         size = zt
     else:
         stream_bytes = packer.download(N, st)
@@ -226,6 +228,34 @@ def fastq_to_bam(index, fq1: str, out_bam: str, bs_r1: str = "C2T", *, fq2: str 
     return {"reads": N, "pairs": N // 2 if paired else 0, "mapped": mapped, "rescued": rescued, "duplicates": dups,
             "bam_stream_bytes": total, "bgzf_bytes": size, "ingest_and_map_s": round(t_ingest, 3),
             "map_kernel_ms": round(t_map, 3), "sort_and_pack_s": round(t_sort_pack, 3), "bgzf_s": round(t_bgzf, 3)}
+
+
+def _pwrite_all(fd: int, buf: np.ndarray, n: int, off: int) -> None:
+    mv = memoryview(buf)[:n]
+    while len(mv):
+        k = os.pwrite(fd, mv, off)
+        mv, off = mv[k:], off + k
+
+
+def _write_from_device(fd: int, d_src: int, nbytes: int, file_off: int, st: Stream, piece: int = 32 << 20,
+                       ring: int = 3) -> None:
+    """nbytes of device memory -> file fd at file_off: pieces copied into a ring
+    of pinned buffers, each written out by a writer thread while the next
+    pieces are copied."""
+    bufs = [pinned_empty((min(piece, max(nbytes, 1)),), np.uint8) for _ in range(min(ring, max(1, -(-nbytes // piece))))]
+    futs = [None] * len(bufs)
+    with ThreadPoolExecutor(max_workers=len(bufs)) as pool:
+        for i, o in enumerate(range(0, nbytes, piece)):
+            k = i % len(bufs)
+            if futs[k] is not None:
+                futs[k].result()  # the buffer's last piece is written
+            n = min(piece, nbytes - o)
+            check(lib().ma_memcpy_d2h(bufs[k].ctypes.data, d_src + o, n, st.handle), "ma_memcpy_d2h")
+            st.sync()
+            futs[k] = pool.submit(_pwrite_all, fd, bufs[k], n, file_off + o)
+        for f in futs:
+            if f is not None:
+                f.result()
 
 
 class _recs_view:
