@@ -80,12 +80,18 @@ __global__ __launch_bounds__(FQ_TB) void fq_count_kernel(const uint8_t *__restri
 }
 
 // nl[L] = position of the newline that ends line L (for L < cap).
+// Newlines a tile stages in LDS before its coalesced write-out (16 KiB of
+// FASTQ text holds about 200 lines; a tile with more writes them directly).
+constexpr int FQ_STAGE = 2048;
+
 __global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restrict__ buf, int64_t n, int64_t tiles,
                                                          const uint64_t *__restrict__ tile_base,
                                                          uint64_t *__restrict__ nl, int64_t cap) {
     __shared__ uint64_t sh[4];
+    __shared__ uint16_t stage[FQ_STAGE];  // the tile's newlines as offsets in the tile, in order
     for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const int64_t p0 = tile * FQ_TILE + (int64_t)threadIdx.x * FQ_BYTES;
+        const int64_t t0 = tile * FQ_TILE;
+        const int64_t p0 = t0 + (int64_t)threadIdx.x * FQ_BYTES;
         // the thread's 64 bytes: newline flags of each word kept for the writes
         const bool vec = p0 + FQ_BYTES <= n && ((p0 | (int64_t)(uintptr_t)buf) & 15) == 0;
         uint32_t hit[FQ_BYTES / 4];
@@ -106,25 +112,50 @@ __global__ __launch_bounds__(FQ_TB) void fq_lines_kernel(const uint8_t *__restri
             c = count_nl(buf, n, p0);
         }
         uint64_t tot;
-        uint64_t L = tile_base[tile] + block_excl_scan_256(c, sh, tot);
-        if (c && L < (uint64_t)cap) {
-            if (vec) {
+        const uint64_t ex = block_excl_scan_256(c, sh, tot);
+        const uint64_t base = tile_base[tile];
+        if (tot <= (uint64_t)FQ_STAGE) {  // block-uniform
+            uint32_t L = (uint32_t)ex;
+            if (c) {
+                if (vec) {
 #pragma unroll
-                for (int i = 0; i < FQ_BYTES / 4; i++) {
-                    uint32_t h = hit[i];
-                    while (h) {
-                        const int b = __builtin_ctz(h) >> 3;
-                        if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 4 * i + b);
-                        L++;
-                        h &= h - 1;
+                    for (int i = 0; i < FQ_BYTES / 4; i++) {
+                        uint32_t h = hit[i];
+                        while (h) {
+                            stage[L++] = (uint16_t)(threadIdx.x * FQ_BYTES + 4 * i + (__builtin_ctz(h) >> 3));
+                            h &= h - 1;
+                        }
                     }
+                } else {
+                    for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
+                        if (buf[p] == '\n') stage[L++] = (uint16_t)(p - t0);
                 }
-            } else {
-                for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
-                    if (buf[p] == '\n') {
-                        if (L < (uint64_t)cap) nl[L] = (uint64_t)p;
-                        L++;
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < (int)tot; i += FQ_TB)
+                if (base + i < (uint64_t)cap) nl[base + i] = (uint64_t)(t0 + stage[i]);
+            // the next tile's scan synchronises the block before stage is written again
+        } else {
+            uint64_t L = base + ex;
+            if (c && L < (uint64_t)cap) {
+                if (vec) {
+#pragma unroll
+                    for (int i = 0; i < FQ_BYTES / 4; i++) {
+                        uint32_t h = hit[i];
+                        while (h) {
+                            const int b = __builtin_ctz(h) >> 3;
+                            if (L < (uint64_t)cap) nl[L] = (uint64_t)(p0 + 4 * i + b);
+                            L++;
+                            h &= h - 1;
+                        }
                     }
+                } else {
+                    for (int64_t p = p0; p < n && p < p0 + FQ_BYTES; p++)
+                        if (buf[p] == '\n') {
+                            if (L < (uint64_t)cap) nl[L] = (uint64_t)p;
+                            L++;
+                        }
+                }
             }
         }
     }
