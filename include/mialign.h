@@ -391,6 +391,16 @@ int ma_bam_pack_dev(void *stream, int64_t n, const ma_bam_fields *fields, const 
 int64_t ma_bgzf_bound(int64_t n);
 int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threads, int32_t add_eof,
                      uint8_t *dst, int64_t dst_cap, int64_t *out_n);
+/* BGZF inflate (replaces the reference's gzip FASTQ pipe, `pigz -dc` or
+ * `gzip -dc`, fq2bam-meth/src/linear_fastq.mojo:141-150, for bgzip-compressed
+ * files): *used = the bytes of the whole members at the front of src[0..n)
+ * (each a gzip header whose extra field holds BC = member bytes - 1, raw
+ * deflate, CRC32, ISIZE), *out_n = their inflated size. With dst null only
+ * those two are set; else the members are inflated into dst (dst_cap >=
+ * *out_n) by `threads` workers, each member's ISIZE and CRC32 checked.
+ * MA_ERR_ARG for bytes that are not a BGZF member, MA_ERR_IO for a corrupt one. */
+int ma_bgzf_inflate(const uint8_t *src, int64_t n, int32_t threads, uint8_t *dst, int64_t dst_cap,
+                    int64_t *used, int64_t *out_n);
 /* BGZF on the device: the same 65280-byte members (header, raw deflate body,
  * CRC32, ISIZE) of a device stream, compressed by the GPU: level 0 = stored
  * blocks, otherwise LZ77 (greedy, matches of 4..64 bytes) and one

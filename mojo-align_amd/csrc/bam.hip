@@ -756,6 +756,87 @@ int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threa
     return MA_OK;
 }
 
+int ma_bgzf_inflate(const uint8_t *src, int64_t n, int32_t threads, uint8_t *dst, int64_t dst_cap,
+                    int64_t *used, int64_t *out_n) {
+    MA_REQUIRE(n >= 0 && (n == 0 || src) && used && out_n, "ma_bgzf_inflate: bad arguments");
+    // the whole members at the front of src: a gzip header whose extra field
+    // holds BC (BSIZE = member bytes - 1), raw deflate, CRC32, ISIZE
+    std::vector<int64_t> soff{0}, doff{0};
+    int64_t at = 0;
+    while (n - at >= 18) {
+        const uint8_t *m = src + at;
+        if (m[0] != 31 || m[1] != 139 || m[2] != 8 || m[3] != 4) {
+            set_error("ma_bgzf_inflate: not a BGZF member at byte " + std::to_string(at));
+            return MA_ERR_ARG;
+        }
+        const int64_t xlen = m[10] | (int64_t)m[11] << 8;
+        if (n - at < 12 + xlen) break;
+        int64_t bsize = -1;
+        for (int64_t x = 12; x + 4 <= 12 + xlen;) {
+            const int64_t slen = m[x + 2] | (int64_t)m[x + 3] << 8;
+            if (m[x] == 66 && m[x + 1] == 67 && slen == 2 && x + 6 <= 12 + xlen) bsize = (m[x + 4] | (int64_t)m[x + 5] << 8) + 1;
+            x += 4 + slen;
+        }
+        if (bsize < 12 + xlen + 8) {
+            set_error("ma_bgzf_inflate: member without a BGZF block size at byte " + std::to_string(at));
+            return MA_ERR_ARG;
+        }
+        if (n - at < bsize) break;
+        const uint8_t *t = m + bsize - 4;
+        const int64_t isize = t[0] | (int64_t)t[1] << 8 | (int64_t)t[2] << 16 | (int64_t)t[3] << 24;
+        at += bsize;
+        soff.push_back(at);
+        doff.push_back(doff.back() + isize);
+    }
+    *used = at;
+    *out_n = doff.back();
+    if (!dst) return MA_OK;
+    MA_REQUIRE(dst_cap >= doff.back(), "ma_bgzf_inflate: output buffer too small");
+    const int64_t nm = (int64_t)soff.size() - 1;
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(std::max(threads, 1), nm));
+    std::vector<int64_t> bad(T, -1);
+    auto work = [&](int w) {
+        for (int64_t i = nm * w / T; i < nm * (w + 1) / T; i++) {
+            const uint8_t *m = src + soff[i];
+            const int64_t xlen = m[10] | (int64_t)m[11] << 8;
+            const int64_t clen = soff[i + 1] - soff[i] - 12 - xlen - 8;
+            const int64_t dlen = doff[i + 1] - doff[i];
+            z_stream zs{};
+            if (inflateInit2(&zs, -15) != Z_OK) {
+                bad[w] = i;
+                return;
+            }
+            zs.next_in = const_cast<Bytef *>(m + 12 + xlen);
+            zs.avail_in = (uInt)clen;
+            zs.next_out = dst + doff[i];
+            zs.avail_out = (uInt)dlen;
+            const int rc = inflate(&zs, Z_FINISH);
+            const bool ok = rc == Z_STREAM_END && (int64_t)zs.total_out == dlen;
+            inflateEnd(&zs);
+            const uint8_t *c = src + soff[i + 1] - 8;
+            const uint32_t crc = c[0] | (uint32_t)c[1] << 8 | (uint32_t)c[2] << 16 | (uint32_t)c[3] << 24;
+            if (!ok || (uint32_t)crc32(0L, dst + doff[i], (uInt)dlen) != crc) {
+                bad[w] = i;
+                return;
+            }
+        }
+    };
+    if (T == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int w = 0; w < T; w++) pool.emplace_back(work, w);
+        for (auto &th : pool) th.join();
+    }
+    for (int w = 0; w < T; w++)
+        if (bad[w] >= 0) {
+            set_error("ma_bgzf_inflate: corrupt BGZF member at byte " + std::to_string(soff[bad[w]]) +
+                      " (inflate, ISIZE or CRC32)");
+            return MA_ERR_IO;
+        }
+    return MA_OK;
+}
+
 int ma_bgzf_open(const char *path, int32_t level, int32_t threads, ma_bgzf **out) {
     MA_REQUIRE(path && out, "ma_bgzf_open: bad arguments");
     FILE *fh = fopen(path, "wb");

@@ -14,7 +14,9 @@ to 'A' (upper case; the Python reader keeps case), rows are padded with 'N'.
 
 from __future__ import annotations
 
+import ctypes as C
 import gzip
+import os
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 from typing import Optional
@@ -102,10 +104,75 @@ class FastqParser:
         return pi
 
 
+def _is_bgzf(path: str) -> bool:
+    """The file starts with a BGZF member (bgzip output): a gzip header with
+    FEXTRA whose first extra subfield is BC with two bytes."""
+    with open(path, "rb") as fh:
+        h = fh.read(16)
+    return len(h) == 16 and h[:4] == b"\x1f\x8b\x08\x04" and h[12:16] == b"BC\x02\x00"
+
+
+class _BgzfFile:
+    """A BGZF-compressed file read as its inflated bytes (readinto, like a
+    file). The members carry their sizes, so a chunk's whole members are
+    inflated at once by `threads` native workers (ma_bgzf_inflate), each
+    member's ISIZE and CRC32 checked; other gzip files go through gzip.open on
+    one thread."""
+
+    def __init__(self, path: str, chunk: int = 32 << 20, threads: Optional[int] = None):
+        self.path = path
+        self.fh = open(path, "rb", buffering=0)
+        self.src = np.empty(max(int(chunk), 1 << 17), np.uint8)  # >= two 64 KiB members
+        self.have = 0  # compressed bytes at the front of src not inflated yet
+        self.out = np.empty(0, np.uint8)
+        self.pos = self.end = 0
+        self.eof = False
+        self.threads = int(threads) if threads else max(1, min(16, os.cpu_count() or 1))
+
+    def _refill(self) -> bool:
+        used, outn = C.c_int64(), C.c_int64()
+        while True:
+            if not self.eof and self.have < self.src.size:
+                k = self.fh.readinto(memoryview(self.src)[self.have:])
+                if k:
+                    self.have += k
+                else:
+                    self.eof = True
+            if self.have == 0 and self.eof:
+                return False
+            check(lib().ma_bgzf_inflate(ptr(self.src), self.have, self.threads, None, 0, C.byref(used),
+                                        C.byref(outn)), "ma_bgzf_inflate")
+            if used.value == 0:
+                if self.eof:
+                    raise MialignError(f"truncated BGZF member at the end of {self.path}")
+                continue
+            if self.out.size < outn.value:
+                self.out = np.empty(outn.value, np.uint8)
+            check(lib().ma_bgzf_inflate(ptr(self.src), used.value, self.threads, ptr(self.out), self.out.size,
+                                        C.byref(used), C.byref(outn)), "ma_bgzf_inflate")
+            rest = self.have - used.value
+            self.src[:rest] = self.src[used.value:self.have].copy()
+            self.have = rest
+            self.pos, self.end = 0, outn.value
+            if self.end:
+                return True
+
+    def readinto(self, mv) -> int:
+        if self.pos == self.end and not self._refill():
+            return 0
+        k = min(len(mv), self.end - self.pos)
+        mv[:k] = memoryview(self.out)[self.pos:self.pos + k]
+        self.pos += k
+        return k
+
+    def close(self):
+        self.fh.close()
+
+
 def _open_stream(path: str):
     low = path.lower()
     if low.endswith(".gz") or low.endswith(".gzip"):
-        return gzip.open(path, "rb")
+        return _BgzfFile(path) if _is_bgzf(path) else gzip.open(path, "rb")
     return open(path, "rb", buffering=0)
 
 

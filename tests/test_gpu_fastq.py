@@ -75,19 +75,23 @@ def test_parse_in_pieces_matches_oracle():
     assert total == 3000
 
 
-@pytest.mark.parametrize("gz", [False, True])
+@pytest.mark.parametrize("gz", ["plain", "gzip", "bgzf"])
 def test_reader_over_files(tmp_path, gz):
     """The streaming reader's batches (small batches and chunks, so tails are
-    carried across buffers) equal one oracle pass over each file; R2 shorter
-    than R1 raises the reference's error."""
+    carried across buffers) equal one oracle pass over each file, plain,
+    gzip or BGZF (members inflated in parallel); R2 shorter than R1 raises the
+    reference's error."""
     say(f"test_reader_over_files gz={gz}")
     d1 = random_fastq(41, 5000, crlf=0.1)
     d2 = random_fastq(42, 5000, crlf=0.1)
-    p1, p2 = tmp_path / ("r1.fq.gz" if gz else "r1.fq"), tmp_path / ("r2.fq.gz" if gz else "r2.fq")
+    sfx = "fq" if gz == "plain" else "fq.gz"
+    p1, p2 = tmp_path / f"r1.{sfx}", tmp_path / f"r2.{sfx}"
     for p, d in ((p1, d1), (p2, d2)):
-        if gz:
+        if gz == "gzip":
             with gzip.open(p, "wb") as fh:
                 fh.write(d)
+        elif gz == "bgzf":
+            p.write_bytes(orc.bgzf_stream(d, 1))
         else:
             p.write_bytes(d)
     w1, w2 = orc.fastq_parse(d1, 256, 1), orc.fastq_parse(d2, 256, 2)
