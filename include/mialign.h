@@ -401,6 +401,15 @@ int ma_bgzf_compress(const uint8_t *src, int64_t n, int32_t level, int32_t threa
  * MA_ERR_ARG for bytes that are not a BGZF member, MA_ERR_IO for a corrupt one. */
 int ma_bgzf_inflate(const uint8_t *src, int64_t n, int32_t threads, uint8_t *dst, int64_t dst_cap,
                     int64_t *used, int64_t *out_n);
+/* gzip FASTQ input of any framing (replaces the same `pigz -dc` / `gzip -dc`
+ * pipe, fq2bam-meth/src/linear_fastq.mojo:141-150): zlib's gz reader over a
+ * file, every member inflated in turn, each CRC32 and ISIZE checked.
+ * ma_gz_read fills dst with up to cap inflated bytes (*got = 0 at the end);
+ * MA_ERR_ARG if the file is not gzip, MA_ERR_IO for a corrupt or truncated one. */
+typedef struct ma_gz ma_gz;
+int ma_gz_open(const char *path, ma_gz **out);
+int ma_gz_read(ma_gz *f, uint8_t *dst, int64_t cap, int64_t *got);
+int ma_gz_close(ma_gz *f);
 /* BGZF on the device: the same 65280-byte members (header, raw deflate body,
  * CRC32, ISIZE) of a device stream, compressed by the GPU: level 0 = stored
  * blocks, otherwise LZ77 (greedy, matches of 4..64 bytes) and one

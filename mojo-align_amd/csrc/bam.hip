@@ -837,6 +837,48 @@ int ma_bgzf_inflate(const uint8_t *src, int64_t n, int32_t threads, uint8_t *dst
     return MA_OK;
 }
 
+int ma_gz_open(const char *path, ma_gz **out) {
+    MA_REQUIRE(path && out, "ma_gz_open: bad arguments");
+    gzFile g = gzopen(path, "rb");
+    if (!g) {
+        set_error(std::string("ma_gz_open: cannot open ") + path);
+        return MA_ERR_IO;
+    }
+    gzbuffer(g, 1u << 20);
+    *out = reinterpret_cast<ma_gz *>(g);
+    return MA_OK;
+}
+
+int ma_gz_read(ma_gz *f, uint8_t *dst, int64_t cap, int64_t *got) {
+    MA_REQUIRE(f && (cap == 0 || dst) && cap >= 0 && got, "ma_gz_read: bad arguments");
+    gzFile g = reinterpret_cast<gzFile>(f);
+    int64_t n = 0;
+    while (n < cap) {
+        const int k = gzread(g, dst + n, (unsigned)std::min<int64_t>(cap - n, 1 << 30));
+        if (k > 0 && n == 0 && gzdirect(g)) {
+            set_error("ma_gz_read: not a gzip file");
+            return MA_ERR_ARG;
+        }
+        if (k <= 0) {
+            int err = Z_OK;
+            const char *msg = gzerror(g, &err);
+            if (k < 0 || (err != Z_OK && err != Z_STREAM_END)) {
+                set_error(std::string("ma_gz_read: ") + (msg ? msg : "inflate failed"));
+                return MA_ERR_IO;
+            }
+            break;
+        }
+        n += k;
+    }
+    *got = n;
+    return MA_OK;
+}
+
+int ma_gz_close(ma_gz *f) {
+    if (f) gzclose_r(reinterpret_cast<gzFile>(f));
+    return MA_OK;
+}
+
 int ma_bgzf_open(const char *path, int32_t level, int32_t threads, ma_bgzf **out) {
     MA_REQUIRE(path && out, "ma_bgzf_open: bad arguments");
     FILE *fh = fopen(path, "wb");

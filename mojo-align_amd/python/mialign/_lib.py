@@ -180,6 +180,9 @@ SIGNATURES = {
     "ma_bgzf_compress_dev": (C.c_int, [P, P, I64, I32, I32, P, I64, P, P, I64]),
     "ma_bgzf_compress": (C.c_int, [P, I64, I32, I32, I32, P, I64, C.POINTER(I64)]),
     "ma_bgzf_inflate": (C.c_int, [P, I64, I32, P, I64, C.POINTER(I64), C.POINTER(I64)]),
+    "ma_gz_open": (C.c_int, [C.c_char_p, C.POINTER(P)]),
+    "ma_gz_read": (C.c_int, [P, P, I64, C.POINTER(I64)]),
+    "ma_gz_close": (C.c_int, [P]),
     "ma_bgzf_open": (C.c_int, [C.c_char_p, I32, I32, C.POINTER(P)]),
     "ma_bgzf_write": (C.c_int, [P, P, I64]),
     "ma_bgzf_close": (C.c_int, [P, C.POINTER(I64)]),

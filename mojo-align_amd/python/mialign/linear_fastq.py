@@ -15,7 +15,6 @@ to 'A' (upper case; the Python reader keeps case), rows are padded with 'N'.
 from __future__ import annotations
 
 import ctypes as C
-import gzip
 import os
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
@@ -116,7 +115,7 @@ class _BgzfFile:
     """A BGZF-compressed file read as its inflated bytes (readinto, like a
     file). The members carry their sizes, so a chunk's whole members are
     inflated at once by `threads` native workers (ma_bgzf_inflate), each
-    member's ISIZE and CRC32 checked; other gzip files go through gzip.open on
+    member's ISIZE and CRC32 checked; other gzip files go through _GzFile on
     one thread."""
 
     def __init__(self, path: str, chunk: int = 32 << 20, threads: Optional[int] = None):
@@ -169,10 +168,33 @@ class _BgzfFile:
         self.fh.close()
 
 
+class _GzFile:
+    """A gzip file of any framing read as its inflated bytes (readinto, like a
+    file) by zlib's gz reader in native code (ma_gz_*): every member in turn,
+    each CRC32 and ISIZE checked, on one thread."""
+
+    def __init__(self, path: str):
+        self.path = path
+        self.h = C.c_void_p()
+        check(lib().ma_gz_open(path.encode(), C.byref(self.h)), "ma_gz_open")
+
+    def readinto(self, mv) -> int:
+        got = C.c_int64()
+        a = np.frombuffer(mv, np.uint8)
+        check(lib().ma_gz_read(self.h, a.ctypes.data if a.size else None, a.size, C.byref(got)),
+              f"ma_gz_read ({self.path})")
+        return got.value
+
+    def close(self):
+        if self.h:
+            lib().ma_gz_close(self.h)
+            self.h = C.c_void_p()
+
+
 def _open_stream(path: str):
     low = path.lower()
     if low.endswith(".gz") or low.endswith(".gzip"):
-        return _BgzfFile(path) if _is_bgzf(path) else gzip.open(path, "rb")
+        return _BgzfFile(path) if _is_bgzf(path) else _GzFile(path)
     return open(path, "rb", buffering=0)
 
 

@@ -1,5 +1,6 @@
-"""BGZF-compressed FASTQ input (bgzip output) on the host: members inflated in
-parallel by ma_bgzf_inflate behind the reader's file interface. Expected bytes
+"""Compressed FASTQ input on the host: BGZF (bgzip output) members inflated in
+parallel by ma_bgzf_inflate, other gzip by zlib's gz reader (ma_gz_*), both
+behind the reader's file interface. Expected bytes
 come from Python's gzip module (multi-member gzip), independent of the native
 inflate; members come from the oracle's BGZF writer and the native compressor."""
 
@@ -54,14 +55,29 @@ def test_bgzf_fastq_reads_back(tmp_path, writer, threads):
     s.close()
 
 
-def test_plain_gzip_goes_through_the_gzip_module(tmp_path):
-    data = fastq_text(8, 500)
+def test_plain_gzip_reads_back_natively(tmp_path):
+    """Single- and multi-member gzip (not BGZF) through zlib's gz reader; a
+    flipped CRC32, a cut file and a file that is not gzip raise."""
+    data = fastq_text(8, 3000)
+    z = gzip.compress(data[:400_000]) + gzip.compress(data[400_000:])  # two members
     p = tmp_path / "r.fq.gz"
-    p.write_bytes(gzip.compress(data))
+    p.write_bytes(z)
     assert not lf._is_bgzf(str(p))
     s = lf._open_stream(str(p))
-    assert not isinstance(s, lf._BgzfFile) and read_all(s, 4096) == data
+    assert isinstance(s, lf._GzFile) and read_all(s, 65_537) == data
     s.close()
+    one = gzip.compress(data)
+    bad = bytearray(one)
+    bad[-6] ^= 0xFF  # CRC32
+    p.write_bytes(bytes(bad))
+    with pytest.raises(MialignError, match="ma_gz_read"):
+        read_all(lf._GzFile(str(p)), 1 << 16)
+    p.write_bytes(one[: len(one) // 2])
+    with pytest.raises(MialignError, match="ma_gz_read"):
+        read_all(lf._GzFile(str(p)), 1 << 16)
+    p.write_bytes(data[:5000])
+    with pytest.raises(MialignError, match="not a gzip file"):
+        read_all(lf._GzFile(str(p)), 1 << 16)
 
 
 def test_bgzf_errors(tmp_path):
