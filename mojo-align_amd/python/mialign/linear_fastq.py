@@ -128,37 +128,42 @@ class _BgzfFile:
         self.eof = False
         self.threads = int(threads) if threads else max(1, min(16, os.cpu_count() or 1))
 
-    def _refill(self) -> bool:
-        used, outn = C.c_int64(), C.c_int64()
-        while True:
-            if not self.eof and self.have < self.src.size:
-                k = self.fh.readinto(memoryview(self.src)[self.have:])
-                if k:
-                    self.have += k
-                else:
-                    self.eof = True
-            if self.have == 0 and self.eof:
-                return False
-            check(lib().ma_bgzf_inflate(ptr(self.src), self.have, self.threads, None, 0, C.byref(used),
-                                        C.byref(outn)), "ma_bgzf_inflate")
-            if used.value == 0:
-                if self.eof:
-                    raise MialignError(f"truncated BGZF member at the end of {self.path}")
-                continue
-            if self.out.size < outn.value:
-                self.out = np.empty(outn.value, np.uint8)
-            check(lib().ma_bgzf_inflate(ptr(self.src), used.value, self.threads, ptr(self.out), self.out.size,
-                                        C.byref(used), C.byref(outn)), "ma_bgzf_inflate")
-            rest = self.have - used.value
-            self.src[:rest] = self.src[used.value:self.have].copy()
-            self.have = rest
-            self.pos, self.end = 0, outn.value
-            if self.end:
-                return True
-
     def readinto(self, mv) -> int:
-        if self.pos == self.end and not self._refill():
-            return 0
+        if self.pos == self.end:
+            used, outn = C.c_int64(), C.c_int64()
+            while True:  # the next whole members, inflated straight into mv when they fit
+                if not self.eof and self.have < self.src.size:
+                    k = self.fh.readinto(memoryview(self.src)[self.have:])
+                    if k:
+                        self.have += k
+                    else:
+                        self.eof = True
+                if self.have == 0 and self.eof:
+                    return 0
+                check(lib().ma_bgzf_inflate(ptr(self.src), self.have, self.threads, None, 0, C.byref(used),
+                                            C.byref(outn)), "ma_bgzf_inflate")
+                if used.value == 0:
+                    if self.eof:
+                        raise MialignError(f"truncated BGZF member at the end of {self.path}")
+                    continue
+                n = outn.value
+                direct = 0 < n <= len(mv)
+                if direct:
+                    dst = np.frombuffer(mv, np.uint8)
+                else:
+                    if self.out.size < n:
+                        self.out = np.empty(n, np.uint8)
+                    dst = self.out
+                check(lib().ma_bgzf_inflate(ptr(self.src), used.value, self.threads, dst.ctypes.data if n else None,
+                                            dst.size, C.byref(used), C.byref(outn)), "ma_bgzf_inflate")
+                rest = self.have - used.value
+                self.src[:rest] = self.src[used.value:self.have].copy()
+                self.have = rest
+                if direct:
+                    return n
+                if n:
+                    self.pos, self.end = 0, n
+                    break
         k = min(len(mv), self.end - self.pos)
         mv[:k] = memoryview(self.out)[self.pos:self.pos + k]
         self.pos += k
