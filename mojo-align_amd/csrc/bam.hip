@@ -784,6 +784,10 @@ int ma_bgzf_inflate(const uint8_t *src, int64_t n, int32_t threads, uint8_t *dst
         if (n - at < bsize) break;
         const uint8_t *t = m + bsize - 4;
         const int64_t isize = t[0] | (int64_t)t[1] << 8 | (int64_t)t[2] << 16 | (int64_t)t[3] << 24;
+        if (isize > 65536) {  // a BGZF member holds at most 64 KiB
+            set_error("ma_bgzf_inflate: corrupt BGZF member at byte " + std::to_string(at) + " (ISIZE)");
+            return MA_ERR_IO;
+        }
         at += bsize;
         soff.push_back(at);
         doff.push_back(doff.back() + isize);

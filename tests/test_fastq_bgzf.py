@@ -92,6 +92,13 @@ def test_bgzf_errors(tmp_path):
     p.write_bytes(bytes(bad))
     with pytest.raises(MialignError, match="corrupt BGZF member"):
         read_all(lf._BgzfFile(str(p), chunk=1 << 17), 1 << 16)
+    # an ISIZE beyond a BGZF member's 64 KiB, rejected before any buffer is sized by it
+    bad = bytearray(z)
+    bad[second - 1] = 0x7F
+    p = tmp_path / "isize.fq.gz"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(MialignError, match="ISIZE"):
+        read_all(lf._BgzfFile(str(p), chunk=1 << 17), 1 << 16)
     # the file cut inside a member
     p = tmp_path / "cut.fq.gz"
     p.write_bytes(bytes(z[: second + 100]))
